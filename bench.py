@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Benchmark of the DVM time-step hot path on one MI355X.
+
+Workload (one "step"): the isolated regularised Biot-Savart direct sum —
+N = 262,144 blobs (x ~ U[-1,15], z ~ U[-3,3], circ ~ N(0, 2e-4) mean-removed,
+sigma = 0.01) evaluated blob-on-blob (computeSelfInduced) plus 1,024 probes on
+z = 0 (computeInducedAt).  Inputs are resident in HBM before timing starts.
+
+value = pair-interactions/s of the FP32-compensated form (the chosen form);
+the plain FP64 form, its error against FP64, and the reference-semantics CPU
+oracle (OpenMP, this host's cores) are reported beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOP_PER_PAIR = 11  # 2 sub, 3 for r^2, 1 reciprocal, 1 for K*circ, 2 FMAs (4 flop)
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector (AMD published)
+PEAK_FP64_TFLOPS = 78.6
+
+
+def make_inputs(n, n_probes, seed=2024):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1.0, 15.0, n)
+    z = rng.uniform(-3.0, 3.0, n)
+    c = rng.normal(0.0, 2e-4, n)
+    c -= c.mean()
+    s = np.full(n, 0.01)
+    tx = np.linspace(-1.0, 15.0, n_probes)
+    tz = np.zeros(n_probes)
+    return x, z, c, s, tx, tz
+
+
+def time_form(ctx, dvm, dev, prec, steps, warmup, outs):
+    import torch
+    X, Z, C, S, TX, TZ = dev
+    (u, w), (pu, pw) = outs
+
+    def step():
+        dvm.bs_self(ctx, X, Z, C, S, prec=prec, out=(u, w))
+        dvm.bs_targets(ctx, X, Z, C, S, TX, TZ, prec=prec, out=(pu, pw))
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.prof_reset()
+    ctx.prof_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.prof_enable(False)
+    name = "f32c" if prec == dvm.F32C else "f64"
+    self_ms, self_n = ctx.prof_read("bs_self_" + name)
+    tgt_ms, tgt_n = ctx.prof_read("bs_targets_" + name)
+    return elapsed, (self_ms / max(self_n, 1)), (tgt_ms / max(tgt_n, 1)), self_n
+
+
+def cpu_baseline(x, z, c, s, budget_s):
+    """The oracle (reference loop order, FP64, OpenMP) on a bounded sample of
+    targets; each target sums over all N sources with the self pair skipped."""
+    import oracle
+    n = x.size
+    order = np.random.default_rng(1).permutation(n)
+    chunk = 4096
+    done = 0
+    t0 = time.perf_counter()
+    while done < n and time.perf_counter() - t0 < budget_s:
+        idx = order[done:done + chunk]
+        oracle.bs_targets(x, z, c, s, x[idx], z[idx])
+        done += idx.size
+    dt = time.perf_counter() - t0
+    return {"value": done * n / dt, "unit": "pair-interactions/s",
+            "cores": int(oracle.lib().orc_num_threads()), "kind": "port",
+            "sample": f"{done} blob targets x {n} sources (self pair skipped), FP64 reference "
+                      f"loop order, OpenMP, {dt:.1f} s"}
+
+
+def load_traffic(kernel):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=262144)
+    ap.add_argument("--probes", type=int, default=1024)
+    ap.add_argument("--f64-steps", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(simulations would shard across GPUs with no communication; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import dvmpp_amd as dvm
+
+    n, npr = args.n, args.probes
+    x, z, c, s, tx, tz = make_inputs(n, npr)
+    ctx = dvm.Context(0)
+    dev = [torch.from_numpy(a).to("cuda") for a in (x, z, c, s, tx, tz)]
+
+    def outs(m):
+        return (torch.empty(m, dtype=torch.float64, device="cuda"),
+                torch.empty(m, dtype=torch.float64, device="cuda"))
+
+    o32 = (outs(n), outs(npr))
+    o64 = (outs(n), outs(npr))
+    pairs_self = n * (n - 1)
+    pairs_probe = npr * n
+    pairs = pairs_self + pairs_probe
+
+    el32, self32_ms, tgt32_ms, launches = time_form(ctx, dvm, dev, dvm.F32C, args.steps,
+                                                    args.warmup, o32)
+    el64, self64_ms, tgt64_ms, _ = time_form(ctx, dvm, dev, dvm.F64, args.f64_steps, 1, o64)
+
+    # accuracy of the timed FP32C result against the FP64 form
+    u32, w32 = (t.cpu().numpy() for t in o32[0])
+    u64, w64 = (t.cpu().numpy() for t in o64[0])
+    v = np.hypot(u64, w64)
+    d = np.hypot(u32 - u64, w32 - w64)
+    rel = d / np.maximum(v, 1e-300)
+
+    ms_step = el32 / args.steps * 1e3
+    value = pairs / (el32 / args.steps)
+    achieved_tf = FLOP_PER_PAIR * pairs_self / (self32_ms * 1e-3) / 1e12
+    line = {
+        "metric": "Biot-Savart pair-interactions/s + sim steps/s at 1 GPU; p99 served latency",
+        "value": value,
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"isolated Biot-Savart direct sum: {n} blobs self + {npr} probes, "
+                        "x~U[-1,15], z~U[-3,3], circ~N(0,2e-4) mean-removed, sigma=0.01",
+            "n_blobs": n, "n_probes": npr, "pairs_per_step": pairs,
+            "form": "FP32 pair terms, tile-recentred Morton-sorted sources, Kahan across tiles",
+        },
+        "roofline": {
+            "bound": "valu",
+            "kernel": "bs_f32c_kernel (blob-on-blob)",
+            "achieved": achieved_tf,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tf / PEAK_FP32_TFLOPS,
+            "traffic": load_traffic("bs_f32c_kernel"),
+            "kernel_ms": self32_ms,
+            "flop_per_launch": FLOP_PER_PAIR * pairs_self,
+        },
+        "f32c_error_vs_f64": {"normwise_max": float(d.max() / v.max()),
+                              "p99_rel": float(np.percentile(rel, 99)),
+                              "p50_rel": float(np.percentile(rel, 50))},
+        "f64_plain": {
+            "pair_interactions_per_s": pairs / (el64 / args.f64_steps),
+            "ms_per_step": el64 / args.f64_steps * 1e3,
+            "self_kernel_ms": self64_ms,
+            "achieved_tflops": FLOP_PER_PAIR * pairs_self / (self64_ms * 1e-3) / 1e12,
+            "frac_of_fp64_peak": FLOP_PER_PAIR * pairs_self / (self64_ms * 1e-3) / 1e12
+            / PEAK_FP64_TFLOPS,
+        },
+        "probe_kernel_ms": {"f32c": tgt32_ms, "f64": tgt64_ms},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(x, z, c, s, args.cpu_seconds)
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,110 @@
+"""MI355X-native DVM time-step hot path (gfx950 HIP kernels behind a C-ABI).
+
+Python face of libdvm_hip.so for tests and benchmarks.  The product is the
+C-ABI in include/dvm_hip.h and the C++17 drop-in classes in dvmpp_amd/host;
+this module only moves torch device tensors across that boundary.
+"""
+import ctypes
+
+from ._lib import F32C, F64, DvmError, check, load  # noqa: F401
+
+__all__ = ["F64", "F32C", "DvmError", "Context", "bs_self", "bs_targets", "load"]
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _dev_f64(t, device, name):
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name}: expected a torch tensor")
+    if t.dtype != torch.float64 or not t.is_contiguous() or t.device != device:
+        raise ValueError(f"{name}: expected a contiguous float64 tensor on {device}")
+    return t
+
+
+class Context:
+    """dvm_ctx_t bound to torch's current stream on `device` (so torch events
+    time the library's kernels)."""
+
+    def __init__(self, device=0, stream=None):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("no GPU visible: the DVM library has no CPU path")
+        lib = load()
+        self.device = torch.device("cuda", device)
+        torch.cuda.set_device(self.device)
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        self.stream = stream
+        h = ctypes.c_void_p()
+        st = lib.dvm_ctx_create(device, ctypes.c_void_p(stream.cuda_stream), ctypes.byref(h))
+        if st != 0:
+            raise DvmError(st, "dvm_ctx_create failed")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().dvm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        check(self.h, load().dvm_ctx_sync(self.h))
+
+    def prof_enable(self, on=True):
+        check(self.h, load().dvm_prof_enable(self.h, 1 if on else 0))
+
+    def prof_reset(self):
+        check(self.h, load().dvm_prof_reset(self.h))
+
+    def prof_read(self, kernel):
+        """(total_ms, launches) of a named kernel since the last reset."""
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        check(self.h, load().dvm_prof_read(self.h, kernel.encode(), ctypes.byref(ms),
+                                           ctypes.byref(n)))
+        return ms.value, n.value
+
+
+def bs_self(ctx, x, z, circ, sigma, prec=F64, out=None):
+    """Blob-on-blob velocities (u, w) as float64 device tensors."""
+    import torch
+    n = x.numel()
+    for t, nm in ((x, "x"), (z, "z"), (circ, "circ"), (sigma, "sigma")):
+        _dev_f64(t, ctx.device, nm)
+        if t.numel() != n:
+            raise ValueError("size mismatch")
+    if out is None:
+        out = (torch.empty(n, dtype=torch.float64, device=ctx.device),
+               torch.empty(n, dtype=torch.float64, device=ctx.device))
+    u, w = out
+    check(ctx.h, load().dvm_bs_self(ctx.h, prec, n, _ptr(x), _ptr(z), _ptr(circ), _ptr(sigma),
+                                    _ptr(u), _ptr(w)))
+    return u, w
+
+
+def bs_targets(ctx, sx, sz, scirc, ssigma, tx, tz, prec=F64, guard=True, r2_skip=1e-15, out=None):
+    """Velocities induced by sources at targets (computeInducedAt when guard)."""
+    import torch
+    ns, nt = sx.numel(), tx.numel()
+    for t, nm in ((sx, "sx"), (sz, "sz"), (scirc, "scirc"), (ssigma, "ssigma")):
+        _dev_f64(t, ctx.device, nm)
+        if t.numel() != ns:
+            raise ValueError("size mismatch")
+    _dev_f64(tx, ctx.device, "tx")
+    _dev_f64(tz, ctx.device, "tz")
+    if out is None:
+        out = (torch.empty(nt, dtype=torch.float64, device=ctx.device),
+               torch.empty(nt, dtype=torch.float64, device=ctx.device))
+    u, w = out
+    check(ctx.h, load().dvm_bs_targets(ctx.h, prec, ns, _ptr(sx), _ptr(sz), _ptr(scirc),
+                                       _ptr(ssigma), nt, _ptr(tx), _ptr(tz), 1 if guard else 0,
+                                       r2_skip, _ptr(u), _ptr(w)))
+    return u, w

@@ -1,0 +1,604 @@
+// Regularised Biot-Savart direct sum for gfx950.
+//
+// Two arithmetic forms behind one entry point (dvm_bs_self / dvm_bs_targets):
+//
+//  DVM_F64  (parity form) — one target per lane, sources streamed with
+//           wave-uniform addresses so they arrive through the scalar cache
+//           into SGPRs; the per-target loop visits sources in index order and
+//           evaluates each pair exactly as src/VelocityKernel.cpp:42-61 does
+//           (FP64, correctly rounded divide, no contraction).  When there are
+//           too few targets to fill 256 CUs the source range is split into
+//           chunks whose partial sums are added in chunk order.
+//
+//  DVM_F32C (fast form) — sources are Morton-sorted, cut into tiles of 256 and
+//           stored as FP32 offsets from their tile centre (FP64).  A target
+//           block (256 targets) is likewise recentred on its own centre, so a
+//           pair's dx is formed from two small FP32 offsets and one per-tile
+//           FP32 centre difference: the cancellation that plain FP32 would
+//           suffer at |x| ~ 15 with 0.01 cores is avoided.  A tile pair whose
+//           bounding boxes are farther apart than sqrt(10)*max(sigma) takes
+//           the far-field loop (1/r^2 only: 7 VALU ops + 1 reciprocal per
+//           pair); the others take the full regularised loop.  Partial sums
+//           within a tile accumulate in FP32 with FMA; tile partials are added
+//           with Kahan compensation and the result is returned in FP64.
+#include "dvm_internal.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <vector>
+
+namespace dvmi {
+
+constexpr int kBlock = 256;  // targets per block (one per lane, 4 waves)
+constexpr int kTile = 256;   // sources per F32C tile
+
+// ----------------------------------------------------------------- FP64 form
+
+template <bool SELF, bool GUARD>
+__global__ __launch_bounds__(kBlock) void bs_f64_kernel(
+    int64_t nt, const double* __restrict__ tx, const double* __restrict__ tz, int64_t ns,
+    const double* __restrict__ sx, const double* __restrict__ sz, const double* __restrict__ sc,
+    const double* __restrict__ ss, int64_t chunk, double r2_skip, double* __restrict__ u,
+    double* __restrict__ w, double* __restrict__ part)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t it = i < nt ? i : nt - 1;
+    const double xi = tx[it];
+    const double zi = tz[it];
+    const int64_t j0 = (int64_t)blockIdx.y * chunk;
+    const int64_t j1 = j0 + chunk < ns ? j0 + chunk : ns;
+    double ui = 0.0;
+    double wi = 0.0;
+
+    // One pair, in the order of src/VelocityKernel.cpp:44-60.
+    auto pair = [&](int64_t j, bool skip_self) {
+        const double xj = sx[j];
+        const double zj = sz[j];
+        const double cj = sc[j];
+        const double sj = ss[j];
+        const double dx = xi - xj;
+        const double dz = zi - zj;
+        const double r2 = dx * dx + dz * dz;
+        bool take = true;
+        if (SELF) take = !(skip_self && j == i);
+        if (GUARD) take = take && (r2 > r2_skip);
+        const double s2 = sj * sj;
+        const double thr = 10.0 * s2;
+        double num = 1.0;
+        if (take && r2 < thr) {
+            const double rs = 1.0 / s2;
+            num = 1.0 - exp(-r2 * rs);
+        }
+        const double K = num / r2;
+        if (take) {
+            ui -= K * dz * cj;
+            wi += K * dx * cj;
+        }
+    };
+
+    if (SELF) {
+        // Only the block's own index range can contain j == i.
+        const int64_t b0 = (int64_t)blockIdx.x * kBlock;
+        const int64_t b1 = b0 + kBlock < nt ? b0 + kBlock : nt;
+        const int64_t a_end = j1 < b0 ? j1 : b0;
+        int64_t j = j0;
+#pragma unroll 4
+        for (; j < a_end; ++j) pair(j, false);
+        const int64_t m_end = j1 < b1 ? j1 : b1;
+        for (j = (j0 > b0 ? j0 : b0); j < m_end; ++j) pair(j, true);
+        j = j0 > b1 ? j0 : b1;
+#pragma unroll 4
+        for (; j < j1; ++j) pair(j, false);
+    } else {
+#pragma unroll 4
+        for (int64_t j = j0; j < j1; ++j) pair(j, false);
+    }
+
+    if (i < nt) {
+        if (part == nullptr) {
+            u[i] = ui * kRpi2;
+            w[i] = wi * kRpi2;
+        } else {
+            part[((int64_t)blockIdx.y * 2 + 0) * nt + i] = ui;
+            part[((int64_t)blockIdx.y * 2 + 1) * nt + i] = wi;
+        }
+    }
+}
+
+// Adds chunk partials in chunk order and applies 1/(2 pi).  If out_index is
+// given, result i goes to out_index[i].
+__global__ void bs_reduce_kernel(int64_t nt, int nchunks, const double* __restrict__ part,
+                                 const uint32_t* __restrict__ out_index, double* __restrict__ u,
+                                 double* __restrict__ w)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    double su = 0.0, sw = 0.0;
+    for (int c = 0; c < nchunks; ++c) {
+        su += part[((int64_t)c * 2 + 0) * nt + i];
+        sw += part[((int64_t)c * 2 + 1) * nt + i];
+    }
+    const int64_t o = out_index ? (int64_t)out_index[i] : i;
+    u[o] = su * kRpi2;
+    w[o] = sw * kRpi2;
+}
+
+// ----------------------------------------------------------------- F32C form
+
+struct TileInfo {
+    double cx, cz;    // tile centre (FP64)
+    float hx, hz;     // half extents, rounded up
+    float thr_max;    // 10 * max(sigma)^2 over the tile's sources
+    float pad;
+};
+
+__global__ void bbox_partial_kernel(int64_t n, const double* __restrict__ x,
+                                    const double* __restrict__ z, double4* __restrict__ part)
+{
+    __shared__ double4 red[kBlock];
+    double4 b = make_double4(INFINITY, -INFINITY, INFINITY, -INFINITY);
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock) {
+        const double xi = x[i], zi = z[i];
+        b.x = fmin(b.x, xi); b.y = fmax(b.y, xi);
+        b.z = fmin(b.z, zi); b.w = fmax(b.w, zi);
+    }
+    red[threadIdx.x] = b;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            double4 o = red[threadIdx.x + s];
+            double4& m = red[threadIdx.x];
+            m.x = fmin(m.x, o.x); m.y = fmax(m.y, o.y);
+            m.z = fmin(m.z, o.z); m.w = fmax(m.w, o.w);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__device__ inline uint32_t spread16(uint32_t v)
+{
+    v &= 0xffffu;
+    v = (v | (v << 8)) & 0x00ff00ffu;
+    v = (v | (v << 4)) & 0x0f0f0f0fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+// Morton key on a square 65536^2 grid spanning the bounding box.
+__global__ void morton_kernel(int64_t n, const double* __restrict__ x, const double* __restrict__ z,
+                              int nparts, const double4* __restrict__ part,
+                              uint32_t* __restrict__ keys, uint32_t* __restrict__ idx)
+{
+    double4 b = part[0];
+    for (int p = 1; p < nparts; ++p) {
+        const double4 o = part[p];
+        b.x = fmin(b.x, o.x); b.y = fmax(b.y, o.y);
+        b.z = fmin(b.z, o.z); b.w = fmax(b.w, o.w);
+    }
+    const double ext = fmax(fmax(b.y - b.x, b.w - b.z), 1e-300);
+    const double scale = 65535.0 / ext;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double qx = (x[i] - b.x) * scale;
+    double qz = (z[i] - b.z) * scale;
+    qx = qx >= 0.0 ? (qx <= 65535.0 ? qx : 65535.0) : 0.0;  // NaN -> 0
+    qz = qz >= 0.0 ? (qz <= 65535.0 ? qz : 65535.0) : 0.0;
+    keys[i] = spread16((uint32_t)qx) | (spread16((uint32_t)qz) << 1);
+    idx[i] = (uint32_t)i;
+}
+
+// One block per tile of kTile consecutive (possibly permuted) points: FP64
+// centre and half extents, FP32 offsets, FP32 near-field constants.
+__global__ __launch_bounds__(kTile) void tile_build_kernel(
+    int64_t n, const uint32_t* __restrict__ perm, const double* __restrict__ x,
+    const double* __restrict__ z, const double* __restrict__ c, const double* __restrict__ s,
+    float4* __restrict__ p32, float2* __restrict__ near32, float2* __restrict__ t32,
+    TileInfo* __restrict__ tiles)
+{
+    __shared__ double red[6][kTile];
+    const int64_t k = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    const bool valid = k < n;
+    const int64_t src = valid ? (perm ? (int64_t)perm[k] : k) : 0;
+    const double xi = valid ? x[src] : 0.0;
+    const double zi = valid ? z[src] : 0.0;
+    const double si = (valid && s) ? s[src] : 0.0;
+    red[0][threadIdx.x] = valid ? xi : INFINITY;
+    red[1][threadIdx.x] = valid ? xi : -INFINITY;
+    red[2][threadIdx.x] = valid ? zi : INFINITY;
+    red[3][threadIdx.x] = valid ? zi : -INFINITY;
+    red[4][threadIdx.x] = si * si;
+    __syncthreads();
+    for (int st = kTile / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            const int o = threadIdx.x + st;
+            red[0][threadIdx.x] = fmin(red[0][threadIdx.x], red[0][o]);
+            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][o]);
+            red[2][threadIdx.x] = fmin(red[2][threadIdx.x], red[2][o]);
+            red[3][threadIdx.x] = fmax(red[3][threadIdx.x], red[3][o]);
+            red[4][threadIdx.x] = fmax(red[4][threadIdx.x], red[4][o]);
+        }
+        __syncthreads();
+    }
+    const double cx = 0.5 * (red[0][0] + red[1][0]);
+    const double cz = 0.5 * (red[2][0] + red[3][0]);
+    if (threadIdx.x == 0) {
+        TileInfo ti;
+        ti.cx = cx;
+        ti.cz = cz;
+        ti.hx = (float)(0.5 * (red[1][0] - red[0][0])) * 1.000001f + 1e-30f;
+        ti.hz = (float)(0.5 * (red[3][0] - red[2][0])) * 1.000001f + 1e-30f;
+        ti.thr_max = (float)(10.0 * red[4][0]);
+        ti.pad = 0.0f;
+        tiles[blockIdx.x] = ti;
+    }
+    if (!valid) return;
+    const float ox = (float)(xi - cx);
+    const float oz = (float)(zi - cz);
+    if (p32) {
+        const double ci = c[src];
+        p32[k] = make_float4(ox, oz, (float)ci, 0.0f);
+        const double s2 = si * si;
+        near32[k] = make_float2((float)(10.0 * s2), (float)(1.0 / s2));
+    }
+    if (t32) t32[k] = make_float2(ox, oz);
+}
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float x)
+{
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+template <bool SELF, bool GUARD>
+__global__ __launch_bounds__(kBlock) void bs_f32c_kernel(
+    int64_t nt, const float2* __restrict__ t32, const TileInfo* __restrict__ tblk, int64_t ns,
+    const float4* __restrict__ s32, const float2* __restrict__ near32,
+    const TileInfo* __restrict__ stile, int ntiles, int tiles_per_chunk, float r2_skip,
+    const uint32_t* __restrict__ out_index, double* __restrict__ u, double* __restrict__ w,
+    double* __restrict__ part)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t it = i < nt ? i : nt - 1;
+    const float2 t = t32[it];
+    const TileInfo tb = tblk[blockIdx.x];
+    float su = 0.0f, cu = 0.0f, sw = 0.0f, cw = 0.0f;
+    const int s0 = blockIdx.y * tiles_per_chunk;
+    const int s1 = s0 + tiles_per_chunk < ntiles ? s0 + tiles_per_chunk : ntiles;
+
+    for (int st = s0; st < s1; ++st) {
+        const TileInfo ts = stile[st];
+        const double Dxd = ts.cx - tb.cx;
+        const double Dzd = ts.cz - tb.cz;
+        const double gx = fmax(fabs(Dxd) - (double)(tb.hx + ts.hx), 0.0);
+        const double gz = fmax(fabs(Dzd) - (double)(tb.hz + ts.hz), 0.0);
+        // Far tile pair: every pair has r2 > 10 sigma_j^2 (and > r2_skip when
+        // guarded), so the 1/r^2 branch holds for all of them.
+        const double lim = 1.0001 * (double)ts.thr_max + 1e-300;
+        const bool near = gx * gx + gz * gz <= (GUARD ? fmax(lim, 1.0001 * (double)r2_skip) : lim);
+        const float txd = t.x - (float)Dxd;
+        const float tzd = t.y - (float)Dzd;
+        const int64_t jb = (int64_t)st * kTile;
+        const int cnt = ns - jb < kTile ? (int)(ns - jb) : kTile;
+        const float4* __restrict__ sp = s32 + jb;
+        float au = 0.0f, aw = 0.0f;
+        if (!near) {
+            if (cnt == kTile) {
+#pragma unroll 16
+                for (int k = 0; k < kTile; ++k) {
+                    const float4 sj = sp[k];
+                    const float dx = txd - sj.x;
+                    const float dz = tzd - sj.y;
+                    const float r2 = __builtin_fmaf(dx, dx, dz * dz);
+                    const float kk = __builtin_amdgcn_rcpf(r2) * sj.z;
+                    au = __builtin_fmaf(kk, dz, au);
+                    aw = __builtin_fmaf(kk, dx, aw);
+                }
+            } else {
+                for (int k = 0; k < cnt; ++k) {
+                    const float4 sj = sp[k];
+                    const float dx = txd - sj.x;
+                    const float dz = tzd - sj.y;
+                    const float r2 = __builtin_fmaf(dx, dx, dz * dz);
+                    const float kk = __builtin_amdgcn_rcpf(r2) * sj.z;
+                    au = __builtin_fmaf(kk, dz, au);
+                    aw = __builtin_fmaf(kk, dx, aw);
+                }
+            }
+        } else {
+            const float2* __restrict__ np = near32 + jb;
+            for (int k = 0; k < cnt; ++k) {
+                const float4 sj = sp[k];
+                const float2 nj = np[k];
+                const float dx = txd - sj.x;
+                const float dz = tzd - sj.y;
+                const float r2 = __builtin_fmaf(dx, dx, dz * dz);
+                const float inv = __builtin_amdgcn_rcpf(r2);
+                float K = inv;
+                if (r2 < nj.x) K = -expm1f(-r2 * nj.y) * inv;
+                bool take = true;
+                if (SELF) take = (jb + k) != i;
+                if (GUARD) take = take && (r2 > r2_skip);
+                const float kk = take ? K * sj.z : 0.0f;
+                au = __builtin_fmaf(kk, dz, au);
+                aw = __builtin_fmaf(kk, dx, aw);
+            }
+        }
+        kahan_add(su, cu, au);
+        kahan_add(sw, cw, aw);
+    }
+
+    if (i < nt) {
+        const double U = (double)su - (double)cu;
+        const double W = (double)sw - (double)cw;
+        if (part == nullptr) {
+            const int64_t o = out_index ? (int64_t)out_index[i] : i;
+            u[o] = -U * kRpi2;
+            w[o] = W * kRpi2;
+        } else {
+            part[((int64_t)blockIdx.y * 2 + 0) * nt + i] = -U;
+            part[((int64_t)blockIdx.y * 2 + 1) * nt + i] = W;
+        }
+    }
+}
+
+// ----------------------------------------------------------------- launch
+
+static int pick_chunks(dvm_ctx_t ctx, int64_t blocks_x, int64_t max_chunks)
+{
+    const int64_t want = 4LL * ctx->num_cus;
+    int64_t c = (want + blocks_x - 1) / blocks_x;
+    if (c > max_chunks) c = max_chunks;
+    if (c < 1) c = 1;
+    return (int)c;
+}
+
+static void launch_f64(dvm_ctx_t ctx, bool self, int64_t ns, const double* sx, const double* sz,
+                       const double* sc, const double* ss, int64_t nt, const double* tx,
+                       const double* tz, int guard, double r2_skip, double* u, double* w)
+{
+    const int64_t bx = cdiv(nt, kBlock);
+    // Keep chunks of >= 2048 sources; a single chunk keeps the reference's
+    // exact summation order.
+    const int nch = pick_chunks(ctx, bx, (ns + 2047) / 2048);
+    const int64_t chunk = (ns + nch - 1) / nch;
+    double* part = nullptr;
+    if (nch > 1) part = static_cast<double*>(ws_get(ctx, sizeof(double) * 2 * nt * nch));
+    dim3 grid((unsigned)bx, (unsigned)nch);
+    {
+        ProfScope ps(ctx, self ? "bs_self_f64" : "bs_targets_f64");
+        if (self) {
+            bs_f64_kernel<true, false><<<grid, kBlock, 0, ctx->stream>>>(
+                nt, tx, tz, ns, sx, sz, sc, ss, chunk, r2_skip, u, w, part);
+        } else if (guard) {
+            bs_f64_kernel<false, true><<<grid, kBlock, 0, ctx->stream>>>(
+                nt, tx, tz, ns, sx, sz, sc, ss, chunk, r2_skip, u, w, part);
+        } else {
+            bs_f64_kernel<false, false><<<grid, kBlock, 0, ctx->stream>>>(
+                nt, tx, tz, ns, sx, sz, sc, ss, chunk, r2_skip, u, w, part);
+        }
+        DVM_LAUNCH_CHECK();
+    }
+    if (part) {
+        bs_reduce_kernel<<<cdiv(nt, 256), 256, 0, ctx->stream>>>(nt, nch, part, nullptr, u, w);
+        DVM_LAUNCH_CHECK();
+    }
+}
+
+static void launch_f32c(dvm_ctx_t ctx, bool self, int64_t ns, const double* sx, const double* sz,
+                        const double* sc, const double* ss, int64_t nt, const double* tx,
+                        const double* tz, int guard, double r2_skip, double* u, double* w)
+{
+    const int ntiles = (int)cdiv(ns, kTile);
+    const int64_t bx = cdiv(nt, kBlock);
+    const int nbb = 64;
+    const int nch = pick_chunks(ctx, bx, ntiles);
+    const int tpc = (ntiles + nch - 1) / nch;
+
+    size_t sort_tmp = 0;
+    DVM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_tmp, (uint32_t*)nullptr,
+                                                     (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                     (uint32_t*)nullptr, (int)ns, 0, 32,
+                                                     ctx->stream));
+    const size_t need = Carver::need({
+        sizeof(uint32_t) * ns * 4, sizeof(double4) * nbb, sort_tmp, sizeof(float4) * ns,
+        sizeof(float2) * ns, sizeof(TileInfo) * ntiles, sizeof(float2) * nt,
+        sizeof(TileInfo) * bx, nch > 1 ? sizeof(double) * 2 * nt * nch : 0});
+    Carver cv(ws_get(ctx, need));
+    uint32_t* keys = cv.take<uint32_t>(ns);
+    uint32_t* keys2 = cv.take<uint32_t>(ns);
+    uint32_t* idx = cv.take<uint32_t>(ns);
+    uint32_t* perm = cv.take<uint32_t>(ns);
+    double4* bb = cv.take<double4>(nbb);
+    void* tmp = cv.take<char>(sort_tmp);
+    float4* s32 = cv.take<float4>(ns);
+    float2* near32 = cv.take<float2>(ns);
+    TileInfo* stile = cv.take<TileInfo>(ntiles);
+    float2* t32 = cv.take<float2>(nt);
+    TileInfo* tblk = cv.take<TileInfo>(bx);
+    double* part = nch > 1 ? cv.take<double>(2 * nt * nch) : nullptr;
+
+    bbox_partial_kernel<<<nbb, kBlock, 0, ctx->stream>>>(ns, sx, sz, bb);
+    DVM_LAUNCH_CHECK();
+    morton_kernel<<<cdiv(ns, 256), 256, 0, ctx->stream>>>(ns, sx, sz, nbb, bb, keys, idx);
+    DVM_LAUNCH_CHECK();
+    DVM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, sort_tmp, keys, keys2, idx, perm,
+                                                     (int)ns, 0, 32, ctx->stream));
+    tile_build_kernel<<<ntiles, kTile, 0, ctx->stream>>>(ns, perm, sx, sz, sc, ss, s32, near32,
+                                                         self ? t32 : nullptr, stile);
+    DVM_LAUNCH_CHECK();
+    const TileInfo* tb = stile;
+    if (!self) {
+        tile_build_kernel<<<(unsigned)bx, kTile, 0, ctx->stream>>>(
+            nt, nullptr, tx, tz, nullptr, nullptr, nullptr, nullptr, t32, tblk);
+        DVM_LAUNCH_CHECK();
+        tb = tblk;
+    }
+    const float skip = (float)r2_skip;
+    const uint32_t* oidx = self ? perm : nullptr;
+    dim3 grid((unsigned)bx, (unsigned)nch);
+    {
+        ProfScope ps(ctx, self ? "bs_self_f32c" : "bs_targets_f32c");
+        if (self) {
+            bs_f32c_kernel<true, false><<<grid, kBlock, 0, ctx->stream>>>(
+                nt, t32, tb, ns, s32, near32, stile, ntiles, tpc, skip, oidx, u, w, part);
+        } else if (guard) {
+            bs_f32c_kernel<false, true><<<grid, kBlock, 0, ctx->stream>>>(
+                nt, t32, tb, ns, s32, near32, stile, ntiles, tpc, skip, oidx, u, w, part);
+        } else {
+            bs_f32c_kernel<false, false><<<grid, kBlock, 0, ctx->stream>>>(
+                nt, t32, tb, ns, s32, near32, stile, ntiles, tpc, skip, oidx, u, w, part);
+        }
+        DVM_LAUNCH_CHECK();
+    }
+    if (part) {
+        bs_reduce_kernel<<<cdiv(nt, 256), 256, 0, ctx->stream>>>(nt, nch, part, oidx, u, w);
+        DVM_LAUNCH_CHECK();
+    }
+}
+
+void bs_self_launch(dvm_ctx_t ctx, int prec, int64_t n, const double* x, const double* z,
+                    const double* c, const double* s, double* u, double* w)
+{
+    require(prec == DVM_F64 || prec == DVM_F32C, DVM_E_INVALID, "unknown precision");
+    require(n >= 0 && n < (int64_t(1) << 31), DVM_E_SIZE, "blob count out of range");
+    if (n == 0) return;
+    require(x && z && c && s && u && w, DVM_E_INVALID, "null array");
+    if (prec == DVM_F64)
+        launch_f64(ctx, true, n, x, z, c, s, n, x, z, 0, 0.0, u, w);
+    else
+        launch_f32c(ctx, true, n, x, z, c, s, n, x, z, 0, 0.0, u, w);
+}
+
+void bs_targets_launch(dvm_ctx_t ctx, int prec, int64_t ns, const double* sx, const double* sz,
+                       const double* sc, const double* ss, int64_t nt, const double* tx,
+                       const double* tz, int guard, double r2_skip, double* u, double* w)
+{
+    require(prec == DVM_F64 || prec == DVM_F32C, DVM_E_INVALID, "unknown precision");
+    require(ns >= 0 && nt >= 0 && ns < (int64_t(1) << 31) && nt < (int64_t(1) << 31), DVM_E_SIZE,
+            "count out of range");
+    if (nt == 0) return;
+    require(tx && tz && u && w, DVM_E_INVALID, "null target array");
+    if (ns == 0) {
+        DVM_HIP_CHECK(hipMemsetAsync(u, 0, sizeof(double) * nt, ctx->stream));
+        DVM_HIP_CHECK(hipMemsetAsync(w, 0, sizeof(double) * nt, ctx->stream));
+        return;
+    }
+    require(sx && sz && sc && ss, DVM_E_INVALID, "null source array");
+    if (prec == DVM_F64)
+        launch_f64(ctx, false, ns, sx, sz, sc, ss, nt, tx, tz, guard, r2_skip, u, w);
+    else
+        launch_f32c(ctx, false, ns, sx, sz, sc, ss, nt, tx, tz, guard, r2_skip, u, w);
+}
+
+}  // namespace dvmi
+
+using namespace dvmi;
+
+extern "C" {
+
+dvm_status dvm_bs_self(dvm_ctx_t ctx, int prec, int64_t n, const double* x, const double* z,
+                       const double* circ, const double* sigma, double* u, double* w)
+{
+    if (!ctx) return DVM_E_INVALID;
+    return guard(ctx, [&] { bs_self_launch(ctx, prec, n, x, z, circ, sigma, u, w); });
+}
+
+dvm_status dvm_bs_targets(dvm_ctx_t ctx, int prec, int64_t ns, const double* sx,
+                          const double* sz, const double* scirc, const double* ssigma,
+                          int64_t nt, const double* tx, const double* tz, int guard_,
+                          double r2_skip, double* u, double* w)
+{
+    if (!ctx) return DVM_E_INVALID;
+    return guard(ctx, [&] {
+        bs_targets_launch(ctx, prec, ns, sx, sz, scirc, ssigma, nt, tx, tz, guard_, r2_skip, u, w);
+    });
+}
+
+// Host AoS forms: upload, split AoS -> SoA on device, sum, AoS results back.
+__global__ void aos4_to_soa_kernel(int64_t n, const double4* __restrict__ p, double* __restrict__ x,
+                                   double* __restrict__ z, double* __restrict__ c,
+                                   double* __restrict__ s)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 v = p[i];
+    x[i] = v.x; z[i] = v.y; c[i] = v.z; s[i] = v.w;
+}
+
+__global__ void soa_to_aos2_kernel(int64_t n, const double* __restrict__ u,
+                                   const double* __restrict__ w, double2* __restrict__ o)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    o[i] = make_double2(u[i], w[i]);
+}
+
+dvm_status dvm_bs_self_host(dvm_ctx_t ctx, int prec, int64_t n, const double* particles4,
+                            double* results2)
+{
+    if (!ctx) return DVM_E_INVALID;
+    return guard(ctx, [&] {
+        require(n >= 0, DVM_E_SIZE, "negative count");
+        if (n == 0) return;
+        require(particles4 && results2, DVM_E_INVALID, "null host array");
+        char* buf = nullptr;
+        const size_t b = sizeof(double) * n;
+        DVM_HIP_CHECK(hipMallocAsync((void**)&buf, b * 11, ctx->stream));
+        double4* p = reinterpret_cast<double4*>(buf);
+        double* x = reinterpret_cast<double*>(buf + 4 * b);
+        double *z = x + n, *c = z + n, *s = c + n, *u = s + n, *w = u + n;
+        double2* o = reinterpret_cast<double2*>(buf);
+        DVM_HIP_CHECK(hipMemcpyAsync(p, particles4, 4 * b, hipMemcpyHostToDevice, ctx->stream));
+        aos4_to_soa_kernel<<<cdiv(n, 256), 256, 0, ctx->stream>>>(n, p, x, z, c, s);
+        DVM_LAUNCH_CHECK();
+        bs_self_launch(ctx, prec, n, x, z, c, s, u, w);
+        soa_to_aos2_kernel<<<cdiv(n, 256), 256, 0, ctx->stream>>>(n, u, w, o);
+        DVM_LAUNCH_CHECK();
+        DVM_HIP_CHECK(hipMemcpyAsync(results2, o, 2 * b, hipMemcpyDeviceToHost, ctx->stream));
+        DVM_HIP_CHECK(hipFreeAsync(buf, ctx->stream));
+        DVM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+dvm_status dvm_bs_targets_host(dvm_ctx_t ctx, int prec, int64_t ns, const double* sources4,
+                               int64_t nt, const double* tx, const double* tz, double* results2)
+{
+    if (!ctx) return DVM_E_INVALID;
+    return guard(ctx, [&] {
+        require(ns >= 0 && nt >= 0, DVM_E_SIZE, "negative count");
+        if (nt == 0) return;
+        require(tx && tz && results2 && (ns == 0 || sources4), DVM_E_INVALID, "null host array");
+        char* buf = nullptr;
+        const size_t bs = sizeof(double) * (ns > 0 ? ns : 1);
+        const size_t bt = sizeof(double) * nt;
+        DVM_HIP_CHECK(hipMallocAsync((void**)&buf, 8 * bs + 6 * bt, ctx->stream));
+        double4* p = reinterpret_cast<double4*>(buf);
+        double* sx = reinterpret_cast<double*>(buf + 4 * bs);
+        double *sz = sx + (bs / 8), *sc = sz + (bs / 8), *ss = sc + (bs / 8);
+        double* t = reinterpret_cast<double*>(buf + 8 * bs);
+        double *dtx = t, *dtz = t + nt, *u = t + 2 * nt, *w = t + 3 * nt;
+        double2* o = reinterpret_cast<double2*>(t + 4 * nt);
+        if (ns > 0) {
+            DVM_HIP_CHECK(hipMemcpyAsync(p, sources4, 4 * sizeof(double) * ns,
+                                         hipMemcpyHostToDevice, ctx->stream));
+            aos4_to_soa_kernel<<<cdiv(ns, 256), 256, 0, ctx->stream>>>(ns, p, sx, sz, sc, ss);
+            DVM_LAUNCH_CHECK();
+        }
+        DVM_HIP_CHECK(hipMemcpyAsync(dtx, tx, bt, hipMemcpyHostToDevice, ctx->stream));
+        DVM_HIP_CHECK(hipMemcpyAsync(dtz, tz, bt, hipMemcpyHostToDevice, ctx->stream));
+        bs_targets_launch(ctx, prec, ns, sx, sz, sc, ss, nt, dtx, dtz, 1, 1e-15, u, w);
+        soa_to_aos2_kernel<<<cdiv(nt, 256), 256, 0, ctx->stream>>>(nt, u, w, o);
+        DVM_LAUNCH_CHECK();
+        DVM_HIP_CHECK(hipMemcpyAsync(results2, o, 2 * bt, hipMemcpyDeviceToHost, ctx->stream));
+        DVM_HIP_CHECK(hipFreeAsync(buf, ctx->stream));
+        DVM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+}  // extern "C"

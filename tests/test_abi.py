@@ -1,0 +1,74 @@
+"""The C-ABI library loads and exports every entry point include/*.h declares
+(CPU only: no compute calls)."""
+import ctypes
+import glob
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+LIB = os.path.join(ROOT, "dvmpp_amd", "libdvm_hip.so")
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(dvm_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert {"dvm_ctx_create", "dvm_bs_self", "dvm_bs_targets"} <= names
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(LIB), "libdvm_hip.so not built"
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    missing = sorted(declared_functions() - exported)
+    assert not missing, missing
+
+
+def test_library_loads_and_reports_version():
+    lib = ctypes.CDLL(LIB)
+    lib.dvm_version.restype = ctypes.c_char_p
+    assert b"gfx950" in lib.dvm_version()
+    lib.dvm_status_string.restype = ctypes.c_char_p
+    lib.dvm_status_string.argtypes = [ctypes.c_int]
+    assert lib.dvm_status_string(3) == b"non-finite release counts"
+
+
+def test_python_binding_covers_header():
+    from dvmpp_amd import _lib
+    missing = sorted(declared_functions() - set(_lib.SIGNATURES))
+    assert not missing, missing
+
+
+def test_null_context_rejected_without_gpu():
+    lib = ctypes.CDLL(LIB)
+    lib.dvm_bs_self.restype = ctypes.c_int
+    # a null context is rejected before any HIP call
+    assert lib.dvm_bs_self(None, 0, ctypes.c_int64(0), None, None, None, None, None, None) == 1
+
+
+def test_oracle_not_linked_into_product():
+    out = subprocess.run(["ldd", LIB], capture_output=True, text=True).stdout
+    assert "oracle" not in out
+    syms = subprocess.run(["nm", "-D", LIB], capture_output=True, text=True).stdout
+    assert "orc_" not in syms
+
+
+@pytest.mark.parametrize("path", ["dvmpp_amd", "include"])
+def test_product_sources_do_not_reference_oracle(path):
+    for f in glob.glob(os.path.join(ROOT, path, "**", "*"), recursive=True):
+        if os.path.isfile(f) and f.endswith((".py", ".hip", ".hpp", ".cpp", ".h")):
+            assert "import oracle" not in open(f).read(), f
+            assert "orc_" not in open(f).read(), f

@@ -5,6 +5,7 @@ C-ABI in include/dvm_hip.h and the C++17 drop-in classes in dvmpp_amd/host;
 this module only moves torch device tensors across that boundary.
 """
 import ctypes
+import weakref
 
 from ._lib import F32C, F64, DvmError, check, load  # noqa: F401
 
@@ -25,8 +26,12 @@ def _dev_f64(t, device, name):
 
 
 class Context:
-    """dvm_ctx_t bound to torch's current stream on `device` (so torch events
-    time the library's kernels)."""
+    """dvm_ctx_t sharing one HIP stream with torch.
+
+    With no stream given, a new torch stream (default priority) is created and
+    made torch's current stream, so torch's copies and the library's kernels
+    are ordered on one queue.  (torch's default stream has handle 0, which the
+    C-ABI reads as "create a private stream".)"""
 
     def __init__(self, device=0, stream=None):
         import torch
@@ -36,16 +41,27 @@ class Context:
         self.device = torch.device("cuda", device)
         torch.cuda.set_device(self.device)
         if stream is None:
-            stream = torch.cuda.current_stream(self.device)
+            stream = torch.cuda.Stream(device=self.device)
+            torch.cuda.set_stream(stream)
+        if stream.cuda_stream == 0:
+            raise ValueError("pass a non-default torch stream (the default stream has handle 0)")
         self.stream = stream
+        self._children = weakref.WeakSet()
         h = ctypes.c_void_p()
         st = lib.dvm_ctx_create(device, ctypes.c_void_p(stream.cuda_stream), ctypes.byref(h))
         if st != 0:
             raise DvmError(st, "dvm_ctx_create failed")
         self.h = h
 
+    def adopt(self, child):
+        """Registers a handle (sheet, simulation) that must be destroyed before
+        the context: the C-ABI requires children to go first."""
+        self._children.add(child)
+
     def close(self):
         if getattr(self, "h", None):
+            for child in list(self._children):
+                child.close()
             load().dvm_ctx_destroy(self.h)
             self.h = None
 

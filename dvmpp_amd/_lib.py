@@ -58,6 +58,68 @@ SIGNATURES = {
     "dvm_bs_targets_host": (_st, [_vp, _int, _i64, _vp, _i64, _vp, _vp, _vp]),
 }
 
+class SheetParams(ctypes.Structure):
+    """dvm_sheet_params."""
+    _fields_ = [("rho", _dbl), ("nu", _dbl), ("dt", _dbl), ("ux", _dbl), ("uz", _dbl),
+                ("cutoff_exp", _dbl), ("max_npv", _int)]
+
+
+class RemeshParams(ctypes.Structure):
+    """dvm_remesh_params."""
+    _fields_ = [("frequency", _int), ("kernel", _int), ("xmin", _dbl), ("xmax", _dbl),
+                ("zmin", _dbl), ("zmax", _dbl), ("dx", _dbl), ("dz", _dbl), ("sigma", _dbl),
+                ("circ_threshold", _dbl)]
+
+
+class SimParams(ctypes.Structure):
+    """dvm_sim_params."""
+    _fields_ = [("sheet", SheetParams), ("scheme", _int), ("crossing", _int), ("seed", _int),
+                ("prec", _int), ("rng_mode", _int), ("remesh", RemeshParams)]
+
+
+_dp = ctypes.POINTER(_dbl)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+SIGNATURES.update({
+    "dvm_sheet_create": (_st, [_vp, _int, _vp, _vp, ctypes.POINTER(SheetParams),
+                               ctypes.POINTER(_vp)]),
+    "dvm_sheet_destroy": (_st, [_vp]),
+    "dvm_sheet_size": (_st, [_vp, ctypes.POINTER(_int)]),
+    "dvm_sheet_geometry": (_st, [_vp] + [_vp] * 9),
+    "dvm_sheet_influence": (_st, [_vp, _vp]),
+    "dvm_sheet_gamma_ptr": (_st, [_vp, ctypes.POINTER(_vp)]),
+    "dvm_sheet_get_gamma": (_st, [_vp, _vp]),
+    "dvm_sheet_set_gamma": (_st, [_vp, _vp]),
+    "dvm_sheet_solve": (_st, [_vp, _int, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "dvm_sheet_on_blobs": (_st, [_vp, _i64, _vp, _vp, _vp, _vp]),
+    "dvm_sheet_loads": (_st, [_vp, _dbl, _vp]),
+    "dvm_sheet_inside": (_st, [_vp, _i64, _vp, _vp, _vp]),
+    "dvm_sheet_reflect": (_st, [_vp, _i64, _vp, _vp]),
+    "dvm_remesh": (_st, [_vp, ctypes.POINTER(RemeshParams), _i64, _vp, _vp, _vp, _vp, _i64,
+                         _vp, _vp, _vp, _vp, ctypes.POINTER(_i64)]),
+    "dvm_advect_euler": (_st, [_vp, _i64, _dbl, _dbl, _dbl] + [_vp] * 6),
+    "dvm_advect_rk3_stage": (_st, [_vp, _i64, _dbl, _dbl, _dbl, _dbl, _dbl] + [_vp] * 8),
+    "dvm_diffuse": (_st, [_vp, _i64, _vp, _vp, _vp]),
+    "dvm_sheet_release_counts": (_st, [_vp, ctypes.POINTER(_i64)]),
+    "dvm_sheet_release_emit": (_st, [_vp, _i64, _vp, ctypes.c_uint32, _i64] + [_vp] * 9),
+    "dvm_sheet_delete_inside": (_st, [_vp, _i64] + [_vp] * 9 + [ctypes.POINTER(_i64)]),
+    "dvm_sim_create": (_st, [_vp, _int, _vp, _vp, ctypes.POINTER(SimParams),
+                             ctypes.POINTER(_vp)]),
+    "dvm_sim_destroy": (_st, [_vp]),
+    "dvm_sim_step": (_st, [_vp]),
+    "dvm_sim_run": (_st, [_vp, _int]),
+    "dvm_sim_substep": (_st, [_vp, _int]),
+    "dvm_sim_size": (_st, [_vp, ctypes.POINTER(_i64)]),
+    "dvm_sim_step_index": (_st, [_vp, ctypes.POINTER(_int)]),
+    "dvm_sim_rng_draws": (_st, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
+    "dvm_sim_get_state": (_st, [_vp] + [_vp] * 9),
+    "dvm_sim_set_state": (_st, [_vp, _i64] + [_vp] * 9),
+    "dvm_sim_get_gamma": (_st, [_vp, _vp]),
+    "dvm_sim_set_gamma": (_st, [_vp, _vp]),
+    "dvm_sim_loads": (_st, [_vp, _vp]),
+    "dvm_sim_blobs": (_st, [_vp] + [ctypes.POINTER(_vp)] * 4),
+})
+
 _lib = None
 
 

@@ -192,11 +192,13 @@ __global__ void morton_kernel(int64_t n, const double* __restrict__ x, const dou
 }
 
 // One block per tile of kTile consecutive (possibly permuted) points: FP64
-// centre and half extents, FP32 offsets, FP32 near-field constants.
+// centre and half extents; for sources also the FP32 offsets from the centre
+// with circulation, the FP32 near-field constants and an FP64 sorted copy of
+// the positions (read by the near-field loop only).
 __global__ __launch_bounds__(kTile) void tile_build_kernel(
     int64_t n, const uint32_t* __restrict__ perm, const double* __restrict__ x,
     const double* __restrict__ z, const double* __restrict__ c, const double* __restrict__ s,
-    float4* __restrict__ p32, float2* __restrict__ near32, float2* __restrict__ t32,
+    float4* __restrict__ p32, float2* __restrict__ near32, double2* __restrict__ p64,
     TileInfo* __restrict__ tiles)
 {
     __shared__ double red[6][kTile];
@@ -235,16 +237,12 @@ __global__ __launch_bounds__(kTile) void tile_build_kernel(
         ti.pad = 0.0f;
         tiles[blockIdx.x] = ti;
     }
-    if (!valid) return;
-    const float ox = (float)(xi - cx);
-    const float oz = (float)(zi - cz);
-    if (p32) {
-        const double ci = c[src];
-        p32[k] = make_float4(ox, oz, (float)ci, 0.0f);
-        const double s2 = si * si;
-        near32[k] = make_float2((float)(10.0 * s2), (float)(1.0 / s2));
-    }
-    if (t32) t32[k] = make_float2(ox, oz);
+    if (!valid || !p32) return;
+    const double ci = c[src];
+    p32[k] = make_float4((float)(xi - cx), (float)(zi - cz), (float)ci, 0.0f);
+    const double s2 = si * si;
+    near32[k] = make_float2((float)(10.0 * s2), (float)(1.0 / s2));
+    p64[k] = make_double2(xi, zi);
 }
 
 __device__ __forceinline__ void kahan_add(float& s, float& c, float x)
@@ -255,17 +253,31 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float x)
     s = t;
 }
 
+// Targets: SELF -> the sorted FP64 copy p64 (target i is source i);
+// otherwise tx, tz in the caller's order.  tblk: bounding box of each target
+// block (the near/far test).  Per source tile each target is recentred on the
+// tile centre in FP64 and rounded once, so a far pair's dx = (x_i - c) - (x_j - c)
+// carries only FP32 rounding relative to |x - c|; near pairs form dx in FP64.
 template <bool SELF, bool GUARD>
 __global__ __launch_bounds__(kBlock) void bs_f32c_kernel(
-    int64_t nt, const float2* __restrict__ t32, const TileInfo* __restrict__ tblk, int64_t ns,
-    const float4* __restrict__ s32, const float2* __restrict__ near32,
+    int64_t nt, const double* __restrict__ tx, const double* __restrict__ tz,
+    const TileInfo* __restrict__ tblk, int64_t ns, const float4* __restrict__ s32,
+    const float2* __restrict__ near32, const double2* __restrict__ p64,
     const TileInfo* __restrict__ stile, int ntiles, int tiles_per_chunk, float r2_skip,
     const uint32_t* __restrict__ out_index, double* __restrict__ u, double* __restrict__ w,
     double* __restrict__ part)
 {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t it = i < nt ? i : nt - 1;
-    const float2 t = t32[it];
+    double xi, zi;
+    if (SELF) {
+        const double2 p = p64[it];
+        xi = p.x;
+        zi = p.y;
+    } else {
+        xi = tx[it];
+        zi = tz[it];
+    }
     const TileInfo tb = tblk[blockIdx.x];
     float su = 0.0f, cu = 0.0f, sw = 0.0f, cw = 0.0f;
     const int s0 = blockIdx.y * tiles_per_chunk;
@@ -281,8 +293,8 @@ __global__ __launch_bounds__(kBlock) void bs_f32c_kernel(
         // guarded), so the 1/r^2 branch holds for all of them.
         const double lim = 1.0001 * (double)ts.thr_max + 1e-300;
         const bool near = gx * gx + gz * gz <= (GUARD ? fmax(lim, 1.0001 * (double)r2_skip) : lim);
-        const float txd = t.x - (float)Dxd;
-        const float tzd = t.y - (float)Dzd;
+        const float txd = (float)(xi - ts.cx);
+        const float tzd = (float)(zi - ts.cz);
         const int64_t jb = (int64_t)st * kTile;
         const int cnt = ns - jb < kTile ? (int)(ns - jb) : kTile;
         const float4* __restrict__ sp = s32 + jb;
@@ -312,11 +324,13 @@ __global__ __launch_bounds__(kBlock) void bs_f32c_kernel(
             }
         } else {
             const float2* __restrict__ np = near32 + jb;
+            const double2* __restrict__ pp = p64 + jb;
             for (int k = 0; k < cnt; ++k) {
                 const float4 sj = sp[k];
                 const float2 nj = np[k];
-                const float dx = txd - sj.x;
-                const float dz = tzd - sj.y;
+                const double2 pj = pp[k];
+                const float dx = (float)(xi - pj.x);
+                const float dz = (float)(zi - pj.y);
                 const float r2 = __builtin_fmaf(dx, dx, dz * dz);
                 const float inv = __builtin_amdgcn_rcpf(r2);
                 float K = inv;
@@ -407,7 +421,7 @@ static void launch_f32c(dvm_ctx_t ctx, bool self, int64_t ns, const double* sx, 
                                                      ctx->stream));
     const size_t need = Carver::need({
         sizeof(uint32_t) * ns * 4, sizeof(double4) * nbb, sort_tmp, sizeof(float4) * ns,
-        sizeof(float2) * ns, sizeof(TileInfo) * ntiles, sizeof(float2) * nt,
+        sizeof(float2) * ns, sizeof(double2) * ns, sizeof(TileInfo) * ntiles,
         sizeof(TileInfo) * bx, nch > 1 ? sizeof(double) * 2 * nt * nch : 0});
     Carver cv(ws_get(ctx, need));
     uint32_t* keys = cv.take<uint32_t>(ns);
@@ -418,8 +432,8 @@ static void launch_f32c(dvm_ctx_t ctx, bool self, int64_t ns, const double* sx, 
     void* tmp = cv.take<char>(sort_tmp);
     float4* s32 = cv.take<float4>(ns);
     float2* near32 = cv.take<float2>(ns);
+    double2* p64 = cv.take<double2>(ns);
     TileInfo* stile = cv.take<TileInfo>(ntiles);
-    float2* t32 = cv.take<float2>(nt);
     TileInfo* tblk = cv.take<TileInfo>(bx);
     double* part = nch > 1 ? cv.take<double>(2 * nt * nch) : nullptr;
 
@@ -430,12 +444,12 @@ static void launch_f32c(dvm_ctx_t ctx, bool self, int64_t ns, const double* sx, 
     DVM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, sort_tmp, keys, keys2, idx, perm,
                                                      (int)ns, 0, 32, ctx->stream));
     tile_build_kernel<<<ntiles, kTile, 0, ctx->stream>>>(ns, perm, sx, sz, sc, ss, s32, near32,
-                                                         self ? t32 : nullptr, stile);
+                                                         p64, stile);
     DVM_LAUNCH_CHECK();
     const TileInfo* tb = stile;
     if (!self) {
         tile_build_kernel<<<(unsigned)bx, kTile, 0, ctx->stream>>>(
-            nt, nullptr, tx, tz, nullptr, nullptr, nullptr, nullptr, t32, tblk);
+            nt, nullptr, tx, tz, nullptr, nullptr, nullptr, nullptr, nullptr, tblk);
         DVM_LAUNCH_CHECK();
         tb = tblk;
     }
@@ -446,13 +460,14 @@ static void launch_f32c(dvm_ctx_t ctx, bool self, int64_t ns, const double* sx, 
         ProfScope ps(ctx, self ? "bs_self_f32c" : "bs_targets_f32c");
         if (self) {
             bs_f32c_kernel<true, false><<<grid, kBlock, 0, ctx->stream>>>(
-                nt, t32, tb, ns, s32, near32, stile, ntiles, tpc, skip, oidx, u, w, part);
+                nt, nullptr, nullptr, tb, ns, s32, near32, p64, stile, ntiles, tpc, skip, oidx,
+                u, w, part);
         } else if (guard) {
             bs_f32c_kernel<false, true><<<grid, kBlock, 0, ctx->stream>>>(
-                nt, t32, tb, ns, s32, near32, stile, ntiles, tpc, skip, oidx, u, w, part);
+                nt, tx, tz, tb, ns, s32, near32, p64, stile, ntiles, tpc, skip, oidx, u, w, part);
         } else {
             bs_f32c_kernel<false, false><<<grid, kBlock, 0, ctx->stream>>>(
-                nt, t32, tb, ns, s32, near32, stile, ntiles, tpc, skip, oidx, u, w, part);
+                nt, tx, tz, tb, ns, s32, near32, p64, stile, ntiles, tpc, skip, oidx, u, w, part);
         }
         DVM_LAUNCH_CHECK();
     }

@@ -124,6 +124,82 @@ struct Carver {
 
 inline unsigned cdiv(int64_t a, int64_t b) { return static_cast<unsigned>((a + b - 1) / b); }
 
+// Per-panel constants read by the sheet-on-blob and release kernels, computed
+// on the host with the C library's libm at set-up (the values the reference
+// recomputes per pair in src/VortexSheet.cpp:225-251 and :325).
+struct PanelConst {
+    double xc, zc, ds, cs, sn, c2, s2, sig;  // cos/sin(theta), cos/sin(-2 theta), ds^cutoff_exp
+};
+
+// Device-resident vortex sheet (one body).
+struct SheetDev {
+    int nl = 0;
+    double rho = 1, nu = 0, dt = 0, ux = 0, uz = 0, cutoff_exp = 0;
+    int max_npv = 1;
+    double aref = 0;
+    std::vector<double> hx, hz;                    // body points (host copy)
+    std::vector<double> hxc, hzc, htheta, hds, henx, henz, hetx, hetz;
+    char* mem = nullptr;                           // one device allocation
+    double *x = nullptr, *z = nullptr;             // nl+1 body points
+    double *enx = nullptr, *enz = nullptr, *etx = nullptr, *etz = nullptr, *ds = nullptr;
+    double *xc = nullptr, *zc = nullptr, *theta = nullptr;
+    PanelConst* panel = nullptr;                   // nl
+    double* A = nullptr;                           // (nl+1) x nl row-major
+    double* P = nullptr;                           // nl x (nl+1) row-major: (A^T A)^-1 A^T
+    double* gamma = nullptr;                       // nl
+    double* b = nullptr;                           // nl+1 right-hand side of the last solve
+    double* uw = nullptr;                          // 2*nl collocation velocities
+    double* scal = nullptr;                        // [0] total circulation, [1..4] fx fz cd cl
+    double* red = nullptr;                         // reduction partials
+    int* cnt = nullptr;                            // release counts per panel
+    int64_t* off = nullptr;                        // release offsets (nl+1), off[nl] = Nrv
+    double* cnew = nullptr;                        // circulation per nascent blob per panel
+    int* flag = nullptr;                           // [0] NaN-count flag
+};
+
+void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const double* pz,
+                const dvm_sheet_params* p);
+void sheet_free(SheetDev& sh);
+// solvevortexsheet: collocation sum + RHS + gamma = P b.
+void sheet_solve(dvm_ctx_t ctx, SheetDev& sh, int prec, int64_t n, const double* bx,
+                 const double* bz, const double* bc, const double* bs);
+// vortexsheetbc
+void sheet_on_blobs(dvm_ctx_t ctx, const SheetDev& sh, int64_t n, const double* bx,
+                    const double* bz, double* uvs, double* wvs);
+void sheet_loads(dvm_ctx_t ctx, SheetDev& sh, double ur);
+// Release counts and offsets; returns nothing (results in sh.cnt/off/flag).
+void sheet_release_count(dvm_ctx_t ctx, SheetDev& sh);
+// Writes nrv nascent blobs at index `base` of the blob arrays, using rrw[k].
+void sheet_release_emit(dvm_ctx_t ctx, const SheetDev& sh, int64_t nrv, const double* rrw,
+                        uint32_t nold, int64_t base, double* x, double* z, double* c, double* s,
+                        double* u, double* w, double* uvs, double* wvs, uint32_t* id);
+// Crossing: flags[i] = inside_body(x_i, z_i).
+void sheet_inside_flags(dvm_ctx_t ctx, const SheetDev& sh, int64_t n, const double* x,
+                        const double* z, int* flags);
+void sheet_reflect(dvm_ctx_t ctx, const SheetDev& sh, int64_t n, const int* flags, double* x,
+                   double* z);
+// Deterministic sum of a device vector into *out (device).
+void device_sum(dvm_ctx_t ctx, int64_t n, const double* v, double* partials, double* out);
+
+// Blob SoA on the device (VortexBlobs fields, src/VortexBlobs.hpp:24-32).
+struct BlobBuf {
+    double *x = nullptr, *z = nullptr, *c = nullptr, *s = nullptr;
+    double *u = nullptr, *w = nullptr, *uvs = nullptr, *wvs = nullptr;
+    uint32_t* id = nullptr;
+};
+
+// Remesh of n blobs (BackgroundMesh specification, see remesh.hip).  Phase 1
+// bins and gathers onto the mesh and returns the output count (host value;
+// synchronizes); phase 2 writes the output into `out` (capacity >= count).
+struct RemeshPlan {
+    int64_t n_in = 0, nx = 0, nz = 0, n_nodes_out = 0, n_kept = 0;
+    char* mem = nullptr;  // grid values, flags, scans (freed by remesh_finish)
+};
+int64_t remesh_count(dvm_ctx_t ctx, const dvm_remesh_params& rp, int64_t n, const BlobBuf& in,
+                     RemeshPlan& plan);
+void remesh_finish(dvm_ctx_t ctx, const dvm_remesh_params& rp, const BlobBuf& in,
+                   RemeshPlan& plan, BlobBuf& out);
+
 // Biot-Savart entry points shared with the sheet and simulation code.
 void bs_self_launch(dvm_ctx_t ctx, int prec, int64_t n, const double* x, const double* z,
                     const double* c, const double* s, double* u, double* w);
@@ -132,3 +208,8 @@ void bs_targets_launch(dvm_ctx_t ctx, int prec, int64_t ns, const double* sx, co
                        const double* tz, int guard, double r2_skip, double* u, double* w);
 
 }  // namespace dvmi
+
+struct dvm_sheet_s {
+    dvm_ctx_t ctx;
+    dvmi::SheetDev sh;
+};

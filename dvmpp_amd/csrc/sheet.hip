@@ -1,0 +1,820 @@
+// Vortex sheet on the device: influence matrix, least-squares operator,
+// per-step gamma solve, sheet-on-blob velocities, loads, nascent release and
+// surface crossing (src/VortexSheet.cpp).
+//
+// Layout: panel data in SoA FP64 arrays plus one 64-byte PanelConst record per
+// panel; the per-pair loops stream panels with wave-uniform addresses (scalar
+// loads), one blob per lane.
+#include "dvm_internal.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <cstring>
+
+namespace dvmi {
+
+// ----------------------------------------------------------------- set-up
+
+// One (i, j) entry of A: src/VortexSheet.cpp:117-161.  Panel j contributes its
+// start-end coefficient p(i,j) and panel j-1 its finish-end coefficient q.
+__device__ void panel_pq(double xci, double zci, double thetai, double xj, double zj,
+                         double thetaj, double dsj, double& p, double& q)
+{
+    const double c1 = -(xci - xj) * cos(thetaj) - (zci - zj) * sin(thetaj);
+    const double c2 = (xci - xj) * (xci - xj) + (zci - zj) * (zci - zj);
+    const double c3 = sin(thetai - thetaj);
+    const double c4 = cos(thetai - thetaj);
+    const double c5 = (xci - xj) * sin(thetaj) - (zci - zj) * cos(thetaj);
+    const double c6 = log(1.0 + dsj * ((dsj + 2 * c1) / c2));
+    const double c7 = atan2((c5 * dsj), (c2 + c1 * dsj));
+    const double c9 = (xci - xj) * cos(thetai - 2.0 * thetaj) - (zci - zj) * sin(thetai - 2.0 * thetaj);
+    q = c4 + 0.5 * c9 * c6 / dsj - ((c1 * c3 + c4 * c5) * c7) / dsj;
+    p = 0.5 * c4 * c6 + c3 * c7 - q;
+}
+
+__global__ void influence_kernel(int nl, const double* __restrict__ x, const double* __restrict__ z,
+                                 const double* __restrict__ xc, const double* __restrict__ zc,
+                                 const double* __restrict__ theta, const double* __restrict__ ds,
+                                 double* __restrict__ A)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (j >= nl) return;
+    if (i == nl) {
+        A[(int64_t)nl * nl + j] = ds[j];
+        return;
+    }
+    const int jm = j == 0 ? nl - 1 : j - 1;
+    double p, q, pd, qd;
+    if (i == j) {
+        p = -1.0;
+    } else {
+        panel_pq(xc[i], zc[i], theta[i], x[j], z[j], theta[j], ds[j], p, qd);
+    }
+    if (i == jm) {
+        q = 1.0;
+    } else {
+        panel_pq(xc[i], zc[i], theta[i], x[jm], z[jm], theta[jm], ds[jm], pd, q);
+    }
+    A[(int64_t)i * nl + j] = -kRpi2 * (p + q);
+}
+
+// M = A^T A (nl x nl), sequential over the nl+1 rows of A.
+__global__ void ata_kernel(int nl, const double* __restrict__ A, double* __restrict__ M)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (j >= nl) return;
+    double s = 0.0;
+    for (int k = 0; k <= nl; ++k) s += A[(int64_t)k * nl + i] * A[(int64_t)k * nl + j];
+    M[(int64_t)i * nl + j] = s;
+}
+
+constexpr int kLuThreads = 1024;
+
+// In-place LU with partial pivoting of the nl x nl matrix M (one workgroup).
+// Pivot: first row with the largest |M[i][k]|.
+__global__ __launch_bounds__(kLuThreads) void lu_kernel(int n, double* __restrict__ M,
+                                                        int* __restrict__ piv)
+{
+    __shared__ double bv[kLuThreads];
+    __shared__ int bi[kLuThreads];
+    const int t = threadIdx.x;
+    for (int k = 0; k < n; ++k) {
+        double best = -1.0;
+        int bidx = n;
+        for (int i = k + t; i < n; i += kLuThreads) {
+            const double v = fabs(M[(int64_t)i * n + k]);
+            if (v > best) { best = v; bidx = i; }
+        }
+        bv[t] = best;
+        bi[t] = bidx;
+        __syncthreads();
+        for (int s = kLuThreads / 2; s > 0; s >>= 1) {
+            if (t < s) {
+                const double ov = bv[t + s];
+                const int oi = bi[t + s];
+                if (ov > bv[t] || (ov == bv[t] && oi < bi[t])) { bv[t] = ov; bi[t] = oi; }
+            }
+            __syncthreads();
+        }
+        const int p = bi[0];
+        __syncthreads();
+        if (t == 0) piv[k] = p;
+        if (p != k) {
+            for (int j = t; j < n; j += kLuThreads) {
+                const double a = M[(int64_t)k * n + j];
+                M[(int64_t)k * n + j] = M[(int64_t)p * n + j];
+                M[(int64_t)p * n + j] = a;
+            }
+        }
+        __syncthreads();
+        const double d = M[(int64_t)k * n + k];
+        for (int i = k + 1 + t; i < n; i += kLuThreads) M[(int64_t)i * n + k] /= d;
+        __syncthreads();
+        const int m = n - k - 1;
+        for (int e = t; e < m * m; e += kLuThreads) {
+            const int i = k + 1 + e / m;
+            const int j = k + 1 + e % m;
+            M[(int64_t)i * n + j] -= M[(int64_t)i * n + k] * M[(int64_t)k * n + j];
+        }
+        __syncthreads();
+    }
+}
+
+// P = M^-1 A^T using the LU factors: column c of A^T is row c of A.
+// Thread c solves for column c; X is stored row-major nl x (nl+1).
+__global__ void lu_solve_columns_kernel(int n, const double* __restrict__ LU,
+                                        const int* __restrict__ piv, const double* __restrict__ A,
+                                        double* __restrict__ X)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncol = n + 1;
+    if (c >= ncol) return;
+    for (int i = 0; i < n; ++i) X[(int64_t)i * ncol + c] = A[(int64_t)c * n + i];
+    for (int k = 0; k < n; ++k) {
+        const int p = piv[k];
+        if (p != k) {
+            const double a = X[(int64_t)k * ncol + c];
+            X[(int64_t)k * ncol + c] = X[(int64_t)p * ncol + c];
+            X[(int64_t)p * ncol + c] = a;
+        }
+    }
+    for (int i = 1; i < n; ++i) {
+        double s = X[(int64_t)i * ncol + c];
+        for (int j = 0; j < i; ++j) s -= LU[(int64_t)i * n + j] * X[(int64_t)j * ncol + c];
+        X[(int64_t)i * ncol + c] = s;
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double s = X[(int64_t)i * ncol + c];
+        for (int j = i + 1; j < n; ++j) s -= LU[(int64_t)i * n + j] * X[(int64_t)j * ncol + c];
+        X[(int64_t)i * ncol + c] = s / LU[(int64_t)i * n + i];
+    }
+}
+
+void sheet_free(SheetDev& sh)
+{
+    if (sh.mem) (void)hipFree(sh.mem);
+    sh.mem = nullptr;
+}
+
+void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const double* pz,
+                const dvm_sheet_params* p)
+{
+    require(npts >= 3, DVM_E_SIZE, "a body needs at least 3 points (2 panels)");
+    require(npts <= 4097, DVM_E_SIZE, "at most 4096 panels per body");
+    require(px && pz && p, DVM_E_INVALID, "null body or parameters");
+    require(p->max_npv > 0 && p->dt > 0.0, DVM_E_INVALID, "max_NumPanelVort and dt must be > 0");
+    const int nl = npts - 1;
+    sh.nl = nl;
+    sh.rho = p->rho; sh.nu = p->nu; sh.dt = p->dt; sh.ux = p->ux; sh.uz = p->uz;
+    sh.cutoff_exp = p->cutoff_exp; sh.max_npv = p->max_npv;
+    sh.hx.assign(px, px + npts);
+    sh.hz.assign(pz, pz + npts);
+    // read_input_coord / form_vortex_sheet (src/VortexSheet.cpp:56-102): O(Nl)
+    // set-up on the host with the C library's libm.
+    double mn = px[0], mx = px[0];
+    for (int i = 1; i < npts; ++i) { mn = px[i] < mn ? px[i] : mn; mx = px[i] > mx ? px[i] : mx; }
+    sh.aref = mx - mn;
+    sh.hxc.resize(nl); sh.hzc.resize(nl); sh.htheta.resize(nl); sh.hds.resize(nl);
+    sh.henx.resize(nl); sh.henz.resize(nl); sh.hetx.resize(nl); sh.hetz.resize(nl);
+    std::vector<PanelConst> pc(nl);
+    for (int i = 0; i < nl; ++i) {
+        sh.hxc[i] = 0.5 * (px[i] + px[i + 1]);
+        sh.hzc[i] = 0.5 * (pz[i] + pz[i + 1]);
+        const double dx = px[i + 1] - px[i];
+        const double dz = pz[i + 1] - pz[i];
+        sh.hds[i] = std::sqrt(dx * dx + dz * dz);
+        sh.htheta[i] = std::atan2(dz, dx);
+        sh.henx[i] = dz / sh.hds[i];
+        sh.henz[i] = -dx / sh.hds[i];
+        sh.hetx[i] = -sh.henz[i];
+        sh.hetz[i] = sh.henx[i];
+        const double th = sh.htheta[i];
+        pc[i] = PanelConst{sh.hxc[i], sh.hzc[i], sh.hds[i], std::cos(th), std::sin(th),
+                           std::cos(-2.0 * th), std::sin(-2.0 * th),
+                           std::pow(sh.hds[i], sh.cutoff_exp)};
+    }
+    const size_t d = sizeof(double);
+    const size_t nred = 3 * (size_t)nl > 1024 ? 3 * (size_t)nl : 1024;
+    const size_t need = Carver::need({d * npts, d * npts, d * nl * 8, sizeof(PanelConst) * nl,
+                                      d * (nl + 1) * nl, d * nl * (nl + 1), d * nl, d * (nl + 1),
+                                      d * 2 * nl, d * 8, d * nred, sizeof(int) * nl,
+                                      sizeof(int64_t) * (nl + 1), d * nl, sizeof(int) * 4});
+    DVM_HIP_CHECK(hipMalloc(&sh.mem, need));
+    Carver cv(sh.mem);
+    sh.x = cv.take<double>(npts);
+    sh.z = cv.take<double>(npts);
+    double* geo = cv.take<double>((size_t)nl * 8);
+    sh.xc = geo; sh.zc = geo + nl; sh.theta = geo + 2 * nl; sh.ds = geo + 3 * nl;
+    sh.enx = geo + 4 * nl; sh.enz = geo + 5 * nl; sh.etx = geo + 6 * nl; sh.etz = geo + 7 * nl;
+    sh.panel = cv.take<PanelConst>(nl);
+    sh.A = cv.take<double>((size_t)(nl + 1) * nl);
+    sh.P = cv.take<double>((size_t)nl * (nl + 1));
+    sh.gamma = cv.take<double>(nl);
+    sh.b = cv.take<double>(nl + 1);
+    sh.uw = cv.take<double>(2 * nl);
+    sh.scal = cv.take<double>(8);
+    sh.red = cv.take<double>(nred);
+    sh.cnt = cv.take<int>(nl);
+    sh.off = cv.take<int64_t>(nl + 1);
+    sh.cnew = cv.take<double>(nl);
+    sh.flag = cv.take<int>(4);
+
+    std::vector<double> g((size_t)nl * 8);
+    std::memcpy(&g[0], sh.hxc.data(), d * nl);
+    std::memcpy(&g[nl], sh.hzc.data(), d * nl);
+    std::memcpy(&g[2 * nl], sh.htheta.data(), d * nl);
+    std::memcpy(&g[3 * nl], sh.hds.data(), d * nl);
+    std::memcpy(&g[4 * nl], sh.henx.data(), d * nl);
+    std::memcpy(&g[5 * nl], sh.henz.data(), d * nl);
+    std::memcpy(&g[6 * nl], sh.hetx.data(), d * nl);
+    std::memcpy(&g[7 * nl], sh.hetz.data(), d * nl);
+    hipStream_t st = ctx->stream;
+    DVM_HIP_CHECK(hipMemcpyAsync(sh.x, px, d * npts, hipMemcpyHostToDevice, st));
+    DVM_HIP_CHECK(hipMemcpyAsync(sh.z, pz, d * npts, hipMemcpyHostToDevice, st));
+    DVM_HIP_CHECK(hipMemcpyAsync(geo, g.data(), d * nl * 8, hipMemcpyHostToDevice, st));
+    DVM_HIP_CHECK(hipMemcpyAsync(sh.panel, pc.data(), sizeof(PanelConst) * nl,
+                                 hipMemcpyHostToDevice, st));
+    DVM_HIP_CHECK(hipMemsetAsync(sh.gamma, 0, d * nl, st));
+    DVM_HIP_CHECK(hipMemsetAsync(sh.scal, 0, d * 8, st));
+
+    // compute_influence_matrix, then the least-squares operator (A^T A)^-1 A^T.
+    double* M = nullptr;
+    int* piv = nullptr;
+    DVM_HIP_CHECK(hipMallocAsync((void**)&M, d * nl * nl + sizeof(int) * nl, st));
+    piv = reinterpret_cast<int*>(M + (size_t)nl * nl);
+    {
+        ProfScope ps(ctx, "panel_influence");
+        influence_kernel<<<dim3(cdiv(nl, 128), nl + 1), 128, 0, st>>>(nl, sh.x, sh.z, sh.xc, sh.zc,
+                                                                      sh.theta, sh.ds, sh.A);
+        DVM_LAUNCH_CHECK();
+    }
+    ata_kernel<<<dim3(cdiv(nl, 128), nl), 128, 0, st>>>(nl, sh.A, M);
+    DVM_LAUNCH_CHECK();
+    lu_kernel<<<1, kLuThreads, 0, st>>>(nl, M, piv);
+    DVM_LAUNCH_CHECK();
+    lu_solve_columns_kernel<<<cdiv(nl + 1, 64), 64, 0, st>>>(nl, M, piv, sh.A, sh.P);
+    DVM_LAUNCH_CHECK();
+    DVM_HIP_CHECK(hipFreeAsync(M, st));
+    DVM_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+// ----------------------------------------------------------------- reductions
+
+__global__ void sum_partial_kernel(int64_t n, const double* __restrict__ v, double* __restrict__ part)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        s += v[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void sum_final_kernel(int np, const double* __restrict__ part, double* __restrict__ out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < np; ++i) s += part[i];
+        *out = s;
+    }
+}
+
+void device_sum(dvm_ctx_t ctx, int64_t n, const double* v, double* partials, double* out)
+{
+    const int nb = 256;
+    sum_partial_kernel<<<nb, 256, 0, ctx->stream>>>(n, v, partials);
+    DVM_LAUNCH_CHECK();
+    sum_final_kernel<<<1, 64, 0, ctx->stream>>>(nb, partials, out);
+    DVM_LAUNCH_CHECK();
+}
+
+// ----------------------------------------------------------------- solve
+
+// b_i = (U + u_i).n_i, b_Nl = -sum circ (src/VortexSheet.cpp:173-175, 209-210);
+// gamma = P b with one wave per row, lanes over columns.
+__global__ __launch_bounds__(256) void gamma_kernel(int nl, const double* __restrict__ P,
+                                                    const double* __restrict__ enx,
+                                                    const double* __restrict__ enz, double ux,
+                                                    double uz, const double* __restrict__ uw,
+                                                    int have_blobs, const double* __restrict__ scal,
+                                                    double* __restrict__ b,
+                                                    double* __restrict__ gamma)
+{
+    extern __shared__ double sb[];
+    for (int i = threadIdx.x; i <= nl; i += 256) {
+        double v;
+        if (i == nl) {
+            v = have_blobs ? -scal[0] : -0.0;
+        } else if (have_blobs) {
+            v = (ux + uw[i]) * enx[i] + (uz + uw[nl + i]) * enz[i];
+        } else {
+            v = ux * enx[i] + uz * enz[i];
+        }
+        sb[i] = v;
+        if (blockIdx.x == 0) b[i] = v;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int ncol = nl + 1;
+    for (int i = blockIdx.x * 4 + wave; i < nl; i += gridDim.x * 4) {
+        double s = 0.0;
+        for (int c = lane; c < ncol; c += 64) s += P[(int64_t)i * ncol + c] * sb[c];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (lane == 0) gamma[i] = s;
+    }
+}
+
+void sheet_solve(dvm_ctx_t ctx, SheetDev& sh, int prec, int64_t n, const double* bx,
+                 const double* bz, const double* bc, const double* bs)
+{
+    const int nl = sh.nl;
+    if (n > 0) {
+        bs_targets_launch(ctx, prec, n, bx, bz, bc, bs, nl, sh.xc, sh.zc, 0, 0.0, sh.uw,
+                          sh.uw + nl);
+        device_sum(ctx, n, bc, sh.red, sh.scal);
+    }
+    const unsigned blocks = cdiv(nl, 4) < 64 ? cdiv(nl, 4) : 64;
+    ProfScope ps(ctx, "sheet_gamma");
+    gamma_kernel<<<blocks, 256, sizeof(double) * (nl + 1), ctx->stream>>>(
+        nl, sh.P, sh.enx, sh.enz, sh.ux, sh.uz, sh.uw, n > 0 ? 1 : 0, sh.scal, sh.b, sh.gamma);
+    DVM_LAUNCH_CHECK();
+}
+
+// ----------------------------------------------------------------- sheet on blobs
+
+// src/VortexSheet.cpp:225-272 fused: per (blob, panel) the coefficients
+// px, qx, py, qy with the collocation midpoint as panel origin, summed against
+// gamma_j and gamma_{j+1} (cyclic).
+__global__ __launch_bounds__(256) void sheet_on_blobs_kernel(
+    int64_t n, int nl, const double* __restrict__ bx, const double* __restrict__ bz,
+    const PanelConst* __restrict__ pc, const double* __restrict__ gamma, double* __restrict__ uvs,
+    double* __restrict__ wvs)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double xi = bx[i];
+    const double zi = bz[i];
+    double su = 0.0, sw = 0.0;
+    for (int j = 0; j < nl; ++j) {
+        const PanelConst q = pc[j];
+        const double gj = gamma[j];
+        const double gj1 = gamma[j + 1 == nl ? 0 : j + 1];
+        const double xj = q.xc, zj = q.zc, dsj = q.ds;
+        const double c1 = -(xi - xj) * q.cs - (zi - zj) * q.sn;
+        const double c2 = (xi - xj) * (xi - xj) + (zi - zj) * (zi - zj);
+        const double c5 = (xi - xj) * q.sn - (zi - zj) * q.cs;
+        const double c6 = log(1.0 + dsj * ((dsj + 2 * c1) / c2));
+        const double c7 = atan2(c5 * dsj, c2 + c1 * dsj);
+        const double c8 = (xi - xj) * q.s2 + (zi - zj) * q.c2;
+        const double c9 = (xi - xj) * q.c2 - (zi - zj) * q.s2;
+        const double qx = -q.sn + 0.5 * c8 * c6 / dsj + (c1 * q.cs + c5 * q.sn) * c7 / dsj;
+        const double px = -0.5 * c6 * q.sn - c7 * q.cs - qx;
+        const double qy = q.cs + 0.5 * c9 * c6 / dsj + (c1 * q.sn - c5 * q.cs) * c7 / dsj;
+        const double py = 0.5 * c6 * q.cs - c7 * q.sn - qy;
+        su += px * gj + qx * gj1;
+        sw += py * gj + qy * gj1;
+    }
+    uvs[i] = su * kRpi2;
+    wvs[i] = sw * kRpi2;
+}
+
+void sheet_on_blobs(dvm_ctx_t ctx, const SheetDev& sh, int64_t n, const double* bx,
+                    const double* bz, double* uvs, double* wvs)
+{
+    if (n <= 0) return;
+    ProfScope ps(ctx, "sheet_on_blobs");
+    sheet_on_blobs_kernel<<<cdiv(n, 256), 256, 0, ctx->stream>>>(n, sh.nl, bx, bz, sh.panel,
+                                                                 sh.gamma, uvs, wvs);
+    DVM_LAUNCH_CHECK();
+}
+
+// ----------------------------------------------------------------- loads
+
+// Armadillo-style two-accumulator sum (see the oracle's note: order unpinned).
+__device__ double pair_sum(const double* a, int n)
+{
+    double acc1 = 0.0, acc2 = 0.0;
+    int j;
+    for (j = 1; j < n; j += 2) {
+        acc1 += a[j - 1];
+        acc2 += a[j];
+    }
+    if ((j - 1) < n) acc1 += a[j - 1];
+    return acc1 + acc2;
+}
+
+// src/VortexSheet.cpp:487-514: sequential pressure integration (one lane).
+// work holds 3*nl doubles.
+__global__ void loads_kernel(int nl, double rho, double dt, double ur, double aref,
+                             const double* __restrict__ gamma, const double* __restrict__ ds,
+                             const double* __restrict__ enx, const double* __restrict__ enz,
+                             double* __restrict__ work, double* __restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double* P = work;
+    double* tx = work + nl;
+    double* tz = work + 2 * nl;
+    const double k = -rho / dt;
+    P[0] = 0.0;
+    for (int i = 0; i < nl - 1; i++) P[i + 1] = P[i] + k * (gamma[i] * ds[i]);
+    for (int i = 0; i < nl - 1; i++) P[i] = 0.5 * (P[i] + P[i + 1]);
+    P[nl - 1] = 0.5 * (P[nl - 1] + P[0]);
+    double pmax = P[0];
+    for (int i = 1; i < nl; ++i) pmax = P[i] > pmax ? P[i] : pmax;
+    const double pref = 0.5 * rho * ur * ur;
+    for (int i = 0; i < nl; ++i) {
+        const double pi = P[i] + (pref - pmax) * 1.0;
+        tx[i] = pi * enx[i] * ds[i];
+        tz[i] = pi * enz[i] * ds[i];
+    }
+    const double fx = -pair_sum(tx, nl);
+    const double fz = -pair_sum(tz, nl);
+    out[0] = fx;
+    out[1] = fz;
+    out[2] = fx / (0.5 * rho * aref * ur * ur);
+    out[3] = fz / (0.5 * rho * aref * ur * ur);
+}
+
+void sheet_loads(dvm_ctx_t ctx, SheetDev& sh, double ur)
+{
+    // sh.red holds max(1024, 3 nl) doubles: the loads work vectors
+    loads_kernel<<<1, 64, 0, ctx->stream>>>(sh.nl, sh.rho, sh.dt, ur, sh.aref, sh.gamma, sh.ds,
+                                            sh.enx, sh.enz, sh.red, sh.scal + 1);
+    DVM_LAUNCH_CHECK();
+}
+
+// ----------------------------------------------------------------- release
+
+// src/VortexSheet.cpp:282-303: counts, NaN check, circulation per nascent blob
+// and the exclusive prefix of counts (one workgroup).
+__global__ __launch_bounds__(1024) void release_count_kernel(
+    int nl, int max_npv, const double* __restrict__ gamma, const double* __restrict__ ds,
+    int* __restrict__ cnt, int64_t* __restrict__ off, double* __restrict__ cnew,
+    int* __restrict__ flag)
+{
+    __shared__ double red[1024];
+    __shared__ int64_t pre[1024];
+    const int t = threadIdx.x;
+    // arma::max: running maximum that only moves on a strictly greater value
+    // (NaN entries never win)
+    double mx = -INFINITY;
+    for (int i = t; i < nl; i += 1024) {
+        const double v = gamma[i] * ds[i];
+        if (v > mx) mx = v;
+    }
+    red[t] = mx;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (t < s) {
+            const double o = red[t + s];
+            if (o > red[t]) red[t] = o;
+        }
+        __syncthreads();
+    }
+    const double m = red[0];
+    __syncthreads();
+    if (t == 0) flag[0] = 0;
+    __syncthreads();
+    // each thread owns a contiguous run of panels for the prefix
+    const int per = (nl + 1023) / 1024;
+    const int i0 = t * per;
+    const int i1 = i0 + per < nl ? i0 + per : nl;
+    int64_t local = 0;
+    int bad = 0;
+    for (int i = i0; i < i1; ++i) {
+        const double pcirc = gamma[i] * ds[i];
+        const double c = floor(fabs(pcirc / m * (double)max_npv)) + 1.0;
+        int ci = 0;
+        if (!(c > 1.0e-15)) {
+            bad |= 1;  // NaN count: the reference throws VortexException
+        } else if (!(c < 1.0e9)) {
+            bad |= 2;  // count too large to emit
+        } else {
+            ci = (int)c;
+        }
+        cnt[i] = ci;
+        cnew[i] = pcirc / c;
+        local += ci;
+    }
+    if (bad) atomicOr(flag, bad);
+    pre[t] = local;
+    __syncthreads();
+    for (int s = 1; s < 1024; s <<= 1) {
+        const int64_t v = t >= s ? pre[t - s] : 0;
+        __syncthreads();
+        pre[t] += v;
+        __syncthreads();
+    }
+    int64_t run = pre[t] - local;
+    for (int i = i0; i < i1; ++i) {
+        off[i] = run;
+        run += cnt[i];
+    }
+    if (t == 1023) off[nl] = pre[1023];
+}
+
+void sheet_release_count(dvm_ctx_t ctx, SheetDev& sh)
+{
+    release_count_kernel<<<1, 1024, 0, ctx->stream>>>(sh.nl, sh.max_npv, sh.gamma, sh.ds, sh.cnt,
+                                                      sh.off, sh.cnew, sh.flag);
+    DVM_LAUNCH_CHECK();
+}
+
+// src/VortexSheet.cpp:305-331: nascent blob k of panel i sits on the outward
+// normal through xm = x_i + ds_i et_i / (n_i + 1), at distance rrw[k].
+__global__ void release_emit_kernel(int64_t nrv, int nl, const int64_t* __restrict__ off,
+                                    const int* __restrict__ cnt, const double* __restrict__ bxp,
+                                    const double* __restrict__ bzp, const double* __restrict__ ds,
+                                    const double* __restrict__ etx, const double* __restrict__ etz,
+                                    const double* __restrict__ enx, const double* __restrict__ enz,
+                                    const double* __restrict__ cnew,
+                                    const PanelConst* __restrict__ pc,
+                                    const double* __restrict__ rrw, uint32_t nold, int64_t base,
+                                    double* __restrict__ x, double* __restrict__ z,
+                                    double* __restrict__ c, double* __restrict__ s,
+                                    double* __restrict__ u, double* __restrict__ w,
+                                    double* __restrict__ uvs, double* __restrict__ wvs,
+                                    uint32_t* __restrict__ id)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nrv) return;
+    int lo = 0, hi = nl;  // find i with off[i] <= k < off[i+1]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= k) lo = mid; else hi = mid;
+    }
+    const int i = lo;
+    const double np1 = (double)cnt[i] + 1;
+    const double xm = bxp[i] + ds[i] * etx[i] / np1;
+    const double zm = bzp[i] + ds[i] * etz[i] / np1;
+    const double r = rrw[k];
+    const int64_t o = base + k;
+    x[o] = xm + r * enx[i];
+    z[o] = zm + r * enz[i];
+    c[o] = cnew[i];
+    s[o] = pc[i].sig;
+    u[o] = 0.0; w[o] = 0.0; uvs[o] = 0.0; wvs[o] = 0.0;
+    id[o] = nold;
+}
+
+void sheet_release_emit(dvm_ctx_t ctx, const SheetDev& sh, int64_t nrv, const double* rrw,
+                        uint32_t nold, int64_t base, double* x, double* z, double* c, double* s,
+                        double* u, double* w, double* uvs, double* wvs, uint32_t* id)
+{
+    if (nrv <= 0) return;
+    release_emit_kernel<<<cdiv(nrv, 256), 256, 0, ctx->stream>>>(
+        nrv, sh.nl, sh.off, sh.cnt, sh.x, sh.z, sh.ds, sh.etx, sh.etz, sh.enx, sh.enz, sh.cnew,
+        sh.panel, rrw, nold, base, x, z, c, s, u, w, uvs, wvs, id);
+    DVM_LAUNCH_CHECK();
+}
+
+// ----------------------------------------------------------------- crossing
+
+// src/VortexSheet.cpp:447-464: crossing number over the Nl edges; inside iff 1.
+__device__ __forceinline__ bool inside_body(int nl, const double* __restrict__ bxp,
+                                            const double* __restrict__ bzp, double xc, double zc)
+{
+    int cn = 0;
+    for (int i = 0; i < nl; i++) {
+        const double zi = bzp[i], zi1 = bzp[i + 1];
+        if (((zi <= zc) && (zi1 > zc)) || ((zi > zc) && (zi1 <= zc))) {
+            const double vt = (zc - zi) / (zi1 - zi);
+            if (xc < bxp[i] + vt * (bxp[i + 1] - bxp[i])) ++cn;
+        }
+    }
+    return cn == 1;
+}
+
+__global__ __launch_bounds__(256) void inside_kernel(int64_t n, int nl, const double* __restrict__ bxp,
+                                                     const double* __restrict__ bzp,
+                                                     const double* __restrict__ x,
+                                                     const double* __restrict__ z,
+                                                     int* __restrict__ flags)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flags[i] = inside_body(nl, bxp, bzp, x[i], z[i]) ? 1 : 0;
+}
+
+void sheet_inside_flags(dvm_ctx_t ctx, const SheetDev& sh, int64_t n, const double* x,
+                        const double* z, int* flags)
+{
+    if (n <= 0) return;
+    ProfScope ps(ctx, "crossing_flags");
+    inside_kernel<<<cdiv(n, 256), 256, 0, ctx->stream>>>(n, sh.nl, sh.x, sh.z, x, z, flags);
+    DVM_LAUNCH_CHECK();
+}
+
+// src/VortexSheet.cpp:345-405 for the flagged (inside) blobs.
+__global__ void reflect_kernel(int64_t n, int nl, const double* __restrict__ bxp,
+                               const double* __restrict__ bzp, const double* __restrict__ enx,
+                               const double* __restrict__ enz, const int* __restrict__ flags,
+                               double* __restrict__ x, double* __restrict__ z)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flags[i]) return;
+    const double px = x[i], pz = z[i];
+    int closest = 0;
+    double min_dist = 1e10;
+    for (int j = 0; j < nl; j++) {
+        const double x0 = bxp[j], z0 = bzp[j], x1 = bxp[j + 1], z1 = bzp[j + 1];
+        const double vx = px - x0, vz = pz - z0;
+        const double dx = x1 - x0, dz = z1 - z0;
+        const double len_sq = dx * dx + dz * dz;
+        double t = (vx * dx + vz * dz) / len_sq;
+        const double tm = (t < 1.0) ? t : 1.0;      // std::min(1.0, t)
+        t = (0.0 < tm) ? tm : 0.0;                   // std::max(0.0, .)
+        const double cx = x0 + t * dx, cz = z0 + t * dz;
+        const double dist = sqrt((px - cx) * (px - cx) + (pz - cz) * (pz - cz));
+        if (dist < min_dist) { min_dist = dist; closest = j; }
+    }
+    const double offset = 2.0 * min_dist + 1e-6;
+    double nx = px + offset * enx[closest];
+    double nz = pz + offset * enz[closest];
+    if (inside_body(nl, bxp, bzp, nx, nz)) {
+        nx = px + 3.0 * offset * enx[closest];
+        nz = pz + 3.0 * offset * enz[closest];
+    }
+    x[i] = nx;
+    z[i] = nz;
+}
+
+void sheet_reflect(dvm_ctx_t ctx, const SheetDev& sh, int64_t n, const int* flags, double* x,
+                   double* z)
+{
+    if (n <= 0) return;
+    reflect_kernel<<<cdiv(n, 256), 256, 0, ctx->stream>>>(n, sh.nl, sh.x, sh.z, sh.enx, sh.enz,
+                                                          flags, x, z);
+    DVM_LAUNCH_CHECK();
+}
+
+}  // namespace dvmi
+
+using namespace dvmi;
+
+extern "C" {
+
+dvm_status dvm_sheet_create(dvm_ctx_t ctx, int npts, const double* px, const double* pz,
+                            const dvm_sheet_params* params, dvm_sheet_t* out)
+{
+    if (!ctx || !out) return DVM_E_INVALID;
+    *out = nullptr;
+    dvm_sheet_t s = new (std::nothrow) dvm_sheet_s();
+    if (!s) return DVM_E_OOM;
+    s->ctx = ctx;
+    dvm_status st = guard(ctx, [&] { sheet_init(ctx, s->sh, npts, px, pz, params); });
+    if (st != DVM_OK) {
+        sheet_free(s->sh);
+        delete s;
+        return st;
+    }
+    *out = s;
+    return DVM_OK;
+}
+
+dvm_status dvm_sheet_destroy(dvm_sheet_t s)
+{
+    if (!s) return DVM_OK;
+    dvm_status st = guard(s->ctx, [&] {
+        DVM_HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+        sheet_free(s->sh);
+    });
+    delete s;
+    return st;
+}
+
+dvm_status dvm_sheet_size(dvm_sheet_t s, int* nl)
+{
+    if (!s || !nl) return DVM_E_INVALID;
+    *nl = s->sh.nl;
+    return DVM_OK;
+}
+
+dvm_status dvm_sheet_geometry(dvm_sheet_t s, double* xc, double* zc, double* theta, double* ds,
+                              double* enx, double* enz, double* etx, double* etz, double* aref)
+{
+    if (!s) return DVM_E_INVALID;
+    const SheetDev& h = s->sh;
+    const size_t b = sizeof(double) * h.nl;
+    if (xc) std::memcpy(xc, h.hxc.data(), b);
+    if (zc) std::memcpy(zc, h.hzc.data(), b);
+    if (theta) std::memcpy(theta, h.htheta.data(), b);
+    if (ds) std::memcpy(ds, h.hds.data(), b);
+    if (enx) std::memcpy(enx, h.henx.data(), b);
+    if (enz) std::memcpy(enz, h.henz.data(), b);
+    if (etx) std::memcpy(etx, h.hetx.data(), b);
+    if (etz) std::memcpy(etz, h.hetz.data(), b);
+    if (aref) *aref = h.aref;
+    return DVM_OK;
+}
+
+dvm_status dvm_sheet_influence(dvm_sheet_t s, double* A)
+{
+    if (!s || !A) return DVM_E_INVALID;
+    return guard(s->ctx, [&] {
+        DVM_HIP_CHECK(hipMemcpyAsync(A, s->sh.A, sizeof(double) * (s->sh.nl + 1) * s->sh.nl,
+                                     hipMemcpyDeviceToHost, s->ctx->stream));
+        DVM_HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+    });
+}
+
+dvm_status dvm_sheet_gamma_ptr(dvm_sheet_t s, double** g)
+{
+    if (!s || !g) return DVM_E_INVALID;
+    *g = s->sh.gamma;
+    return DVM_OK;
+}
+
+dvm_status dvm_sheet_get_gamma(dvm_sheet_t s, double* g)
+{
+    if (!s || !g) return DVM_E_INVALID;
+    return guard(s->ctx, [&] {
+        DVM_HIP_CHECK(hipMemcpyAsync(g, s->sh.gamma, sizeof(double) * s->sh.nl,
+                                     hipMemcpyDeviceToHost, s->ctx->stream));
+        DVM_HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+    });
+}
+
+dvm_status dvm_sheet_set_gamma(dvm_sheet_t s, const double* g)
+{
+    if (!s || !g) return DVM_E_INVALID;
+    return guard(s->ctx, [&] {
+        DVM_HIP_CHECK(hipMemcpyAsync(s->sh.gamma, g, sizeof(double) * s->sh.nl,
+                                     hipMemcpyHostToDevice, s->ctx->stream));
+        DVM_HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+    });
+}
+
+dvm_status dvm_sheet_solve(dvm_sheet_t s, int prec, int64_t n, const double* x, const double* z,
+                           const double* circ, const double* sigma, double* b_host)
+{
+    if (!s) return DVM_E_INVALID;
+    return guard(s->ctx, [&] {
+        require(n >= 0, DVM_E_SIZE, "negative blob count");
+        require(n == 0 || (x && z && circ && sigma), DVM_E_INVALID, "null blob array");
+        sheet_solve(s->ctx, s->sh, prec, n, x, z, circ, sigma);
+        if (b_host) {
+            DVM_HIP_CHECK(hipMemcpyAsync(b_host, s->sh.b, sizeof(double) * (s->sh.nl + 1),
+                                         hipMemcpyDeviceToHost, s->ctx->stream));
+            DVM_HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+        }
+    });
+}
+
+dvm_status dvm_sheet_on_blobs(dvm_sheet_t s, int64_t n, const double* x, const double* z,
+                              double* uvs, double* wvs)
+{
+    if (!s) return DVM_E_INVALID;
+    return guard(s->ctx, [&] {
+        require(n >= 0, DVM_E_SIZE, "negative blob count");
+        require(n == 0 || (x && z && uvs && wvs), DVM_E_INVALID, "null array");
+        sheet_on_blobs(s->ctx, s->sh, n, x, z, uvs, wvs);
+    });
+}
+
+dvm_status dvm_sheet_loads(dvm_sheet_t s, double ur, double* out4)
+{
+    if (!s) return DVM_E_INVALID;
+    return guard(s->ctx, [&] {
+        sheet_loads(s->ctx, s->sh, ur);
+        if (out4) {
+            DVM_HIP_CHECK(hipMemcpyAsync(out4, s->sh.scal + 1, sizeof(double) * 4,
+                                         hipMemcpyDeviceToHost, s->ctx->stream));
+            DVM_HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+        }
+    });
+}
+
+dvm_status dvm_sheet_inside(dvm_sheet_t s, int64_t n, const double* x, const double* z, int* flags)
+{
+    if (!s) return DVM_E_INVALID;
+    return guard(s->ctx, [&] {
+        require(n >= 0, DVM_E_SIZE, "negative count");
+        require(n == 0 || (x && z && flags), DVM_E_INVALID, "null array");
+        sheet_inside_flags(s->ctx, s->sh, n, x, z, flags);
+    });
+}
+
+dvm_status dvm_sheet_reflect(dvm_sheet_t s, int64_t n, double* x, double* z)
+{
+    if (!s) return DVM_E_INVALID;
+    return guard(s->ctx, [&] {
+        require(n >= 0, DVM_E_SIZE, "negative count");
+        if (n == 0) return;
+        require(x && z, DVM_E_INVALID, "null array");
+        int* flags = static_cast<int*>(ws_get(s->ctx, sizeof(int) * n));
+        sheet_inside_flags(s->ctx, s->sh, n, x, z, flags);
+        sheet_reflect(s->ctx, s->sh, n, flags, x, z);
+    });
+}
+
+}  // extern "C"

@@ -112,6 +112,184 @@ dvm_status dvm_bs_targets_host(dvm_ctx_t ctx, int prec, int64_t ns, const double
                                int64_t nt, const double* tx, const double* tz,
                                double* results2);
 
+/* ------------------------------------------------------------------ vortex sheet
+ * One body's vortex sheet, resident on the device: panel geometry, the
+ * influence matrix A ((Nl+1) x Nl), the least-squares operator
+ * (A^T A)^-1 A^T formed once (A is time-invariant), and the surface
+ * vorticity gamma. */
+typedef struct dvm_sheet_s* dvm_sheet_t;
+
+/* Physical constants of src/VortexSheet.cpp:12-54 (XML constants/, flow/, time/). */
+typedef struct dvm_sheet_params {
+    double rho;        /* constants/density */
+    double nu;         /* constants/nu */
+    double dt;         /* time/dt */
+    double ux, uz;     /* flow/ux, flow/uz */
+    double cutoff_exp; /* constants/cutoff_exp: nascent sigma = ds^cutoff_exp */
+    int max_npv;       /* constants/max_NumPanelVort */
+} dvm_sheet_params;
+
+/* VortexSheet(const XmlHandler&, stamp) minus file I/O: read_input_coord,
+ * form_vortex_sheet and compute_influence_matrix (src/VortexSheet.cpp:12-165).
+ * px, pz: HOST arrays of npts body points, closed (last == first), so
+ * Nl = npts - 1 panels. */
+dvm_status dvm_sheet_create(dvm_ctx_t ctx, int npts, const double* px, const double* pz,
+                            const dvm_sheet_params* params, dvm_sheet_t* out);
+dvm_status dvm_sheet_destroy(dvm_sheet_t sheet);
+/* VortexSheet::size (src/VortexSheet.cpp:516-530). */
+dvm_status dvm_sheet_size(dvm_sheet_t sheet, int* nl);
+/* Geometry to HOST arrays (each Nl long; any may be NULL); aref = x-extent
+ * (src/VortexSheet.cpp:74). */
+dvm_status dvm_sheet_geometry(dvm_sheet_t sheet, double* xc, double* zc, double* theta,
+                              double* ds, double* enx, double* enz, double* etx, double* etz,
+                              double* aref);
+/* Influence matrix to a HOST array, row-major (Nl+1) x Nl (src/VortexSheet.cpp:104-165). */
+dvm_status dvm_sheet_influence(dvm_sheet_t sheet, double* A);
+/* Device pointer to gamma (Nl doubles), valid until dvm_sheet_destroy. */
+dvm_status dvm_sheet_gamma_ptr(dvm_sheet_t sheet, double** gamma);
+/* gamma to / from HOST memory. */
+dvm_status dvm_sheet_get_gamma(dvm_sheet_t sheet, double* gamma);
+dvm_status dvm_sheet_set_gamma(dvm_sheet_t sheet, const double* gamma);
+/* VortexSheet::solvevortexsheet (src/VortexSheet.cpp:167-215): collocation
+ * sum of the n device blobs (unguarded), RHS b = (U+u).n, b_Nl = -sum circ,
+ * gamma = (A^T A)^-1 A^T b.  b_host (Nl+1, may be NULL) receives b. */
+dvm_status dvm_sheet_solve(dvm_sheet_t sheet, int prec, int64_t n, const double* x,
+                           const double* z, const double* circ, const double* sigma,
+                           double* b_host);
+/* VortexSheet::vortexsheetbc (src/VortexSheet.cpp:217-273): sheet-induced
+ * velocity at n device points, fused (no N x Nl coefficient matrices). */
+dvm_status dvm_sheet_on_blobs(dvm_sheet_t sheet, int64_t n, const double* x, const double* z,
+                              double* uvs, double* wvs);
+/* VortexSheet::compute_loads (src/VortexSheet.cpp:487-514) and get_forces
+ * (:552-555): out4 (HOST) = {fx, fz, cd, cl}. */
+dvm_status dvm_sheet_loads(dvm_sheet_t sheet, double ur, double* out4);
+/* VortexSheet::inside_body (src/VortexSheet.cpp:447-464) for n device points:
+ * flags[i] = 1 if inside (crossing number == 1). */
+dvm_status dvm_sheet_inside(dvm_sheet_t sheet, int64_t n, const double* x, const double* z,
+                            int* flags);
+/* VortexSheet::reflect (src/VortexSheet.cpp:345-405), in place on device x, z. */
+dvm_status dvm_sheet_reflect(dvm_sheet_t sheet, int64_t n, double* x, double* z);
+
+/* ------------------------------------------------------------------ blob operations
+ * Single steps of the time loop on caller-owned device SoA arrays. */
+
+/* Euler convection, src/DVMBase.cpp:179-180: x += ((u + uvs) + ux) dt. */
+dvm_status dvm_advect_euler(dvm_ctx_t ctx, int64_t n, double ux, double uz, double dt,
+                            const double* u, const double* w, const double* uvs,
+                            const double* wvs, double* x, double* z);
+/* One low-storage RK3 stage, src/DVMBase.cpp:201-205:
+ * q = a q + ((u + uvs) + U) dt; x += b q (q1, q2 zeroed by the caller before
+ * stage 0, as src/DVMBase.cpp:188-189 does). */
+dvm_status dvm_advect_rk3_stage(dvm_ctx_t ctx, int64_t n, double a, double b, double ux,
+                                double uz, double dt, const double* u, const double* w,
+                                const double* uvs, const double* wvs, double* q1, double* q2,
+                                double* x, double* z);
+/* Random-walk diffusion with given increments (VortexBlobs::diffusion_random_walk,
+ * src/VortexBlobs.cpp:251-265): x_i += dxdz[2i], z_i += dxdz[2i+1]; the
+ * increments are r cos(theta), r sin(theta) of the reference's Box-Muller step. */
+dvm_status dvm_diffuse(dvm_ctx_t ctx, int64_t n, const double* dxdz, double* x, double* z);
+/* Nascent-vortex release, split around the random draws
+ * (VortexSheet::release_nascent_vortices_rw, src/VortexSheet.cpp:275-333):
+ * _counts computes the per-panel counts from gamma and returns their total
+ * (DVM_E_NAN on non-finite counts); _emit writes the nrv new blobs at index
+ * `base` of the given device arrays, blob k offset by rrw[k] (device) along the
+ * panel's outward normal; IDs are set to `id_value` (Nold in append_vortices,
+ * src/VortexBlobs.cpp:215-218). */
+dvm_status dvm_sheet_release_counts(dvm_sheet_t sheet, int64_t* nrv);
+dvm_status dvm_sheet_release_emit(dvm_sheet_t sheet, int64_t nrv, const double* rrw,
+                                  uint32_t id_value, int64_t base, double* x, double* z,
+                                  double* circ, double* sigma, double* u, double* w,
+                                  double* uvs, double* wvs, uint32_t* id);
+/* VortexSheet::delete_inside (src/VortexSheet.cpp:407-445) in place on device
+ * arrays: stable compaction of the blobs outside the body, IDs renumbered
+ * 0..M-1 (unchanged when nothing is deleted); *n_out = M. */
+dvm_status dvm_sheet_delete_inside(dvm_sheet_t sheet, int64_t n, double* x, double* z,
+                                   double* circ, double* sigma, double* u, double* w,
+                                   double* uvs, double* wvs, uint32_t* id, int64_t* n_out);
+
+/* ------------------------------------------------------------------ simulation
+ * One DVM simulation (one body) with all blob and sheet state resident on the
+ * device: DVMBase's time loop (src/DVMBase.cpp:67-209) without its text I/O. */
+typedef struct dvm_sim_s* dvm_sim_t;
+
+/* <remesh> block of the XML (src/XmlHandler.cpp:192-201); frequency 0 disables. */
+typedef struct dvm_remesh_params {
+    int frequency;        /* remesh every `frequency` steps */
+    int kernel;           /* 0 = M4prime, 1 = gaussian, 2 = M6 */
+    double xmin, xmax, zmin, zmax, dx, dz;
+    double sigma;         /* core size of the new blobs */
+    double circ_threshold;/* nodes with |circ| <= threshold create no blob */
+} dvm_remesh_params;
+
+/* BackgroundMesh::remesh (use site src/DVMBase.cpp:160-165; the reference's
+ * BackgroundMesh source is missing, the API is that of
+ * tests/unit/test_BackgroundMesh.cpp:14-18,105-111,167-204).  Redistributes n
+ * device blobs onto the mesh: nodes with |circ| > circ_threshold in linear
+ * order k*nx + i (sigma = params->sigma), then the blobs whose interpolation
+ * stencil leaves the mesh, unchanged.  *n_out receives the output count; if it
+ * exceeds out_cap nothing is written and DVM_E_SIZE is returned. */
+dvm_status dvm_remesh(dvm_ctx_t ctx, const dvm_remesh_params* params, int64_t n,
+                      const double* x, const double* z, const double* circ,
+                      const double* sigma, int64_t out_cap, double* ox, double* oz,
+                      double* ocirc, double* osigma, int64_t* n_out);
+
+typedef enum dvm_scheme { DVM_EULER = 0, DVM_RK3 = 1 } dvm_scheme;
+typedef enum dvm_crossing { DVM_DELETE = 0, DVM_ABSORB = 1, DVM_REFLECT = 2 } dvm_crossing;
+/* Random stream of the diffusion and release steps.
+ *  DVM_RNG_REFERENCE: std::mt19937 + uniform_real_distribution<double> seeded
+ *    as src/Random.cpp:3-18, consumed in the reference's order (two draws per
+ *    blob for diffusion, one per nascent blob); the host turns the draws into
+ *    displacement increments with the C library's libm and the device applies
+ *    them, so positions match the reference CPU path bit for bit.
+ *  DVM_RNG_DEVICE: counter-based Philox4x32-10 on the device keyed by
+ *    (seed, step, blob index): same distributions, no host round trip. */
+typedef enum dvm_rng_mode { DVM_RNG_REFERENCE = 0, DVM_RNG_DEVICE = 1 } dvm_rng_mode;
+
+typedef struct dvm_sim_params {
+    dvm_sheet_params sheet;
+    int scheme;        /* dvm_scheme (time/scheme) */
+    int crossing;      /* dvm_crossing (algorithms/surface_crossing) */
+    int seed;          /* constants/seed (< 0: std::random_device) */
+    int prec;          /* dvm_precision of the pair sums */
+    int rng_mode;      /* dvm_rng_mode */
+    dvm_remesh_params remesh;
+} dvm_sim_params;
+
+/* DVMBase(XmlHandler&, stamp) (src/DVMBase.cpp:8-65): body points px, pz on
+ * the HOST (closed contour), zero blobs. */
+dvm_status dvm_sim_create(dvm_ctx_t ctx, int npts, const double* px, const double* pz,
+                          const dvm_sim_params* params, dvm_sim_t* out);
+dvm_status dvm_sim_destroy(dvm_sim_t sim);
+/* increment_step + compute_step (src/DVMBase.cpp:70-71, 86-133): convect
+ * (Euler or RK3), remesh, diffusion, release + append, crossing, loads.
+ * DVM_E_NAN on the VortexException path (non-finite release counts). */
+dvm_status dvm_sim_step(dvm_sim_t sim);
+/* `steps` calls of dvm_sim_step (DVMBase::solve without output). */
+dvm_status dvm_sim_run(dvm_sim_t sim, int steps);
+/* Sub-steps of compute_step, in its order, for lock-step checking:
+ * 0 = increment_step + convect, 1 = remesh, 2 = diffusion,
+ * 3 = release + append, 4 = crossing, 5 = loads. */
+dvm_status dvm_sim_substep(dvm_sim_t sim, int which);
+dvm_status dvm_sim_size(dvm_sim_t sim, int64_t* n);
+dvm_status dvm_sim_step_index(dvm_sim_t sim, int* step);
+/* Uniform draws consumed so far from the reference stream. */
+dvm_status dvm_sim_rng_draws(dvm_sim_t sim, uint64_t* draws);
+/* Blob state to / from HOST arrays (VortexBlobs fields, src/VortexBlobs.hpp:24-32);
+ * any output may be NULL. */
+dvm_status dvm_sim_get_state(dvm_sim_t sim, uint32_t* id, double* x, double* z, double* circ,
+                             double* sigma, double* u, double* w, double* uvs, double* wvs);
+dvm_status dvm_sim_set_state(dvm_sim_t sim, int64_t n, const uint32_t* id, const double* x,
+                             const double* z, const double* circ, const double* sigma,
+                             const double* u, const double* w, const double* uvs,
+                             const double* wvs);
+/* gamma (Nl) to / from HOST memory. */
+dvm_status dvm_sim_get_gamma(dvm_sim_t sim, double* gamma);
+dvm_status dvm_sim_set_gamma(dvm_sim_t sim, const double* gamma);
+/* Loads of the last step to HOST: {fx, fz, cd, cl}. */
+dvm_status dvm_sim_loads(dvm_sim_t sim, double* out4);
+/* Device pointers to the blob SoA (valid until the next call that changes N). */
+dvm_status dvm_sim_blobs(dvm_sim_t sim, double** x, double** z, double** circ, double** sigma);
+
 #ifdef __cplusplus
 }
 #endif

@@ -105,6 +105,9 @@ def main():
     ap.add_argument("--f64-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this size (reference kernel "
+                         "for scaling HBM byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -189,6 +192,14 @@ def main():
         },
         "probe_kernel_ms": {"f32c": tgt32_ms, "f64": tgt64_ms},
     }
+    if args.copy_calibration_mib > 0:
+        nbytes = args.copy_calibration_mib << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        line["copy_calibration_bytes"] = nbytes
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(x, z, c, s, args.cpu_seconds)
     ctx.close()

@@ -1,0 +1,56 @@
+// Vortex blobs (src/VortexBlobs.hpp:21-109): public SoA fields in host memory;
+// the compute methods run on the GPU through the C-ABI.  Output-file members
+// of the reference are not part of this layer.
+#ifndef DVMPP_AMD_VORTEXBLOBS_HPP
+#define DVMPP_AMD_VORTEXBLOBS_HPP
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "BaseTypes.hpp"
+#include "Exceptions.hpp"
+#include "Random.hpp"
+
+namespace dvm {
+class VelocityKernel;
+}
+
+class VortexBlobs {
+public:
+    std::vector<unsigned> m_ID;
+    Vector m_x;
+    Vector m_z;
+    Vector m_circ;
+    Vector m_sigma;
+    Vector m_u;
+    Vector m_w;
+    Vector m_uvs;
+    Vector m_wvs;
+
+    VortexBlobs();
+    explicit VortexBlobs(unsigned N);
+    ~VortexBlobs();
+    VortexBlobs(VortexBlobs&& other) noexcept;
+    VortexBlobs& operator=(VortexBlobs&& other) noexcept;
+    VortexBlobs(const VortexBlobs& other);
+    VortexBlobs& operator=(const VortexBlobs& other);
+
+    /// src/VortexBlobs.cpp:211-227: appended IDs are Nold + the new blob's ID.
+    void append_vortices(VortexBlobs& NewVortBlobs);
+    /// src/VortexBlobs.cpp:229-249 through the selected velocity kernel.
+    void biotsavart();
+    /// src/VortexBlobs.cpp:251-265: draws R1, R2 per blob from `rand` in the
+    /// reference's order; the displacements are applied on the GPU.
+    void diffusion_random_walk(Random& rand, double nu, double dt);
+    [[nodiscard]] double totalcirc() const;
+    /// Throws SizeMismatchException if the fields disagree (src/VortexBlobs.cpp:272-285).
+    [[nodiscard]] unsigned size() const;
+    void setKernelType(const std::string& type);
+    [[nodiscard]] std::string kernelType() const;
+
+private:
+    std::unique_ptr<dvm::VelocityKernel> m_kernel;
+};
+
+#endif

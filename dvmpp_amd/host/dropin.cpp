@@ -1,0 +1,527 @@
+// Drop-in C++17 classes over the C-ABI (include/dvm_hip.h).
+#include <cmath>
+#include <cstdlib>
+#include <fstream>
+#include <mutex>
+#include <sstream>
+
+#include "BackgroundMesh.hpp"
+#include "DVMBase.hpp"
+#include "Device.hpp"
+#include "Exceptions.hpp"
+#include "VelocityKernel.hpp"
+#include "VortexBlobs.hpp"
+#include "VortexSheet.hpp"
+
+namespace dvm {
+
+// ---------------------------------------------------------------- errors
+
+void throw_status(dvm_status st, const std::string& where, dvm_ctx_t ctx)
+{
+    const std::string msg = where + ": " + (ctx ? dvm_last_error(ctx) : dvm_status_string(st));
+    switch (st) {
+    case DVM_E_INVALID: throw AlgorithmException(msg);
+    case DVM_E_SIZE: throw SizeMismatchException(msg);
+    case DVM_E_NAN: throw VortexException(msg);
+    default: throw DVMException(msg);
+    }
+}
+
+// ---------------------------------------------------------------- device
+
+namespace {
+std::mutex g_ctx_mutex;
+dvm_ctx_t g_ctx = nullptr;  // lives until process exit (the OS reclaims it)
+}  // namespace
+
+dvm_ctx_t device_context()
+{
+    std::lock_guard<std::mutex> lock(g_ctx_mutex);
+    if (!g_ctx) {
+        const char* env = std::getenv("DVM_DEVICE");
+        const int dev = env ? std::atoi(env) : 0;
+        check(dvm_ctx_create(dev, nullptr, &g_ctx), "dvm_ctx_create", nullptr);
+    }
+    return g_ctx;
+}
+
+DeviceBuffer::~DeviceBuffer()
+{
+    if (ptr_) (void)dvm_free(device_context(), ptr_);
+}
+
+void DeviceBuffer::reset(std::size_t bytes)
+{
+    dvm_ctx_t ctx = device_context();
+    if (ptr_) check(dvm_free(ctx, ptr_), "dvm_free", ctx);
+    ptr_ = nullptr;
+    bytes_ = bytes;
+    if (bytes) check(dvm_malloc(ctx, bytes, &ptr_), "dvm_malloc", ctx);
+}
+
+void DeviceBuffer::upload(const void* src, std::size_t bytes, std::size_t offset)
+{
+    if (!bytes) return;
+    dvm_ctx_t ctx = device_context();
+    check(dvm_memcpy_h2d(ctx, static_cast<char*>(ptr_) + offset, src, bytes), "upload", ctx);
+}
+
+void DeviceBuffer::download(void* dst, std::size_t bytes, std::size_t offset) const
+{
+    if (!bytes) return;
+    dvm_ctx_t ctx = device_context();
+    check(dvm_memcpy_d2h(ctx, dst, static_cast<char*>(ptr_) + offset, bytes), "download", ctx);
+}
+
+// ---------------------------------------------------------------- VelocityKernel
+
+std::unique_ptr<VelocityKernel> VelocityKernel::create(const std::string& type)
+{
+    if (type == "direct") return std::make_unique<DirectKernel>();
+    if (type == "direct_f32c") return std::make_unique<DirectF32CKernel>();
+    throw AlgorithmException("Unknown velocity kernel type: " + type +
+                             ". Available: direct, direct_f32c");
+}
+
+int DirectKernel::precision() const { return DVM_F64; }
+int DirectF32CKernel::precision() const { return DVM_F32C; }
+
+static_assert(sizeof(VortexParticle) == 4 * sizeof(double), "AoS particle layout");
+static_assert(sizeof(VelocityResult) == 2 * sizeof(double), "AoS result layout");
+
+void DirectKernel::computeSelfInduced(const std::vector<VortexParticle>& particles,
+                                      std::vector<VelocityResult>& velocities)
+{
+    velocities.resize(particles.size());
+    if (particles.empty()) return;
+    dvm_ctx_t ctx = device_context();
+    check(dvm_bs_self_host(ctx, precision(), (int64_t)particles.size(),
+                           reinterpret_cast<const double*>(particles.data()),
+                           reinterpret_cast<double*>(velocities.data())),
+          "computeSelfInduced", ctx);
+}
+
+void DirectKernel::computeInducedAt(const std::vector<VortexParticle>& sources,
+                                    const std::vector<double>& target_x,
+                                    const std::vector<double>& target_z,
+                                    std::vector<VelocityResult>& velocities)
+{
+    if (target_x.size() != target_z.size()) throw SizeMismatchException("computeInducedAt targets");
+    velocities.resize(target_x.size());
+    if (target_x.empty()) return;
+    dvm_ctx_t ctx = device_context();
+    check(dvm_bs_targets_host(ctx, precision(), (int64_t)sources.size(),
+                              reinterpret_cast<const double*>(sources.data()),
+                              (int64_t)target_x.size(), target_x.data(), target_z.data(),
+                              reinterpret_cast<double*>(velocities.data())),
+          "computeInducedAt", ctx);
+}
+
+}  // namespace dvm
+
+// ---------------------------------------------------------------- VortexBlobs
+
+VortexBlobs::VortexBlobs() : m_kernel(dvm::VelocityKernel::create("direct")) {}
+
+VortexBlobs::VortexBlobs(unsigned N) : VortexBlobs()
+{
+    m_ID.assign(N, 0u);
+    for (Vector* v : {&m_x, &m_z, &m_circ, &m_sigma, &m_u, &m_w, &m_uvs, &m_wvs}) v->zeros(N);
+}
+
+VortexBlobs::~VortexBlobs() = default;
+VortexBlobs::VortexBlobs(VortexBlobs&& other) noexcept = default;
+VortexBlobs& VortexBlobs::operator=(VortexBlobs&& other) noexcept = default;
+
+VortexBlobs::VortexBlobs(const VortexBlobs& o)
+    : m_ID(o.m_ID), m_x(o.m_x), m_z(o.m_z), m_circ(o.m_circ), m_sigma(o.m_sigma), m_u(o.m_u),
+      m_w(o.m_w), m_uvs(o.m_uvs), m_wvs(o.m_wvs),
+      m_kernel(dvm::VelocityKernel::create(o.m_kernel ? o.m_kernel->name() : "direct"))
+{
+}
+
+VortexBlobs& VortexBlobs::operator=(const VortexBlobs& o)
+{
+    if (this != &o) {
+        m_ID = o.m_ID; m_x = o.m_x; m_z = o.m_z; m_circ = o.m_circ; m_sigma = o.m_sigma;
+        m_u = o.m_u; m_w = o.m_w; m_uvs = o.m_uvs; m_wvs = o.m_wvs;
+        m_kernel = dvm::VelocityKernel::create(o.m_kernel ? o.m_kernel->name() : "direct");
+    }
+    return *this;
+}
+
+void VortexBlobs::setKernelType(const std::string& type) { m_kernel = dvm::VelocityKernel::create(type); }
+
+std::string VortexBlobs::kernelType() const { return m_kernel->name(); }
+
+void VortexBlobs::append_vortices(VortexBlobs& nb)
+{
+    const unsigned nold = size();
+    const unsigned nnew = nb.size();
+    for (unsigned i = 0; i < nnew; i++) m_ID.push_back(nold + nb.m_ID[i]);
+    m_x.insert_rows(m_x.size(), nb.m_x);
+    m_z.insert_rows(m_z.size(), nb.m_z);
+    m_circ.insert_rows(m_circ.size(), nb.m_circ);
+    m_sigma.insert_rows(m_sigma.size(), nb.m_sigma);
+    m_u.insert_rows(m_u.size(), nb.m_u);
+    m_w.insert_rows(m_w.size(), nb.m_w);
+    m_uvs.insert_rows(m_uvs.size(), nb.m_uvs);
+    m_wvs.insert_rows(m_wvs.size(), nb.m_wvs);
+}
+
+void VortexBlobs::biotsavart()
+{
+    const unsigned N = size();
+    if (N == 0) return;
+    std::vector<dvm::VortexParticle> particles(N);
+    for (unsigned i = 0; i < N; ++i) particles[i] = {m_x(i), m_z(i), m_circ(i), m_sigma(i)};
+    std::vector<dvm::VelocityResult> velocities;
+    m_kernel->computeSelfInduced(particles, velocities);
+    for (unsigned i = 0; i < N; ++i) {
+        m_u(i) = velocities[i].u;
+        m_w(i) = velocities[i].w;
+    }
+}
+
+void VortexBlobs::diffusion_random_walk(Random& rand, double nu, double dt)
+{
+    const unsigned N = size();
+    if (N == 0) return;
+    // The reference's draw order and Box-Muller transform; the increments are
+    // applied to the positions on the device.
+    std::vector<double> tape(2 * (size_t)N);
+    for (unsigned i = 0; i < N; i++) {
+        const double R1 = rand.rand();
+        const double R2 = rand.rand();
+        const double rrw = std::sqrt(4.0 * nu * dt * std::log(1.0 / R1));
+        const double thetarw = 2.0 * math::pi * R2;
+        tape[2 * i] = rrw * std::cos(thetarw);
+        tape[2 * i + 1] = rrw * std::sin(thetarw);
+    }
+    const size_t b = sizeof(double) * N;
+    dvm::DeviceBuffer buf(4 * b);
+    buf.upload(tape.data(), 2 * b, 0);
+    buf.upload(m_x.memptr(), b, 2 * b);
+    buf.upload(m_z.memptr(), b, 3 * b);
+    dvm_ctx_t ctx = dvm::device_context();
+    double* d = buf.as<double>();
+    dvm::check(dvm_diffuse(ctx, N, d, d + 2 * (size_t)N, d + 3 * (size_t)N), "diffusion_random_walk", ctx);
+    buf.download(m_x.memptr(), b, 2 * b);
+    buf.download(m_z.memptr(), b, 3 * b);
+}
+
+double VortexBlobs::totalcirc() const
+{
+    // Armadillo's accumulate order: two interleaved partial sums.
+    double a1 = 0.0, a2 = 0.0;
+    const size_t n = m_circ.size();
+    size_t j;
+    for (j = 1; j < n; j += 2) {
+        a1 += m_circ(j - 1);
+        a2 += m_circ(j);
+    }
+    if ((j - 1) < n) a1 += m_circ(j - 1);
+    return a1 + a2;
+}
+
+unsigned VortexBlobs::size() const
+{
+    const size_t n = m_ID.size();
+    if (n != m_x.size() || n != m_z.size() || n != m_circ.size() || n != m_sigma.size() ||
+        n != m_u.size() || n != m_w.size() || n != m_uvs.size() || n != m_wvs.size())
+        throw dvm::SizeMismatchException("VortexBlobs");
+    return static_cast<unsigned>(n);
+}
+
+// ---------------------------------------------------------------- VortexSheet
+
+static void read_body(const std::string& file, std::vector<double>& x, std::vector<double>& z)
+{
+    std::ifstream in(file);
+    if (!in) throw dvm::FileIOException::cannotRead(file);
+    std::string line;
+    while (std::getline(in, line)) {
+        std::istringstream ss(line);
+        double a, b;
+        if (ss >> a >> b) {
+            x.push_back(a);
+            z.push_back(b);
+        }
+    }
+    if (x.size() < 3) throw dvm::FileIOException::cannotRead(file);
+}
+
+VortexSheet::VortexSheet(const std::vector<double>& x, const std::vector<double>& z,
+                         const dvm_sheet_params& params, int precision)
+    : m_params(params), m_prec(precision)
+{
+    init(x, z);
+}
+
+VortexSheet::VortexSheet(const std::string& body_file, const dvm_sheet_params& params, int precision)
+    : m_params(params), m_prec(precision)
+{
+    std::vector<double> x, z;
+    read_body(body_file, x, z);
+    init(x, z);
+}
+
+void VortexSheet::init(const std::vector<double>& x, const std::vector<double>& z)
+{
+    if (x.size() != z.size()) throw dvm::SizeMismatchException("VortexSheet body");
+    dvm_ctx_t ctx = dvm::device_context();
+    dvm::check(dvm_sheet_create(ctx, (int)x.size(), x.data(), z.data(), &m_params, &m_sheet),
+               "VortexSheet", ctx);
+    int nl = 0;
+    dvm::check(dvm_sheet_size(m_sheet, &nl), "VortexSheet::size", ctx);
+    m_nl = (unsigned)nl;
+}
+
+VortexSheet::~VortexSheet()
+{
+    if (m_sheet) (void)dvm_sheet_destroy(m_sheet);
+}
+
+unsigned VortexSheet::size() const { return m_nl; }
+
+std::tuple<double, double> VortexSheet::get_forces() const { return {m_fx, m_fz}; }
+std::tuple<double, double> VortexSheet::get_coefficients() const { return {m_cd, m_cl}; }
+
+Vector VortexSheet::gamma() const
+{
+    Vector g(m_nl);
+    dvm::check(dvm_sheet_get_gamma(m_sheet, g.memptr()), "gamma", dvm::device_context());
+    return g;
+}
+
+void VortexSheet::set_gamma(const Vector& g)
+{
+    if (g.size() != m_nl) throw dvm::SizeMismatchException("VortexSheet gamma");
+    dvm::check(dvm_sheet_set_gamma(m_sheet, g.memptr()), "set_gamma", dvm::device_context());
+}
+
+void VortexSheet::solvevortexsheet(VortexBlobs& blobs)
+{
+    const unsigned N = blobs.size();
+    const size_t b = sizeof(double) * N;
+    dvm::DeviceBuffer buf(4 * b + 8);
+    buf.upload(blobs.m_x.memptr(), b, 0);
+    buf.upload(blobs.m_z.memptr(), b, b);
+    buf.upload(blobs.m_circ.memptr(), b, 2 * b);
+    buf.upload(blobs.m_sigma.memptr(), b, 3 * b);
+    double* d = buf.as<double>();
+    dvm::check(dvm_sheet_solve(m_sheet, m_prec, N, d, d + N, d + 2 * (size_t)N, d + 3 * (size_t)N,
+                               nullptr),
+               "solvevortexsheet", dvm::device_context());
+    dvm::check(dvm_ctx_sync(dvm::device_context()), "solvevortexsheet", dvm::device_context());
+}
+
+void VortexSheet::vortexsheetbc(VortexBlobs& blobs)
+{
+    const unsigned N = blobs.size();
+    if (N == 0) return;
+    const size_t b = sizeof(double) * N;
+    dvm::DeviceBuffer buf(4 * b);
+    buf.upload(blobs.m_x.memptr(), b, 0);
+    buf.upload(blobs.m_z.memptr(), b, b);
+    double* d = buf.as<double>();
+    dvm::check(dvm_sheet_on_blobs(m_sheet, N, d, d + N, d + 2 * (size_t)N, d + 3 * (size_t)N),
+               "vortexsheetbc", dvm::device_context());
+    buf.download(blobs.m_uvs.memptr(), b, 2 * b);
+    buf.download(blobs.m_wvs.memptr(), b, 3 * b);
+}
+
+VortexBlobs VortexSheet::release_nascent_vortices_rw(Random& rand)
+{
+    int64_t nrv = 0;
+    dvm::check(dvm_sheet_release_counts(m_sheet, &nrv), "VortexSheet::release_nascent_vortices_rw",
+               dvm::device_context());
+    VortexBlobs out((unsigned)nrv);
+    if (nrv == 0) return out;
+    std::vector<double> rrw((size_t)nrv);
+    const double k = 4.0 * m_params.nu * m_params.dt;
+    for (int64_t i = 0; i < nrv; ++i) {
+        const double R = rand.rand();
+        rrw[(size_t)i] = std::fabs(std::sqrt(k * std::log(1.0 / R)));
+    }
+    const size_t b = sizeof(double) * nrv;
+    dvm::DeviceBuffer buf(9 * b + sizeof(uint32_t) * nrv);
+    buf.upload(rrw.data(), b, 0);
+    double* d = buf.as<double>();
+    uint32_t* id = reinterpret_cast<uint32_t*>(d + 9 * nrv);
+    dvm::check(dvm_sheet_release_emit(m_sheet, nrv, d, 0u, 0, d + nrv, d + 2 * nrv, d + 3 * nrv,
+                                      d + 4 * nrv, d + 5 * nrv, d + 6 * nrv, d + 7 * nrv,
+                                      d + 8 * nrv, id),
+               "release_nascent_vortices_rw", dvm::device_context());
+    Vector* f[8] = {&out.m_x, &out.m_z, &out.m_circ, &out.m_sigma, &out.m_u, &out.m_w, &out.m_uvs, &out.m_wvs};
+    for (int k2 = 0; k2 < 8; ++k2) buf.download(f[k2]->memptr(), b, (k2 + 1) * b);
+    return out;
+}
+
+void VortexSheet::reflect(VortexBlobs& v)
+{
+    const unsigned N = v.size();
+    if (N == 0) return;
+    const size_t b = sizeof(double) * N;
+    dvm::DeviceBuffer buf(2 * b);
+    buf.upload(v.m_x.memptr(), b, 0);
+    buf.upload(v.m_z.memptr(), b, b);
+    double* d = buf.as<double>();
+    dvm::check(dvm_sheet_reflect(m_sheet, N, d, d + N), "reflect", dvm::device_context());
+    buf.download(v.m_x.memptr(), b, 0);
+    buf.download(v.m_z.memptr(), b, b);
+}
+
+void VortexSheet::delete_inside(VortexBlobs& v)
+{
+    const unsigned N = v.size();
+    if (N == 0) return;
+    const size_t b = sizeof(double) * N;
+    dvm::DeviceBuffer buf(8 * b + sizeof(uint32_t) * N);
+    Vector* f[8] = {&v.m_x, &v.m_z, &v.m_circ, &v.m_sigma, &v.m_u, &v.m_w, &v.m_uvs, &v.m_wvs};
+    for (int k = 0; k < 8; ++k) buf.upload(f[k]->memptr(), b, k * b);
+    buf.upload(v.m_ID.data(), sizeof(uint32_t) * N, 8 * b);
+    double* d = buf.as<double>();
+    uint32_t* id = reinterpret_cast<uint32_t*>(d + 8 * (size_t)N);
+    int64_t m = 0;
+    dvm::check(dvm_sheet_delete_inside(m_sheet, N, d, d + N, d + 2 * (size_t)N, d + 3 * (size_t)N,
+                                       d + 4 * (size_t)N, d + 5 * (size_t)N, d + 6 * (size_t)N,
+                                       d + 7 * (size_t)N, id, &m),
+               "delete_inside", dvm::device_context());
+    if (m == (int64_t)N) return;
+    VortexBlobs nv((unsigned)m);
+    nv.setKernelType(v.kernelType());
+    Vector* g[8] = {&nv.m_x, &nv.m_z, &nv.m_circ, &nv.m_sigma, &nv.m_u, &nv.m_w, &nv.m_uvs, &nv.m_wvs};
+    for (int k = 0; k < 8; ++k) buf.download(g[k]->memptr(), sizeof(double) * m, k * b);
+    buf.download(nv.m_ID.data(), sizeof(uint32_t) * m, 8 * b);
+    v = std::move(nv);
+}
+
+void VortexSheet::compute_loads(double Urel)
+{
+    double out[4];
+    dvm::check(dvm_sheet_loads(m_sheet, Urel, out), "compute_loads", dvm::device_context());
+    m_fx = out[0];
+    m_fz = out[1];
+    m_cd = out[2];
+    m_cl = out[3];
+}
+
+// ---------------------------------------------------------------- BackgroundMesh
+
+BackgroundMesh::BackgroundMesh() = default;
+
+BackgroundMesh::BackgroundMesh(const dvm_remesh_params& p) : m_p(p), m_enabled(p.frequency > 0)
+{
+    if (m_enabled) {
+        m_nx = (unsigned)(std::llround((p.xmax - p.xmin) / p.dx) + 1);
+        m_nz = (unsigned)(std::llround((p.zmax - p.zmin) / p.dz) + 1);
+    }
+}
+
+bool BackgroundMesh::should_remesh(unsigned step) const
+{
+    return m_enabled && step > 0 && step % (unsigned)m_p.frequency == 0;
+}
+
+VortexBlobs BackgroundMesh::remesh(const VortexBlobs& blobs) const
+{
+    if (!m_enabled) return blobs;
+    const unsigned N = blobs.size();
+    const size_t b = sizeof(double) * N;
+    dvm::DeviceBuffer in(4 * b + 8);
+    in.upload(blobs.m_x.memptr(), b, 0);
+    in.upload(blobs.m_z.memptr(), b, b);
+    in.upload(blobs.m_circ.memptr(), b, 2 * b);
+    in.upload(blobs.m_sigma.memptr(), b, 3 * b);
+    double* d = in.as<double>();
+    dvm_ctx_t ctx = dvm::device_context();
+    int64_t m = 0;
+    dvm_status st = dvm_remesh(ctx, &m_p, N, d, d + N, d + 2 * (size_t)N, d + 3 * (size_t)N, 0,
+                               nullptr, nullptr, nullptr, nullptr, &m);
+    if (st != DVM_OK && st != DVM_E_SIZE) dvm::throw_status(st, "remesh", ctx);
+    VortexBlobs out((unsigned)m);
+    out.setKernelType(blobs.kernelType());
+    if (m == 0) return out;
+    const size_t bm = sizeof(double) * m;
+    dvm::DeviceBuffer o(4 * bm);
+    double* od = o.as<double>();
+    dvm::check(dvm_remesh(ctx, &m_p, N, d, d + N, d + 2 * (size_t)N, d + 3 * (size_t)N, m, od,
+                          od + m, od + 2 * m, od + 3 * m, &m),
+               "remesh", ctx);
+    o.download(out.m_x.memptr(), bm, 0);
+    o.download(out.m_z.memptr(), bm, bm);
+    o.download(out.m_circ.memptr(), bm, 2 * bm);
+    o.download(out.m_sigma.memptr(), bm, 3 * bm);
+    for (unsigned i = 0; i < (unsigned)m; ++i) out.m_ID[i] = i;
+    return out;
+}
+
+// ---------------------------------------------------------------- DVMBase
+
+DVMBase::DVMBase(const std::vector<double>& x, const std::vector<double>& z,
+                 const dvm_sim_params& params, unsigned steps)
+    : m_p(params), m_steps(steps)
+{
+    init(x, z);
+}
+
+DVMBase::DVMBase(const std::string& body_file, const dvm_sim_params& params, unsigned steps)
+    : m_p(params), m_steps(steps)
+{
+    std::vector<double> x, z;
+    read_body(body_file, x, z);
+    init(x, z);
+}
+
+void DVMBase::init(const std::vector<double>& x, const std::vector<double>& z)
+{
+    if (x.size() != z.size()) throw dvm::SizeMismatchException("DVMBase body");
+    dvm_ctx_t ctx = dvm::device_context();
+    dvm::check(dvm_sim_create(ctx, (int)x.size(), x.data(), z.data(), &m_p, &m_sim), "DVMBase", ctx);
+}
+
+DVMBase::~DVMBase()
+{
+    if (m_sim) (void)dvm_sim_destroy(m_sim);
+}
+
+void DVMBase::solve() { dvm::check(dvm_sim_run(m_sim, (int)m_steps), "DVMBase::solve", dvm::device_context()); }
+
+void DVMBase::step() { dvm::check(dvm_sim_step(m_sim), "DVMBase::step", dvm::device_context()); }
+
+unsigned DVMBase::current_step() const
+{
+    int s = 0;
+    dvm::check(dvm_sim_step_index(m_sim, &s), "step_index", dvm::device_context());
+    return (unsigned)s;
+}
+
+double DVMBase::time() const { return current_step() * m_p.sheet.dt; }
+
+unsigned DVMBase::num_blobs() const
+{
+    int64_t n = 0;
+    dvm::check(dvm_sim_size(m_sim, &n), "size", dvm::device_context());
+    return (unsigned)n;
+}
+
+VortexBlobs DVMBase::blobs() const
+{
+    const unsigned n = num_blobs();
+    VortexBlobs v(n);
+    if (n == 0) return v;
+    dvm::check(dvm_sim_get_state(m_sim, v.m_ID.data(), v.m_x.memptr(), v.m_z.memptr(),
+                                 v.m_circ.memptr(), v.m_sigma.memptr(), v.m_u.memptr(),
+                                 v.m_w.memptr(), v.m_uvs.memptr(), v.m_wvs.memptr()),
+               "blobs", dvm::device_context());
+    return v;
+}
+
+std::tuple<double, double, double, double> DVMBase::loads() const
+{
+    double o[4];
+    dvm::check(dvm_sim_loads(m_sim, o), "loads", dvm::device_context());
+    return {o[0], o[1], o[2], o[3]};
+}

@@ -1,0 +1,500 @@
+// C++ tests of the drop-in classes (dvmpp_amd/host), mirroring the reference's
+// GoogleTest suites case by case (tests/unit, tests/integration,
+// tests/regression) with a minimal runner, since GoogleTest is not available
+// offline.  Needs a GPU: the classes run the HIP kernels.
+#include <cmath>
+#include <cstdio>
+#include <functional>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "BackgroundMesh.hpp"
+#include "DVMBase.hpp"
+#include "Exceptions.hpp"
+#include "Random.hpp"
+#include "VelocityKernel.hpp"
+#include "VortexBlobs.hpp"
+#include "VortexSheet.hpp"
+
+namespace {
+struct Case {
+    const char* name;
+    std::function<void()> fn;
+};
+std::vector<Case>& cases()
+{
+    static std::vector<Case> c;
+    return c;
+}
+int g_fail = 0;
+struct Reg {
+    Reg(const char* n, std::function<void()> f) { cases().push_back({n, f}); }
+};
+}  // namespace
+
+#define TEST(name)                          \
+    static void name();                     \
+    static Reg reg_##name(#name, name);     \
+    static void name()
+#define EXPECT(cond)                                                              \
+    do {                                                                          \
+        if (!(cond)) {                                                            \
+            std::printf("  FAIL %s:%d: %s\n", __FILE__, __LINE__, #cond);        \
+            ++g_fail;                                                             \
+        }                                                                         \
+    } while (0)
+#define EXPECT_NEAR(a, b, tol) EXPECT(std::fabs((a) - (b)) <= (tol))
+#define EXPECT_THROW(stmt, exc)             \
+    do {                                    \
+        bool thrown_ = false;               \
+        try {                               \
+            stmt;                           \
+        } catch (const exc&) {              \
+            thrown_ = true;                 \
+        }                                   \
+        EXPECT(thrown_);                    \
+    } while (0)
+
+static std::string g_golden;
+
+// ---------------------------------------------------------------- VelocityKernel
+// tests/unit/test_VelocityKernel.cpp:12-125
+
+TEST(CreateDirectKernel)
+{
+    auto k = dvm::VelocityKernel::create("direct");
+    EXPECT(k != nullptr);
+    EXPECT(k->name() == "direct");
+    EXPECT(dvm::VelocityKernel::create("direct_f32c")->name() == "direct_f32c");
+}
+
+TEST(CreateInvalidKernelThrows) { EXPECT_THROW((void)dvm::VelocityKernel::create("invalid"), dvm::AlgorithmException); }
+
+TEST(DirectKernelComputesSelfInducedVelocity)
+{
+    auto k = dvm::VelocityKernel::create("direct");
+    std::vector<dvm::VortexParticle> p = {{0.0, 0.0, 1.0, 0.1}, {1.0, 0.0, -1.0, 0.1}};
+    std::vector<dvm::VelocityResult> v;
+    k->computeSelfInduced(p, v);
+    EXPECT(v.size() == 2u);
+    EXPECT(v[0].w != 0.0 && v[1].w != 0.0);
+    EXPECT(v[0].w * v[1].w > 0.0);
+}
+
+TEST(DirectKernelEmptyParticlesReturnsEmpty)
+{
+    auto k = dvm::VelocityKernel::create("direct");
+    std::vector<dvm::VortexParticle> p;
+    std::vector<dvm::VelocityResult> v(3);
+    k->computeSelfInduced(p, v);
+    EXPECT(v.empty());
+}
+
+TEST(DirectKernelSingleParticleHasZeroSelfVelocity)
+{
+    for (const char* name : {"direct", "direct_f32c"}) {
+        auto k = dvm::VelocityKernel::create(name);
+        std::vector<dvm::VortexParticle> p = {{0.0, 0.0, 1.0, 0.1}};
+        std::vector<dvm::VelocityResult> v;
+        k->computeSelfInduced(p, v);
+        EXPECT(v.size() == 1u);
+        EXPECT(v[0].u == 0.0 && v[0].w == 0.0);
+    }
+}
+
+TEST(DirectKernelComputesInducedAtTargets)
+{
+    auto k = dvm::VelocityKernel::create("direct");
+    std::vector<dvm::VortexParticle> s = {{0.0, 0.0, 1.0, 0.1}};
+    std::vector<dvm::VelocityResult> v;
+    k->computeInducedAt(s, {1.0}, {0.0}, v);
+    EXPECT(v.size() == 1u);
+    EXPECT_NEAR(v[0].u, 0.0, 1e-10);
+    EXPECT_NEAR(v[0].w, 1.0 / (2.0 * math::pi), 0.01);
+}
+
+TEST(BiotSavartVortexPairAnalytical)
+{
+    for (const char* name : {"direct", "direct_f32c"}) {
+        auto k = dvm::VelocityKernel::create(name);
+        const double d = 2.0, G = 1.0;
+        std::vector<dvm::VortexParticle> p = {{-d / 2, 0.0, G, 0.01}, {d / 2, 0.0, -G, 0.01}};
+        std::vector<dvm::VelocityResult> v;
+        k->computeSelfInduced(p, v);
+        const double ew = G / (2.0 * math::pi * d);
+        EXPECT_NEAR(v[0].w, ew, 0.01 * ew);
+        EXPECT_NEAR(v[1].w, ew, 0.01 * ew);
+        EXPECT_NEAR(v[0].u, 0.0, 1e-6);
+        EXPECT_NEAR(v[1].u, 0.0, 1e-6);
+    }
+}
+
+// ---------------------------------------------------------------- BiotSavart integration
+// tests/integration/test_BiotSavart.cpp:15-233
+
+static VortexBlobs pair(double d, double g0, double g1, double sigma)
+{
+    VortexBlobs b(2);
+    b.m_ID[0] = 0; b.m_ID[1] = 1;
+    b.m_x(0) = -d / 2; b.m_x(1) = d / 2;
+    b.m_z(0) = 0.0; b.m_z(1) = 0.0;
+    b.m_circ(0) = g0; b.m_circ(1) = g1;
+    b.m_sigma(0) = sigma; b.m_sigma(1) = sigma;
+    return b;
+}
+
+TEST(VortexPairTranslation)
+{
+    VortexBlobs b = pair(2.0, 1.0, -1.0, 0.01);
+    b.biotsavart();
+    const double ew = 1.0 / (2.0 * math::pi * 2.0);
+    EXPECT(b.m_w(0) > 0.0 && b.m_w(1) > 0.0);
+    EXPECT_NEAR(b.m_w(0), ew, 0.01 * ew);
+    EXPECT_NEAR(b.m_w(1), ew, 0.01 * ew);
+    EXPECT_NEAR(b.m_u(0), 0.0, 1e-10);
+    EXPECT_NEAR(b.m_u(1), 0.0, 1e-10);
+}
+
+TEST(CoRotatingVortexPair)
+{
+    VortexBlobs b = pair(2.0, 1.0, 1.0, 0.01);
+    b.biotsavart();
+    const double ew = 1.0 / (2.0 * math::pi * 2.0);
+    EXPECT_NEAR(b.m_w(0), -ew, 0.01 * ew);
+    EXPECT_NEAR(b.m_w(1), ew, 0.01 * ew);
+    EXPECT_NEAR(b.m_u(0), 0.0, 1e-10);
+    EXPECT_NEAR(b.m_u(1), 0.0, 1e-10);
+}
+
+TEST(RegularizationPreventsInfinity)
+{
+    VortexBlobs b = pair(0.001, 1.0, -1.0, 0.1);
+    b.biotsavart();
+    for (unsigned i = 0; i < 2; ++i) EXPECT(std::isfinite(b.m_u(i)) && std::isfinite(b.m_w(i)));
+}
+
+TEST(FourVortexLeapfrog)
+{
+    VortexBlobs b(4);
+    const double xs[4] = {-1, -1, 1, 1}, zs[4] = {0.5, -0.5, 0.5, -0.5}, gs[4] = {1, -1, 1, -1};
+    for (unsigned i = 0; i < 4; ++i) {
+        b.m_ID[i] = i; b.m_x(i) = xs[i]; b.m_z(i) = zs[i]; b.m_circ(i) = gs[i]; b.m_sigma(i) = 0.05;
+    }
+    b.biotsavart();
+    for (unsigned i = 0; i < 4; ++i) EXPECT(std::isfinite(b.m_u(i)) && std::isfinite(b.m_w(i)));
+    EXPECT_NEAR(b.m_u(0), b.m_u(2), 1e-10);
+    EXPECT_NEAR(b.m_u(1), b.m_u(3), 1e-10);
+}
+
+TEST(CircularRingApproximation)
+{
+    const int N = 20;
+    VortexBlobs b(N);
+    for (int i = 0; i < N; ++i) {
+        const double th = 2.0 * math::pi * i / N;
+        b.m_ID[i] = i; b.m_x(i) = std::cos(th); b.m_z(i) = std::sin(th);
+        b.m_circ(i) = 1.0 / N; b.m_sigma(i) = 0.05;
+    }
+    b.biotsavart();
+    EXPECT_NEAR(b.totalcirc(), 1.0, 1e-10);
+    for (int i = 0; i < N; ++i) EXPECT(std::isfinite(b.m_u(i)) && std::isfinite(b.m_w(i)));
+}
+
+TEST(KernelConsistencyWithVortexBlobs)
+{
+    VortexBlobs b(3);
+    const double xs[3] = {0.0, 1.0, 0.5}, zs[3] = {0.0, 0.0, 1.0}, gs[3] = {1.0, -0.5, 0.3};
+    for (unsigned i = 0; i < 3; ++i) {
+        b.m_ID[i] = i; b.m_x(i) = xs[i]; b.m_z(i) = zs[i]; b.m_circ(i) = gs[i]; b.m_sigma(i) = 0.1;
+    }
+    b.biotsavart();
+    auto k = dvm::VelocityKernel::create("direct");
+    std::vector<dvm::VortexParticle> p(3);
+    for (int i = 0; i < 3; ++i) p[i] = {b.m_x(i), b.m_z(i), b.m_circ(i), b.m_sigma(i)};
+    std::vector<dvm::VelocityResult> v;
+    k->computeSelfInduced(p, v);
+    for (int i = 0; i < 3; ++i) EXPECT(b.m_u(i) == v[i].u && b.m_w(i) == v[i].w);
+}
+
+// ---------------------------------------------------------------- VortexBlobs unit
+// tests/unit/test_VortexBlobs.cpp:15-141
+
+TEST(VortexBlobsConstruction)
+{
+    VortexBlobs a;
+    EXPECT(a.size() == 0u);
+    VortexBlobs b(10);
+    EXPECT(b.size() == 10u);
+    EXPECT(b.m_x.size() == 10u && b.m_sigma.size() == 10u && b.m_ID.size() == 10u);
+}
+
+TEST(AppendVorticesIncreasesSize)
+{
+    VortexBlobs a(3), b(2);
+    for (unsigned i = 0; i < 2; ++i) {
+        b.m_ID[i] = i; b.m_x(i) = i * 1.0; b.m_z(i) = i * 2.0; b.m_circ(i) = 0.1; b.m_sigma(i) = 0.05;
+    }
+    a.append_vortices(b);
+    EXPECT(a.size() == 5u);
+    EXPECT(a.m_ID[3] == 3u && a.m_ID[4] == 4u);
+}
+
+TEST(BiotSavartTwoVorticesOppositeCirculation)
+{
+    VortexBlobs b = pair(1.0, 1.0, -1.0, 0.1);
+    b.biotsavart();
+    const double ew = 1.0 / (2.0 * math::pi);
+    EXPECT_NEAR(b.m_w(0), ew, 0.1 * ew);
+    EXPECT_NEAR(b.m_w(1), ew, 0.1 * ew);
+}
+
+TEST(BiotSavartEmptyBlobsDoesNotCrash)
+{
+    VortexBlobs b;
+    b.biotsavart();
+    EXPECT(b.size() == 0u);
+}
+
+TEST(TotalCirculation)
+{
+    VortexBlobs b(3);
+    b.m_circ(0) = 1.0; b.m_circ(1) = 2.0; b.m_circ(2) = -0.5;
+    EXPECT(b.totalcirc() == 2.5);
+    VortexBlobs e;
+    EXPECT(e.totalcirc() == 0.0);
+}
+
+TEST(CopyAndMove)
+{
+    VortexBlobs a(3);
+    a.m_x(0) = 1.0; a.m_x(1) = 2.0; a.m_x(2) = 3.0;
+    VortexBlobs c(a);
+    EXPECT(c.size() == 3u && c.m_x(0) == 1.0);
+    a.m_x(0) = 100.0;
+    EXPECT(c.m_x(0) == 1.0);
+    VortexBlobs m(std::move(c));
+    EXPECT(m.size() == 3u && m.m_x(0) == 1.0);
+}
+
+TEST(KernelType)
+{
+    VortexBlobs b;
+    b.setKernelType("direct");
+    EXPECT_THROW(b.setKernelType("invalid"), dvm::AlgorithmException);
+}
+
+TEST(SizeMismatchThrows)
+{
+    VortexBlobs b(3);
+    b.m_x.set_size(2);
+    EXPECT_THROW((void)b.size(), dvm::SizeMismatchException);
+}
+
+// ---------------------------------------------------------------- CylinderFlow regression
+// tests/regression/test_CylinderFlow.cpp:26-167
+
+TEST(DiffusionRandomWalkStatistics)
+{
+    const int N = 1000;
+    VortexBlobs b(N);
+    Random rng;
+    rng.seed(42);
+    for (int i = 0; i < N; ++i) { b.m_ID[i] = i; b.m_circ(i) = 1.0; b.m_sigma(i) = 0.1; }
+    b.diffusion_random_walk(rng, 0.001, 0.01);
+    double mx = 0, mz = 0;
+    for (int i = 0; i < N; ++i) { mx += b.m_x(i); mz += b.m_z(i); }
+    mx /= N; mz /= N;
+    EXPECT_NEAR(mx, 0.0, 0.1);
+    EXPECT_NEAR(mz, 0.0, 0.1);
+    double vx = 0, vz = 0;
+    for (int i = 0; i < N; ++i) { vx += (b.m_x(i) - mx) * (b.m_x(i) - mx); vz += (b.m_z(i) - mz) * (b.m_z(i) - mz); }
+    const double sx = std::sqrt(vx / (N - 1)), sz = std::sqrt(vz / (N - 1));
+    const double spread = std::sqrt(4.0 * 0.001 * 0.01);
+    EXPECT(sx > 0.5 * spread && sx < 2.0 * spread);
+    EXPECT(sz > 0.5 * spread && sz < 2.0 * spread);
+}
+
+TEST(CirculationConservation)
+{
+    const int N = 50;
+    VortexBlobs b(N);
+    Random rng;
+    rng.seed(123);
+    double total = 0.0;
+    for (int i = 0; i < N; ++i) {
+        b.m_ID[i] = i; b.m_x(i) = rng.rand() * 10.0 - 5.0; b.m_z(i) = rng.rand() * 10.0 - 5.0;
+        b.m_circ(i) = rng.rand() * 2.0 - 1.0; b.m_sigma(i) = 0.1; total += b.m_circ(i);
+    }
+    const double before = b.totalcirc();
+    b.biotsavart();
+    EXPECT(b.totalcirc() == before);
+    EXPECT_NEAR(before, total, 1e-12);
+}
+
+TEST(VortexAppendPreservesData)
+{
+    VortexBlobs a(3), b(2);
+    for (int i = 0; i < 3; ++i) { a.m_ID[i] = i; a.m_x(i) = i * 1.0; a.m_z(i) = i * 2.0; a.m_circ(i) = i * 0.1; a.m_sigma(i) = 0.1; }
+    for (int i = 0; i < 2; ++i) { b.m_ID[i] = i; b.m_x(i) = 10.0 + i; b.m_z(i) = 20.0 + i; b.m_circ(i) = 1.0 + i * 0.1; b.m_sigma(i) = 0.2; }
+    const double before = a.totalcirc() + b.totalcirc();
+    a.append_vortices(b);
+    EXPECT(a.size() == 5u);
+    EXPECT_NEAR(a.totalcirc(), before, 1e-15);
+    EXPECT(a.m_x(0) == 0.0 && a.m_x(1) == 1.0 && a.m_x(2) == 2.0 && a.m_x(3) == 10.0 && a.m_x(4) == 11.0);
+}
+
+TEST(ReproducibilityWithSeed)
+{
+    auto run = [](int seed) {
+        VortexBlobs b(10);
+        Random rng;
+        rng.seed(seed);
+        for (int i = 0; i < 10; ++i) {
+            b.m_ID[i] = i; b.m_x(i) = rng.rand(); b.m_z(i) = rng.rand();
+            b.m_circ(i) = rng.rand() - 0.5; b.m_sigma(i) = 0.1;
+        }
+        b.biotsavart();
+        b.diffusion_random_walk(rng, 0.001, 0.01);
+        return b.m_x(0);
+    };
+    EXPECT(run(42) == run(42));
+    EXPECT(run(42) != run(123));
+}
+
+// ---------------------------------------------------------------- Random
+// tests/unit/test_Random.cpp:10-78
+
+TEST(RandomRangeAndSeeds)
+{
+    Random r;
+    for (int i = 0; i < 1000; ++i) { const double v = r.rand(); EXPECT(v >= 0.0 && v < 1.0); }
+    Random a, b;
+    a.seed(42); b.seed(42);
+    for (int i = 0; i < 100; ++i) EXPECT(a.rand() == b.rand());
+    Random c, d;
+    c.seed(42); d.seed(123);
+    bool diff = false;
+    for (int i = 0; i < 100; ++i) diff = diff || c.rand() != d.rand();
+    EXPECT(diff);
+    Random e;
+    e.seed(42);
+    std::set<double> vals;
+    for (int i = 0; i < 100; ++i) vals.insert(e.rand());
+    EXPECT(vals.size() > 90u);
+    Random z;
+    z.seed(0);
+    (void)z.rand();
+}
+
+// ---------------------------------------------------------------- BackgroundMesh
+// tests/unit/test_BackgroundMesh.cpp:14-18,105-111,167-204
+
+TEST(DisabledMesh)
+{
+    BackgroundMesh m;
+    EXPECT(!m.is_enabled());
+    for (unsigned s : {0u, 1u, 10u, 100u}) EXPECT(!m.should_remesh(s));
+    EXPECT(m.nx() == 0u && m.nz() == 0u);
+    VortexBlobs empty;
+    EXPECT(m.remesh(empty).size() == 0u);
+    VortexBlobs p(2);
+    p.m_x(0) = 1.0; p.m_x(1) = 2.0; p.m_z(0) = 1.0; p.m_z(1) = 2.0;
+    p.m_circ(0) = 0.5; p.m_circ(1) = 0.3; p.m_sigma(0) = 0.1; p.m_sigma(1) = 0.1;
+    VortexBlobs r = m.remesh(p);
+    EXPECT(r.size() == p.size() && r.m_x(0) == p.m_x(0) && r.totalcirc() == p.totalcirc());
+}
+
+TEST(EnabledMeshConservesCirculation)
+{
+    dvm_remesh_params rp{5, 0, -1.0, 2.0, -1.0, 2.0, 0.25, 0.25, 0.05, 0.0};
+    BackgroundMesh m(rp);
+    EXPECT(m.is_enabled() && m.nx() == 13u && m.nz() == 13u);
+    EXPECT(m.should_remesh(10) && !m.should_remesh(11));
+    VortexBlobs p(4);
+    const double xs[4] = {0.25, 0.75, 0.25, 0.75}, zs[4] = {0.25, 0.25, 0.75, 0.75}, gs[4] = {0.5, 0.3, -0.2, 0.4};
+    for (unsigned i = 0; i < 4; ++i) { p.m_ID[i] = i; p.m_x(i) = xs[i]; p.m_z(i) = zs[i]; p.m_circ(i) = gs[i]; p.m_sigma(i) = 0.1; }
+    EXPECT(p.totalcirc() == 1.0);
+    VortexBlobs r = m.remesh(p);
+    EXPECT_NEAR(r.totalcirc(), 1.0, 1e-14);
+}
+
+// ---------------------------------------------------------------- VortexSheet + DVMBase
+
+static dvm_sheet_params sheet_params()
+{
+    // tests/fixtures/test_configs/cylinder_test.xml
+    return dvm_sheet_params{1.0, 0.001, 0.005, 3.0, 0.0, 0.675, 3};
+}
+
+TEST(VortexSheetMethods)
+{
+    VortexSheet sh(g_golden + "/simple_body.txt", sheet_params());
+    EXPECT(sh.size() == 20u);
+    VortexBlobs none;
+    sh.solvevortexsheet(none);
+    Random rng;
+    rng.seed(42);
+    VortexBlobs nascent = sh.release_nascent_vortices_rw(rng);
+    EXPECT(nascent.size() >= 20u);
+    double released = nascent.totalcirc();
+    EXPECT(std::isfinite(released));
+    VortexBlobs v;
+    v.append_vortices(nascent);
+    v.biotsavart();
+    sh.solvevortexsheet(v);
+    sh.vortexsheetbc(v);
+    for (unsigned i = 0; i < v.size(); ++i) EXPECT(std::isfinite(v.m_uvs(i)) && std::isfinite(v.m_wvs(i)));
+    // push a blob inside, reflect it out; delete another inside one
+    v.m_x(0) = 0.1; v.m_z(0) = 0.0;
+    sh.reflect(v);
+    EXPECT(std::hypot(v.m_x(0), v.m_z(0)) > 0.45);
+    v.m_x(1) = 0.0; v.m_z(1) = 0.1;
+    const unsigned n0 = v.size();
+    sh.delete_inside(v);
+    EXPECT(v.size() == n0 - 1u && v.m_ID[0] == 0u && v.m_ID[v.size() - 1] == v.size() - 1);
+    sh.compute_loads(3.0);
+    auto [fx, fz] = sh.get_forces();
+    EXPECT(std::isfinite(fx) && std::isfinite(fz));
+}
+
+TEST(DVMBaseRunsCylinder)
+{
+    dvm_sim_params p{};
+    p.sheet = sheet_params();
+    p.scheme = DVM_EULER;
+    p.crossing = DVM_REFLECT;
+    p.seed = 42;
+    p.prec = DVM_F64;
+    p.rng_mode = DVM_RNG_REFERENCE;
+    DVMBase dvm(g_golden + "/simple_body.txt", p, 10);
+    dvm.solve();
+    EXPECT(dvm.current_step() == 10u);
+    EXPECT(dvm.num_blobs() >= 200u);
+    VortexBlobs b = dvm.blobs();
+    EXPECT(b.size() == dvm.num_blobs());
+    auto [fx, fz, cd, cl] = dvm.loads();
+    EXPECT(std::isfinite(cd) && std::isfinite(cl) && std::isfinite(fx) && std::isfinite(fz));
+    EXPECT_NEAR(dvm.time(), 0.05, 1e-15);
+}
+
+int main(int argc, char** argv)
+{
+    g_golden = argc > 1 ? argv[1] : "tests/golden";
+    int failed_cases = 0;
+    for (auto& c : cases()) {
+        const int before = g_fail;
+        try {
+            c.fn();
+        } catch (const std::exception& e) {
+            std::printf("  FAIL %s: unexpected exception: %s\n", c.name, e.what());
+            ++g_fail;
+        }
+        const bool ok = g_fail == before;
+        failed_cases += ok ? 0 : 1;
+        std::printf("[%s] %s\n", ok ? "  OK  " : " FAIL ", c.name);
+        std::fflush(stdout);
+    }
+    std::printf("%zu cases, %d failed\n", cases().size(), failed_cases);
+    return failed_cases == 0 ? 0 : 1;
+}

@@ -77,6 +77,13 @@ class SimParams(ctypes.Structure):
                 ("prec", _int), ("rng_mode", _int), ("remesh", RemeshParams)]
 
 
+class BatchConfig(ctypes.Structure):
+    """dvm_batch_config."""
+    _fields_ = [("slots", _int), ("slot_capacity", _i64), ("max_panels", _int), ("scheme", _int),
+                ("crossing", _int), ("prec", _int), ("use_graph", _int),
+                ("remesh", RemeshParams)]
+
+
 _dp = ctypes.POINTER(_dbl)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 
@@ -118,6 +125,16 @@ SIGNATURES.update({
     "dvm_sim_set_gamma": (_st, [_vp, _vp]),
     "dvm_sim_loads": (_st, [_vp, _vp]),
     "dvm_sim_blobs": (_st, [_vp] + [ctypes.POINTER(_vp)] * 4),
+    "dvm_batch_footprint": (_st, [ctypes.POINTER(BatchConfig), ctypes.POINTER(_sz)]),
+    "dvm_batch_create": (_st, [_vp, ctypes.POINTER(BatchConfig), ctypes.POINTER(_vp)]),
+    "dvm_batch_destroy": (_st, [_vp]),
+    "dvm_batch_admit": (_st, [_vp, _int, _vp, _vp, ctypes.POINTER(SheetParams), _int, _int,
+                              ctypes.POINTER(_int)]),
+    "dvm_batch_step": (_st, [_vp, _int]),
+    "dvm_batch_poll": (_st, [_vp, _int, _vp, _vp, ctypes.POINTER(_int)]),
+    "dvm_batch_slot_info": (_st, [_vp, _int, ctypes.POINTER(_i64), ctypes.POINTER(_int),
+                                  ctypes.POINTER(_int), ctypes.POINTER(_int), _vp]),
+    "dvm_batch_get_state": (_st, [_vp, _int] + [_vp] * 6),
 })
 
 _lib = None

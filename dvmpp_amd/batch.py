@@ -1,0 +1,91 @@
+"""Python handle for the batched ensemble (dvm_batch_* in include/dvm_hip.h)."""
+import ctypes
+
+import numpy as np
+
+from ._lib import BatchConfig, check, load
+from .sim import DELETE, EULER, REFLECT, remesh_params
+
+
+def batch_config(slots=8, slot_capacity=1 << 16, max_panels=512, scheme=EULER, crossing=REFLECT,
+                 prec=1, use_graph=True, remesh=None):
+    return BatchConfig(slots, slot_capacity, max_panels, scheme, crossing, prec,
+                       1 if use_graph else 0, remesh if remesh is not None else remesh_params())
+
+
+def footprint(cfg):
+    n = ctypes.c_size_t()
+    st = load().dvm_batch_footprint(ctypes.byref(cfg), ctypes.byref(n))
+    if st != 0:
+        raise RuntimeError("dvm_batch_footprint failed")
+    return n.value
+
+
+class Batch:
+    """dvm_batch_t: simulations stepped together on one device."""
+
+    def __init__(self, ctx, cfg):
+        self.ctx = ctx
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        check(ctx.h, load().dvm_batch_create(ctx.h, ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        ctx.adopt(self)
+
+    def close(self):
+        if getattr(self, "h", None):
+            if getattr(self.ctx, "h", None):
+                load().dvm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def admit(self, px, pz, sheet_params, seed, steps):
+        px = np.ascontiguousarray(px, dtype=np.float64)
+        pz = np.ascontiguousarray(pz, dtype=np.float64)
+        slot = ctypes.c_int()
+        check(self.ctx.h, load().dvm_batch_admit(self.h, px.size, ctypes.c_void_p(px.ctypes.data),
+                                                 ctypes.c_void_p(pz.ctypes.data),
+                                                 ctypes.byref(sheet_params), seed, steps,
+                                                 ctypes.byref(slot)))
+        return slot.value
+
+    def step(self, steps=1):
+        check(self.ctx.h, load().dvm_batch_step(self.h, steps))
+
+    def poll(self, max_n=1024):
+        slots = np.empty(max_n, dtype=np.int32)
+        status = np.empty(max_n, dtype=np.int32)
+        cnt = ctypes.c_int()
+        check(self.ctx.h, load().dvm_batch_poll(self.h, max_n, ctypes.c_void_p(slots.ctypes.data),
+                                                ctypes.c_void_p(status.ctypes.data),
+                                                ctypes.byref(cnt)))
+        return list(zip(slots[:cnt.value].tolist(), status[:cnt.value].tolist()))
+
+    def info(self, slot):
+        n = ctypes.c_int64()
+        step = ctypes.c_int()
+        active = ctypes.c_int()
+        status = ctypes.c_int()
+        loads = np.empty(4)
+        check(self.ctx.h, load().dvm_batch_slot_info(self.h, slot, ctypes.byref(n),
+                                                     ctypes.byref(step), ctypes.byref(active),
+                                                     ctypes.byref(status),
+                                                     ctypes.c_void_p(loads.ctypes.data)))
+        return dict(n=n.value, step=step.value, active=active.value, status=status.value,
+                    loads=tuple(loads))
+
+    def state(self, slot):
+        n = self.info(slot)["n"]
+        arrs = {k: np.empty(n) for k in ("x", "z", "circ", "sigma", "u", "w")}
+        check(self.ctx.h, load().dvm_batch_get_state(
+            self.h, slot, *[ctypes.c_void_p(arrs[k].ctypes.data)
+                            for k in ("x", "z", "circ", "sigma", "u", "w")]))
+        return arrs
+
+
+__all__ = ["Batch", "batch_config", "footprint", "DELETE", "REFLECT"]

@@ -1,0 +1,1116 @@
+// Batched ensemble: many independent simulations (bodies, viscosities, step
+// counts, blob counts) advanced together.
+//
+// Storage: slot s owns blob range [s*cap, (s+1)*cap) of one SoA and panel range
+// [s*mp, (s+1)*mp) of the sheet arrays; live counts, step counters and status
+// words live on the device, so a whole time step is a fixed sequence of
+// launches that reads its sizes from device memory — it is captured once into
+// a hipGraph and replayed.  Every launch covers all slots: per-blob kernels
+// stride over the concatenated live ranges (slot found by binary search of a
+// device prefix), per-slot kernels use one workgroup per slot, and the
+// pair sum is one persistent launch whose workgroups pull (slot, target tile)
+// items from a device work counter, largest slots first.  Each item's sum
+// order is fixed, so results do not depend on which workgroup ran it or on
+// what else is in the batch.
+#include "dvm_device.hpp"
+#include "dvm_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace dvmi {
+
+constexpr int kBT = 256;       // targets per item / sources per tile
+constexpr int kColChunks = 64;  // source chunks of the collocation sum per slot
+
+struct SlotParam {
+    double rho, nu, dt, ux, uz, ur, aref, four_nu_dt;
+    int nl, max_npv, steps_target, pad0;
+    uint32_t seed, pad1;
+    double pad2;
+};
+
+struct SlotState {
+    int64_t n;       // live blobs
+    int step;        // completed steps
+    int active;      // 1 while stepping
+    int status;      // dvm_status of the failure that stopped the slot, else 0
+    int nrv;         // nascent blobs of the current step
+    double loads[4]; // fx fz cd cl of the last step
+};
+
+// Work lists rebuilt by bt_prefix_kernel at every velocity evaluation.
+struct WorkList {
+    int total_tiles;
+    int work_counter;
+    int64_t total_blobs;
+};
+
+struct BatchDev {
+    int S = 0;
+    int64_t cap = 0;
+    int mp = 0;
+    int max_tiles = 0;
+    SlotParam* prm = nullptr;
+    SlotState* st = nullptr;
+    WorkList* wl = nullptr;
+    int* tile_pre = nullptr;    // S+1, slot order
+    int* order = nullptr;       // S, slots sorted by tile count (desc)
+    int* otile_pre = nullptr;   // S+1, in `order`
+    int64_t* blob_pre = nullptr;  // S+1
+    BlobBuf b, alt;
+    double *q1 = nullptr, *q2 = nullptr;
+    float4* s32 = nullptr;
+    float2* near32 = nullptr;
+    TileInfo* tiles = nullptr;  // S * max_tiles (compact, slot order)
+    int* flags = nullptr;
+    // sheet
+    double *bx = nullptr, *bz = nullptr;  // S*(mp+1)
+    double *enx = nullptr, *enz = nullptr, *etx = nullptr, *etz = nullptr, *ds = nullptr;
+    PanelConst* panel = nullptr;
+    double* P = nullptr;        // S * mp*(mp+1)
+    double* gamma = nullptr;
+    double* col = nullptr;      // S * kColChunks * (2*mp + 1)
+    int* cnt = nullptr;
+    int64_t* off = nullptr;     // S*(mp+1)
+    double* cnew = nullptr;
+    double* lwork = nullptr;    // S * 3*mp
+};
+
+// ----------------------------------------------------------------- work lists
+
+// One workgroup: per-slot tile and blob counts, their prefixes, the slot order
+// by decreasing tile count, and the work-counter reset.
+__global__ __launch_bounds__(1024) void bt_prefix_kernel(int S, const SlotState* __restrict__ st,
+                                                         WorkList* wl, int* tile_pre, int* order,
+                                                         int* otile_pre, int64_t* blob_pre)
+{
+    __shared__ int tiles_s[1024];
+    const int t = threadIdx.x;
+    for (int s = t; s < S; s += 1024) {
+        const int64_t n = st[s].active ? st[s].n : 0;
+        tiles_s[s] = (int)((n + kBT - 1) / kBT);
+    }
+    __syncthreads();
+    // rank slots by tile count, ties by index
+    for (int s = t; s < S; s += 1024) {
+        int r = 0;
+        for (int o = 0; o < S; ++o)
+            r += (tiles_s[o] > tiles_s[s]) || (tiles_s[o] == tiles_s[s] && o < s);
+        order[r] = s;
+    }
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0, oacc = 0;
+        int64_t bacc = 0;
+        for (int s = 0; s < S; ++s) {
+            tile_pre[s] = acc;
+            acc += tiles_s[s];
+            blob_pre[s] = bacc;
+            bacc += st[s].active ? st[s].n : 0;
+            otile_pre[s] = oacc;
+            oacc += tiles_s[order[s]];
+        }
+        tile_pre[S] = acc;
+        otile_pre[S] = oacc;
+        blob_pre[S] = bacc;
+        wl->total_tiles = acc;
+        wl->work_counter = 0;
+        wl->total_blobs = bacc;
+    }
+}
+
+template <class T>
+__device__ __forceinline__ int find_slot(const T* __restrict__ pre, int S, T g)
+{
+    int lo = 0, hi = S;  // pre[lo] <= g < pre[lo+1]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Grid-stride loop over the live blobs of all slots: calls f(slot, blob index).
+template <class F>
+__device__ __forceinline__ void for_each_blob(const WorkList* wl, const int64_t* blob_pre, int S,
+                                              int64_t cap, F&& f)
+{
+    const int64_t total = wl->total_blobs;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int s = find_slot(blob_pre, S, g);
+        f(s, (int64_t)s * cap + (g - blob_pre[s]));
+    }
+}
+
+// ----------------------------------------------------------------- pair sums
+
+__global__ __launch_bounds__(kBT) void bt_tiles_kernel(int S, int64_t cap, const WorkList* wl,
+                                                       const int* __restrict__ tile_pre,
+                                                       const SlotState* __restrict__ st,
+                                                       BlobBuf b, float4* __restrict__ s32,
+                                                       float2* __restrict__ near32,
+                                                       TileInfo* __restrict__ tiles)
+{
+    __shared__ double red[5][kBT];
+    const int total = wl->total_tiles;
+    for (int g = blockIdx.x; g < total; g += gridDim.x) {
+        const int s = find_slot(tile_pre, S, g);
+        const int64_t n = st[s].n;
+        const int64_t k = (int64_t)(g - tile_pre[s]) * kBT + threadIdx.x;
+        const bool valid = k < n;
+        const int64_t o = (int64_t)s * cap + k;
+        const double xi = valid ? b.x[o] : 0.0, zi = valid ? b.z[o] : 0.0;
+        const double si = valid ? b.s[o] : 0.0;
+        red[0][threadIdx.x] = valid ? xi : INFINITY;
+        red[1][threadIdx.x] = valid ? xi : -INFINITY;
+        red[2][threadIdx.x] = valid ? zi : INFINITY;
+        red[3][threadIdx.x] = valid ? zi : -INFINITY;
+        red[4][threadIdx.x] = si * si;
+        __syncthreads();
+        for (int w = kBT / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) {
+                const int p = threadIdx.x + w;
+                red[0][threadIdx.x] = fmin(red[0][threadIdx.x], red[0][p]);
+                red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][p]);
+                red[2][threadIdx.x] = fmin(red[2][threadIdx.x], red[2][p]);
+                red[3][threadIdx.x] = fmax(red[3][threadIdx.x], red[3][p]);
+                red[4][threadIdx.x] = fmax(red[4][threadIdx.x], red[4][p]);
+            }
+            __syncthreads();
+        }
+        const double cx = 0.5 * (red[0][0] + red[1][0]);
+        const double cz = 0.5 * (red[2][0] + red[3][0]);
+        if (threadIdx.x == 0) {
+            TileInfo ti;
+            ti.cx = cx;
+            ti.cz = cz;
+            ti.hx = (float)(0.5 * (red[1][0] - red[0][0])) * 1.000001f + 1e-30f;
+            ti.hz = (float)(0.5 * (red[3][0] - red[2][0])) * 1.000001f + 1e-30f;
+            ti.thr_max = (float)(10.0 * red[4][0]);
+            ti.pad = 0.0f;
+            tiles[g] = ti;
+        }
+        if (valid) {
+            s32[o] = make_float4((float)(xi - cx), (float)(zi - cz), (float)b.c[o], 0.0f);
+            const double s2 = si * si;
+            near32[o] = make_float2((float)(10.0 * s2), (float)(1.0 / s2));
+        }
+        __syncthreads();
+    }
+}
+
+// Persistent pair sum over (slot, target tile) items, largest slots first.
+// F32C: the bs_f32c_kernel arithmetic (per-tile FP64 recentring, far / near
+// loops, Kahan across tiles).  F64: the reference pair in source order.
+template <bool F32C>
+__global__ __launch_bounds__(kBT) void bt_bs_kernel(int S, int64_t cap, WorkList* wl,
+                                                    const int* __restrict__ tile_pre,
+                                                    const int* __restrict__ order,
+                                                    const int* __restrict__ otile_pre,
+                                                    const SlotState* __restrict__ st, BlobBuf b,
+                                                    const float4* __restrict__ s32,
+                                                    const float2* __restrict__ near32,
+                                                    const TileInfo* __restrict__ tiles)
+{
+    __shared__ int item_sh;
+    const int total = wl->total_tiles;
+    for (;;) {
+        if (threadIdx.x == 0) item_sh = atomicAdd(&wl->work_counter, 1);
+        __syncthreads();
+        const int item = item_sh;
+        __syncthreads();
+        if (item >= total) break;
+        const int p = find_slot(otile_pre, S, item);
+        const int s = order[p];
+        const int tt = item - otile_pre[p];
+        const int64_t n = st[s].n;
+        const int64_t base = (int64_t)s * cap;
+        const int64_t i = (int64_t)tt * kBT + threadIdx.x;
+        const int64_t it = i < n ? i : n - 1;
+        const double xi = b.x[base + it];
+        const double zi = b.z[base + it];
+        const int nt = (int)((n + kBT - 1) / kBT);
+        if (F32C) {
+            const TileInfo tb = tiles[tile_pre[s] + tt];
+            float su = 0.0f, cu = 0.0f, sw = 0.0f, cw = 0.0f;
+            for (int sti = 0; sti < nt; ++sti) {
+                const TileInfo ts = tiles[tile_pre[s] + sti];
+                const double gx = fmax(fabs(ts.cx - tb.cx) - (double)(tb.hx + ts.hx), 0.0);
+                const double gz = fmax(fabs(ts.cz - tb.cz) - (double)(tb.hz + ts.hz), 0.0);
+                const bool near = gx * gx + gz * gz <= 1.0001 * (double)ts.thr_max + 1e-300;
+                const float txd = (float)(xi - ts.cx);
+                const float tzd = (float)(zi - ts.cz);
+                const int64_t jb = base + (int64_t)sti * kBT;
+                const int cnt = n - (int64_t)sti * kBT < kBT ? (int)(n - (int64_t)sti * kBT) : kBT;
+                const float4* __restrict__ sp = s32 + jb;
+                float au = 0.0f, aw = 0.0f;
+                if (!near) {
+                    if (cnt == kBT) {
+#pragma unroll 16
+                        for (int k = 0; k < kBT; ++k) {
+                            const float4 sj = sp[k];
+                            const float dx = txd - sj.x;
+                            const float dz = tzd - sj.y;
+                            const float r2 = __builtin_fmaf(dx, dx, dz * dz);
+                            const float kk = __builtin_amdgcn_rcpf(r2) * sj.z;
+                            au = __builtin_fmaf(kk, dz, au);
+                            aw = __builtin_fmaf(kk, dx, aw);
+                        }
+                    } else {
+                        for (int k = 0; k < cnt; ++k) {
+                            const float4 sj = sp[k];
+                            const float dx = txd - sj.x;
+                            const float dz = tzd - sj.y;
+                            const float r2 = __builtin_fmaf(dx, dx, dz * dz);
+                            const float kk = __builtin_amdgcn_rcpf(r2) * sj.z;
+                            au = __builtin_fmaf(kk, dz, au);
+                            aw = __builtin_fmaf(kk, dx, aw);
+                        }
+                    }
+                } else {
+                    const float2* __restrict__ np = near32 + jb;
+                    for (int k = 0; k < cnt; ++k) {
+                        const float4 sj = sp[k];
+                        const float2 nj = np[k];
+                        const float dx = (float)(xi - b.x[jb + k]);
+                        const float dz = (float)(zi - b.z[jb + k]);
+                        const float r2 = __builtin_fmaf(dx, dx, dz * dz);
+                        const float inv = __builtin_amdgcn_rcpf(r2);
+                        float K = inv;
+                        if (r2 < nj.x) K = -expm1f(-r2 * nj.y) * inv;
+                        const bool take = ((int64_t)sti * kBT + k) != i;
+                        const float kk = take ? K * sj.z : 0.0f;
+                        au = __builtin_fmaf(kk, dz, au);
+                        aw = __builtin_fmaf(kk, dx, aw);
+                    }
+                }
+                kahan_add(su, cu, au);
+                kahan_add(sw, cw, aw);
+            }
+            if (i < n) {
+                b.u[base + i] = -((double)su - (double)cu) * kRpi2;
+                b.w[base + i] = ((double)sw - (double)cw) * kRpi2;
+            }
+        } else {
+            double ui = 0.0, wi = 0.0;
+            for (int64_t j = 0; j < n; ++j) {
+                if (j == i) continue;
+                bs_pair_f64(xi, zi, b.x[base + j], b.z[base + j], b.c[base + j], b.s[base + j], ui, wi);
+            }
+            if (i < n) {
+                b.u[base + i] = ui * kRpi2;
+                b.w[base + i] = wi * kRpi2;
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------- sheet
+
+// Collocation sums (unguarded) per (slot, source chunk): u, w partials for the
+// slot's Nl panels and the chunk's circulation sum.
+__global__ __launch_bounds__(256) void bt_colloc_kernel(int64_t cap, int mp,
+                                                        const SlotParam* __restrict__ prm,
+                                                        const SlotState* __restrict__ st, BlobBuf b,
+                                                        const PanelConst* __restrict__ panel,
+                                                        double* __restrict__ col)
+{
+    const int s = blockIdx.y;
+    const int c = blockIdx.x;
+    if (!st[s].active) return;
+    const int nl = prm[s].nl;
+    const int64_t n = st[s].n;
+    const int64_t chunk = (n + kColChunks - 1) / kColChunks;
+    const int64_t j0 = (int64_t)c * chunk;
+    const int64_t j1 = j0 + chunk < n ? j0 + chunk : n;
+    const int64_t base = (int64_t)s * cap;
+    double* out = col + ((int64_t)s * kColChunks + c) * (2 * mp + 1);
+    for (int pj = threadIdx.x; pj < nl; pj += 256) {
+        const PanelConst q = panel[(int64_t)s * mp + pj];
+        double ui = 0.0, wi = 0.0;
+        for (int64_t j = j0; j < j1; ++j)
+            bs_pair_f64(q.xc, q.zc, b.x[base + j], b.z[base + j], b.c[base + j], b.s[base + j], ui, wi);
+        out[pj] = ui;
+        out[mp + pj] = wi;
+    }
+    __shared__ double red[256];
+    double cs = 0.0;
+    for (int64_t j = j0 + threadIdx.x; j < j1; j += 256) cs += b.c[base + j];
+    red[threadIdx.x] = cs;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[2 * mp] = red[0];
+}
+
+// One workgroup per slot: b = (U + u).n, b_Nl = -sum circ; gamma = P b.
+__global__ __launch_bounds__(256) void bt_gamma_kernel(int mp, const SlotParam* __restrict__ prm,
+                                                       const SlotState* __restrict__ st,
+                                                       const double* __restrict__ enx,
+                                                       const double* __restrict__ enz,
+                                                       const double* __restrict__ P,
+                                                       const double* __restrict__ col,
+                                                       double* __restrict__ gamma)
+{
+    const int s = blockIdx.x;
+    if (!st[s].active) return;
+    extern __shared__ double sb[];
+    const int nl = prm[s].nl;
+    const bool have = st[s].n > 0;
+    const double* cs = col + (int64_t)s * kColChunks * (2 * mp + 1);
+    for (int i = threadIdx.x; i <= nl; i += 256) {
+        double v;
+        if (i == nl) {
+            double tot = 0.0;
+            for (int c = 0; c < kColChunks; ++c) tot += cs[(int64_t)c * (2 * mp + 1) + 2 * mp];
+            v = have ? -tot : -0.0;
+        } else {
+            double u = 0.0, w = 0.0;
+            if (have)
+                for (int c = 0; c < kColChunks; ++c) {
+                    u += cs[(int64_t)c * (2 * mp + 1) + i];
+                    w += cs[(int64_t)c * (2 * mp + 1) + mp + i];
+                }
+            const double ex = enx[(int64_t)s * mp + i], ez = enz[(int64_t)s * mp + i];
+            v = have ? (prm[s].ux + u * kRpi2) * ex + (prm[s].uz + w * kRpi2) * ez
+                     : prm[s].ux * ex + prm[s].uz * ez;
+        }
+        sb[i] = v;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ncol = nl + 1;
+    const double* Ps = P + (int64_t)s * mp * (mp + 1);
+    for (int i = wave; i < nl; i += 4) {
+        double acc = 0.0;
+        for (int c = lane; c < ncol; c += 64) acc += Ps[(int64_t)i * ncol + c] * sb[c];
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+        if (lane == 0) gamma[(int64_t)s * mp + i] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void bt_sheet_kernel(int S, int64_t cap, int mp,
+                                                       const WorkList* wl,
+                                                       const int64_t* __restrict__ blob_pre,
+                                                       const SlotParam* __restrict__ prm, BlobBuf b,
+                                                       const PanelConst* __restrict__ panel,
+                                                       const double* __restrict__ gamma)
+{
+    for_each_blob(wl, blob_pre, S, cap, [&](int s, int64_t o) {
+        const int nl = prm[s].nl;
+        const PanelConst* pc = panel + (int64_t)s * mp;
+        const double* g = gamma + (int64_t)s * mp;
+        const double xi = b.x[o], zi = b.z[o];
+        double su = 0.0, sw = 0.0;
+        for (int j = 0; j < nl; ++j) sheet_pair(xi, zi, pc[j], g[j], g[j + 1 == nl ? 0 : j + 1], su, sw);
+        b.uvs[o] = su * kRpi2;
+        b.wvs[o] = sw * kRpi2;
+    });
+}
+
+// ----------------------------------------------------------------- blob updates
+
+__global__ void bt_zero_q_kernel(int S, int64_t cap, const WorkList* wl,
+                                 const int64_t* __restrict__ blob_pre, double* q1, double* q2)
+{
+    for_each_blob(wl, blob_pre, S, cap, [&](int, int64_t o) { q1[o] = 0.0; q2[o] = 0.0; });
+}
+
+__global__ void bt_advect_kernel(int S, int64_t cap, const WorkList* wl,
+                                 const int64_t* __restrict__ blob_pre,
+                                 const SlotParam* __restrict__ prm, int rk, double a, double bb,
+                                 BlobBuf b, double* q1, double* q2)
+{
+    for_each_blob(wl, blob_pre, S, cap, [&](int s, int64_t o) {
+        const double dt = prm[s].dt, ux = prm[s].ux, uz = prm[s].uz;
+        if (!rk) {
+            b.x[o] += (b.u[o] + b.uvs[o] + ux) * dt;
+            b.z[o] += (b.w[o] + b.wvs[o] + uz) * dt;
+        } else {
+            const double n1 = a * q1[o] + (b.u[o] + b.uvs[o] + ux) * dt;
+            const double n2 = a * q2[o] + (b.w[o] + b.wvs[o] + uz) * dt;
+            q1[o] = n1;
+            q2[o] = n2;
+            b.x[o] += bb * n1;
+            b.z[o] += bb * n2;
+        }
+    });
+}
+
+// Box-Muller random walk on Philox draws keyed by (seed, step, blob index):
+// the same stream as the single-simulation device mode.
+__global__ void bt_diffuse_kernel(int S, int64_t cap, const WorkList* wl,
+                                  const int64_t* __restrict__ blob_pre,
+                                  const SlotParam* __restrict__ prm,
+                                  const SlotState* __restrict__ st, BlobBuf b)
+{
+    for_each_blob(wl, blob_pre, S, cap, [&](int s, int64_t o) {
+        const int64_t i = o - (int64_t)s * cap;
+        const uint32_t step = (uint32_t)(st[s].step + 1);
+        const uint4 r = philox4x32(make_uint4((uint32_t)i, (uint32_t)(i >> 32), step, 0u),
+                                   make_uint2(prm[s].seed, 0x5eed0001u));
+        const double rrw = sqrt(prm[s].four_nu_dt * log(1.0 / u01(r.x, r.y)));
+        const double th = 2.0 * 3.14159265358979323846 * u01(r.z, r.w);
+        b.x[o] += rrw * cos(th);
+        b.z[o] += rrw * sin(th);
+    });
+}
+
+// One workgroup per slot: release counts, offsets, circulation per nascent
+// blob (src/VortexSheet.cpp:282-303); stops the slot on NaN counts or overflow.
+__global__ __launch_bounds__(1024) void bt_release_count_kernel(
+    int64_t cap, int mp, const SlotParam* __restrict__ prm, SlotState* __restrict__ st,
+    const double* __restrict__ gamma, const double* __restrict__ ds, int* __restrict__ cnt,
+    int64_t* __restrict__ off, double* __restrict__ cnew)
+{
+    const int s = blockIdx.x;
+    if (!st[s].active) return;
+    __shared__ double red[1024];
+    __shared__ int64_t pre[1024];
+    __shared__ int bad_sh;
+    const int t = threadIdx.x;
+    const int nl = prm[s].nl;
+    const double* g = gamma + (int64_t)s * mp;
+    const double* d = ds + (int64_t)s * mp;
+    double mx = -INFINITY;
+    for (int i = t; i < nl; i += 1024) {
+        const double v = g[i] * d[i];
+        if (v > mx) mx = v;
+    }
+    red[t] = mx;
+    if (t == 0) bad_sh = 0;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (t < w && red[t + w] > red[t]) red[t] = red[t + w];
+        __syncthreads();
+    }
+    const double m = red[0];
+    const int per = (nl + 1023) / 1024;
+    const int i0 = t * per;
+    const int i1 = i0 + per < nl ? i0 + per : nl;
+    int64_t local = 0;
+    int bad = 0;
+    for (int i = i0; i < i1; ++i) {
+        const double pcirc = g[i] * d[i];
+        const double c = floor(fabs(pcirc / m * (double)prm[s].max_npv)) + 1.0;
+        int ci = 0;
+        if (!(c > 1.0e-15)) bad |= 1;
+        else if (!(c < 1.0e9)) bad |= 2;
+        else ci = (int)c;
+        cnt[(int64_t)s * mp + i] = ci;
+        cnew[(int64_t)s * mp + i] = pcirc / c;
+        local += ci;
+    }
+    if (bad) atomicOr(&bad_sh, bad);
+    pre[t] = local;
+    __syncthreads();
+    for (int w = 1; w < 1024; w <<= 1) {
+        const int64_t v = t >= w ? pre[t - w] : 0;
+        __syncthreads();
+        pre[t] += v;
+        __syncthreads();
+    }
+    int64_t run = pre[t] - local;
+    for (int i = i0; i < i1; ++i) {
+        off[(int64_t)s * (mp + 1) + i] = run;
+        run += cnt[(int64_t)s * mp + i];
+    }
+    if (t == 1023) {
+        const int64_t nrv = pre[1023];
+        off[(int64_t)s * (mp + 1) + nl] = nrv;
+        int status = 0;
+        if (bad_sh & 1) status = DVM_E_NAN;
+        else if (bad_sh & 2 || st[s].n + nrv > cap) status = DVM_E_SIZE;
+        st[s].nrv = status ? 0 : (int)nrv;
+        if (status) {
+            st[s].status = status;
+            st[s].active = 0;
+        }
+    }
+}
+
+// Nascent blobs of every slot: grid (chunks, S); blob k of slot s goes to
+// index n_s + k.
+__global__ void bt_release_emit_kernel(int64_t cap, int mp, const SlotParam* __restrict__ prm,
+                                       const SlotState* __restrict__ st,
+                                       const double* __restrict__ bx, const double* __restrict__ bz,
+                                       const double* __restrict__ ds, const double* __restrict__ etx,
+                                       const double* __restrict__ etz, const double* __restrict__ enx,
+                                       const double* __restrict__ enz, const int* __restrict__ cnt,
+                                       const int64_t* __restrict__ off,
+                                       const double* __restrict__ cnew,
+                                       const PanelConst* __restrict__ panel, BlobBuf b)
+{
+    const int s = blockIdx.y;
+    if (!st[s].active) return;
+    const int nrv = st[s].nrv;
+    const int nl = prm[s].nl;
+    const int64_t pb = (int64_t)s * mp, ob = (int64_t)s * (mp + 1), xb = (int64_t)s * (mp + 1);
+    const uint32_t step = (uint32_t)(st[s].step + 1);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nrv; k += gridDim.x * blockDim.x) {
+        int lo = 0, hi = nl;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (off[ob + mid] <= k) lo = mid; else hi = mid;
+        }
+        const int i = lo;
+        const uint4 r = philox4x32(make_uint4((uint32_t)k, 0u, step, 1u),
+                                   make_uint2(prm[s].seed, 0x5eed0002u));
+        const double rrw = fabs(sqrt(prm[s].four_nu_dt * log(1.0 / u01(r.x, r.y))));
+        const double np1 = (double)cnt[pb + i] + 1;
+        const double xm = bx[xb + i] + ds[pb + i] * etx[pb + i] / np1;
+        const double zm = bz[xb + i] + ds[pb + i] * etz[pb + i] / np1;
+        const int64_t o = (int64_t)s * cap + st[s].n + k;
+        b.x[o] = xm + rrw * enx[pb + i];
+        b.z[o] = zm + rrw * enz[pb + i];
+        b.c[o] = cnew[pb + i];
+        b.s[o] = panel[pb + i].sig;
+        b.u[o] = 0.0; b.w[o] = 0.0; b.uvs[o] = 0.0; b.wvs[o] = 0.0;
+        b.id[o] = (uint32_t)st[s].n;
+    }
+}
+
+__global__ void bt_commit_release_kernel(int S, SlotState* __restrict__ st)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S || !st[s].active) return;
+    st[s].n += st[s].nrv;
+    st[s].nrv = 0;
+}
+
+__global__ void bt_inside_kernel(int S, int64_t cap, int mp, const WorkList* wl,
+                                 const int64_t* __restrict__ blob_pre,
+                                 const SlotParam* __restrict__ prm, const double* __restrict__ bx,
+                                 const double* __restrict__ bz, const double* __restrict__ enx,
+                                 const double* __restrict__ enz, int reflect, BlobBuf b,
+                                 int* __restrict__ flags)
+{
+    for_each_blob(wl, blob_pre, S, cap, [&](int s, int64_t o) {
+        const int nl = prm[s].nl;
+        const double* px = bx + (int64_t)s * (mp + 1);
+        const double* pz = bz + (int64_t)s * (mp + 1);
+        const double x0 = b.x[o], z0 = b.z[o];
+        const bool in = inside_body(nl, px, pz, x0, z0);
+        flags[o] = in ? 1 : 0;
+        if (!in || !reflect) return;
+        // src/VortexSheet.cpp:345-405
+        int closest = 0;
+        double min_dist = 1e10;
+        for (int j = 0; j < nl; j++) {
+            const double xa = px[j], za = pz[j], xb2 = px[j + 1], zb2 = pz[j + 1];
+            const double vx = x0 - xa, vz = z0 - za;
+            const double dx = xb2 - xa, dz = zb2 - za;
+            const double len_sq = dx * dx + dz * dz;
+            double t = (vx * dx + vz * dz) / len_sq;
+            const double tm = (t < 1.0) ? t : 1.0;
+            t = (0.0 < tm) ? tm : 0.0;
+            const double cx = xa + t * dx, cz = za + t * dz;
+            const double dist = sqrt((x0 - cx) * (x0 - cx) + (z0 - cz) * (z0 - cz));
+            if (dist < min_dist) { min_dist = dist; closest = j; }
+        }
+        const double offset = 2.0 * min_dist + 1e-6;
+        const double ex = enx[(int64_t)s * mp + closest], ez = enz[(int64_t)s * mp + closest];
+        double nx = x0 + offset * ex, nz = z0 + offset * ez;
+        if (inside_body(nl, px, pz, nx, nz)) {
+            nx = x0 + 3.0 * offset * ex;
+            nz = z0 + 3.0 * offset * ez;
+        }
+        b.x[o] = nx;
+        b.z[o] = nz;
+    });
+}
+
+// One workgroup per slot: stable compaction of the blobs outside the body
+// (src/VortexSheet.cpp:407-445) through the alternate buffers, IDs renumbered.
+__global__ __launch_bounds__(1024) void bt_delete_kernel(int64_t cap, SlotState* __restrict__ st,
+                                                         const int* __restrict__ flags, BlobBuf b,
+                                                         BlobBuf alt)
+{
+    const int s = blockIdx.x;
+    if (!st[s].active) return;
+    __shared__ int scan[1024];
+    __shared__ int64_t base_out;
+    const int t = threadIdx.x;
+    const int64_t n = st[s].n;
+    const int64_t sb = (int64_t)s * cap;
+    if (t == 0) base_out = 0;
+    __syncthreads();
+    for (int64_t c0 = 0; c0 < n; c0 += 1024) {
+        const int64_t i = c0 + t;
+        const int keep = (i < n && !flags[sb + i]) ? 1 : 0;
+        scan[t] = keep;
+        __syncthreads();
+        for (int w = 1; w < 1024; w <<= 1) {
+            const int v = t >= w ? scan[t - w] : 0;
+            __syncthreads();
+            scan[t] += v;
+            __syncthreads();
+        }
+        if (keep) {
+            const int64_t o = sb + base_out + scan[t] - 1;
+            const int64_t q = sb + i;
+            alt.x[o] = b.x[q]; alt.z[o] = b.z[q]; alt.c[o] = b.c[q]; alt.s[o] = b.s[q];
+            alt.u[o] = b.u[q]; alt.w[o] = b.w[q]; alt.uvs[o] = b.uvs[q]; alt.wvs[o] = b.wvs[q];
+            alt.id[o] = (uint32_t)(base_out + scan[t] - 1);
+        }
+        __syncthreads();
+        if (t == 1023) base_out += scan[1023];
+        __syncthreads();
+    }
+    const int64_t m = base_out;
+    if (m == n) return;
+    for (int64_t i = t; i < m; i += 1024) {
+        const int64_t q = sb + i;
+        b.x[q] = alt.x[q]; b.z[q] = alt.z[q]; b.c[q] = alt.c[q]; b.s[q] = alt.s[q];
+        b.u[q] = alt.u[q]; b.w[q] = alt.w[q]; b.uvs[q] = alt.uvs[q]; b.wvs[q] = alt.wvs[q];
+        b.id[q] = alt.id[q];
+    }
+    __syncthreads();
+    if (t == 0) st[s].n = m;
+}
+
+// One lane per slot: loads (src/VortexSheet.cpp:487-514), then the step
+// counter; a slot that reaches its step count stops.
+__global__ void bt_loads_finish_kernel(int S, int mp, const SlotParam* __restrict__ prm,
+                                       SlotState* __restrict__ st, const double* __restrict__ gamma,
+                                       const double* __restrict__ ds, const double* __restrict__ enx,
+                                       const double* __restrict__ enz, double* __restrict__ work)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S || !st[s].active) return;
+    const int nl = prm[s].nl;
+    const double* g = gamma + (int64_t)s * mp;
+    const double* d = ds + (int64_t)s * mp;
+    const double* ex = enx + (int64_t)s * mp;
+    const double* ez = enz + (int64_t)s * mp;
+    double* P = work + (int64_t)s * 3 * mp;
+    const double rho = prm[s].rho, ur = prm[s].ur;
+    const double k = -rho / prm[s].dt;
+    P[0] = 0.0;
+    for (int i = 0; i < nl - 1; i++) P[i + 1] = P[i] + k * (g[i] * d[i]);
+    for (int i = 0; i < nl - 1; i++) P[i] = 0.5 * (P[i] + P[i + 1]);
+    P[nl - 1] = 0.5 * (P[nl - 1] + P[0]);
+    double pmax = P[0];
+    for (int i = 1; i < nl; ++i) pmax = P[i] > pmax ? P[i] : pmax;
+    const double pref = 0.5 * rho * ur * ur;
+    double ax1 = 0.0, ax2 = 0.0, az1 = 0.0, az2 = 0.0;
+    int i;
+    for (i = 1; i < nl; i += 2) {
+        const double p0 = P[i - 1] + (pref - pmax) * 1.0, p1 = P[i] + (pref - pmax) * 1.0;
+        ax1 += p0 * ex[i - 1] * d[i - 1];
+        ax2 += p1 * ex[i] * d[i];
+        az1 += p0 * ez[i - 1] * d[i - 1];
+        az2 += p1 * ez[i] * d[i];
+    }
+    if ((i - 1) < nl) {
+        const double p0 = P[i - 1] + (pref - pmax) * 1.0;
+        ax1 += p0 * ex[i - 1] * d[i - 1];
+        az1 += p0 * ez[i - 1] * d[i - 1];
+    }
+    const double fx = -(ax1 + ax2), fz = -(az1 + az2);
+    st[s].loads[0] = fx;
+    st[s].loads[1] = fz;
+    st[s].loads[2] = fx / (0.5 * rho * prm[s].aref * ur * ur);
+    st[s].loads[3] = fz / (0.5 * rho * prm[s].aref * ur * ur);
+    st[s].step += 1;
+    if (st[s].step >= prm[s].steps_target) st[s].active = 0;
+}
+
+}  // namespace dvmi
+
+// ======================================================================= host
+
+using namespace dvmi;
+
+struct dvm_batch_s {
+    dvm_ctx_t ctx = nullptr;
+    dvm_batch_config cfg{};
+    BatchDev d;
+    char* mem = nullptr;
+    std::vector<int> busy;            // admitted and not yet reported finished
+    std::vector<int> host_steps;      // steps completed while busy (mirror for remesh timing)
+    std::vector<SlotState> hst;       // last polled states
+    hipGraph_t graph_a = nullptr, graph_b = nullptr;
+    hipGraphExec_t exec_a = nullptr, exec_b = nullptr;
+};
+
+namespace {
+
+void carve_blob_set(Carver& cv, BlobBuf& b, size_t n)
+{
+    b.x = cv.take<double>(n); b.z = cv.take<double>(n); b.c = cv.take<double>(n);
+    b.s = cv.take<double>(n); b.u = cv.take<double>(n); b.w = cv.take<double>(n);
+    b.uvs = cv.take<double>(n); b.wvs = cv.take<double>(n); b.id = cv.take<uint32_t>(n);
+}
+
+size_t batch_bytes(const dvm_batch_config& c, BatchDev* d, char* mem)
+{
+    const size_t S = c.slots, cap = c.slot_capacity, mp = c.max_panels;
+    const size_t nb = S * cap, np = S * mp, np1 = S * (mp + 1);
+    const size_t maxt = (cap + kBT - 1) / kBT;
+    Carver cv(mem);
+    BatchDev t;
+    BatchDev& o = d ? *d : t;
+    o.prm = cv.take<SlotParam>(S);
+    o.st = cv.take<SlotState>(S);
+    o.wl = cv.take<WorkList>(1);
+    o.tile_pre = cv.take<int>(S + 1);
+    o.order = cv.take<int>(S);
+    o.otile_pre = cv.take<int>(S + 1);
+    o.blob_pre = cv.take<int64_t>(S + 1);
+    carve_blob_set(cv, o.b, nb);
+    carve_blob_set(cv, o.alt, nb);
+    o.q1 = cv.take<double>(nb);
+    o.q2 = cv.take<double>(nb);
+    o.s32 = cv.take<float4>(nb);
+    o.near32 = cv.take<float2>(nb);
+    o.tiles = cv.take<TileInfo>(S * maxt);
+    o.flags = cv.take<int>(nb);
+    o.bx = cv.take<double>(np1);
+    o.bz = cv.take<double>(np1);
+    o.enx = cv.take<double>(np); o.enz = cv.take<double>(np);
+    o.etx = cv.take<double>(np); o.etz = cv.take<double>(np);
+    o.ds = cv.take<double>(np);
+    o.panel = cv.take<PanelConst>(np);
+    o.P = cv.take<double>(S * mp * (mp + 1));
+    o.gamma = cv.take<double>(np);
+    o.col = cv.take<double>(S * kColChunks * (2 * mp + 1));
+    o.cnt = cv.take<int>(np);
+    o.off = cv.take<int64_t>(np1);
+    o.cnew = cv.take<double>(np);
+    o.lwork = cv.take<double>(S * 3 * mp);
+    return cv.off + 256;
+}
+
+void record_convect(dvm_batch_t B)
+{
+    BatchDev& d = B->d;
+    hipStream_t st = B->ctx->stream;
+    const int S = d.S;
+    const int64_t cap = d.cap;
+    const int mp = d.mp;
+    const unsigned gs = 4096;  // grid-stride launches
+    const unsigned persistent = (unsigned)B->ctx->num_cus * 6;
+    const bool f32c = B->cfg.prec == DVM_F32C;
+    const bool rk = B->cfg.scheme == DVM_RK3;
+    const double a[3] = {0, -17. / 32, -32. / 27};
+    const double bb[3] = {1. / 4, 8. / 9, 3. / 4};
+    const int stages = rk ? 3 : 1;
+    for (int k = 0; k < stages; ++k) {
+        bt_prefix_kernel<<<1, 1024, 0, st>>>(S, d.st, d.wl, d.tile_pre, d.order, d.otile_pre, d.blob_pre);
+        if (rk && k == 0) bt_zero_q_kernel<<<gs, 256, 0, st>>>(S, cap, d.wl, d.blob_pre, d.q1, d.q2);
+        if (f32c) {
+            bt_tiles_kernel<<<gs, kBT, 0, st>>>(S, cap, d.wl, d.tile_pre, d.st, d.b, d.s32, d.near32, d.tiles);
+            bt_bs_kernel<true><<<persistent, kBT, 0, st>>>(S, cap, d.wl, d.tile_pre, d.order, d.otile_pre,
+                                                           d.st, d.b, d.s32, d.near32, d.tiles);
+        } else {
+            bt_bs_kernel<false><<<persistent, kBT, 0, st>>>(S, cap, d.wl, d.tile_pre, d.order, d.otile_pre,
+                                                            d.st, d.b, d.s32, d.near32, d.tiles);
+        }
+        bt_colloc_kernel<<<dim3(kColChunks, S), 256, 0, st>>>(cap, mp, d.prm, d.st, d.b, d.panel, d.col);
+        bt_gamma_kernel<<<S, 256, sizeof(double) * (mp + 1), st>>>(mp, d.prm, d.st, d.enx, d.enz, d.P,
+                                                                    d.col, d.gamma);
+        bt_sheet_kernel<<<gs, 256, 0, st>>>(S, cap, mp, d.wl, d.blob_pre, d.prm, d.b, d.panel, d.gamma);
+        bt_advect_kernel<<<gs, 256, 0, st>>>(S, cap, d.wl, d.blob_pre, d.prm, rk ? 1 : 0, a[k], bb[k],
+                                             d.b, d.q1, d.q2);
+    }
+    DVM_LAUNCH_CHECK();
+}
+
+void record_rest(dvm_batch_t B)
+{
+    BatchDev& d = B->d;
+    hipStream_t st = B->ctx->stream;
+    const int S = d.S;
+    const int64_t cap = d.cap;
+    const int mp = d.mp;
+    const unsigned gs = 4096;
+    bt_prefix_kernel<<<1, 1024, 0, st>>>(S, d.st, d.wl, d.tile_pre, d.order, d.otile_pre, d.blob_pre);
+    bt_diffuse_kernel<<<gs, 256, 0, st>>>(S, cap, d.wl, d.blob_pre, d.prm, d.st, d.b);
+    bt_release_count_kernel<<<S, 1024, 0, st>>>(cap, mp, d.prm, d.st, d.gamma, d.ds, d.cnt, d.off, d.cnew);
+    bt_release_emit_kernel<<<dim3(16, S), 256, 0, st>>>(cap, mp, d.prm, d.st, d.bx, d.bz, d.ds, d.etx,
+                                                         d.etz, d.enx, d.enz, d.cnt, d.off, d.cnew,
+                                                         d.panel, d.b);
+    bt_commit_release_kernel<<<(S + 255) / 256, 256, 0, st>>>(S, d.st);
+    bt_prefix_kernel<<<1, 1024, 0, st>>>(S, d.st, d.wl, d.tile_pre, d.order, d.otile_pre, d.blob_pre);
+    const int reflect = B->cfg.crossing != DVM_DELETE ? 1 : 0;
+    bt_inside_kernel<<<gs, 256, 0, st>>>(S, cap, mp, d.wl, d.blob_pre, d.prm, d.bx, d.bz, d.enx, d.enz,
+                                         reflect, d.b, d.flags);
+    if (!reflect) bt_delete_kernel<<<S, 1024, 0, st>>>(cap, d.st, d.flags, d.b, d.alt);
+    bt_loads_finish_kernel<<<(S + 63) / 64, 64, 0, st>>>(S, mp, d.prm, d.st, d.gamma, d.ds, d.enx, d.enz,
+                                                         d.lwork);
+    DVM_LAUNCH_CHECK();
+}
+
+void build_graphs(dvm_batch_t B)
+{
+    hipStream_t st = B->ctx->stream;
+    DVM_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    record_convect(B);
+    DVM_HIP_CHECK(hipStreamEndCapture(st, &B->graph_a));
+    DVM_HIP_CHECK(hipGraphInstantiate(&B->exec_a, B->graph_a, nullptr, nullptr, 0));
+    DVM_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    record_rest(B);
+    DVM_HIP_CHECK(hipStreamEndCapture(st, &B->graph_b));
+    DVM_HIP_CHECK(hipGraphInstantiate(&B->exec_b, B->graph_b, nullptr, nullptr, 0));
+}
+
+void poll_states(dvm_batch_t B)
+{
+    DVM_HIP_CHECK(hipMemcpyAsync(B->hst.data(), B->d.st, sizeof(SlotState) * B->d.S,
+                                 hipMemcpyDeviceToHost, B->ctx->stream));
+    DVM_HIP_CHECK(hipStreamSynchronize(B->ctx->stream));
+}
+
+// Remesh of the slots whose next step is a remesh step (BackgroundMesh
+// specification of remesh.hip), one slot at a time between the two graphs.
+void remesh_due(dvm_batch_t B)
+{
+    const dvm_remesh_params& rp = B->cfg.remesh;
+    if (rp.frequency <= 0) return;
+    bool any = false;
+    for (int s = 0; s < B->d.S; ++s)
+        if (B->busy[s] && (B->host_steps[s] + 1) % rp.frequency == 0) any = true;
+    if (!any) return;
+    poll_states(B);
+    hipStream_t st = B->ctx->stream;
+    for (int s = 0; s < B->d.S; ++s) {
+        if (!B->busy[s] || !B->hst[s].active || (B->host_steps[s] + 1) % rp.frequency != 0) continue;
+        const int64_t off = (int64_t)s * B->d.cap;
+        auto view = [off](const BlobBuf& b) {
+            BlobBuf v;
+            v.x = b.x + off; v.z = b.z + off; v.c = b.c + off; v.s = b.s + off;
+            v.u = b.u + off; v.w = b.w + off; v.uvs = b.uvs + off; v.wvs = b.wvs + off;
+            v.id = b.id + off;
+            return v;
+        };
+        BlobBuf in = view(B->d.b), out = view(B->d.alt);
+        RemeshPlan plan;
+        const int64_t m = remesh_count(B->ctx, rp, B->hst[s].n, in, plan);
+        if (m > B->d.cap) {
+            DVM_HIP_CHECK(hipFreeAsync(plan.mem, st));
+            SlotState bad = B->hst[s];
+            bad.active = 0;
+            bad.status = DVM_E_SIZE;
+            DVM_HIP_CHECK(hipMemcpyAsync(B->d.st + s, &bad, sizeof(SlotState), hipMemcpyHostToDevice, st));
+            DVM_HIP_CHECK(hipStreamSynchronize(st));
+            continue;
+        }
+        remesh_finish(B->ctx, rp, in, plan, out);
+        const size_t b = sizeof(double) * m;
+        const double* fs[8] = {out.x, out.z, out.c, out.s, out.u, out.w, out.uvs, out.wvs};
+        double* ts[8] = {in.x, in.z, in.c, in.s, in.u, in.w, in.uvs, in.wvs};
+        for (int k = 0; k < 8; ++k)
+            if (m) DVM_HIP_CHECK(hipMemcpyAsync(ts[k], fs[k], b, hipMemcpyDeviceToDevice, st));
+        if (m) DVM_HIP_CHECK(hipMemcpyAsync(in.id, out.id, sizeof(uint32_t) * m, hipMemcpyDeviceToDevice, st));
+        DVM_HIP_CHECK(hipMemcpyAsync(&B->d.st[s].n, &m, sizeof(int64_t), hipMemcpyHostToDevice, st));
+        DVM_HIP_CHECK(hipStreamSynchronize(st));
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+dvm_status dvm_batch_footprint(const dvm_batch_config* cfg, size_t* bytes)
+{
+    if (!cfg || !bytes) return DVM_E_INVALID;
+    *bytes = batch_bytes(*cfg, nullptr, nullptr);
+    return DVM_OK;
+}
+
+dvm_status dvm_batch_create(dvm_ctx_t ctx, const dvm_batch_config* cfg, dvm_batch_t* out)
+{
+    if (!ctx || !cfg || !out) return DVM_E_INVALID;
+    *out = nullptr;
+    dvm_batch_t B = new (std::nothrow) dvm_batch_s();
+    if (!B) return DVM_E_OOM;
+    B->ctx = ctx;
+    dvm_status st = guard(ctx, [&] {
+        const dvm_batch_config& c = *cfg;
+        require(c.slots >= 1 && c.slots <= 1024, DVM_E_INVALID, "slots must be in [1, 1024]");
+        require(c.slot_capacity >= kBT && c.slot_capacity < (int64_t(1) << 31), DVM_E_INVALID,
+                "slot capacity out of range");
+        require(c.max_panels >= 2 && c.max_panels <= 1024, DVM_E_INVALID, "max_panels must be in [2, 1024]");
+        require(c.scheme == DVM_EULER || c.scheme == DVM_RK3, DVM_E_INVALID, "unknown scheme");
+        require(c.crossing >= DVM_DELETE && c.crossing <= DVM_REFLECT, DVM_E_INVALID, "unknown crossing");
+        require(c.prec == DVM_F64 || c.prec == DVM_F32C, DVM_E_INVALID, "unknown precision");
+        B->cfg = c;
+        B->d.S = c.slots;
+        B->d.cap = c.slot_capacity;
+        B->d.mp = c.max_panels;
+        B->d.max_tiles = (int)((c.slot_capacity + kBT - 1) / kBT);
+        const size_t bytes = batch_bytes(c, nullptr, nullptr);
+        DVM_HIP_CHECK(hipMalloc(&B->mem, bytes));
+        batch_bytes(c, &B->d, B->mem);
+        DVM_HIP_CHECK(hipMemsetAsync(B->d.st, 0, sizeof(SlotState) * c.slots, ctx->stream));
+        DVM_HIP_CHECK(hipMemsetAsync(B->d.prm, 0, sizeof(SlotParam) * c.slots, ctx->stream));
+        DVM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        B->busy.assign(c.slots, 0);
+        B->host_steps.assign(c.slots, 0);
+        B->hst.resize(c.slots);
+        if (c.use_graph) build_graphs(B);
+    });
+    if (st != DVM_OK) {
+        if (B->mem) (void)hipFree(B->mem);
+        delete B;
+        return st;
+    }
+    *out = B;
+    return DVM_OK;
+}
+
+dvm_status dvm_batch_destroy(dvm_batch_t B)
+{
+    if (!B) return DVM_OK;
+    dvm_status st = guard(B->ctx, [&] {
+        DVM_HIP_CHECK(hipStreamSynchronize(B->ctx->stream));
+        if (B->exec_a) DVM_HIP_CHECK(hipGraphExecDestroy(B->exec_a));
+        if (B->exec_b) DVM_HIP_CHECK(hipGraphExecDestroy(B->exec_b));
+        if (B->graph_a) DVM_HIP_CHECK(hipGraphDestroy(B->graph_a));
+        if (B->graph_b) DVM_HIP_CHECK(hipGraphDestroy(B->graph_b));
+        if (B->mem) DVM_HIP_CHECK(hipFree(B->mem));
+    });
+    delete B;
+    return st;
+}
+
+dvm_status dvm_batch_admit(dvm_batch_t B, int npts, const double* px, const double* pz,
+                           const dvm_sheet_params* p, int seed, int steps, int* slot)
+{
+    if (!B || !slot) return DVM_E_INVALID;
+    dvm_ctx_t ctx = B->ctx;
+    return guard(ctx, [&] {
+        *slot = -1;
+        require(steps >= 1, DVM_E_INVALID, "steps must be >= 1");
+        require(npts - 1 <= B->d.mp, DVM_E_SIZE, "body has more panels than max_panels");
+        int s = -1;
+        for (int k = 0; k < B->d.S; ++k)
+            if (!B->busy[k]) { s = k; break; }
+        require(s >= 0, DVM_E_STATE, "no free slot");
+        SheetDev sh;
+        try {
+            sheet_init(ctx, sh, npts, px, pz, p);
+        } catch (...) {
+            sheet_free(sh);
+            throw;
+        }
+        BatchDev& d = B->d;
+        const int nl = sh.nl, mp = d.mp;
+        hipStream_t st = ctx->stream;
+        const size_t db = sizeof(double);
+        auto d2d = [&](void* dst, const void* src, size_t bytes) {
+            DVM_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
+        };
+        d2d(d.bx + (size_t)s * (mp + 1), sh.x, db * npts);
+        d2d(d.bz + (size_t)s * (mp + 1), sh.z, db * npts);
+        d2d(d.enx + (size_t)s * mp, sh.enx, db * nl);
+        d2d(d.enz + (size_t)s * mp, sh.enz, db * nl);
+        d2d(d.etx + (size_t)s * mp, sh.etx, db * nl);
+        d2d(d.etz + (size_t)s * mp, sh.etz, db * nl);
+        d2d(d.ds + (size_t)s * mp, sh.ds, db * nl);
+        d2d(d.panel + (size_t)s * mp, sh.panel, sizeof(PanelConst) * nl);
+        d2d(d.P + (size_t)s * mp * (mp + 1), sh.P, db * nl * (nl + 1));
+        DVM_HIP_CHECK(hipMemsetAsync(d.gamma + (size_t)s * mp, 0, db * nl, st));
+        SlotParam sp{};
+        sp.rho = p->rho; sp.nu = p->nu; sp.dt = p->dt; sp.ux = p->ux; sp.uz = p->uz;
+        sp.ur = std::sqrt(p->ux * p->ux + p->uz * p->uz);
+        sp.aref = sh.aref;
+        sp.four_nu_dt = 4.0 * p->nu * p->dt;
+        sp.nl = nl; sp.max_npv = p->max_npv; sp.steps_target = steps;
+        sp.seed = (uint32_t)seed;
+        SlotState ss{};
+        ss.active = 1;
+        DVM_HIP_CHECK(hipMemcpyAsync(d.prm + s, &sp, sizeof(sp), hipMemcpyHostToDevice, st));
+        DVM_HIP_CHECK(hipMemcpyAsync(d.st + s, &ss, sizeof(ss), hipMemcpyHostToDevice, st));
+        DVM_HIP_CHECK(hipStreamSynchronize(st));
+        sheet_free(sh);
+        B->busy[s] = 1;
+        B->host_steps[s] = 0;
+        B->hst[s] = ss;
+        *slot = s;
+    });
+}
+
+dvm_status dvm_batch_step(dvm_batch_t B, int steps)
+{
+    if (!B) return DVM_E_INVALID;
+    return guard(B->ctx, [&] {
+        for (int k = 0; k < steps; ++k) {
+            if (B->cfg.use_graph) {
+                ProfScope ps(B->ctx, "batch_step");
+                DVM_HIP_CHECK(hipGraphLaunch(B->exec_a, B->ctx->stream));
+                remesh_due(B);
+                DVM_HIP_CHECK(hipGraphLaunch(B->exec_b, B->ctx->stream));
+            } else {
+                ProfScope ps(B->ctx, "batch_step");
+                record_convect(B);
+                remesh_due(B);
+                record_rest(B);
+            }
+            for (int s = 0; s < B->d.S; ++s)
+                if (B->busy[s]) B->host_steps[s]++;
+        }
+    });
+}
+
+dvm_status dvm_batch_poll(dvm_batch_t B, int max, int* slots, int* status, int* count)
+{
+    if (!B || !count) return DVM_E_INVALID;
+    return guard(B->ctx, [&] {
+        poll_states(B);
+        int c = 0;
+        for (int s = 0; s < B->d.S && c < max; ++s) {
+            if (B->busy[s] && !B->hst[s].active) {
+                if (slots) slots[c] = s;
+                if (status) status[c] = B->hst[s].status;
+                B->busy[s] = 0;
+                ++c;
+            }
+        }
+        *count = c;
+    });
+}
+
+dvm_status dvm_batch_slot_info(dvm_batch_t B, int slot, int64_t* n, int* step, int* active,
+                               int* status, double* loads4)
+{
+    if (!B || slot < 0 || slot >= B->d.S) return DVM_E_INVALID;
+    return guard(B->ctx, [&] {
+        SlotState ss;
+        DVM_HIP_CHECK(hipMemcpyAsync(&ss, B->d.st + slot, sizeof(ss), hipMemcpyDeviceToHost, B->ctx->stream));
+        DVM_HIP_CHECK(hipStreamSynchronize(B->ctx->stream));
+        if (n) *n = ss.n;
+        if (step) *step = ss.step;
+        if (active) *active = ss.active;
+        if (status) *status = ss.status;
+        if (loads4) std::memcpy(loads4, ss.loads, sizeof(ss.loads));
+    });
+}
+
+dvm_status dvm_batch_get_state(dvm_batch_t B, int slot, double* x, double* z, double* circ,
+                               double* sigma, double* u, double* w)
+{
+    if (!B || slot < 0 || slot >= B->d.S) return DVM_E_INVALID;
+    return guard(B->ctx, [&] {
+        SlotState ss;
+        hipStream_t st = B->ctx->stream;
+        DVM_HIP_CHECK(hipMemcpyAsync(&ss, B->d.st + slot, sizeof(ss), hipMemcpyDeviceToHost, st));
+        DVM_HIP_CHECK(hipStreamSynchronize(st));
+        const int64_t off = (int64_t)slot * B->d.cap;
+        const size_t b = sizeof(double) * ss.n;
+        double* outs[6] = {x, z, circ, sigma, u, w};
+        const double* srcs[6] = {B->d.b.x, B->d.b.z, B->d.b.c, B->d.b.s, B->d.b.u, B->d.b.w};
+        for (int k = 0; k < 6; ++k)
+            if (outs[k] && ss.n) DVM_HIP_CHECK(hipMemcpyAsync(outs[k], srcs[k] + off, b, hipMemcpyDeviceToHost, st));
+        DVM_HIP_CHECK(hipStreamSynchronize(st));
+    });
+}
+
+}  // extern "C"

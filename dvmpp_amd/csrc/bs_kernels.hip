@@ -21,6 +21,7 @@
 //           pair); the others take the full regularised loop.  Partial sums
 //           within a tile accumulate in FP32 with FMA; tile partials are added
 //           with Kahan compensation and the result is returned in FP64.
+#include "dvm_device.hpp"
 #include "dvm_internal.hpp"
 
 #include <hipcub/hipcub.hpp>
@@ -125,13 +126,6 @@ __global__ void bs_reduce_kernel(int64_t nt, int nchunks, const double* __restri
 }
 
 // ----------------------------------------------------------------- F32C form
-
-struct TileInfo {
-    double cx, cz;    // tile centre (FP64)
-    float hx, hz;     // half extents, rounded up
-    float thr_max;    // 10 * max(sigma)^2 over the tile's sources
-    float pad;
-};
 
 __global__ void bbox_partial_kernel(int64_t n, const double* __restrict__ x,
                                     const double* __restrict__ z, double4* __restrict__ part)
@@ -243,14 +237,6 @@ __global__ __launch_bounds__(kTile) void tile_build_kernel(
     const double s2 = si * si;
     near32[k] = make_float2((float)(10.0 * s2), (float)(1.0 / s2));
     p64[k] = make_double2(xi, zi);
-}
-
-__device__ __forceinline__ void kahan_add(float& s, float& c, float x)
-{
-    const float y = x - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
 }
 
 // Targets: SELF -> the sorted FP64 copy p64 (target i is source i);

@@ -5,6 +5,7 @@
 // Layout: panel data in SoA FP64 arrays plus one 64-byte PanelConst record per
 // panel; the per-pair loops stream panels with wave-uniform addresses (scalar
 // loads), one blob per lane.
+#include "dvm_device.hpp"
 #include "dvm_internal.hpp"
 
 #include <hipcub/hipcub.hpp>
@@ -364,25 +365,8 @@ __global__ __launch_bounds__(256) void sheet_on_blobs_kernel(
     const double xi = bx[i];
     const double zi = bz[i];
     double su = 0.0, sw = 0.0;
-    for (int j = 0; j < nl; ++j) {
-        const PanelConst q = pc[j];
-        const double gj = gamma[j];
-        const double gj1 = gamma[j + 1 == nl ? 0 : j + 1];
-        const double xj = q.xc, zj = q.zc, dsj = q.ds;
-        const double c1 = -(xi - xj) * q.cs - (zi - zj) * q.sn;
-        const double c2 = (xi - xj) * (xi - xj) + (zi - zj) * (zi - zj);
-        const double c5 = (xi - xj) * q.sn - (zi - zj) * q.cs;
-        const double c6 = log(1.0 + dsj * ((dsj + 2 * c1) / c2));
-        const double c7 = atan2(c5 * dsj, c2 + c1 * dsj);
-        const double c8 = (xi - xj) * q.s2 + (zi - zj) * q.c2;
-        const double c9 = (xi - xj) * q.c2 - (zi - zj) * q.s2;
-        const double qx = -q.sn + 0.5 * c8 * c6 / dsj + (c1 * q.cs + c5 * q.sn) * c7 / dsj;
-        const double px = -0.5 * c6 * q.sn - c7 * q.cs - qx;
-        const double qy = q.cs + 0.5 * c9 * c6 / dsj + (c1 * q.sn - c5 * q.cs) * c7 / dsj;
-        const double py = 0.5 * c6 * q.cs - c7 * q.sn - qy;
-        su += px * gj + qx * gj1;
-        sw += py * gj + qy * gj1;
-    }
+    for (int j = 0; j < nl; ++j)
+        sheet_pair(xi, zi, pc[j], gamma[j], gamma[j + 1 == nl ? 0 : j + 1], su, sw);
     uvs[i] = su * kRpi2;
     wvs[i] = sw * kRpi2;
 }
@@ -578,21 +562,6 @@ void sheet_release_emit(dvm_ctx_t ctx, const SheetDev& sh, int64_t nrv, const do
 }
 
 // ----------------------------------------------------------------- crossing
-
-// src/VortexSheet.cpp:447-464: crossing number over the Nl edges; inside iff 1.
-__device__ __forceinline__ bool inside_body(int nl, const double* __restrict__ bxp,
-                                            const double* __restrict__ bzp, double xc, double zc)
-{
-    int cn = 0;
-    for (int i = 0; i < nl; i++) {
-        const double zi = bzp[i], zi1 = bzp[i + 1];
-        if (((zi <= zc) && (zi1 > zc)) || ((zi > zc) && (zi1 <= zc))) {
-            const double vt = (zc - zi) / (zi1 - zi);
-            if (xc < bxp[i] + vt * (bxp[i + 1] - bxp[i])) ++cn;
-        }
-    }
-    return cn == 1;
-}
 
 __global__ __launch_bounds__(256) void inside_kernel(int64_t n, int nl, const double* __restrict__ bxp,
                                                      const double* __restrict__ bzp,

@@ -7,6 +7,7 @@
 // with the C library's libm (so that diffusion and release positions match the
 // reference bit for bit); the increments are generated while the device runs
 // the convection stage and are applied by device kernels.
+#include "dvm_device.hpp"
 #include "dvm_internal.hpp"
 
 #include <hipcub/hipcub.hpp>
@@ -57,28 +58,6 @@ __global__ void diffuse_tape_kernel(int64_t n, const double* __restrict__ tape,
     const double2 d = reinterpret_cast<const double2*>(tape)[i];
     x[i] += d.x;
     z[i] += d.y;
-}
-
-// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): counter-based generator.
-__device__ __forceinline__ uint4 philox4x32(uint4 ctr, uint2 key)
-{
-    for (int r = 0; r < 10; ++r) {
-        const uint32_t hi0 = __umulhi(0xD2511F53u, ctr.x);
-        const uint32_t lo0 = 0xD2511F53u * ctr.x;
-        const uint32_t hi1 = __umulhi(0xCD9E8D57u, ctr.z);
-        const uint32_t lo1 = 0xCD9E8D57u * ctr.z;
-        ctr = make_uint4(hi1 ^ ctr.y ^ key.x, lo1, hi0 ^ ctr.w ^ key.y, lo0);
-        key.x += 0x9E3779B9u;
-        key.y += 0xBB67AE85u;
-    }
-    return ctr;
-}
-
-// 53-bit uniform in (0, 1) from two 32-bit words.
-__device__ __forceinline__ double u01(uint32_t a, uint32_t b)
-{
-    const uint64_t m = ((uint64_t)(a >> 5) << 26) | (uint64_t)(b >> 6);
-    return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
 }
 
 // Diffusion with device random numbers: the reference's Box-Muller step

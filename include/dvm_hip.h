@@ -290,6 +290,50 @@ dvm_status dvm_sim_loads(dvm_sim_t sim, double* out4);
 /* Device pointers to the blob SoA (valid until the next call that changes N). */
 dvm_status dvm_sim_blobs(dvm_sim_t sim, double** x, double** z, double** circ, double** sigma);
 
+/* ------------------------------------------------------------------ batched ensemble
+ * Many independent simulations (different bodies, viscosities, time steps,
+ * seeds and step counts; ragged blob counts) stepped together: each launch
+ * covers every active simulation, the pair sum is one persistent launch over
+ * (simulation, target tile) work items, and blob counts, step counters and
+ * status words stay on the device so a step is replayed from a hipGraph.
+ * Diffusion and release draw from the device Philox stream (DVM_RNG_DEVICE).
+ * No counterpart in the reference, which runs one simulation per process
+ * (src/main.cpp:20-115); each simulation follows DVMBase::compute_step
+ * (src/DVMBase.cpp:86-133). */
+typedef struct dvm_batch_s* dvm_batch_t;
+
+typedef struct dvm_batch_config {
+    int slots;              /* simulations held at once (<= 1024) */
+    int64_t slot_capacity;  /* blob capacity of each slot */
+    int max_panels;         /* largest body (panels) a slot accepts (<= 1024) */
+    int scheme;             /* dvm_scheme, shared by all slots */
+    int crossing;           /* dvm_crossing, shared */
+    int prec;               /* dvm_precision of the pair sums */
+    int use_graph;          /* 1: replay each step from a captured hipGraph */
+    dvm_remesh_params remesh; /* shared remesh settings (frequency 0 = none) */
+} dvm_batch_config;
+
+/* Device bytes a batch with this configuration allocates. */
+dvm_status dvm_batch_footprint(const dvm_batch_config* cfg, size_t* bytes);
+dvm_status dvm_batch_create(dvm_ctx_t ctx, const dvm_batch_config* cfg, dvm_batch_t* out);
+dvm_status dvm_batch_destroy(dvm_batch_t batch);
+/* Starts a simulation of `steps` steps in a free slot (DVM_E_STATE if none):
+ * the body (HOST points, closed) gets its sheet set up as dvm_sheet_create
+ * does; zero blobs. */
+dvm_status dvm_batch_admit(dvm_batch_t batch, int npts, const double* px, const double* pz,
+                           const dvm_sheet_params* params, int seed, int steps, int* slot);
+/* Advances every active simulation by `steps` time steps. */
+dvm_status dvm_batch_step(dvm_batch_t batch, int steps);
+/* Reports up to `max` simulations that stopped since the last poll (finished
+ * their steps, or failed: status DVM_E_NAN / DVM_E_SIZE); their slots become
+ * free, their state stays readable until the slot is admitted again. */
+dvm_status dvm_batch_poll(dvm_batch_t batch, int max, int* slots, int* status, int* count);
+dvm_status dvm_batch_slot_info(dvm_batch_t batch, int slot, int64_t* n, int* step, int* active,
+                               int* status, double* loads4);
+/* A slot's blob fields to HOST arrays (n = dvm_batch_slot_info's n). */
+dvm_status dvm_batch_get_state(dvm_batch_t batch, int slot, double* x, double* z, double* circ,
+                               double* sigma, double* u, double* w);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,123 @@
+"""Batched ensemble (dvm_batch_*): each simulation's result is the same alone
+and in a mix (bit-for-bit: per-item sum orders are fixed), the graph replay
+equals eager launches, a batch slot follows the single-simulation engine with
+the same device random stream (FP64: to 1e-10; FP32C: to the FP32C
+tolerance), and a failing simulation stops alone."""
+import numpy as np
+import pytest
+
+import dvmpp_amd as dvm
+from dvmpp_amd import bodies
+from dvmpp_amd import sim as dsim
+from dvmpp_amd.batch import Batch, batch_config
+
+pytestmark = pytest.mark.gpu
+
+CASES = [  # body, sheet kw, seed, steps
+    (bodies.cylinder(64), dict(nu=1e-3, dt=0.005), 11, 12),
+    (bodies.ellipse(48, aspect=2.5, aoa_deg=15), dict(nu=5e-4, dt=0.004, ux=0.8), 12, 9),
+    (bodies.rounded_square(40), dict(nu=2e-3, dt=0.006), 13, 15),
+]
+
+
+def run_batch(ctx, cases, prec=dvm.F32C, scheme=dsim.EULER, crossing=dsim.REFLECT, graph=True,
+              extra_steps=0):
+    cfg = batch_config(slots=8, slot_capacity=1 << 15, max_panels=128, scheme=scheme,
+                       crossing=crossing, prec=prec, use_graph=graph)
+    b = Batch(ctx, cfg)
+    slots = [b.admit(px, pz, dsim.sheet_params(**kw), seed, steps)
+             for (px, pz), kw, seed, steps in cases]
+    b.step(max(c[3] for c in cases) + extra_steps)
+    out = [b.state(s) for s in slots]
+    infos = [b.info(s) for s in slots]
+    done = b.poll()
+    b.close()
+    return out, infos, done
+
+
+def test_alone_equals_in_mix(gpu_ctx):
+    mix, infos, done = run_batch(gpu_ctx, CASES)
+    assert sorted(s for s, _ in done) == [0, 1, 2] and all(st == 0 for _, st in done)
+    for k, case in enumerate(CASES):
+        assert infos[k]["step"] == case[3] and infos[k]["active"] == 0
+        alone, _, _ = run_batch(gpu_ctx, [case])
+        for f in ("x", "z", "circ", "sigma", "u", "w"):
+            assert np.array_equal(mix[k][f], alone[0][f]), (k, f)
+
+
+def test_graph_equals_eager(gpu_ctx):
+    a, _, _ = run_batch(gpu_ctx, CASES, graph=True)
+    b, _, _ = run_batch(gpu_ctx, CASES, graph=False)
+    for k in range(len(CASES)):
+        for f in ("x", "z", "circ", "u", "w"):
+            assert np.array_equal(a[k][f], b[k][f]), (k, f)
+
+
+@pytest.mark.parametrize("prec,scheme,crossing", [(dvm.F64, dsim.EULER, dsim.REFLECT),
+                                                  (dvm.F64, dsim.RK3, dsim.DELETE),
+                                                  (dvm.F32C, dsim.EULER, dsim.REFLECT)])
+def test_slot_matches_single_simulation(gpu_ctx, prec, scheme, crossing):
+    (px, pz), kw, seed, steps = CASES[0]
+    out, infos, _ = run_batch(gpu_ctx, [CASES[0]], prec=prec, scheme=scheme, crossing=crossing)
+    sp = dsim.sim_params(scheme=scheme, crossing=crossing, seed=seed, prec=prec,
+                         rng_mode=dsim.RNG_DEVICE, **kw)
+    s = dsim.Sim(gpu_ctx, px, pz, sp)
+    s.run(steps)
+    ref = s.get_state()
+    assert ref["x"].size == out[0]["x"].size
+    tol = 1e-10 if prec == dvm.F64 else 1e-6
+    scale = np.max(np.abs(ref["x"]))
+    assert np.max(np.abs(ref["x"] - out[0]["x"])) <= tol * scale
+    assert np.max(np.abs(ref["z"] - out[0]["z"])) <= tol * scale
+    assert np.array_equal(ref["sigma"], out[0]["sigma"])
+    lo = s.loads()
+    assert abs(lo[2] - infos[0]["loads"][2]) <= max(1e-6, 1e3 * tol) * max(1.0, abs(lo[2]))
+    s.close()
+
+
+def test_batch_remesh_matches_single_simulation(gpu_ctx):
+    rp = dsim.remesh_params(frequency=4, kernel=dsim.M4PRIME, xmin=-2.0, xmax=4.0, zmin=-2.0,
+                            zmax=2.0, dx=0.05, dz=0.05, sigma=0.06, circ_threshold=1e-9)
+    (px, pz), kw, seed, _ = CASES[0]
+    cfg = batch_config(slots=2, slot_capacity=1 << 15, max_panels=128, prec=dvm.F64, remesh=rp)
+    b = Batch(gpu_ctx, cfg)
+    slot = b.admit(px, pz, dsim.sheet_params(**kw), seed, 10)
+    b.step(10)
+    out = b.state(slot)
+    b.close()
+    s = dsim.Sim(gpu_ctx, px, pz, dsim.sim_params(seed=seed, prec=dvm.F64, rng_mode=dsim.RNG_DEVICE,
+                                                   remesh=rp, **kw))
+    s.run(10)
+    ref = s.get_state()
+    assert ref["x"].size == out["x"].size
+    assert np.max(np.abs(ref["x"] - out["x"])) <= 1e-10 * np.max(np.abs(ref["x"]))
+    assert np.max(np.abs(ref["circ"] - out["circ"])) <= 1e-10 * np.max(np.abs(ref["circ"]))
+    s.close()
+
+
+def test_failure_stops_one_slot(gpu_ctx):
+    cases = list(CASES[:2]) + [(bodies.cylinder(32), dict(ux=0.0, uz=0.0), 5, 10)]
+    out, infos, done = run_batch(gpu_ctx, cases)
+    status = dict(done)
+    assert status[2] == 3  # DVM_E_NAN: zero free stream -> NaN release counts
+    assert status[0] == 0 and status[1] == 0
+    alone, _, _ = run_batch(gpu_ctx, CASES[:2])
+    for k in range(2):
+        assert np.array_equal(out[k]["x"], alone[k]["x"])
+
+
+def test_slot_reuse_and_capacity_error(gpu_ctx):
+    cfg = batch_config(slots=2, slot_capacity=1024, max_panels=64)
+    b = Batch(gpu_ctx, cfg)
+    s0 = b.admit(*bodies.cylinder(64), dsim.sheet_params(), 1, 30)  # overflows 1024 blobs
+    s1 = b.admit(*bodies.cylinder(16), dsim.sheet_params(), 2, 3)
+    with pytest.raises(dvm.DvmError):
+        b.admit(*bodies.cylinder(16), dsim.sheet_params(), 3, 3)  # no free slot
+    b.step(30)
+    status = dict(b.poll())
+    assert status[s1] == 0 and status[s0] == 2  # DVM_E_SIZE
+    s2 = b.admit(*bodies.cylinder(16), dsim.sheet_params(), 4, 2)
+    assert s2 in (s0, s1)
+    b.step(2)
+    assert dict(b.poll())[s2] == 0
+    b.close()

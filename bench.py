@@ -86,6 +86,140 @@ def cpu_baseline(x, z, c, s, budget_s):
                       f"loop order, OpenMP, {dt:.1f} s"}
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def plain_cylinder_phase(ctx, steps, cpu_steps):
+    """Plain cylinder (256 panels, Euler, dt 0.005, REFLECT, seed 0): steps/s of
+    the FP64 parity form (reference random stream), of the chosen form (FP32C
+    sums, device random stream), and of the CPU oracle over the first
+    `cpu_steps` steps (same window timed on the GPU)."""
+    import torch
+
+    import dvmpp_amd as dvm
+    from dvmpp_amd import sim as dsim
+    from dvmpp_amd.workloads import plain_cylinder
+    px, pz, sp = plain_cylinder()
+    out = {"workload": f"circular cylinder r=0.5, 256 panels, Re=1000, Euler dt=0.005, REFLECT, "
+                       f"{steps} steps from rest"}
+
+    def run(prec, rng):
+        p = dsim.SimParams(sp, dsim.EULER, dsim.REFLECT, 0, prec, rng, dsim.remesh_params())
+        s = dsim.Sim(ctx, px, pz, p)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.run(cpu_steps)
+        torch.cuda.synchronize()
+        t_win = time.perf_counter() - t0
+        s.run(steps - cpu_steps)
+        torch.cuda.synchronize()
+        t_all = time.perf_counter() - t0
+        n, cd = s.size(), s.loads()[2]
+        s.close()
+        return t_win, t_all, n, cd
+
+    w64, a64, n64, cd64 = run(dvm.F64, dsim.RNG_REFERENCE)
+    log(f"plain cylinder FP64: {steps} steps in {a64:.2f} s, N = {n64}")
+    w32, a32, n32, cd32 = run(dvm.F32C, dsim.RNG_DEVICE)
+    log(f"plain cylinder FP32C: {steps} steps in {a32:.2f} s, N = {n32}")
+    out["f64_reference_stream"] = {"steps_per_s": steps / a64, "final_blobs": n64, "cd": cd64,
+                                   "window_steps_per_s": cpu_steps / w64}
+    out["f32c_device_stream"] = {"steps_per_s": steps / a32, "final_blobs": n32, "cd": cd32,
+                                 "window_steps_per_s": cpu_steps / w32}
+    import oracle
+    sim = oracle.Sim(px, pz, oracle.make_params(rho=sp.rho, nu=sp.nu, dt=sp.dt, ux=sp.ux,
+                                                uz=sp.uz, cutoff_exp=sp.cutoff_exp,
+                                                max_npv=sp.max_npv, scheme=0, crossing=2, seed=0))
+    t0 = time.perf_counter()
+    for _ in range(cpu_steps):
+        sim.step()
+    tc = time.perf_counter() - t0
+    out["cpu_oracle_window"] = {"steps": cpu_steps, "steps_per_s": cpu_steps / tc,
+                                "final_blobs": sim.size(),
+                                "cores": int(oracle.lib().orc_num_threads())}
+    log(f"CPU oracle: {cpu_steps} steps in {tc:.2f} s")
+    return out
+
+
+def ensemble_phase(ctx, count, steps, plain_count):
+    """Uneven ensemble (ellipses and rounded squares, 128-512 panels, nu
+    log-uniform in [2e-4, 2e-3], dt 0.005) advanced `steps` steps from rest:
+    one batch (FP32C, graph-replayed steps) against the plain FP64 form run one
+    simulation at a time (measured on the first `plain_count` bodies)."""
+    import torch
+
+    import dvmpp_amd as dvm
+    from dvmpp_amd import sim as dsim
+    from dvmpp_amd.batch import Batch, batch_config
+    from dvmpp_amd.workloads import ensemble
+    jobs = ensemble(count, steps_range=(steps, steps))
+    cap = 1 << 18
+    b = Batch(ctx, batch_config(slots=count, slot_capacity=cap, max_panels=512, prec=dvm.F32C,
+                                use_graph=True))
+    for px, pz, sp, seed, st in jobs:
+        b.admit(px, pz, sp, seed, st)
+    torch.cuda.synchronize()
+    ctx.prof_reset()
+    ctx.prof_enable(True)
+    t0 = time.perf_counter()
+    b.step(steps)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    ctx.prof_enable(False)
+    done = b.poll()
+    blobs = [b.info(s)["n"] for s, _ in done]
+    ok = sum(1 for _, st in done if st == 0)
+    b.close()
+    log(f"ensemble batch: {count} x {steps} steps in {tb:.2f} s")
+    t0 = time.perf_counter()
+    for px, pz, sp, seed, st in jobs[:plain_count]:
+        p = dsim.SimParams(sp, dsim.EULER, dsim.REFLECT, seed, dvm.F64, dsim.RNG_REFERENCE,
+                           dsim.remesh_params())
+        s = dsim.Sim(ctx, px, pz, p)
+        s.run(st)
+        s.close()
+    torch.cuda.synchronize()
+    tp = (time.perf_counter() - t0) * count / plain_count
+    log(f"ensemble plain FP64: {plain_count} x {steps} steps in {tp * plain_count / count:.2f} s")
+    return {"workload": f"{count} bodies (ellipses AR~U[1,4], AoA~U[0,30] deg; rounded squares), "
+                        f"128-512 panels, nu log-U[2e-4,2e-3], dt 0.005, {steps} steps from rest",
+            "batched_f32c_graph": {"aggregate_steps_per_s": count * steps / tb, "seconds": tb,
+                                   "completed_ok": ok,
+                                   "final_blobs_min_max": [min(blobs), max(blobs)]},
+            "plain_f64_per_simulation": {"aggregate_steps_per_s": count * steps / tp,
+                                         "seconds_extrapolated": tp,
+                                         "measured_simulations": plain_count},
+            "speedup": tp / tb}
+
+
+def served_phase(ctx, count, rate, steps_range, slots):
+    """Poisson arrivals at `rate`/s of ensemble-distribution requests, served by
+    continuous batching (FP32C, graph-replayed steps) with admission by
+    projected HBM footprint against a 200 GB budget."""
+    import numpy as np
+
+    import dvmpp_amd as dvm
+    from dvmpp_amd.batch import batch_config
+    from dvmpp_amd.serve import Request, Server
+    from dvmpp_amd.workloads import request_trace
+    trace = request_trace(count, rate=rate, steps_range=steps_range)
+    reqs = [Request(i, t, px, pz, sp, seed, st) for i, (t, (px, pz, sp, seed, st)) in enumerate(trace)]
+    srv = Server(ctx, batch_config(slots=slots, slot_capacity=1 << 18, max_panels=512,
+                                   prec=dvm.F32C, use_graph=True), budget_bytes=200 << 30)
+    done, wall = srv.run(reqs)
+    srv.close()
+    lat = np.array([r.latency for r in done])
+    steps = sum(r.steps for r in done if r.status == 0)
+    log(f"served: {len(done)} requests in {wall:.2f} s")
+    return {"workload": f"{count} requests, Poisson {rate}/s, ensemble bodies, steps ~ "
+                        f"U{{{steps_range[0]}..{steps_range[1]}}}, {slots} slots, 200 GB budget",
+            "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "aggregate_steps_per_s": steps / wall, "wall_s": wall,
+            "completed_ok": sum(1 for r in done if r.status == 0)}
+
+
 def load_traffic(kernel):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -108,7 +242,17 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (reference kernel "
                          "for scaling HBM byte counters)")
+    ap.add_argument("--phases", default="sum,cylinder,ensemble,served",
+                    help="comma list: sum (isolated sum, always run), cylinder, ensemble, served")
+    ap.add_argument("--cylinder-steps", type=int, default=400)
+    ap.add_argument("--cylinder-cpu-steps", type=int, default=40)
+    ap.add_argument("--ensemble-count", type=int, default=128)
+    ap.add_argument("--ensemble-steps", type=int, default=40)
+    ap.add_argument("--ensemble-plain", type=int, default=32)
+    ap.add_argument("--served-count", type=int, default=24)
+    ap.add_argument("--served-steps", default="40,160")
     args = ap.parse_args()
+    phases = set(args.phases.split(","))
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -200,8 +344,21 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         line["copy_calibration_bytes"] = nbytes
+    log(f"isolated sum: FP32C {ms_step:.2f} ms/step, FP64 {el64 / args.f64_steps * 1e3:.2f} ms/step")
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(x, z, c, s, args.cpu_seconds)
+        log("cpu baseline done")
+    del dev, o32, o64
+    torch.cuda.empty_cache()
+    if "cylinder" in phases:
+        line["sim_steps_plain_cylinder"] = plain_cylinder_phase(ctx, args.cylinder_steps,
+                                                                args.cylinder_cpu_steps)
+    if "ensemble" in phases:
+        line["ensemble"] = ensemble_phase(ctx, args.ensemble_count, args.ensemble_steps,
+                                          args.ensemble_plain)
+    if "served" in phases:
+        lo, hi = (int(v) for v in args.served_steps.split(","))
+        line["served"] = served_phase(ctx, args.served_count, 20.0, (lo, hi), slots=32)
     ctx.close()
     print(json.dumps(line), flush=True)
 

@@ -45,11 +45,16 @@ def time_form(ctx, dvm, dev, prec, steps, warmup, outs):
     X, Z, C, S, TX, TZ = dev
     (u, w), (pu, pw) = outs
 
+    variants = {}
+
     def step():
         dvm.bs_self(ctx, X, Z, C, S, prec=prec, out=(u, w))
+        variants["self"] = ctx.f32c_variant()
         dvm.bs_targets(ctx, X, Z, C, S, TX, TZ, prec=prec, out=(pu, pw))
+        variants["targets"] = ctx.f32c_variant()
 
-    for _ in range(warmup):
+    # at least one untimed pass: under measured tuning it picks the variants
+    for _ in range(max(warmup, 1)):
         step()
     torch.cuda.synchronize()
     ctx.prof_reset()
@@ -63,7 +68,7 @@ def time_form(ctx, dvm, dev, prec, steps, warmup, outs):
     name = "f32c" if prec == dvm.F32C else "f64"
     self_ms, self_n = ctx.prof_read("bs_self_" + name)
     tgt_ms, tgt_n = ctx.prof_read("bs_targets_" + name)
-    return elapsed, (self_ms / max(self_n, 1)), (tgt_ms / max(tgt_n, 1)), self_n
+    return elapsed, (self_ms / max(self_n, 1)), (tgt_ms / max(tgt_n, 1)), variants
 
 
 def cpu_baseline(x, z, c, s, budget_s):
@@ -239,6 +244,9 @@ def main():
     ap.add_argument("--f64-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tune", choices=("measure", "table"), default="measure",
+                    help="F32C launch variant: timed on first use per size (measure) or the "
+                         "built-in size table")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (reference kernel "
                          "for scaling HBM byte counters)")
@@ -279,7 +287,8 @@ def main():
     pairs_probe = npr * n
     pairs = pairs_self + pairs_probe
 
-    el32, self32_ms, tgt32_ms, launches = time_form(ctx, dvm, dev, dvm.F32C, args.steps,
+    ctx.set_tuning(dvm.TUNE_MEASURE if args.tune == "measure" else dvm.TUNE_TABLE)
+    el32, self32_ms, tgt32_ms, variants = time_form(ctx, dvm, dev, dvm.F32C, args.steps,
                                                     args.warmup, o32)
     el64, self64_ms, tgt64_ms, _ = time_form(ctx, dvm, dev, dvm.F64, args.f64_steps, 1, o64)
 
@@ -311,6 +320,9 @@ def main():
                         "x~U[-1,15], z~U[-3,3], circ~N(0,2e-4) mean-removed, sigma=0.01",
             "n_blobs": n, "n_probes": npr, "pairs_per_step": pairs,
             "form": "FP32 pair terms, tile-recentred Morton-sorted sources, Kahan across tiles",
+            "f32c_variant": {k: dict(zip(("tile", "block", "share", "occupancy"), v))
+                             for k, v in variants.items()},
+            "tuning": args.tune,
         },
         "roofline": {
             "bound": "valu",
@@ -350,6 +362,9 @@ def main():
         log("cpu baseline done")
     del dev, o32, o64
     torch.cuda.empty_cache()
+    # The simulation phases grow N every step: they use the size table, so no
+    # variant timing runs inside their timed windows.
+    ctx.set_tuning(dvm.TUNE_TABLE)
     if "cylinder" in phases:
         line["sim_steps_plain_cylinder"] = plain_cylinder_phase(ctx, args.cylinder_steps,
                                                                 args.cylinder_cpu_steps)

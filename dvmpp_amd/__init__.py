@@ -7,9 +7,15 @@ this module only moves torch device tensors across that boundary.
 import ctypes
 import weakref
 
-from ._lib import F32C, F64, DvmError, check, load  # noqa: F401
+from ._lib import F32C, F64, TUNE_MEASURE, TUNE_TABLE, DvmError, check, load  # noqa: F401
 
-__all__ = ["F64", "F32C", "DvmError", "Context", "bs_self", "bs_targets", "load"]
+__all__ = ["F64", "F32C", "TUNE_TABLE", "TUNE_MEASURE", "DvmError", "Context", "bs_self",
+           "bs_targets", "load"]
+
+# Compiled F32C variants (source tile, target block, sources per reciprocal)
+# and occupancy targets, as in dvmpp_amd/csrc/bs_kernels.hip.
+F32C_VARIANTS = [(t, b, s) for t in (128, 256) for b in (64, 256) for s in (1, 2, 4)]
+F32C_OCCUPANCY = (2, 4, 8)
 
 
 def _ptr(t):
@@ -87,6 +93,21 @@ class Context:
         check(self.h, load().dvm_prof_read(self.h, kernel.encode(), ctypes.byref(ms),
                                            ctypes.byref(n)))
         return ms.value, n.value
+
+    def set_tuning(self, mode):
+        """TUNE_TABLE (fixed F32C variant per size) or TUNE_MEASURE (time
+        every variant on first use per size bucket, keep the fastest)."""
+        check(self.h, load().dvm_set_tuning(self.h, mode))
+
+    def set_f32c_variant(self, tile=0, block=0, share=0, occupancy=0):
+        """Force an F32C variant; tile=0 clears the forced choice."""
+        check(self.h, load().dvm_set_f32c_variant(self.h, tile, block, share, occupancy))
+
+    def f32c_variant(self):
+        """(tile, block, share, occupancy) of the last F32C call."""
+        out = (ctypes.c_int32 * 4)()
+        check(self.h, load().dvm_get_f32c_variant(self.h, out))
+        return tuple(out)
 
 
 def bs_self(ctx, x, z, circ, sigma, prec=F64, out=None):

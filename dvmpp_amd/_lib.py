@@ -56,7 +56,13 @@ SIGNATURES = {
                              _vp, _vp]),
     "dvm_bs_self_host": (_st, [_vp, _int, _i64, _vp, _vp]),
     "dvm_bs_targets_host": (_st, [_vp, _int, _i64, _vp, _i64, _vp, _vp, _vp]),
+    "dvm_set_tuning": (_st, [_vp, _int]),
+    "dvm_set_f32c_variant": (_st, [_vp, _int, _int, _int, _int]),
+    "dvm_get_f32c_variant": (_st, [_vp, ctypes.POINTER(ctypes.c_int32)]),
 }
+
+TUNE_TABLE = 0
+TUNE_MEASURE = 1
 
 class SheetParams(ctypes.Structure):
     """dvm_sheet_params."""

@@ -10,17 +10,21 @@
 //           too few targets to fill 256 CUs the source range is split into
 //           chunks whose partial sums are added in chunk order.
 //
-//  DVM_F32C (fast form) — sources are Morton-sorted, cut into tiles of 256 and
-//           stored as FP32 offsets from their tile centre (FP64).  A target
-//           block (256 targets) is likewise recentred on its own centre, so a
-//           pair's dx is formed from two small FP32 offsets and one per-tile
-//           FP32 centre difference: the cancellation that plain FP32 would
-//           suffer at |x| ~ 15 with 0.01 cores is avoided.  A tile pair whose
-//           bounding boxes are farther apart than sqrt(10)*max(sigma) takes
-//           the far-field loop (1/r^2 only: 7 VALU ops + 1 reciprocal per
-//           pair); the others take the full regularised loop.  Partial sums
-//           within a tile accumulate in FP32 with FMA; tile partials are added
-//           with Kahan compensation and the result is returned in FP64.
+//  DVM_F32C (fast form) — sources are Morton-sorted, cut into tiles (128 or
+//           256) and stored as FP32 offsets from their tile centre (FP64).
+//           For each source tile every target is recentred on that centre in
+//           FP64 and rounded once, so a pair's dx is the difference of two
+//           small FP32 offsets: the cancellation that plain FP32 would suffer
+//           at |x| ~ 15 with 0.01 cores is avoided.  A tile pair whose
+//           bounding boxes (source tile, target block of 64 or 256) are
+//           farther apart than sqrt(10)*max(sigma) takes the far-field loop
+//           (1/r^2 only: 7 VALU ops per pair and one reciprocal per 1, 2 or
+//           4 pairs); the others take the full regularised loop with dx in
+//           FP64.  Partial sums within a tile accumulate in FP32 with FMA;
+//           tile partials are added with Kahan compensation and the result
+//           is returned in FP64.  The variant (tile, block, sources per
+//           reciprocal, occupancy target) comes from a size table or is
+//           measured on first use (dvm_set_tuning).
 #include "dvm_device.hpp"
 #include "dvm_internal.hpp"
 
@@ -31,8 +35,7 @@
 
 namespace dvmi {
 
-constexpr int kBlock = 256;  // targets per block (one per lane, 4 waves)
-constexpr int kTile = 256;   // sources per F32C tile
+constexpr int kBlock = 256;  // targets per FP64 block (one per lane, 4 waves)
 
 // ----------------------------------------------------------------- FP64 form
 
@@ -185,18 +188,19 @@ __global__ void morton_kernel(int64_t n, const double* __restrict__ x, const dou
     idx[i] = (uint32_t)i;
 }
 
-// One block per tile of kTile consecutive (possibly permuted) points: FP64
-// centre and half extents; for sources also the FP32 offsets from the centre
-// with circulation, the FP32 near-field constants and an FP64 sorted copy of
-// the positions (read by the near-field loop only).
-__global__ __launch_bounds__(kTile) void tile_build_kernel(
+// One block per tile of T consecutive (possibly permuted) points: FP64 centre
+// and half extents; for sources also the FP32 offsets from the centre with
+// circulation, the FP32 near-field constants and an FP64 sorted copy of the
+// positions (read by the near-field loop only).
+template <int T>
+__global__ __launch_bounds__(T) void tile_build_kernel(
     int64_t n, const uint32_t* __restrict__ perm, const double* __restrict__ x,
     const double* __restrict__ z, const double* __restrict__ c, const double* __restrict__ s,
     float4* __restrict__ p32, float2* __restrict__ near32, double2* __restrict__ p64,
     TileInfo* __restrict__ tiles)
 {
-    __shared__ double red[6][kTile];
-    const int64_t k = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    __shared__ double red[5][T];
+    const int64_t k = (int64_t)blockIdx.x * T + threadIdx.x;
     const bool valid = k < n;
     const int64_t src = valid ? (perm ? (int64_t)perm[k] : k) : 0;
     const double xi = valid ? x[src] : 0.0;
@@ -208,7 +212,7 @@ __global__ __launch_bounds__(kTile) void tile_build_kernel(
     red[3][threadIdx.x] = valid ? zi : -INFINITY;
     red[4][threadIdx.x] = si * si;
     __syncthreads();
-    for (int st = kTile / 2; st > 0; st >>= 1) {
+    for (int st = T / 2; st > 0; st >>= 1) {
         if ((int)threadIdx.x < st) {
             const int o = threadIdx.x + st;
             red[0][threadIdx.x] = fmin(red[0][threadIdx.x], red[0][o]);
@@ -239,13 +243,95 @@ __global__ __launch_bounds__(kTile) void tile_build_kernel(
     p64[k] = make_double2(xi, zi);
 }
 
-// Targets: SELF -> the sorted FP64 copy p64 (target i is source i);
-// otherwise tx, tz in the caller's order.  tblk: bounding box of each target
-// block (the near/far test).  Per source tile each target is recentred on the
-// tile centre in FP64 and rounded once, so a far pair's dx = (x_i - c) - (x_j - c)
-// carries only FP32 rounding relative to |x - c|; near pairs form dx in FP64.
-template <bool SELF, bool GUARD>
-__global__ __launch_bounds__(kBlock) void bs_f32c_kernel(
+// Far-field loops over one source tile: sum of circ_j * (dz, dx) / r^2.
+// The shared forms take one reciprocal for 2 (or 4) sources:
+// 1/ra = rb * (1/(ra rb)); they are used only when every r^2 of the tile pair
+// lies in a range where the product stays a normal FP32 number.
+
+__device__ __forceinline__ void far_pair(float4 sj, float txd, float tzd, float& au, float& aw)
+{
+    const float dx = txd - sj.x;
+    const float dz = tzd - sj.y;
+    const float r2 = __builtin_fmaf(dx, dx, dz * dz);
+    const float kk = __builtin_amdgcn_rcpf(r2) * sj.z;
+    au = __builtin_fmaf(kk, dz, au);
+    aw = __builtin_fmaf(kk, dx, aw);
+}
+
+template <int T>
+__device__ __forceinline__ void far_tile(const float4* __restrict__ sp, float txd, float tzd,
+                                         float& au, float& aw)
+{
+#pragma unroll 16
+    for (int k = 0; k < T; ++k) far_pair(sp[k], txd, tzd, au, aw);
+}
+
+template <int T>
+__device__ __forceinline__ void far_tile_share2(const float4* __restrict__ sp, float txd,
+                                                float tzd, float& au, float& aw)
+{
+#pragma unroll 8
+    for (int k = 0; k < T; k += 2) {
+        const float4 sa = sp[k];
+        const float4 sb = sp[k + 1];
+        const float dxa = txd - sa.x, dza = tzd - sa.y;
+        const float dxb = txd - sb.x, dzb = tzd - sb.y;
+        const float ra = __builtin_fmaf(dxa, dxa, dza * dza);
+        const float rb = __builtin_fmaf(dxb, dxb, dzb * dzb);
+        const float inv = __builtin_amdgcn_rcpf(ra * rb);
+        const float ka = (rb * inv) * sa.z;
+        const float kb = (ra * inv) * sb.z;
+        au = __builtin_fmaf(ka, dza, au);
+        aw = __builtin_fmaf(ka, dxa, aw);
+        au = __builtin_fmaf(kb, dzb, au);
+        aw = __builtin_fmaf(kb, dxb, aw);
+    }
+}
+
+template <int T>
+__device__ __forceinline__ void far_tile_share4(const float4* __restrict__ sp, float txd,
+                                                float tzd, float& au, float& aw)
+{
+#pragma unroll 4
+    for (int k = 0; k < T; k += 4) {
+        const float4 sa = sp[k], sb = sp[k + 1], sc = sp[k + 2], sd = sp[k + 3];
+        const float dxa = txd - sa.x, dza = tzd - sa.y;
+        const float dxb = txd - sb.x, dzb = tzd - sb.y;
+        const float dxc = txd - sc.x, dzc = tzd - sc.y;
+        const float dxd = txd - sd.x, dzd = tzd - sd.y;
+        const float ra = __builtin_fmaf(dxa, dxa, dza * dza);
+        const float rb = __builtin_fmaf(dxb, dxb, dzb * dzb);
+        const float rc = __builtin_fmaf(dxc, dxc, dzc * dzc);
+        const float rd = __builtin_fmaf(dxd, dxd, dzd * dzd);
+        const float pab = ra * rb;
+        const float pcd = rc * rd;
+        const float inv = __builtin_amdgcn_rcpf(pab * pcd);
+        const float iab = inv * pcd;
+        const float icd = inv * pab;
+        const float ka = (rb * iab) * sa.z;
+        const float kb = (ra * iab) * sb.z;
+        const float kc = (rd * icd) * sc.z;
+        const float kd = (rc * icd) * sd.z;
+        au = __builtin_fmaf(ka, dza, au);
+        aw = __builtin_fmaf(ka, dxa, aw);
+        au = __builtin_fmaf(kb, dzb, au);
+        aw = __builtin_fmaf(kb, dxb, aw);
+        au = __builtin_fmaf(kc, dzc, au);
+        aw = __builtin_fmaf(kc, dxc, aw);
+        au = __builtin_fmaf(kd, dzd, au);
+        aw = __builtin_fmaf(kd, dxd, aw);
+    }
+}
+
+// TILE sources per source tile, BLOCK targets per block (one per lane),
+// SHARE sources per far-field reciprocal.  Targets: SELF -> the sorted FP64
+// copy p64 (target i is source i); otherwise tx, tz in the caller's order.
+// tblk: bounding box of each target block (the near/far test).  Per source
+// tile each target is recentred on the tile centre in FP64 and rounded once,
+// so a far pair's dx = (x_i - c) - (x_j - c) carries only FP32 rounding
+// relative to |x - c|; near pairs form dx in FP64.
+template <int TILE, int BLOCK, int SHARE, bool SELF, bool GUARD>
+__global__ __launch_bounds__(BLOCK) void bs_f32c_kernel(
     int64_t nt, const double* __restrict__ tx, const double* __restrict__ tz,
     const TileInfo* __restrict__ tblk, int64_t ns, const float4* __restrict__ s32,
     const float2* __restrict__ near32, const double2* __restrict__ p64,
@@ -253,7 +339,9 @@ __global__ __launch_bounds__(kBlock) void bs_f32c_kernel(
     const uint32_t* __restrict__ out_index, double* __restrict__ u, double* __restrict__ w,
     double* __restrict__ part)
 {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    static_assert(TILE % 16 == 0 && BLOCK % 64 == 0, "tile and block shapes");
+    static_assert(SHARE == 1 || SHARE == 2 || SHARE == 4, "sources per reciprocal");
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     const int64_t it = i < nt ? i : nt - 1;
     double xi, zi;
     if (SELF) {
@@ -281,32 +369,29 @@ __global__ __launch_bounds__(kBlock) void bs_f32c_kernel(
         const bool near = gx * gx + gz * gz <= (GUARD ? fmax(lim, 1.0001 * (double)r2_skip) : lim);
         const float txd = (float)(xi - ts.cx);
         const float tzd = (float)(zi - ts.cz);
-        const int64_t jb = (int64_t)st * kTile;
-        const int cnt = ns - jb < kTile ? (int)(ns - jb) : kTile;
+        const int64_t jb = (int64_t)st * TILE;
+        const int cnt = ns - jb < TILE ? (int)(ns - jb) : TILE;
         const float4* __restrict__ sp = s32 + jb;
         float au = 0.0f, aw = 0.0f;
         if (!near) {
-            if (cnt == kTile) {
-#pragma unroll 16
-                for (int k = 0; k < kTile; ++k) {
-                    const float4 sj = sp[k];
-                    const float dx = txd - sj.x;
-                    const float dz = tzd - sj.y;
-                    const float r2 = __builtin_fmaf(dx, dx, dz * dz);
-                    const float kk = __builtin_amdgcn_rcpf(r2) * sj.z;
-                    au = __builtin_fmaf(kk, dz, au);
-                    aw = __builtin_fmaf(kk, dx, aw);
-                }
+            bool shared = false;
+            if (SHARE > 1) {
+                // r^2 of every pair lies in [gap^2, ext^2]; a product of
+                // SHARE of them stays within [1e-34, 1e34].
+                const double ex = fabs(Dxd) + (double)(tb.hx + ts.hx);
+                const double ez = fabs(Dzd) + (double)(tb.hz + ts.hz);
+                const double lo = SHARE == 2 ? 1e-17 : 1e-8;
+                const double hi = SHARE == 2 ? 1e17 : 1e8;
+                shared = cnt == TILE && gx * gx + gz * gz > lo && ex * ex + ez * ez < hi;
+            }
+            if (SHARE == 2 && shared) {
+                far_tile_share2<TILE>(sp, txd, tzd, au, aw);
+            } else if (SHARE == 4 && shared) {
+                far_tile_share4<TILE>(sp, txd, tzd, au, aw);
+            } else if (cnt == TILE) {
+                far_tile<TILE>(sp, txd, tzd, au, aw);
             } else {
-                for (int k = 0; k < cnt; ++k) {
-                    const float4 sj = sp[k];
-                    const float dx = txd - sj.x;
-                    const float dz = tzd - sj.y;
-                    const float r2 = __builtin_fmaf(dx, dx, dz * dz);
-                    const float kk = __builtin_amdgcn_rcpf(r2) * sj.z;
-                    au = __builtin_fmaf(kk, dz, au);
-                    aw = __builtin_fmaf(kk, dx, aw);
-                }
+                for (int k = 0; k < cnt; ++k) far_pair(sp[k], txd, tzd, au, aw);
             }
         } else {
             const float2* __restrict__ np = near32 + jb;
@@ -390,14 +475,99 @@ static void launch_f64(dvm_ctx_t ctx, bool self, int64_t ns, const double* sx, c
     }
 }
 
-static void launch_f32c(dvm_ctx_t ctx, bool self, int64_t ns, const double* sx, const double* sz,
-                        const double* sc, const double* ss, int64_t nt, const double* tx,
-                        const double* tz, int guard, double r2_skip, double* u, double* w)
+// Arguments of one F32C pair-sum launch (host side).
+struct F32cArgs {
+    int64_t nt;
+    const double* tx;
+    const double* tz;
+    const TileInfo* tblk;
+    int64_t ns;
+    const float4* s32;
+    const float2* near32;
+    const double2* p64;
+    const TileInfo* stile;
+    int ntiles;
+    int tpc;
+    float r2_skip;
+    const uint32_t* out_index;
+    double* u;
+    double* w;
+    double* part;
+};
+
+// kind: 0 blob-on-blob, 1 targets, 2 targets with the r2_skip guard.
+template <int TILE, int BLOCK, int SHARE>
+static void f32c_dispatch(int kind, dim3 grid, hipStream_t st, const F32cArgs& a)
 {
-    const int ntiles = (int)cdiv(ns, kTile);
-    const int64_t bx = cdiv(nt, kBlock);
+#define DVM_F32C_ARGS                                                                        \
+    a.nt, a.tx, a.tz, a.tblk, a.ns, a.s32, a.near32, a.p64, a.stile, a.ntiles, a.tpc,        \
+        a.r2_skip, a.out_index, a.u, a.w, a.part
+    if (kind == 0)
+        bs_f32c_kernel<TILE, BLOCK, SHARE, true, false><<<grid, BLOCK, 0, st>>>(DVM_F32C_ARGS);
+    else if (kind == 2)
+        bs_f32c_kernel<TILE, BLOCK, SHARE, false, true><<<grid, BLOCK, 0, st>>>(DVM_F32C_ARGS);
+    else
+        bs_f32c_kernel<TILE, BLOCK, SHARE, false, false><<<grid, BLOCK, 0, st>>>(DVM_F32C_ARGS);
+#undef DVM_F32C_ARGS
+}
+
+template <int T>
+static void tile_build(hipStream_t st, int64_t n, const uint32_t* perm, const double* x,
+                       const double* z, const double* c, const double* s, float4* p32,
+                       float2* near32, double2* p64, TileInfo* tiles)
+{
+    tile_build_kernel<T><<<(unsigned)cdiv(n, T), T, 0, st>>>(n, perm, x, z, c, s, p32, near32,
+                                                             p64, tiles);
+}
+
+using F32cFn = void (*)(int, dim3, hipStream_t, const F32cArgs&);
+using TileFn = void (*)(hipStream_t, int64_t, const uint32_t*, const double*, const double*,
+                        const double*, const double*, float4*, float2*, double2*, TileInfo*);
+
+struct F32cKernel {
+    int tile, block, share;
+    F32cFn fn;
+};
+
+// The compiled variants: source tile x target block x sources per reciprocal.
+static const F32cKernel kF32c[] = {
+    {128, 64, 1, f32c_dispatch<128, 64, 1>},   {128, 64, 2, f32c_dispatch<128, 64, 2>},
+    {128, 64, 4, f32c_dispatch<128, 64, 4>},   {128, 256, 1, f32c_dispatch<128, 256, 1>},
+    {128, 256, 2, f32c_dispatch<128, 256, 2>}, {128, 256, 4, f32c_dispatch<128, 256, 4>},
+    {256, 64, 1, f32c_dispatch<256, 64, 1>},   {256, 64, 2, f32c_dispatch<256, 64, 2>},
+    {256, 64, 4, f32c_dispatch<256, 64, 4>},   {256, 256, 1, f32c_dispatch<256, 256, 1>},
+    {256, 256, 2, f32c_dispatch<256, 256, 2>}, {256, 256, 4, f32c_dispatch<256, 256, 4>},
+};
+constexpr int kNumF32c = sizeof(kF32c) / sizeof(kF32c[0]);
+constexpr int kOccupancy[] = {2, 4, 8};  // target resident blocks per CU for source splitting
+
+static TileFn tile_fn(int t)
+{
+    return t == 64 ? tile_build<64> : t == 128 ? tile_build<128> : tile_build<256>;
+}
+
+static int f32c_find(int tile, int block, int share)
+{
+    for (int k = 0; k < kNumF32c; ++k)
+        if (kF32c[k].tile == tile && kF32c[k].block == block && kF32c[k].share == share) return k;
+    return -1;
+}
+
+// One complete F32C evaluation with variant v (index into kF32c) and occupancy
+// target occ: Morton sort of the sources, tile build, pair sum, chunk reduce.
+static void f32c_run(dvm_ctx_t ctx, int kind, int v, int occ, int64_t ns, const double* sx,
+                     const double* sz, const double* sc, const double* ss, int64_t nt,
+                     const double* tx, const double* tz, double r2_skip, double* u, double* w)
+{
+    const F32cKernel& K = kF32c[v];
+    const bool self = kind == 0;
+    const int ntiles = (int)cdiv(ns, K.tile);
+    const int64_t bx = cdiv(nt, K.block);
     const int nbb = 64;
-    const int nch = pick_chunks(ctx, bx, ntiles);
+    const int64_t want = (int64_t)occ * ctx->num_cus * 256 / K.block;
+    int64_t nch64 = (want + bx - 1) / bx;
+    nch64 = nch64 > ntiles ? ntiles : (nch64 < 1 ? 1 : nch64);
+    const int nch = (int)nch64;
     const int tpc = (ntiles + nch - 1) / nch;
 
     size_t sort_tmp = 0;
@@ -429,38 +599,148 @@ static void launch_f32c(dvm_ctx_t ctx, bool self, int64_t ns, const double* sx, 
     DVM_LAUNCH_CHECK();
     DVM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, sort_tmp, keys, keys2, idx, perm,
                                                      (int)ns, 0, 32, ctx->stream));
-    tile_build_kernel<<<ntiles, kTile, 0, ctx->stream>>>(ns, perm, sx, sz, sc, ss, s32, near32,
-                                                         p64, stile);
+    tile_fn(K.tile)(ctx->stream, ns, perm, sx, sz, sc, ss, s32, near32, p64, stile);
     DVM_LAUNCH_CHECK();
-    const TileInfo* tb = stile;
-    if (!self) {
-        tile_build_kernel<<<(unsigned)bx, kTile, 0, ctx->stream>>>(
-            nt, nullptr, tx, tz, nullptr, nullptr, nullptr, nullptr, nullptr, tblk);
-        DVM_LAUNCH_CHECK();
-        tb = tblk;
-    }
-    const float skip = (float)r2_skip;
-    const uint32_t* oidx = self ? perm : nullptr;
+    // Target blocks: the sorted sources themselves (self) or the targets in
+    // the caller's order; bounding boxes only.
+    tile_fn(K.block)(ctx->stream, nt, self ? perm : nullptr, self ? sx : tx, self ? sz : tz,
+                     nullptr, nullptr, nullptr, nullptr, nullptr, tblk);
+    DVM_LAUNCH_CHECK();
+    F32cArgs a{nt,    self ? nullptr : tx, self ? nullptr : tz, tblk, ns, s32, near32, p64, stile,
+               ntiles, tpc, (float)r2_skip,   self ? perm : nullptr, u, w, part};
     dim3 grid((unsigned)bx, (unsigned)nch);
     {
         ProfScope ps(ctx, self ? "bs_self_f32c" : "bs_targets_f32c");
-        if (self) {
-            bs_f32c_kernel<true, false><<<grid, kBlock, 0, ctx->stream>>>(
-                nt, nullptr, nullptr, tb, ns, s32, near32, p64, stile, ntiles, tpc, skip, oidx,
-                u, w, part);
-        } else if (guard) {
-            bs_f32c_kernel<false, true><<<grid, kBlock, 0, ctx->stream>>>(
-                nt, tx, tz, tb, ns, s32, near32, p64, stile, ntiles, tpc, skip, oidx, u, w, part);
-        } else {
-            bs_f32c_kernel<false, false><<<grid, kBlock, 0, ctx->stream>>>(
-                nt, tx, tz, tb, ns, s32, near32, p64, stile, ntiles, tpc, skip, oidx, u, w, part);
-        }
+        K.fn(kind, grid, ctx->stream, a);
         DVM_LAUNCH_CHECK();
     }
     if (part) {
-        bs_reduce_kernel<<<cdiv(nt, 256), 256, 0, ctx->stream>>>(nt, nch, part, oidx, u, w);
+        bs_reduce_kernel<<<cdiv(nt, 256), 256, 0, ctx->stream>>>(nt, nch, part, a.out_index, u,
+                                                                 w);
         DVM_LAUNCH_CHECK();
     }
+}
+
+static int lg2_bucket(int64_t n)
+{
+    int b = 0;
+    while ((int64_t(1) << b) < n) ++b;
+    return b;
+}
+
+// Fixed choice (DVM_TUNE_TABLE), from sweeps of every variant on MI355X
+// (uniform and wake-like blob sets, 4K..1M blobs; 257 and 1024 targets):
+// small sums favour few, wide blocks; probe and collocation sums 1-reciprocal
+// tiles of 128; large sums 64-target blocks (tight near/far boxes), 4
+// sources per reciprocal and the source range split to 8 blocks per CU.
+static void f32c_table(int kind, int64_t nt, int64_t ns, int& v, int& occ)
+{
+    if ((double)nt * (double)ns < 6.7e7) {
+        v = f32c_find(128, 256, 2);
+        occ = 2;
+    } else if (kind != 0 && nt < 8192) {
+        v = f32c_find(128, 64, 1);
+        occ = 2;
+    } else if (ns >= 196608) {
+        v = f32c_find(256, 64, 4);
+        occ = 8;
+    } else {
+        v = f32c_find(128, 64, 4);
+        occ = 8;
+    }
+}
+
+// DVM_TUNE_MEASURE: time every variant once at occupancy 4 (after an untimed
+// run), then the best one at the other occupancy targets; keep the fastest.
+static void f32c_measure(dvm_ctx_t ctx, int kind, int64_t ns, const double* sx, const double* sz,
+                         const double* sc, const double* ss, int64_t nt, const double* tx,
+                         const double* tz, double r2_skip, double* u, double* w, int& v_best,
+                         int& occ_best)
+{
+    hipEvent_t e0, e1;
+    DVM_HIP_CHECK(hipEventCreate(&e0));
+    DVM_HIP_CHECK(hipEventCreate(&e1));
+    const bool prof = ctx->prof;
+    ctx->prof = false;
+    float best = 3.0e38f;
+    auto time_one = [&](int v, int occ) {
+        f32c_run(ctx, kind, v, occ, ns, sx, sz, sc, ss, nt, tx, tz, r2_skip, u, w);
+        float t = 3.0e38f;
+        for (int rep = 0; rep < 2; ++rep) {
+            DVM_HIP_CHECK(hipEventRecord(e0, ctx->stream));
+            f32c_run(ctx, kind, v, occ, ns, sx, sz, sc, ss, nt, tx, tz, r2_skip, u, w);
+            DVM_HIP_CHECK(hipEventRecord(e1, ctx->stream));
+            DVM_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            DVM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            t = ms < t ? ms : t;
+        }
+        if (t < best) {
+            best = t;
+            v_best = v;
+            occ_best = occ;
+        }
+    };
+    try {
+        for (int v = 0; v < kNumF32c; ++v) time_one(v, 4);
+        const int v0 = v_best;
+        for (int occ : kOccupancy)
+            if (occ != 4) time_one(v0, occ);
+    } catch (...) {
+        ctx->prof = prof;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    ctx->prof = prof;
+    DVM_HIP_CHECK(hipEventDestroy(e0));
+    DVM_HIP_CHECK(hipEventDestroy(e1));
+}
+
+static void launch_f32c(dvm_ctx_t ctx, bool self, int64_t ns, const double* sx, const double* sz,
+                        const double* sc, const double* ss, int64_t nt, const double* tx,
+                        const double* tz, int guard, double r2_skip, double* u, double* w)
+{
+    const int kind = self ? 0 : (guard ? 2 : 1);
+    int v = -1, occ = 4;
+    if (ctx->f32c_forced[0] != 0) {
+        v = f32c_find(ctx->f32c_forced[0], ctx->f32c_forced[1], ctx->f32c_forced[2]);
+        occ = ctx->f32c_forced[3];
+    } else if (ctx->tuning == DVM_TUNE_MEASURE) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        DVM_HIP_CHECK(hipStreamIsCapturing(ctx->stream, &cs));
+        if (cs == hipStreamCaptureStatusNone) {
+            const auto key = std::make_tuple(kind, lg2_bucket(nt), lg2_bucket(ns));
+            auto it = ctx->f32c_tune.find(key);
+            if (it == ctx->f32c_tune.end()) {
+                f32c_measure(ctx, kind, ns, sx, sz, sc, ss, nt, tx, tz, r2_skip, u, w, v, occ);
+                it = ctx->f32c_tune.emplace(key, std::make_pair(v, occ)).first;
+            }
+            v = it->second.first;
+            occ = it->second.second;
+        }
+    }
+    if (v < 0) f32c_table(kind, nt, ns, v, occ);
+    const F32cKernel& K = kF32c[v];
+    ctx->f32c_last[0] = K.tile;
+    ctx->f32c_last[1] = K.block;
+    ctx->f32c_last[2] = K.share;
+    ctx->f32c_last[3] = occ;
+    f32c_run(ctx, kind, v, occ, ns, sx, sz, sc, ss, nt, tx, tz, r2_skip, u, w);
+}
+
+void f32c_set_variant(dvm_ctx_t ctx, int tile, int block, int share, int occupancy)
+{
+    if (tile == 0) {
+        ctx->f32c_forced[0] = 0;
+        return;
+    }
+    require(f32c_find(tile, block, share) >= 0, DVM_E_INVALID, "no such F32C variant");
+    require(occupancy >= 1 && occupancy <= 64, DVM_E_INVALID, "occupancy target out of range");
+    ctx->f32c_forced[0] = tile;
+    ctx->f32c_forced[1] = block;
+    ctx->f32c_forced[2] = share;
+    ctx->f32c_forced[3] = occupancy;
 }
 
 void bs_self_launch(dvm_ctx_t ctx, int prec, int64_t n, const double* x, const double* z,
@@ -600,6 +880,29 @@ dvm_status dvm_bs_targets_host(dvm_ctx_t ctx, int prec, int64_t ns, const double
         DVM_HIP_CHECK(hipFreeAsync(buf, ctx->stream));
         DVM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     });
+}
+
+dvm_status dvm_set_tuning(dvm_ctx_t ctx, int mode)
+{
+    if (!ctx) return DVM_E_INVALID;
+    return guard(ctx, [&] {
+        require(mode == DVM_TUNE_TABLE || mode == DVM_TUNE_MEASURE, DVM_E_INVALID,
+                "unknown tuning mode");
+        ctx->tuning = mode;
+    });
+}
+
+dvm_status dvm_set_f32c_variant(dvm_ctx_t ctx, int tile, int block, int share, int occupancy)
+{
+    if (!ctx) return DVM_E_INVALID;
+    return guard(ctx, [&] { f32c_set_variant(ctx, tile, block, share, occupancy); });
+}
+
+dvm_status dvm_get_f32c_variant(dvm_ctx_t ctx, int32_t* out4)
+{
+    if (!ctx || !out4) return DVM_E_INVALID;
+    for (int k = 0; k < 4; ++k) out4[k] = ctx->f32c_last[k];
+    return DVM_OK;
 }
 
 }  // extern "C"

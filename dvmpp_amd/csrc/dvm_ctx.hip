@@ -124,6 +124,8 @@ dvm_status dvm_ctx_create(int device, void* stream, dvm_ctx_t* out)
         hipDeviceProp_t prop;
         DVM_HIP_CHECK(hipGetDeviceProperties(&prop, device));
         ctx->num_cus = prop.multiProcessorCount;
+        const char* tune = std::getenv("DVM_TUNE");
+        if (tune && std::string(tune) == "measure") ctx->tuning = DVM_TUNE_MEASURE;
         if (stream) {
             ctx->stream = static_cast<hipStream_t>(stream);
             ctx->own_stream = false;

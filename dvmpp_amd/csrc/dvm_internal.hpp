@@ -8,6 +8,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/dvm_hip.h"
@@ -25,8 +26,14 @@ struct dvm_ctx_s {
     // Pinned host staging for small device->host reads.
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
-    // Launch-shape cache keyed by (kernel family, size bucket).
-    std::map<std::pair<int, int>, int> tune;
+    // F32C pair-sum variant choice: DVM_TUNE_TABLE or DVM_TUNE_MEASURE; a
+    // forced variant (tile, block, share, occupancy; tile 0 = none); the
+    // variant of the last call; measured choices keyed by (kind, log2 of
+    // the target count, log2 of the source count) -> (variant, occupancy).
+    int tuning = DVM_TUNE_TABLE;
+    int f32c_forced[4] = {0, 0, 0, 0};
+    int f32c_last[4] = {0, 0, 0, 0};
+    std::map<std::tuple<int, int, int>, std::pair<int, int>> f32c_tune;
     // Kernel timing (dvm_prof_*): pending event pairs and totals per kernel.
     bool prof = false;
     struct ProfEntry {
@@ -206,6 +213,7 @@ void bs_self_launch(dvm_ctx_t ctx, int prec, int64_t n, const double* x, const d
 void bs_targets_launch(dvm_ctx_t ctx, int prec, int64_t ns, const double* sx, const double* sz,
                        const double* sc, const double* ss, int64_t nt, const double* tx,
                        const double* tz, int guard, double r2_skip, double* u, double* w);
+void f32c_set_variant(dvm_ctx_t ctx, int tile, int block, int share, int occupancy);
 
 }  // namespace dvmi
 

@@ -112,6 +112,29 @@ dvm_status dvm_bs_targets_host(dvm_ctx_t ctx, int prec, int64_t ns, const double
                                int64_t nt, const double* tx, const double* tz,
                                double* results2);
 
+/* DVM_F32C launch variants: source tile (128 | 256 sources), target block
+ * (64 | 256 targets, i.e. 1 or 4 waves per source tile visit), sources per
+ * far-field reciprocal (1 | 2 | 4) and an occupancy target (resident
+ * 256-lane blocks per CU that the source range is split to reach).  Every
+ * variant meets the same accuracy bound; their rounding differs, so results
+ * are bitwise reproducible only for a fixed variant.
+ *   DVM_TUNE_TABLE   (default) a fixed variant per problem size.
+ *   DVM_TUNE_MEASURE the first call of each kind (self / targets / guarded
+ *                    targets) and size bucket (log2 of target and source
+ *                    counts) times every variant on the context's stream,
+ *                    synchronising, and the context keeps the fastest.
+ *                    Calls made while the stream is being captured use the
+ *                    table.  The environment variable DVM_TUNE=measure
+ *                    selects this mode at context creation.
+ * dvm_set_f32c_variant forces one variant (tile 0 clears it: DVM_E_INVALID
+ * for a variant that does not exist); dvm_get_f32c_variant reports the
+ * variant of the last F32C call {tile, block, share, occupancy}.  No
+ * counterpart in the reference (its kernel has no launch shape). */
+enum { DVM_TUNE_TABLE = 0, DVM_TUNE_MEASURE = 1 };
+dvm_status dvm_set_tuning(dvm_ctx_t ctx, int mode);
+dvm_status dvm_set_f32c_variant(dvm_ctx_t ctx, int tile, int block, int share, int occupancy);
+dvm_status dvm_get_f32c_variant(dvm_ctx_t ctx, int32_t* out4);
+
 /* ------------------------------------------------------------------ vortex sheet
  * One body's vortex sheet, resident on the device: panel geometry, the
  * influence matrix A ((Nl+1) x Nl), the least-squares operator

@@ -138,6 +138,98 @@ def test_f32c_targets_accuracy(gpu_ctx):
     assert norm_max <= F32C_TOL and p99 <= F32C_TOL, (norm_max, p99)
 
 
+def _mixed_blobs(n, seed):
+    """Uniform background plus a dense patch with wider cores: tile pairs of
+    both the near (regularised) and the far (1/r^2) kind; n is not a multiple
+    of any tile size, so the partial last tile is exercised."""
+    x, z, c, s = config2_blobs(n, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    k = n // 4
+    x[:k] = rng.uniform(3.0, 3.4, k)
+    z[:k] = rng.uniform(-0.2, 0.2, k)
+    s[:k] = rng.uniform(0.01, 0.03, k)
+    return x, z, c, s
+
+
+def _with_variant(ctx, variant, fn):
+    ctx.set_f32c_variant(*variant)
+    try:
+        return fn()
+    finally:
+        ctx.set_f32c_variant()
+
+
+@pytest.mark.parametrize("variant", dvm.F32C_VARIANTS)
+def test_f32c_every_variant(gpu_ctx, variant):
+    # Each compiled launch variant (tile, block, sources per reciprocal) at the
+    # smallest and largest occupancy target meets the accuracy bound for the
+    # self sum and both target sums.
+    x, z, c, s = _mixed_blobs(6001, seed=31)
+    tx = np.linspace(-1, 15, 777)
+    tz = np.linspace(-0.5, 0.5, 777)
+    uo, wo = oracle.bs_self(x, z, c, s)
+    pu, pw = oracle.bs_targets(x, z, c, s, tx, tz)
+    for occ in (dvm.F32C_OCCUPANCY[0], dvm.F32C_OCCUPANCY[-1]):
+        v = tuple(variant) + (occ,)
+        ug, wg = _with_variant(gpu_ctx, v, lambda: gpu_self(gpu_ctx, x, z, c, s, dvm.F32C))
+        norm_max, p99 = _f32c_errors(ug, wg, uo, wo)
+        assert norm_max <= F32C_TOL and p99 <= F32C_TOL, (v, norm_max, p99)
+        for guard in (True, False):
+            tg = _with_variant(gpu_ctx, v, lambda: gpu_targets(gpu_ctx, x, z, c, s, tx, tz,
+                                                               dvm.F32C, guard=guard))
+            assert gpu_ctx.f32c_variant() == v
+            norm_max, p99 = _f32c_errors(tg[0], tg[1], pu, pw)
+            assert norm_max <= F32C_TOL and p99 <= F32C_TOL, (v, guard, norm_max, p99)
+
+
+@pytest.mark.parametrize("share", [1, 2, 4])
+def test_f32c_shared_reciprocal_wide_domain(gpu_ctx, share):
+    # Separations up to ~1.7e4: r^2 products of 4 leave the range where one
+    # reciprocal may be shared (1e-34..1e34), so those tile pairs take the
+    # per-pair loop while the closer ones share.
+    rng = np.random.default_rng(5)
+    n = 1000
+    x = rng.uniform(0.0, 1.2e4, n)
+    z = rng.uniform(0.0, 1.2e4, n)
+    c = rng.normal(0.0, 1.0, n)
+    s = np.full(n, 5.0)
+    uo, wo = oracle.bs_self(x, z, c, s)
+    ug, wg = _with_variant(gpu_ctx, (128, 64, share, 4),
+                           lambda: gpu_self(gpu_ctx, x, z, c, s, dvm.F32C))
+    norm_max, p99 = _f32c_errors(ug, wg, uo, wo)
+    assert norm_max <= F32C_TOL and p99 <= F32C_TOL, (norm_max, p99)
+
+
+def test_f32c_measured_tuning(gpu_ctx):
+    # The first call of a size bucket times every variant and keeps the
+    # fastest; later calls in the bucket reuse it, and its results are the
+    # forced variant's, bit for bit.
+    x, z, c, s = _mixed_blobs(20001, seed=44)
+    gpu_ctx.set_tuning(dvm.TUNE_MEASURE)
+    try:
+        u1, w1 = gpu_self(gpu_ctx, x, z, c, s, dvm.F32C)
+        v = gpu_ctx.f32c_variant()
+        assert v[:3] in [tuple(t) for t in dvm.F32C_VARIANTS] and v[3] in dvm.F32C_OCCUPANCY
+        u2, w2 = gpu_self(gpu_ctx, x, z, c, s, dvm.F32C)
+        assert gpu_ctx.f32c_variant() == v
+    finally:
+        gpu_ctx.set_tuning(dvm.TUNE_TABLE)
+    u3, w3 = _with_variant(gpu_ctx, v, lambda: gpu_self(gpu_ctx, x, z, c, s, dvm.F32C))
+    for a, b in ((u1, u2), (w1, w2), (u1, u3), (w1, w3)):
+        assert np.array_equal(a, b)
+    uo, wo = oracle.bs_self(x, z, c, s)
+    norm_max, p99 = _f32c_errors(u1, w1, uo, wo)
+    assert norm_max <= F32C_TOL and p99 <= F32C_TOL, (norm_max, p99)
+
+
+def test_f32c_variant_rejected(gpu_ctx):
+    with pytest.raises(dvm.DvmError) as e:
+        gpu_ctx.set_f32c_variant(512, 64, 2, 4)
+    assert e.value.status == 1
+    with pytest.raises(dvm.DvmError):
+        gpu_ctx.set_tuning(5)
+
+
 def test_full_size_isolated_sum(gpu_ctx):
     # N = 262,144 (the isolated-sum workload): FP64 against the oracle on 256
     # sampled targets (computeInducedAt at a blob's own position skips only the

@@ -12,9 +12,11 @@ from ._lib import F32C, F64, TUNE_MEASURE, TUNE_TABLE, DvmError, check, load  # 
 __all__ = ["F64", "F32C", "TUNE_TABLE", "TUNE_MEASURE", "DvmError", "Context", "bs_self",
            "bs_targets", "load"]
 
-# Compiled F32C variants (source tile, target block, sources per reciprocal)
-# and occupancy targets, as in dvmpp_amd/csrc/bs_kernels.hip.
-F32C_VARIANTS = [(t, b, s) for t in (128, 256) for b in (64, 256) for s in (1, 2, 4)]
+# Compiled F32C variants (source tile, target block, sources per reciprocal;
+# 0 = matrix-core far field with 32-target boxes) and occupancy targets, as in
+# dvmpp_amd/csrc/bs_kernels.hip.
+F32C_VARIANTS = ([(t, b, s) for t in (128, 256) for b in (64, 256) for s in (1, 2, 4)]
+                 + [(128, 32, 0), (256, 32, 0)])
 F32C_OCCUPANCY = (2, 4, 8)
 
 

@@ -432,6 +432,251 @@ __global__ __launch_bounds__(BLOCK) void bs_f32c_kernel(
     }
 }
 
+// ------------------------------------------------- F32C, matrix-core far field
+//
+// A wave owns 32 targets (8 quads of 4); a block of 4 waves stages each
+// source tile once into LDS (double-buffered, one barrier per tile).  For a
+// tile that is far from the wave's target box and well separated from it
+// (gap >= kappa * the larger of the tile's half-diagonal and the reach of the
+// wave's box from the block's reference point), the tile's pair sums go
+// through v_mfma_f32_4x4x1f32 (16 blocks of 4x4, K = 1; each step rounds
+// once, like fmaf):
+//   (1) r^2 for 64 sources x 4 targets: four accumulating K = 1 steps of
+//       r^2 = |t'|^2 + |s'|^2 - 2 t'.s' with t', s' relative to the source
+//       tile centre (A: source features, lane = source; B: target features,
+//       lane & 3 = target);
+//   (2) W = 1/r^2 on the VALU, in place in the result registers;
+//   (3) sum_j W * circ_j * (1, x_j - c, z_j - c) with c the block's
+//       reference point: four K = 1 steps per 64 sources, A = W (register r
+//       holds source 4*(lane/4) + r, lane & 3 = target), B = the staged
+//       features (lane & 3 = feature).  Results stay in 16 per-block
+//       partials that are folded into FP64 accumulators after each tile and
+//       summed over the blocks at the end: u = -(z'' F0 - F2),
+//       w = x'' F0 - F1 with x'', z'' the target's offset from c.
+// The separation bounds the rounding of both products to a few FP32 units
+// relative to the pair term.  Other tiles take the VALU loops of the vector
+// form, each lane covering one target and half of the tile's sources.
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int kMT = 32;       // targets per wave
+constexpr int kMQ = kMT / 4;  // target quads per wave
+// Matrix form only for gap^2 >= kKappa2 * reach^2: with |t'| <= r + reach and
+// |s'| <= reach the products round to within ~(1 + 2/kappa)^2 FP32 units.
+constexpr float kKappa2 = 4.0f;
+
+template <int TILE, bool SELF, bool GUARD>
+__global__ __launch_bounds__(256) void bs_f32m_kernel(
+    int64_t nt, const double* __restrict__ tx, const double* __restrict__ tz,
+    const TileInfo* __restrict__ tgrp, int64_t ns, const float4* __restrict__ s32,
+    const float2* __restrict__ near32, const double2* __restrict__ p64,
+    const TileInfo* __restrict__ stile, int ntiles, int tiles_per_chunk, float r2_skip,
+    float kappa2, const uint32_t* __restrict__ out_index, double* __restrict__ u,
+    double* __restrict__ w, double* __restrict__ part)
+{
+    static_assert(TILE == 128 || TILE == 256, "a tile is 2 or 4 groups of 64 sources");
+    __shared__ float4 la[2][TILE];         // (-2x', -2z', |s'|^2, 0) per source
+    __shared__ float lf[2][4][TILE + 1];   // circ * (1, x - c, z - c, 0), feature-major
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int64_t grp = (int64_t)blockIdx.x * 4 + (tid >> 6);
+    const int64_t t0 = grp * kMT;
+    const bool active = t0 < nt;  // wave-uniform; idle waves still stage and synchronise
+    const int c4 = lane & 3;
+    const int tl = lane & 31;
+    const int half = lane >> 5;
+
+    for (int k = tid; k < 2 * (TILE + 1); k += 256) lf[k / (TILE + 1)][3][k % (TILE + 1)] = 0.0f;
+
+    auto target = [&](int64_t idx, double& x, double& z) {
+        idx = idx < nt ? idx : nt - 1;
+        if (SELF) {
+            const double2 p = p64[idx];
+            x = p.x;
+            z = p.y;
+        } else {
+            x = tx[idx];
+            z = tz[idx];
+        }
+    };
+    const TileInfo tref = tgrp[(int64_t)blockIdx.x * 4];
+    const TileInfo tg = tgrp[active ? grp : (int64_t)blockIdx.x * 4];
+    const int64_t i = t0 + tl;
+    double xi, zi;
+    target(i, xi, zi);
+    float tqx[kMQ], tqz[kMQ];
+#pragma unroll
+    for (int q = 0; q < kMQ; ++q) {
+        double x, z;
+        target(t0 + 4 * q + c4, x, z);
+        tqx[q] = (float)(x - tg.cx);
+        tqz[q] = (float)(z - tg.cz);
+    }
+    // reach of this wave's targets from the reference point, squared
+    const double ox = fabs(tg.cx - tref.cx) + (double)tg.hx;
+    const double oz = fabs(tg.cz - tref.cz) + (double)tg.hz;
+    const double reach_t2 = ox * ox + oz * oz;
+
+    double E[kMQ][4];
+#pragma unroll
+    for (int q = 0; q < kMQ; ++q)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) E[q][r] = 0.0;
+    float su = 0.0f, cu = 0.0f, sw = 0.0f, cw = 0.0f;
+    const int s0 = blockIdx.y * tiles_per_chunk;
+    const int s1 = s0 + tiles_per_chunk < ntiles ? s0 + tiles_per_chunk : ntiles;
+    const v4f zero4 = {0.0f, 0.0f, 0.0f, 0.0f};
+
+    auto stage_load = [&](int st) {
+        const int64_t j = (int64_t)st * TILE + tid;
+        return (tid < TILE && j < ns) ? s32[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    };
+    float4 pre = stage_load(s0);
+
+    for (int st = s0; st < s1; ++st) {
+        const int b = (st - s0) & 1;
+        const TileInfo ts = stile[st];
+        if (tid < TILE) {
+            const float rx = (float)(ts.cx - tref.cx);
+            const float rz = (float)(ts.cz - tref.cz);
+            la[b][tid] = make_float4(-2.0f * pre.x, -2.0f * pre.y,
+                                     __builtin_fmaf(pre.x, pre.x, pre.y * pre.y), 0.0f);
+            lf[b][0][tid] = pre.z;
+            lf[b][1][tid] = pre.z * (pre.x + rx);
+            lf[b][2][tid] = pre.z * (pre.y + rz);
+        }
+        __syncthreads();
+        if (st + 1 < s1) pre = stage_load(st + 1);
+        if (!active) continue;
+
+        const double Dxd = ts.cx - tg.cx;
+        const double Dzd = ts.cz - tg.cz;
+        const double gx = fmax(fabs(Dxd) - (double)(tg.hx + ts.hx), 0.0);
+        const double gz = fmax(fabs(Dzd) - (double)(tg.hz + ts.hz), 0.0);
+        const double g2 = gx * gx + gz * gz;
+        const double lim = 1.0001 * (double)ts.thr_max + 1e-300;
+        const bool near = g2 <= (GUARD ? fmax(lim, 1.0001 * (double)r2_skip) : lim);
+        const int64_t jb = (int64_t)st * TILE;
+        const int cnt = ns - jb < TILE ? (int)(ns - jb) : TILE;
+        const double ext_s2 = (double)ts.hx * ts.hx + (double)ts.hz * ts.hz;
+        const bool mat = !near && cnt == TILE && g2 > 1e-30 &&
+                         g2 >= (double)kappa2 * fmax(ext_s2, reach_t2);
+        if (mat) {
+            // target box centre relative to the source tile centre
+            const float dtx = (float)(tg.cx - ts.cx);
+            const float dtz = (float)(tg.cz - ts.cz);
+            float bx[kMQ], bz[kMQ], br[kMQ];
+#pragma unroll
+            for (int q = 0; q < kMQ; ++q) {
+                bx[q] = tqx[q] + dtx;
+                bz[q] = tqz[q] + dtz;
+                br[q] = __builtin_fmaf(bx[q], bx[q], bz[q] * bz[q]);
+            }
+            v4f D2[kMQ];
+#pragma unroll
+            for (int q = 0; q < kMQ; ++q) D2[q] = zero4;
+#pragma unroll 1
+            for (int gg = 0; gg < TILE / 64; ++gg) {
+                const float4 a = la[b][gg * 64 + lane];
+                float bv[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) bv[r] = lf[b][c4][gg * 64 + 4 * (lane >> 2) + r];
+#pragma unroll
+                for (int q = 0; q < kMQ; ++q) {
+                    v4f d = __builtin_amdgcn_mfma_f32_4x4x1f32(1.0f, br[q], zero4, 0, 0, 0);
+                    d = __builtin_amdgcn_mfma_f32_4x4x1f32(a.z, 1.0f, d, 0, 0, 0);
+                    d = __builtin_amdgcn_mfma_f32_4x4x1f32(a.x, bx[q], d, 0, 0, 0);
+                    d = __builtin_amdgcn_mfma_f32_4x4x1f32(a.y, bz[q], d, 0, 0, 0);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) d[r] = __builtin_amdgcn_rcpf(d[r]);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        D2[q] = __builtin_amdgcn_mfma_f32_4x4x1f32(d[r], bv[r], D2[q], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kMQ; ++q)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) E[q][r] += (double)D2[q][r];
+        } else {
+            // VALU: this lane's target, sources [k0, k1) of the tile
+            const int hs = TILE / 2;
+            const int k0 = half * hs;
+            const int k1 = cnt < k0 + hs ? cnt : k0 + hs;
+            const float4* __restrict__ sp = s32 + jb;
+            float au = 0.0f, aw = 0.0f;
+            if (!near) {
+                const float txd = (float)(xi - ts.cx);
+                const float tzd = (float)(zi - ts.cz);
+                for (int k = k0; k < k1; ++k) far_pair(sp[k], txd, tzd, au, aw);
+            } else {
+                const float2* __restrict__ np = near32 + jb;
+                const double2* __restrict__ pp = p64 + jb;
+                for (int k = k0; k < k1; ++k) {
+                    const float4 sj = sp[k];
+                    const float2 nj = np[k];
+                    const double2 pj = pp[k];
+                    const float dx = (float)(xi - pj.x);
+                    const float dz = (float)(zi - pj.y);
+                    const float r2 = __builtin_fmaf(dx, dx, dz * dz);
+                    const float inv = __builtin_amdgcn_rcpf(r2);
+                    float K = inv;
+                    if (r2 < nj.x) K = -expm1f(-r2 * nj.y) * inv;
+                    bool take = true;
+                    if (SELF) take = (jb + k) != i;
+                    if (GUARD) take = take && (r2 > r2_skip);
+                    const float kk = take ? K * sj.z : 0.0f;
+                    au = __builtin_fmaf(kk, dz, au);
+                    aw = __builtin_fmaf(kk, dx, aw);
+                }
+            }
+            kahan_add(su, cu, au);
+            kahan_add(sw, cw, aw);
+        }
+    }
+    if (!active) return;
+
+    // VALU part: add the two source halves of each target.
+    double U = (double)su - (double)cu;
+    double W = (double)sw - (double)cw;
+    U += __shfl_xor(U, 32);
+    W += __shfl_xor(W, 32);
+    // Matrix part: sum the 16 blocks (lanes of equal lane & 3), then lane t
+    // (< 32) collects features 0..2 of its target 4q + r from lanes 0..2.
+    double F0 = 0.0, F1 = 0.0, F2 = 0.0;
+#pragma unroll
+    for (int q = 0; q < kMQ; ++q) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            double e = E[q][r];
+            e += __shfl_xor(e, 4);
+            e += __shfl_xor(e, 8);
+            e += __shfl_xor(e, 16);
+            e += __shfl_xor(e, 32);
+            const double v0 = __shfl(e, 0);
+            const double v1 = __shfl(e, 1);
+            const double v2 = __shfl(e, 2);
+            if (tl == 4 * q + r) {
+                F0 = v0;
+                F1 = v1;
+                F2 = v2;
+            }
+        }
+    }
+    U += (zi - tref.cz) * F0 - F2;
+    W += (xi - tref.cx) * F0 - F1;
+    if (half == 0 && i < nt) {
+        if (part == nullptr) {
+            const int64_t o = out_index ? (int64_t)out_index[i] : i;
+            u[o] = -U * kRpi2;
+            w[o] = W * kRpi2;
+        } else {
+            part[((int64_t)blockIdx.y * 2 + 0) * nt + i] = -U;
+            part[((int64_t)blockIdx.y * 2 + 1) * nt + i] = W;
+        }
+    }
+}
+
 // ----------------------------------------------------------------- launch
 
 static int pick_chunks(dvm_ctx_t ctx, int64_t blocks_x, int64_t max_chunks)
@@ -493,6 +738,7 @@ struct F32cArgs {
     double* u;
     double* w;
     double* part;
+    float kappa2;  // matrix form only
 };
 
 // kind: 0 blob-on-blob, 1 targets, 2 targets with the r2_skip guard.
@@ -509,6 +755,22 @@ static void f32c_dispatch(int kind, dim3 grid, hipStream_t st, const F32cArgs& a
     else
         bs_f32c_kernel<TILE, BLOCK, SHARE, false, false><<<grid, BLOCK, 0, st>>>(DVM_F32C_ARGS);
 #undef DVM_F32C_ARGS
+}
+
+// Matrix far-field form: 256-lane blocks of 4 waves x 32 targets.
+template <int TILE>
+static void f32m_dispatch(int kind, dim3 grid, hipStream_t st, const F32cArgs& a)
+{
+#define DVM_F32M_ARGS                                                                        \
+    a.nt, a.tx, a.tz, a.tblk, a.ns, a.s32, a.near32, a.p64, a.stile, a.ntiles, a.tpc,        \
+        a.r2_skip, a.kappa2, a.out_index, a.u, a.w, a.part
+    if (kind == 0)
+        bs_f32m_kernel<TILE, true, false><<<grid, 256, 0, st>>>(DVM_F32M_ARGS);
+    else if (kind == 2)
+        bs_f32m_kernel<TILE, false, true><<<grid, 256, 0, st>>>(DVM_F32M_ARGS);
+    else
+        bs_f32m_kernel<TILE, false, false><<<grid, 256, 0, st>>>(DVM_F32M_ARGS);
+#undef DVM_F32M_ARGS
 }
 
 template <int T>
@@ -537,13 +799,18 @@ static const F32cKernel kF32c[] = {
     {256, 64, 1, f32c_dispatch<256, 64, 1>},   {256, 64, 2, f32c_dispatch<256, 64, 2>},
     {256, 64, 4, f32c_dispatch<256, 64, 4>},   {256, 256, 1, f32c_dispatch<256, 256, 1>},
     {256, 256, 2, f32c_dispatch<256, 256, 2>}, {256, 256, 4, f32c_dispatch<256, 256, 4>},
+    // share 0: matrix-core far field, 32-target boxes
+    {128, 32, 0, f32m_dispatch<128>},          {256, 32, 0, f32m_dispatch<256>},
 };
 constexpr int kNumF32c = sizeof(kF32c) / sizeof(kF32c[0]);
 constexpr int kOccupancy[] = {2, 4, 8};  // target resident blocks per CU for source splitting
 
 static TileFn tile_fn(int t)
 {
-    return t == 64 ? tile_build<64> : t == 128 ? tile_build<128> : tile_build<256>;
+    return t == 32    ? tile_build<32>
+           : t == 64  ? tile_build<64>
+           : t == 128 ? tile_build<128>
+                      : tile_build<256>;
 }
 
 static int f32c_find(int tile, int block, int share)
@@ -561,11 +828,15 @@ static void f32c_run(dvm_ctx_t ctx, int kind, int v, int occ, int64_t ns, const 
 {
     const F32cKernel& K = kF32c[v];
     const bool self = kind == 0;
+    const bool matrix = K.share == 0;
     const int ntiles = (int)cdiv(ns, K.tile);
-    const int64_t bx = cdiv(nt, K.block);
+    const int64_t bx = cdiv(nt, K.block);  // target boxes
+    // launch blocks: one box each, or 4 boxes of 32 (one per wave) in the
+    // matrix form; the occupancy target counts 256-lane blocks per CU
+    const int64_t gx = matrix ? cdiv(nt, 4 * kMT) : bx;
     const int nbb = 64;
-    const int64_t want = (int64_t)occ * ctx->num_cus * 256 / K.block;
-    int64_t nch64 = (want + bx - 1) / bx;
+    const int64_t want = (int64_t)occ * ctx->num_cus * 256 / (matrix ? 256 : K.block);
+    int64_t nch64 = (want + gx - 1) / gx;
     nch64 = nch64 > ntiles ? ntiles : (nch64 < 1 ? 1 : nch64);
     const int nch = (int)nch64;
     const int tpc = (ntiles + nch - 1) / nch;
@@ -606,9 +877,9 @@ static void f32c_run(dvm_ctx_t ctx, int kind, int v, int occ, int64_t ns, const 
     tile_fn(K.block)(ctx->stream, nt, self ? perm : nullptr, self ? sx : tx, self ? sz : tz,
                      nullptr, nullptr, nullptr, nullptr, nullptr, tblk);
     DVM_LAUNCH_CHECK();
-    F32cArgs a{nt,    self ? nullptr : tx, self ? nullptr : tz, tblk, ns, s32, near32, p64, stile,
-               ntiles, tpc, (float)r2_skip,   self ? perm : nullptr, u, w, part};
-    dim3 grid((unsigned)bx, (unsigned)nch);
+    F32cArgs a{nt,     self ? nullptr : tx, self ? nullptr : tz,  tblk, ns, s32, near32, p64,
+               stile,  ntiles, tpc, (float)r2_skip, self ? perm : nullptr, u, w, part, kKappa2};
+    dim3 grid((unsigned)gx, (unsigned)nch);
     {
         ProfScope ps(ctx, self ? "bs_self_f32c" : "bs_targets_f32c");
         K.fn(kind, grid, ctx->stream, a);

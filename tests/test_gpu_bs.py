@@ -182,6 +182,29 @@ def test_f32c_every_variant(gpu_ctx, variant):
             assert norm_max <= F32C_TOL and p99 <= F32C_TOL, (v, guard, norm_max, p99)
 
 
+@pytest.mark.parametrize("tile", [128, 256])
+def test_f32c_matrix_form_large(gpu_ctx, tile):
+    # 65,536 blobs: most tile pairs are well separated and go through the
+    # matrix-core far field; FP32C against the FP64 form on every target,
+    # self sum and both target sums.
+    x, z, c, s = _mixed_blobs(65537, seed=77)
+    X, Z, C, S = to_dev(x, z, c, s)
+    tx = np.linspace(-1, 15, 1024)
+    tz = np.full(1024, 0.1)
+    TX, TZ = to_dev(tx, tz)
+    u64, w64 = (t.cpu().numpy() for t in dvm.bs_self(gpu_ctx, X, Z, C, S, prec=dvm.F64))
+    p64 = [t.cpu().numpy() for t in dvm.bs_targets(gpu_ctx, X, Z, C, S, TX, TZ, prec=dvm.F64)]
+    v = (tile, 32, 0, 8)
+    u32, w32 = _with_variant(gpu_ctx, v, lambda: [t.cpu().numpy() for t in dvm.bs_self(
+        gpu_ctx, X, Z, C, S, prec=dvm.F32C)])
+    norm_max, p99 = _f32c_errors(u32, w32, u64, w64)
+    assert norm_max <= F32C_TOL and p99 <= F32C_TOL, (norm_max, p99)
+    p32 = _with_variant(gpu_ctx, v, lambda: [t.cpu().numpy() for t in dvm.bs_targets(
+        gpu_ctx, X, Z, C, S, TX, TZ, prec=dvm.F32C)])
+    norm_max, p99 = _f32c_errors(p32[0], p32[1], p64[0], p64[1])
+    assert norm_max <= F32C_TOL and p99 <= F32C_TOL, (norm_max, p99)
+
+
 @pytest.mark.parametrize("share", [1, 2, 4])
 def test_f32c_shared_reciprocal_wide_domain(gpu_ctx, share):
     # Separations up to ~1.7e4: r^2 products of 4 leave the range where one

@@ -244,9 +244,9 @@ def main():
     ap.add_argument("--f64-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--tune", choices=("measure", "table"), default="measure",
-                    help="F32C launch variant: timed on first use per size (measure) or the "
-                         "built-in size table")
+    ap.add_argument("--tune", choices=("measure", "table"), default="table",
+                    help="F32C launch variant: the built-in size table (the library default) "
+                         "or timed on first use per size (measure)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (reference kernel "
                          "for scaling HBM byte counters)")

@@ -106,6 +106,8 @@ SIGNATURES.update({
     "dvm_sheet_solve": (_st, [_vp, _int, _i64, _vp, _vp, _vp, _vp, _vp]),
     "dvm_sheet_on_blobs": (_st, [_vp, _i64, _vp, _vp, _vp, _vp]),
     "dvm_sheet_loads": (_st, [_vp, _dbl, _vp]),
+    "dvm_sheet_probe_quirk": (_st, [_vp, _int, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
+                                    _vp, _vp, _vp]),
     "dvm_sheet_inside": (_st, [_vp, _i64, _vp, _vp, _vp]),
     "dvm_sheet_reflect": (_st, [_vp, _i64, _vp, _vp]),
     "dvm_remesh": (_st, [_vp, ctypes.POINTER(RemeshParams), _i64, _vp, _vp, _vp, _vp, _i64,

@@ -174,6 +174,11 @@ void sheet_solve(dvm_ctx_t ctx, SheetDev& sh, int prec, int64_t n, const double*
 void sheet_on_blobs(dvm_ctx_t ctx, const SheetDev& sh, int64_t n, const double* bx,
                     const double* bz, double* uvs, double* wvs);
 void sheet_loads(dvm_ctx_t ctx, SheetDev& sh, double ur);
+// probe_velocities in the reference's (quirk) form.
+void sheet_probe_quirk(dvm_ctx_t ctx, const SheetDev& sh, int prec, int64_t n, const double* bx,
+                       const double* bz, const double* bc, const double* bs, int64_t np,
+                       const double* px, const double* pz, double* u, double* w, double* uvs,
+                       double* wvs);
 // Release counts and offsets; returns nothing (results in sh.cnt/off/flag).
 void sheet_release_count(dvm_ctx_t ctx, SheetDev& sh);
 // Writes nrv nascent blobs at index `base` of the blob arrays, using rrw[k].

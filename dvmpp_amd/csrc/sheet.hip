@@ -381,6 +381,70 @@ void sheet_on_blobs(dvm_ctx_t ctx, const SheetDev& sh, int64_t n, const double* 
     DVM_LAUNCH_CHECK();
 }
 
+// ----------------------------------------------------------------- probes (quirk form)
+
+// Sheet part of VortexSheet::probe_velocities (src/VortexSheet.cpp:597-638):
+// the panel loop runs j = 1 .. Nl-1 and assigns (does not accumulate), so
+// only panel Nl-1 survives, paired with gamma_0 and with the qy term
+// subtracted (:630-634); its coefficients differ from vortexsheetbc in c9,
+// qy and py (:612-626) and carry no 1/(2 pi).  u, w arrive holding the blob
+// part already scaled by 1/(2 pi); the result is (u + uvs) + Ux (:641-644).
+__global__ void probe_quirk_kernel(int64_t np, const double* __restrict__ px,
+                                   const double* __restrict__ pz, int nl,
+                                   const PanelConst* __restrict__ panel,
+                                   const double* __restrict__ gamma, double ux, double uz,
+                                   double* __restrict__ u, double* __restrict__ w,
+                                   double* __restrict__ uvs, double* __restrict__ wvs)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    double su = 0.0, sw = 0.0;
+    if (nl >= 2) {
+        const PanelConst q = panel[nl - 1];
+        const double dx = px[i] - q.xc;
+        const double dz = pz[i] - q.zc;
+        const double dsj = q.ds;
+        const double c1 = -dx * q.cs - dz * q.sn;
+        const double c2 = dx * dx + dz * dz;
+        const double c5 = dx * q.sn - dz * q.cs;
+        const double c6 = log(1.0 + dsj * ((dsj + 2 * c1) / c2));
+        const double c7 = atan2(c5 * dsj, c2 + c1 * dsj);
+        const double c8 = dx * q.s2 + dz * q.c2;
+        const double c9 = dx * q.c2 + dz * q.s2;
+        const double qx = -q.sn + 0.5 * c8 * c6 / dsj + (c1 * q.cs + c5 * q.sn) * c7 / dsj;
+        const double pxj = -0.5 * c6 * q.sn - c7 * q.cs - qx;
+        const double qy = q.cs + 0.5 * c9 * c6 / dsj + (c1 * q.cs - c5 * q.sn) * c7 / dsj;
+        const double pyj = -0.5 * c6 * q.cs - c7 * q.sn - qy;
+        su = pxj * gamma[nl - 1] + qx * gamma[0];
+        sw = pyj * gamma[nl - 1] - qy * gamma[0];
+    }
+    uvs[i] = su;
+    wvs[i] = sw;
+    u[i] = u[i] + su + ux;
+    w[i] = w[i] + sw + uz;
+}
+
+void sheet_probe_quirk(dvm_ctx_t ctx, const SheetDev& sh, int prec, int64_t n, const double* bx,
+                       const double* bz, const double* bc, const double* bs, int64_t np,
+                       const double* px, const double* pz, double* u, double* w, double* uvs,
+                       double* wvs)
+{
+    if (np <= 0) return;
+    // Blob part: blobs 1 .. n-1 (the reference loop starts at j = 1,
+    // src/VortexSheet.cpp:573), the A1 pair with no self or r^2 guard.
+    if (n >= 2) {
+        bs_targets_launch(ctx, prec, n - 1, bx + 1, bz + 1, bc + 1, bs + 1, np, px, pz, 0, 0.0,
+                          u, w);
+    } else {
+        DVM_HIP_CHECK(hipMemsetAsync(u, 0, sizeof(double) * np, ctx->stream));
+        DVM_HIP_CHECK(hipMemsetAsync(w, 0, sizeof(double) * np, ctx->stream));
+    }
+    probe_quirk_kernel<<<cdiv(np, 256), 256, 0, ctx->stream>>>(np, px, pz, sh.nl, sh.panel,
+                                                              sh.gamma, sh.ux, sh.uz, u, w,
+                                                              uvs, wvs);
+    DVM_LAUNCH_CHECK();
+}
+
 // ----------------------------------------------------------------- loads
 
 // Armadillo-style two-accumulator sum (see the oracle's note: order unpinned).
@@ -760,6 +824,22 @@ dvm_status dvm_sheet_loads(dvm_sheet_t s, double ur, double* out4)
                                          hipMemcpyDeviceToHost, s->ctx->stream));
             DVM_HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
         }
+    });
+}
+
+dvm_status dvm_sheet_probe_quirk(dvm_sheet_t s, int prec, int64_t n, const double* x,
+                                 const double* z, const double* circ, const double* sigma,
+                                 int64_t np, const double* px, const double* pz, double* u,
+                                 double* w, double* uvs, double* wvs)
+{
+    if (!s) return DVM_E_INVALID;
+    return guard(s->ctx, [&] {
+        require(prec == DVM_F64 || prec == DVM_F32C, DVM_E_INVALID, "unknown precision");
+        require(n >= 0 && np >= 0, DVM_E_SIZE, "negative count");
+        require(n < 2 || (x && z && circ && sigma), DVM_E_INVALID, "null blob array");
+        require(np == 0 || (px && pz && u && w && uvs && wvs), DVM_E_INVALID,
+                "null probe array");
+        sheet_probe_quirk(s->ctx, s->sh, prec, n, x, z, circ, sigma, np, px, pz, u, w, uvs, wvs);
     });
 }
 

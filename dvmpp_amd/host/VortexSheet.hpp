@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "BaseTypes.hpp"
+#include "Probe.hpp"
 #include "Random.hpp"
 #include "VortexBlobs.hpp"
 #include "dvm_hip.h"
@@ -32,6 +33,9 @@ public:
     void reflect(VortexBlobs& vortex);
     void delete_inside(VortexBlobs& vortex);
     void compute_loads(double Urel);
+    /// src/VortexSheet.cpp:559-646 with the reference's quirks (never called
+    /// there): fills probe.m_u, m_w, m_uvs, m_wvs.
+    void probe_velocities(const VortexBlobs& blobs, Probe& probe);
     [[nodiscard]] unsigned size() const;
     [[nodiscard]] std::tuple<double, double> get_forces() const;
     /// Drag and lift coefficients of the last compute_loads (printed by the reference).

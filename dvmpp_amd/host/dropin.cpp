@@ -332,6 +332,36 @@ void VortexSheet::vortexsheetbc(VortexBlobs& blobs)
     buf.download(blobs.m_wvs.memptr(), b, 3 * b);
 }
 
+void VortexSheet::probe_velocities(const VortexBlobs& blobs, Probe& probe)
+{
+    const unsigned N = blobs.size();
+    const unsigned P = probe.size();
+    if (probe.m_z.size() != P) throw dvm::SizeMismatchException("Probe");
+    probe.m_u.zeros(P);
+    probe.m_w.zeros(P);
+    probe.m_uvs.zeros(P);
+    probe.m_wvs.zeros(P);
+    if (P == 0) return;
+    const size_t b = sizeof(double) * N, pb = sizeof(double) * P;
+    dvm::DeviceBuffer buf(4 * b + 6 * pb + 8);
+    buf.upload(blobs.m_x.memptr(), b, 0);
+    buf.upload(blobs.m_z.memptr(), b, b);
+    buf.upload(blobs.m_circ.memptr(), b, 2 * b);
+    buf.upload(blobs.m_sigma.memptr(), b, 3 * b);
+    buf.upload(probe.m_x.memptr(), pb, 4 * b);
+    buf.upload(probe.m_z.memptr(), pb, 4 * b + pb);
+    double* d = buf.as<double>();
+    double* p = d + 4 * (size_t)N;
+    dvm::check(dvm_sheet_probe_quirk(m_sheet, m_prec, N, d, d + N, d + 2 * (size_t)N,
+                                     d + 3 * (size_t)N, P, p, p + P, p + 2 * (size_t)P,
+                                     p + 3 * (size_t)P, p + 4 * (size_t)P, p + 5 * (size_t)P),
+               "probe_velocities", dvm::device_context());
+    buf.download(probe.m_u.memptr(), pb, 4 * b + 2 * pb);
+    buf.download(probe.m_w.memptr(), pb, 4 * b + 3 * pb);
+    buf.download(probe.m_uvs.memptr(), pb, 4 * b + 4 * pb);
+    buf.download(probe.m_wvs.memptr(), pb, 4 * b + 5 * pb);
+}
+
 VortexBlobs VortexSheet::release_nascent_vortices_rw(Random& rand)
 {
     int64_t nrv = 0;

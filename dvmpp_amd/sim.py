@@ -4,7 +4,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import RemeshParams, SheetParams, SimParams, check, load
+from ._lib import F64, RemeshParams, SheetParams, SimParams, check, load
 
 STATE_FIELDS = ("x", "z", "circ", "sigma", "u", "w", "uvs", "wvs")
 
@@ -112,6 +112,18 @@ class Sheet:
         out = np.empty(4)
         check(self.ctx.h, load().dvm_sheet_loads(self.h, ur, _hp(out)))
         return tuple(out)
+
+    def probe_quirk(self, x, z, circ, sigma, px, pz, prec=F64):
+        """probe_velocities in the reference's form: (u, w, uvs, wvs) at the
+        probes (device tensors in, device tensors out)."""
+        import torch
+        n, npr = x.numel(), px.numel()
+        outs = [torch.empty(npr, dtype=torch.float64, device=px.device) for _ in range(4)]
+        ptr = (lambda t: ctypes.c_void_p(t.data_ptr()))
+        check(self.ctx.h, load().dvm_sheet_probe_quirk(
+            self.h, prec, n, ptr(x), ptr(z), ptr(circ), ptr(sigma), npr, ptr(px), ptr(pz),
+            *[ptr(o) for o in outs]))
+        return tuple(outs)
 
     def inside(self, x, z):
         import torch

@@ -186,6 +186,19 @@ dvm_status dvm_sheet_on_blobs(dvm_sheet_t sheet, int64_t n, const double* x, con
 /* VortexSheet::compute_loads (src/VortexSheet.cpp:487-514) and get_forces
  * (:552-555): out4 (HOST) = {fx, fz, cd, cl}. */
 dvm_status dvm_sheet_loads(dvm_sheet_t sheet, double ur, double* out4);
+/* VortexSheet::probe_velocities (src/VortexSheet.cpp:559-646, decl
+ * src/VortexSheet.hpp:126) in the reference's form.  The reference never
+ * calls it (DVMBase::probe_velocities is declared, src/DVMBase.hpp:98, but not
+ * defined); the isolated-sum probes use dvm_bs_targets instead.  Kept quirks:
+ * the blob sum starts at blob 1 and has no self or r^2 guard; the sheet term
+ * is that of panel Nl-1 alone (the panel loop assigns instead of adding),
+ * paired with gamma_0 with the qy term subtracted, without 1/(2 pi).  Device
+ * arrays: n blobs, np probes; u, w (Probe::m_u, m_w) = (1/2pi) blob sum +
+ * sheet term + U; uvs, wvs (Probe::m_uvs, m_wvs) = the sheet term. */
+dvm_status dvm_sheet_probe_quirk(dvm_sheet_t sheet, int prec, int64_t n, const double* x,
+                                 const double* z, const double* circ, const double* sigma,
+                                 int64_t np, const double* px, const double* pz, double* u,
+                                 double* w, double* uvs, double* wvs);
 /* VortexSheet::inside_body (src/VortexSheet.cpp:447-464) for n device points:
  * flags[i] = 1 if inside (crossing number == 1). */
 dvm_status dvm_sheet_inside(dvm_sheet_t sheet, int64_t n, const double* x, const double* z,

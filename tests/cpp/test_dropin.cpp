@@ -456,6 +456,17 @@ TEST(VortexSheetMethods)
     sh.compute_loads(3.0);
     auto [fx, fz] = sh.get_forces();
     EXPECT(std::isfinite(fx) && std::isfinite(fz));
+    // probe_velocities: the sheet part is independent of the blobs (only the
+    // last panel's term survives) and the total adds the free stream
+    Probe probe({2.0, 3.0, -2.0}, {0.0, 0.5, 1.0});
+    sh.probe_velocities(v, probe);
+    Probe alone({2.0, 3.0, -2.0}, {0.0, 0.5, 1.0});
+    sh.probe_velocities(VortexBlobs(), alone);
+    for (unsigned i = 0; i < probe.size(); ++i) {
+        EXPECT(std::isfinite(probe.m_u(i)) && std::isfinite(probe.m_w(i)));
+        EXPECT(probe.m_uvs(i) == alone.m_uvs(i) && probe.m_wvs(i) == alone.m_wvs(i));
+        EXPECT(alone.m_u(i) == alone.m_uvs(i) + sheet_params().ux);
+    }
 }
 
 TEST(DVMBaseRunsCylinder)

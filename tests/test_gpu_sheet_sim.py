@@ -92,6 +92,31 @@ def test_sheet_on_blobs_and_loads(gpu_ctx):
         assert abs(a - b) <= TOL * max(abs(b), 1e-12)
 
 
+@pytest.mark.parametrize("n", [0, 1, 2, 3000])
+def test_probe_velocities_quirk_form(gpu_ctx, n):
+    # probe_velocities (src/VortexSheet.cpp:559-646) with its quirks: blobs
+    # from index 1, no guard, the last panel's assigned sheet term, no 1/(2pi)
+    # on it.  FP64 against the oracle's restatement of those loops.
+    px, pz = load_body("bodycoordinatesN40.txt")
+    op, gp = both_params(ux=1.0, uz=0.25)
+    osh = oracle.Sheet(px, pz, op)
+    gsh = dsim.Sheet(gpu_ctx, px, pz, gp.sheet)
+    gam = np.random.default_rng(5).normal(size=osh.nl)
+    osh.gamma = gam
+    gsh.gamma = gam
+    x, z, c, s = wake_blobs(n, seed=11)
+    prx = np.linspace(-1.0, 4.0, 101)
+    prz = np.linspace(-1.5, 1.5, 101)
+    ref = osh.probe_quirk(x, z, c, s, prx, prz)
+    X, Z, C, S, PX, PZ = to_dev(x, z, c, s, prx, prz)
+    got = [t.cpu().numpy() for t in gsh.probe_quirk(X, Z, C, S, PX, PZ)]
+    for g, o, name in zip(got, ref, ("u", "w", "uvs", "wvs")):
+        assert rel_err(g, o) <= TOL, name
+    # the sheet term is the last panel's alone: it does not depend on the blobs
+    assert np.array_equal(got[2], gsh.probe_quirk(X[:0], Z[:0], C[:0], S[:0], PX, PZ)[2].cpu()
+                          .numpy())
+
+
 def test_inside_and_reflect_bit_exact(gpu_ctx):
     px, pz = oracle.cylinder_points(64)
     op, gp = both_params()

@@ -91,6 +91,26 @@ def cpu_baseline(x, z, c, s, budget_s):
                       f"loop order, OpenMP, {dt:.1f} s"}
 
 
+def host_aos_rate(ctx, dvm, x, z, c, s):
+    """Blob-on-blob sum through the host-pointer entry point (AoS particles in
+    host memory, results back in host memory): the PCIe-inclusive rate."""
+    import ctypes
+    n = x.size
+    parts = np.ascontiguousarray(np.stack([x, z, c, s], axis=1))
+    res = np.empty((n, 2))
+    lib = dvm.load()
+    args = (ctx.h, dvm.F32C, n, parts.ctypes.data_as(ctypes.c_void_p),
+            res.ctypes.data_as(ctypes.c_void_p))
+    dvm.check(ctx.h, lib.dvm_bs_self_host(*args))
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dvm.check(ctx.h, lib.dvm_bs_self_host(*args))
+    dt = (time.perf_counter() - t0) / reps
+    return {"pair_interactions_per_s": n * (n - 1) / dt, "ms_per_call": dt * 1e3,
+            "note": "dvm_bs_self_host: pageable host AoS in and out, synchronous"}
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -348,6 +368,7 @@ def main():
         },
         "probe_kernel_ms": {"f32c": tgt32_ms, "f64": tgt64_ms},
     }
+    line["host_aos_f32c"] = host_aos_rate(ctx, dvm, x, z, c, s)
     if args.copy_calibration_mib > 0:
         nbytes = args.copy_calibration_mib << 20
         src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")

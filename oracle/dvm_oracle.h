@@ -10,6 +10,13 @@
  * (Armadillo, Boost.program_options and BackgroundMesh.{hpp,cpp} are absent),
  * so the restatement is pinned by the reference's own known-answer tests,
  * closed forms, invariants and long-double brute force (tests/test_oracle_*.py).
+ *
+ * Parity unpinned (nothing outside the reading of the reference fixes them):
+ * the summation order of arma::sum (totalcirc, the force sums), the BLAS
+ * order of A^T A and A^T b and the factorisation arma::solve uses, and the
+ * w component of the sheet term of orc_sheet_probe_quirk (probe_velocities'
+ * py, qy coefficients, src/VortexSheet.cpp:621-626, match no panel-velocity
+ * closed form; its u component is pinned by quadrature).
  */
 #ifndef DVM_ORACLE_H
 #define DVM_ORACLE_H

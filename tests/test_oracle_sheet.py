@@ -265,3 +265,29 @@ def test_probe_quirk_runs(params):
     ub, wb = oracle.bs_targets([2.0], [0.5], [0.2], [0.05], [3.0], [0.0])
     assert abs((u[0] - uvs[0] - 1.0) - ub[0]) < 1e-15
     assert abs((w[0] - wvs[0] - 0.0) - wb[0]) < 1e-15
+
+
+def test_probe_quirk_sheet_term(params):
+    # probe_velocities' sheet term is that of panel Nl-1 alone, paired with
+    # gamma_0 (the loop assigns, src/VortexSheet.cpp:628-636).  Its u part is
+    # 2 pi times the vortexsheetbc coefficient pair, i.e. minus 2 pi times the
+    # quadrature velocity of the midpoint-origin linear panel (no 1/(2 pi)).
+    x, z = load_body("bodycoordinatesN40.txt")
+    sh = oracle.Sheet(x, z, params)
+    g = sh.geometry()
+    nl = sh.nl
+    gam = np.random.default_rng(3).normal(size=nl)
+    sh.gamma = gam
+    prx = np.array([1.3, -0.2, 0.9, 3.0])
+    prz = np.array([0.4, 1.1, -0.8, -2.0])
+    _, _, uvs, wvs = sh.probe_quirk([], [], [], [], prx, prz)
+    for k in range(prx.size):
+        du, _ = panel_velocity(prx[k], prz[k], g["xc"][nl - 1], g["zc"][nl - 1],
+                               g["theta"][nl - 1], g["ds"][nl - 1], gam[nl - 1], gam[0])
+        assert abs(uvs[k] + du / RP2) < 1e-11
+    # the term does not depend on gamma_1 .. gamma_{Nl-2}
+    gam2 = gam.copy()
+    gam2[1:nl - 1] = 7.0
+    sh.gamma = gam2
+    _, _, uvs2, wvs2 = sh.probe_quirk([], [], [], [], prx, prz)
+    assert np.array_equal(uvs, uvs2) and np.array_equal(wvs, wvs2)

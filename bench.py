@@ -245,6 +245,51 @@ def served_phase(ctx, count, rate, steps_range, slots):
             "completed_ok": sum(1 for r in done if r.status == 0)}
 
 
+def rk3_plateau_phase(ctx, steps):
+    """RK3 cylinder (512 panels, nu 5e-4, dt 0.0025, remesh M4' every 20
+    steps) timed per step at a fixed-N snapshot of the plateau: a remeshed
+    wake of ~1.5 M blobs is loaded into the simulation, one step runs untimed,
+    then `steps` full RK3 steps (3 Biot-Savart sums, 3 sheet solves and
+    sheet-on-blob sums, diffusion, release, crossing) are timed; one remesh of
+    the same state is timed separately."""
+    import torch
+
+    import dvmpp_amd as dvm
+    from dvmpp_amd import sim as dsim
+    from dvmpp_amd.workloads import rk3_cylinder, wake_snapshot
+    px, pz, sp, sig = rk3_cylinder()
+    rm = dsim.remesh_params(frequency=20, kernel=dsim.M4PRIME, xmin=-5.0, xmax=25.0, zmin=-10.0,
+                            zmax=10.0, dx=0.01, dz=0.01, sigma=sig, circ_threshold=1e-10)
+    snap = wake_snapshot(sigma=sig)
+    n0 = snap["x"].size
+    p = dsim.SimParams(sp, dsim.RK3, dsim.REFLECT, 0, dvm.F32C, dsim.RNG_DEVICE, rm)
+    s = dsim.Sim(ctx, px, pz, p)
+    s.set_state(snap)
+    s.run(1)
+    torch.cuda.synchronize()
+    n1 = s.size()
+    t0 = time.perf_counter()
+    s.run(steps)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    n2 = s.size()
+    s.close()
+    log(f"RK3 plateau: {steps} steps at N = {n1}..{n2} in {dt * steps:.2f} s")
+    X, Z, C, S = (torch.from_numpy(snap[k]).cuda() for k in ("x", "z", "circ", "sigma"))
+    dsim.remesh(ctx, rm, X, Z, C, S)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = dsim.remesh(ctx, rm, X, Z, C, S)
+    torch.cuda.synchronize()
+    tr = time.perf_counter() - t0
+    nmean = 0.5 * (n1 + n2)
+    return {"workload": "RK3 cylinder, 512 panels, nu 5e-4, dt 0.0025, REFLECT, FP32C sums, "
+                        f"device random stream; remeshed-wake snapshot of {n0} blobs",
+            "seconds_per_step": dt, "steps_timed": steps, "blobs": [n1, n2],
+            "pair_interactions_per_s_in_step": 3.0 * nmean * (nmean - 1) / dt,
+            "remesh_ms": tr * 1e3, "remesh_blobs_out": int(out[0].numel())}
+
+
 def load_traffic(kernel):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -270,8 +315,10 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (reference kernel "
                          "for scaling HBM byte counters)")
-    ap.add_argument("--phases", default="sum,cylinder,ensemble,served",
-                    help="comma list: sum (isolated sum, always run), cylinder, ensemble, served")
+    ap.add_argument("--phases", default="sum,cylinder,ensemble,served,rk3",
+                    help="comma list: sum (isolated sum, always run), cylinder, ensemble, "
+                         "served, rk3")
+    ap.add_argument("--rk3-steps", type=int, default=2)
     ap.add_argument("--cylinder-steps", type=int, default=400)
     ap.add_argument("--cylinder-cpu-steps", type=int, default=40)
     ap.add_argument("--ensemble-count", type=int, default=128)
@@ -395,6 +442,8 @@ def main():
     if "served" in phases:
         lo, hi = (int(v) for v in args.served_steps.split(","))
         line["served"] = served_phase(ctx, args.served_count, 20.0, (lo, hi), slots=32)
+    if "rk3" in phases:
+        line["rk3_cylinder_plateau"] = rk3_plateau_phase(ctx, args.rk3_steps)
     ctx.close()
     print(json.dumps(line), flush=True)
 

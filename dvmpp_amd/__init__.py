@@ -17,7 +17,7 @@ __all__ = ["F64", "F32C", "TUNE_TABLE", "TUNE_MEASURE", "DvmError", "Context", "
 # dvmpp_amd/csrc/bs_kernels.hip.
 F32C_VARIANTS = ([(t, b, s) for t in (128, 256) for b in (64, 256) for s in (1, 2, 4)]
                  + [(128, 32, 0), (256, 32, 0)])
-F32C_OCCUPANCY = (2, 4, 8)
+F32C_OCCUPANCY = (8, 32, 128)
 
 
 def _ptr(t):

@@ -119,7 +119,23 @@ __global__ void bs_reduce_kernel(int64_t nt, int nchunks, const double* __restri
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nt) return;
     double su = 0.0, sw = 0.0;
-    for (int c = 0; c < nchunks; ++c) {
+    // Loads of 8 chunks are issued together; the adds stay in chunk order.
+    constexpr int kU = 8;
+    int c = 0;
+    for (; c + kU <= nchunks; c += kU) {
+        double a[kU], b[kU];
+#pragma unroll
+        for (int k = 0; k < kU; ++k) {
+            a[k] = part[((int64_t)(c + k) * 2 + 0) * nt + i];
+            b[k] = part[((int64_t)(c + k) * 2 + 1) * nt + i];
+        }
+#pragma unroll
+        for (int k = 0; k < kU; ++k) {
+            su += a[k];
+            sw += b[k];
+        }
+    }
+    for (; c < nchunks; ++c) {
         su += part[((int64_t)c * 2 + 0) * nt + i];
         sw += part[((int64_t)c * 2 + 1) * nt + i];
     }
@@ -803,7 +819,14 @@ static const F32cKernel kF32c[] = {
     {128, 32, 0, f32m_dispatch<128>},          {256, 32, 0, f32m_dispatch<256>},
 };
 constexpr int kNumF32c = sizeof(kF32c) / sizeof(kF32c[0]);
-constexpr int kOccupancy[] = {2, 4, 8};  // target resident blocks per CU for source splitting
+// Occupancy targets: 256-lane blocks per CU that splitting the source range
+// into chunks aims for.  Beyond filling the chip, finer chunks keep the
+// source slice that co-resident waves stream small (blocks are dispatched
+// chunk by chunk), so it stays in the scalar cache and L2: on MI355X the
+// 262,144-blob sum drops from 16.3 ms at 8 to 11.9 ms at 128.
+constexpr int kOccupancy[] = {8, 32, 128};
+constexpr int kOccupancyFirst = 32;          // used while every variant is timed
+constexpr size_t kPartialBytes = 256u << 20;  // cap on the FP64 chunk partials
 
 static TileFn tile_fn(int t)
 {
@@ -837,6 +860,8 @@ static void f32c_run(dvm_ctx_t ctx, int kind, int v, int occ, int64_t ns, const 
     const int nbb = 64;
     const int64_t want = (int64_t)occ * ctx->num_cus * 256 / (matrix ? 256 : K.block);
     int64_t nch64 = (want + gx - 1) / gx;
+    const int64_t nch_mem = (int64_t)(kPartialBytes / (2 * sizeof(double) * (size_t)nt));
+    nch64 = nch64 > nch_mem ? nch_mem : nch64;
     nch64 = nch64 > ntiles ? ntiles : (nch64 < 1 ? 1 : nch64);
     const int nch = (int)nch64;
     const int tpc = (ntiles + nch - 1) / nch;
@@ -899,30 +924,31 @@ static int lg2_bucket(int64_t n)
     return b;
 }
 
-// Fixed choice (DVM_TUNE_TABLE), from sweeps of every variant on MI355X
-// (uniform and wake-like blob sets, 4K..1M blobs; 257 and 1024 targets):
-// small sums favour few, wide blocks; probe and collocation sums 1-reciprocal
-// tiles of 128; large sums 64-target blocks (tight near/far boxes), 4
-// sources per reciprocal and the source range split to 8 blocks per CU.
+// Fixed choice (DVM_TUNE_TABLE), from sweeps of every variant and
+// occupancy target on MI355X (uniform and cylinder-wake blob sets, 4K..1M
+// blobs; 257 and 1024 targets over 16K and 262K sources): small sums favour
+// few, wide blocks; probe and collocation sums 1-reciprocal tiles of 128 and
+// 64-target blocks; large sums 64-target blocks (tight near/far boxes) and 4
+// sources per reciprocal; everything gains from a finely split source range.
 static void f32c_table(int kind, int64_t nt, int64_t ns, int& v, int& occ)
 {
     if ((double)nt * (double)ns < 6.7e7) {
         v = f32c_find(128, 256, 2);
-        occ = 2;
+        occ = 8;
     } else if (kind != 0 && nt < 8192) {
         v = f32c_find(128, 64, 1);
-        occ = 2;
+        occ = 32;
     } else if (ns >= 196608) {
         v = f32c_find(256, 64, 4);
-        occ = 8;
+        occ = 128;
     } else {
         v = f32c_find(128, 64, 4);
-        occ = 8;
+        occ = 128;
     }
 }
 
-// DVM_TUNE_MEASURE: time every variant once at occupancy 4 (after an untimed
-// run), then the best one at the other occupancy targets; keep the fastest.
+// DVM_TUNE_MEASURE: time every variant at the first occupancy target (after
+// an untimed run), then the best one at the other targets; keep the fastest.
 static void f32c_measure(dvm_ctx_t ctx, int kind, int64_t ns, const double* sx, const double* sz,
                          const double* sc, const double* ss, int64_t nt, const double* tx,
                          const double* tz, double r2_skip, double* u, double* w, int& v_best,
@@ -953,10 +979,10 @@ static void f32c_measure(dvm_ctx_t ctx, int kind, int64_t ns, const double* sx, 
         }
     };
     try {
-        for (int v = 0; v < kNumF32c; ++v) time_one(v, 4);
+        for (int v = 0; v < kNumF32c; ++v) time_one(v, kOccupancyFirst);
         const int v0 = v_best;
         for (int occ : kOccupancy)
-            if (occ != 4) time_one(v0, occ);
+            if (occ != kOccupancyFirst) time_one(v0, occ);
     } catch (...) {
         ctx->prof = prof;
         (void)hipEventDestroy(e0);
@@ -1007,7 +1033,7 @@ void f32c_set_variant(dvm_ctx_t ctx, int tile, int block, int share, int occupan
         return;
     }
     require(f32c_find(tile, block, share) >= 0, DVM_E_INVALID, "no such F32C variant");
-    require(occupancy >= 1 && occupancy <= 64, DVM_E_INVALID, "occupancy target out of range");
+    require(occupancy >= 1 && occupancy <= 1024, DVM_E_INVALID, "occupancy target out of range");
     ctx->f32c_forced[0] = tile;
     ctx->f32c_forced[1] = block;
     ctx->f32c_forced[2] = share;

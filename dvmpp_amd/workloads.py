@@ -49,3 +49,39 @@ def projected_blobs(nl, steps, max_npv=3):
     least one blob per step and about 2.3 on average for a bluff body
     (src/VortexSheet.cpp:285-287)."""
     return int(2.3 * nl * steps) + nl * (max_npv + 1)
+
+
+def rk3_cylinder():
+    """RK3 cylinder: r = 0.5, 512 equal panels, U = (1, 0), nu = 5e-4,
+    dt = 0.0025, cutoff_exp = 0.675 (nascent sigma = ds^0.675 = 0.0321),
+    max_npv = 3, REFLECT; remesh with M4' every 20 steps at dx = dz = 0.01 on
+    [-5, 25] x [-10, 10] with the nascent sigma."""
+    px, pz = bodies.cylinder(512, 0.5)
+    ds = float(np.hypot(px[1] - px[0], pz[1] - pz[0]))
+    return px, pz, sheet_params(rho=1.0, nu=5e-4, dt=0.0025, ux=1.0, uz=0.0, cutoff_exp=0.675,
+                                max_npv=3), ds ** 0.675
+
+
+def wake_snapshot(n_target=1_500_000, dx=0.01, sigma=0.0321, seed=5):
+    """A remeshed-wake state of about n_target blobs: blobs on the mesh nodes
+    (spacing dx) of a rectangle behind and around the r = 0.5 body, outside
+    it, with the circulation of two shed shear layers plus a vortex street
+    (omega * dx^2).  Returns dict(x, z, circ, sigma, id, u, w, uvs, wvs)."""
+    rng = np.random.default_rng(seed)
+    width = 10.0
+    length = n_target * dx * dx / width
+    xs = np.arange(-0.7, -0.7 + length, dx)
+    zs = np.arange(-0.5 * width, 0.5 * width, dx)
+    X, Z = np.meshgrid(xs, zs, indexing="xy")
+    x, z = X.ravel(), Z.ravel()
+    keep = x * x + z * z > 0.55 ** 2
+    x, z = x[keep], z[keep]
+    layer = np.exp(-((np.abs(z) - 0.5 - 0.05 * np.maximum(x, 0.0)) / 0.15) ** 2)
+    street = np.sin(2.0 * np.pi * x / 4.0) * np.exp(-z * z / 2.0)
+    omega = -np.sign(z) * 8.0 * layer * np.exp(-0.1 * np.maximum(x, 0.0)) + 2.0 * street
+    omega += rng.normal(0.0, 0.05, x.size)
+    n = x.size
+    zero = np.zeros(n)
+    return {"x": x, "z": z, "circ": omega * dx * dx, "sigma": np.full(n, sigma),
+            "id": np.arange(n, dtype=np.uint32), "u": zero, "w": zero.copy(),
+            "uvs": zero.copy(), "wvs": zero.copy()}

@@ -15,6 +15,8 @@
 #include "dvm_device.hpp"
 #include "dvm_internal.hpp"
 
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -22,8 +24,12 @@
 
 namespace dvmi {
 
-constexpr int kBT = 256;       // targets per item / sources per tile
+constexpr int kBT = 256;       // sources per tile; targets per FP64 item
 constexpr int kColChunks = 64;  // source chunks of the collocation sum per slot
+constexpr int kTB = 64;         // targets per FP32C item (one wave)
+constexpr int kMaxChunks = 4;   // source chunks per slot in the FP32C sum
+constexpr int kMinChunkTiles = 8;  // a chunk spans at least this many tiles
+constexpr int kSortBits = 24;   // Morton key bits sorted (12 per axis)
 
 struct SlotParam {
     double rho, nu, dt, ux, uz, ur, aref, four_nu_dt;
@@ -46,6 +52,8 @@ struct WorkList {
     int total_tiles;
     int work_counter;
     int64_t total_blobs;
+    int total_items;  // FP32C work items (slot, chunk, 64-target block)
+    int pad;
 };
 
 struct BatchDev {
@@ -62,9 +70,20 @@ struct BatchDev {
     int64_t* blob_pre = nullptr;  // S+1
     BlobBuf b, alt;
     double *q1 = nullptr, *q2 = nullptr;
-    float4* s32 = nullptr;
-    float2* near32 = nullptr;
+    float4* s32 = nullptr;      // sorted order
+    float4* near32 = nullptr;   // sorted order
+    double2* p64 = nullptr;     // sorted FP64 positions
     TileInfo* tiles = nullptr;  // S * max_tiles (compact, slot order)
+    // FP32C sum: per-slot Morton sort and chunked work items
+    int* item_pre = nullptr;    // S+1, in `order`
+    double4* sbox = nullptr;    // S bounding boxes
+    uint32_t *keys = nullptr, *keys2 = nullptr, *vals = nullptr, *perm = nullptr;  // S*cap
+    int *seg_begin = nullptr, *seg_end = nullptr;  // S
+    TileInfo* tboxes = nullptr; // S * max_tboxes, 64-target boxes
+    int max_tboxes = 0;
+    double* part = nullptr;     // 2 * kMaxChunks * S * cap chunk partials
+    void* sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
     int* flags = nullptr;
     // sheet
     double *bx = nullptr, *bz = nullptr;  // S*(mp+1)
@@ -83,9 +102,21 @@ struct BatchDev {
 
 // One workgroup: per-slot tile and blob counts, their prefixes, the slot order
 // by decreasing tile count, and the work-counter reset.
-__global__ __launch_bounds__(1024) void bt_prefix_kernel(int S, const SlotState* __restrict__ st,
+// FP32C source chunks of a slot with n live blobs (a function of n alone, so
+// a slot's sum does not depend on the rest of the batch).
+__device__ __forceinline__ int slot_chunks(int64_t n)
+{
+    const int ntl = (int)((n + kBT - 1) / kBT);
+    int c = ntl / kMinChunkTiles;
+    return c < 1 ? 1 : (c > kMaxChunks ? kMaxChunks : c);
+}
+
+__global__ __launch_bounds__(1024) void bt_prefix_kernel(int S, int64_t cap,
+                                                         const SlotState* __restrict__ st,
                                                          WorkList* wl, int* tile_pre, int* order,
-                                                         int* otile_pre, int64_t* blob_pre)
+                                                         int* otile_pre, int64_t* blob_pre,
+                                                         int* item_pre, int* seg_begin,
+                                                         int* seg_end)
 {
     __shared__ int tiles_s[1024];
     const int t = threadIdx.x;
@@ -119,6 +150,20 @@ __global__ __launch_bounds__(1024) void bt_prefix_kernel(int S, const SlotState*
         wl->total_tiles = acc;
         wl->work_counter = 0;
         wl->total_blobs = bacc;
+        int iacc = 0;
+        for (int r = 0; r < S; ++r) {
+            const int s = order[r];
+            const int64_t n = st[s].active ? st[s].n : 0;
+            item_pre[r] = iacc;
+            iacc += (int)((n + kTB - 1) / kTB) * (n > 0 ? slot_chunks(n) : 0);
+        }
+        item_pre[S] = iacc;
+        wl->total_items = iacc;
+    }
+    for (int s = t; s < S; s += 1024) {
+        const int64_t n = st[s].active ? st[s].n : 0;
+        seg_begin[s] = (int)(s * cap);
+        seg_end[s] = (int)(s * cap + n);
     }
 }
 
@@ -147,24 +192,112 @@ __device__ __forceinline__ void for_each_blob(const WorkList* wl, const int64_t*
 }
 
 // ----------------------------------------------------------------- pair sums
+//
+// FP32C (the form of bs_kernels.hip, per slot): each slot's live blobs are
+// Morton-sorted (bounding box, 24-bit keys, segmented radix sort over the
+// slot ranges), cut into 256-source tiles stored as FP32 offsets from their
+// FP64 centres, and summed by a persistent kernel whose waves pull
+// (slot, source chunk, 64-target block) items, largest slots first and chunk
+// by chunk, so co-resident waves stream the same source slice.  Each item
+// writes FP64 chunk partials; a reduce adds them in chunk order and scatters
+// to the storage order.  Chunking and order depend on the slot's own count
+// only.  FP64: one 256-target item per workgroup over all sources, in index
+// order (the reference's summation order).
 
-__global__ __launch_bounds__(kBT) void bt_tiles_kernel(int S, int64_t cap, const WorkList* wl,
+// Per-slot bounding box of the live blobs.
+__global__ __launch_bounds__(256) void bt_bbox_kernel(int64_t cap, const SlotState* __restrict__ st,
+                                                      BlobBuf b, double4* __restrict__ sbox)
+{
+    __shared__ double4 red[256];
+    const int s = blockIdx.x;
+    const int64_t n = st[s].active ? st[s].n : 0;
+    const int64_t base = (int64_t)s * cap;
+    double4 m = make_double4(INFINITY, -INFINITY, INFINITY, -INFINITY);
+    for (int64_t k = threadIdx.x; k < n; k += 256) {
+        const double x = b.x[base + k], z = b.z[base + k];
+        m.x = fmin(m.x, x); m.y = fmax(m.y, x);
+        m.z = fmin(m.z, z); m.w = fmax(m.w, z);
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            const double4 o = red[threadIdx.x + w];
+            double4& r = red[threadIdx.x];
+            r.x = fmin(r.x, o.x); r.y = fmax(r.y, o.y);
+            r.z = fmin(r.z, o.z); r.w = fmax(r.w, o.w);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sbox[s] = red[0];
+}
+
+// Morton key (12 bits per axis over the slot's box) and local index.
+__global__ void bt_morton_kernel(int S, int64_t cap, const WorkList* wl,
+                                 const int64_t* __restrict__ blob_pre,
+                                 const double4* __restrict__ sbox, BlobBuf b,
+                                 uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    for_each_blob(wl, blob_pre, S, cap, [&](int s, int64_t o) {
+        const double4 bb = sbox[s];
+        const double ext = fmax(fmax(bb.y - bb.x, bb.w - bb.z), 1e-300);
+        const double scale = 4095.0 / ext;
+        double qx = (b.x[o] - bb.x) * scale;
+        double qz = (b.z[o] - bb.z) * scale;
+        qx = qx >= 0.0 ? (qx <= 4095.0 ? qx : 4095.0) : 0.0;  // NaN -> 0
+        qz = qz >= 0.0 ? (qz <= 4095.0 ? qz : 4095.0) : 0.0;
+        keys[o] = spread16((uint32_t)qx) | (spread16((uint32_t)qz) << 1);
+        vals[o] = (uint32_t)(o - (int64_t)s * cap);
+    });
+}
+
+// One workgroup per 256-source tile (sorted order): tile record, FP32
+// offsets, near-field constants, sorted FP64 target positions, and the four
+// 64-target boxes of the tile.
+__global__ __launch_bounds__(kBT) void bt_tiles_kernel(int S, int64_t cap, int max_tboxes,
+                                                       const WorkList* wl,
                                                        const int* __restrict__ tile_pre,
                                                        const SlotState* __restrict__ st,
+                                                       const uint32_t* __restrict__ perm,
                                                        BlobBuf b, float4* __restrict__ s32,
-                                                       float2* __restrict__ near32,
-                                                       TileInfo* __restrict__ tiles)
+                                                       float4* __restrict__ near32,
+                                                       double2* __restrict__ p64,
+                                                       TileInfo* __restrict__ tiles,
+                                                       TileInfo* __restrict__ tboxes)
 {
     __shared__ double red[5][kBT];
     const int total = wl->total_tiles;
+    const int lane = threadIdx.x & 63;
     for (int g = blockIdx.x; g < total; g += gridDim.x) {
         const int s = find_slot(tile_pre, S, g);
         const int64_t n = st[s].n;
-        const int64_t k = (int64_t)(g - tile_pre[s]) * kBT + threadIdx.x;
+        const int tile = g - tile_pre[s];
+        const int64_t k = (int64_t)tile * kBT + threadIdx.x;  // sorted rank
         const bool valid = k < n;
-        const int64_t o = (int64_t)s * cap + k;
-        const double xi = valid ? b.x[o] : 0.0, zi = valid ? b.z[o] : 0.0;
-        const double si = valid ? b.s[o] : 0.0;
+        const int64_t base = (int64_t)s * cap;
+        const int64_t src = valid ? base + perm[base + k] : base;
+        const double xi = valid ? b.x[src] : 0.0, zi = valid ? b.z[src] : 0.0;
+        const double si = valid ? b.s[src] : 0.0;
+        // 64-target box of this wave
+        double mnx = valid ? xi : INFINITY, mxx = valid ? xi : -INFINITY;
+        double mnz = valid ? zi : INFINITY, mxz = valid ? zi : -INFINITY;
+        for (int o = 32; o > 0; o >>= 1) {
+            mnx = fmin(mnx, __shfl_xor(mnx, o));
+            mxx = fmax(mxx, __shfl_xor(mxx, o));
+            mnz = fmin(mnz, __shfl_xor(mnz, o));
+            mxz = fmax(mxz, __shfl_xor(mxz, o));
+        }
+        const int64_t w0 = (int64_t)tile * kBT + (threadIdx.x & ~63);
+        if (lane == 0 && w0 < n) {
+            TileInfo tb;
+            tb.cx = 0.5 * (mnx + mxx);
+            tb.cz = 0.5 * (mnz + mxz);
+            tb.hx = (float)(0.5 * (mxx - mnx)) * 1.000001f + 1e-30f;
+            tb.hz = (float)(0.5 * (mxz - mnz)) * 1.000001f + 1e-30f;
+            tb.thr_max = 0.0f;
+            tb.pad = 0.0f;
+            tboxes[(int64_t)s * max_tboxes + w0 / kTB] = tb;
+        }
         red[0][threadIdx.x] = valid ? xi : INFINITY;
         red[1][threadIdx.x] = valid ? xi : -INFINITY;
         red[2][threadIdx.x] = valid ? zi : INFINITY;
@@ -195,26 +328,126 @@ __global__ __launch_bounds__(kBT) void bt_tiles_kernel(int S, int64_t cap, const
             tiles[g] = ti;
         }
         if (valid) {
-            s32[o] = make_float4((float)(xi - cx), (float)(zi - cz), (float)b.c[o], 0.0f);
+            const int64_t o = base + k;
+            s32[o] = make_float4((float)(xi - cx), (float)(zi - cz), (float)b.c[src], 0.0f);
             const double s2 = si * si;
-            near32[o] = make_float2((float)(10.0 * s2), (float)(1.0 / s2));
+            const double ox = xi - cx, oz = zi - cz;
+            near32[o] = make_float4((float)(10.0 * s2), (float)(1.0 / s2),
+                                    (float)(ox - (double)(float)ox), (float)(oz - (double)(float)oz));
+            p64[o] = make_double2(xi, zi);
         }
         __syncthreads();
     }
 }
 
-// Persistent pair sum over (slot, target tile) items, largest slots first.
-// F32C: the bs_f32c_kernel arithmetic (per-tile FP64 recentring, far / near
-// loops, Kahan across tiles).  F64: the reference pair in source order.
-template <bool F32C>
-__global__ __launch_bounds__(kBT) void bt_bs_kernel(int S, int64_t cap, WorkList* wl,
-                                                    const int* __restrict__ tile_pre,
-                                                    const int* __restrict__ order,
-                                                    const int* __restrict__ otile_pre,
-                                                    const SlotState* __restrict__ st, BlobBuf b,
-                                                    const float4* __restrict__ s32,
-                                                    const float2* __restrict__ near32,
-                                                    const TileInfo* __restrict__ tiles)
+// FP32C pair sum, one wave per (slot, chunk, 64-target block) item.
+__global__ __launch_bounds__(256) void bt_bs32_kernel(int S, int64_t cap, int max_tboxes,
+                                                      WorkList* wl,
+                                                      const int* __restrict__ tile_pre,
+                                                      const int* __restrict__ order,
+                                                      const int* __restrict__ item_pre,
+                                                      const SlotState* __restrict__ st,
+                                                      const float4* __restrict__ s32,
+                                                      const float4* __restrict__ near32,
+                                                      const double2* __restrict__ p64,
+                                                      const TileInfo* __restrict__ tiles,
+                                                      const TileInfo* __restrict__ tboxes,
+                                                      double* __restrict__ part)
+{
+    const int lane = threadIdx.x & 63;
+    const int total = wl->total_items;
+    const int64_t nb = (int64_t)S * cap;
+    for (;;) {
+        int item = 0;
+        if (lane == 0) item = atomicAdd(&wl->work_counter, 1);
+        item = __shfl(item, 0);
+        if (item >= total) break;
+        const int p = find_slot(item_pre, S, item);
+        const int s = order[p];
+        const int local = item - item_pre[p];
+        const int64_t n = st[s].n;
+        const int nblk = (int)((n + kTB - 1) / kTB);
+        const int nch = slot_chunks(n);
+        const int c = local / nblk;
+        const int tbk = local - c * nblk;
+        const int ntl = (int)((n + kBT - 1) / kBT);
+        const int tpc = (ntl + nch - 1) / nch;
+        const int s0 = c * tpc;
+        const int s1 = s0 + tpc < ntl ? s0 + tpc : ntl;
+        const int64_t base = (int64_t)s * cap;
+        const int64_t i = (int64_t)tbk * kTB + lane;  // sorted rank
+        const int64_t it = i < n ? i : n - 1;
+        const double2 pi = p64[base + it];
+        const double xi = pi.x, zi = pi.y;
+        const TileInfo tb = tboxes[(int64_t)s * max_tboxes + tbk];
+        float su = 0.0f, cu = 0.0f, sw = 0.0f, cw = 0.0f;
+        for (int sti = s0; sti < s1; ++sti) {
+            const TileInfo ts = tiles[tile_pre[s] + sti];
+            const double Dxd = ts.cx - tb.cx;
+            const double Dzd = ts.cz - tb.cz;
+            const double gx = fmax(fabs(Dxd) - (double)(tb.hx + ts.hx), 0.0);
+            const double gz = fmax(fabs(Dzd) - (double)(tb.hz + ts.hz), 0.0);
+            const double g2 = gx * gx + gz * gz;
+            const bool near = g2 <= 1.0001 * (double)ts.thr_max + 1e-300;
+            const float txd = (float)(xi - ts.cx);
+            const float tzd = (float)(zi - ts.cz);
+            const int64_t jr = (int64_t)sti * kBT;  // sorted rank of the tile's first source
+            const int cnt = n - jr < kBT ? (int)(n - jr) : kBT;
+            const float4* __restrict__ sp = s32 + base + jr;
+            float au = 0.0f, aw = 0.0f;
+            if (!near) {
+                const double ex = fabs(Dxd) + (double)(tb.hx + ts.hx);
+                const double ez = fabs(Dzd) + (double)(tb.hz + ts.hz);
+                if (cnt == kBT && g2 > 1e-8 && ex * ex + ez * ez < 1e8)
+                    far_tile_share4<kBT>(sp, txd, tzd, au, aw);
+                else if (cnt == kBT)
+                    far_tile<kBT>(sp, txd, tzd, au, aw);
+                else
+                    for (int k = 0; k < cnt; ++k) far_pair(sp[k], txd, tzd, au, aw);
+            } else {
+                const float txl = (float)((xi - ts.cx) - (double)txd);
+                const float tzl = (float)((zi - ts.cz) - (double)tzd);
+                near_range<true, false>(sp, near32 + base + jr, 0, cnt, txd, tzd, txl, tzl, jr, i,
+                                        0.0f, au, aw);
+            }
+            kahan_add(su, cu, au);
+            kahan_add(sw, cw, aw);
+        }
+        if (i < n) {
+            part[((int64_t)c * 2 + 0) * nb + base + i] = -((double)su - (double)cu);
+            part[((int64_t)c * 2 + 1) * nb + base + i] = (double)sw - (double)cw;
+        }
+    }
+}
+
+// Adds a blob's chunk partials in chunk order, scales by 1/(2 pi) and writes
+// the velocity at the blob's storage position.
+__global__ void bt_reduce32_kernel(int S, int64_t cap, const WorkList* wl,
+                                   const int64_t* __restrict__ blob_pre,
+                                   const SlotState* __restrict__ st,
+                                   const uint32_t* __restrict__ perm,
+                                   const double* __restrict__ part, BlobBuf b)
+{
+    const int64_t nb = (int64_t)S * cap;
+    for_each_blob(wl, blob_pre, S, cap, [&](int s, int64_t o) {
+        const int nch = slot_chunks(st[s].n);
+        double su = 0.0, sw = 0.0;
+        for (int c = 0; c < nch; ++c) {
+            su += part[((int64_t)c * 2 + 0) * nb + o];
+            sw += part[((int64_t)c * 2 + 1) * nb + o];
+        }
+        const int64_t dst = (int64_t)s * cap + perm[o];
+        b.u[dst] = su * kRpi2;
+        b.w[dst] = sw * kRpi2;
+    });
+}
+
+// FP64 pair sum: one workgroup per (slot, 256-target tile), sources in index
+// order (src/VelocityKernel.cpp:37-61 per slot).
+__global__ __launch_bounds__(kBT) void bt_bs64_kernel(int S, int64_t cap, WorkList* wl,
+                                                      const int* __restrict__ order,
+                                                      const int* __restrict__ otile_pre,
+                                                      const SlotState* __restrict__ st, BlobBuf b)
 {
     __shared__ int item_sh;
     const int total = wl->total_tiles;
@@ -233,78 +466,14 @@ __global__ __launch_bounds__(kBT) void bt_bs_kernel(int S, int64_t cap, WorkList
         const int64_t it = i < n ? i : n - 1;
         const double xi = b.x[base + it];
         const double zi = b.z[base + it];
-        const int nt = (int)((n + kBT - 1) / kBT);
-        if (F32C) {
-            const TileInfo tb = tiles[tile_pre[s] + tt];
-            float su = 0.0f, cu = 0.0f, sw = 0.0f, cw = 0.0f;
-            for (int sti = 0; sti < nt; ++sti) {
-                const TileInfo ts = tiles[tile_pre[s] + sti];
-                const double gx = fmax(fabs(ts.cx - tb.cx) - (double)(tb.hx + ts.hx), 0.0);
-                const double gz = fmax(fabs(ts.cz - tb.cz) - (double)(tb.hz + ts.hz), 0.0);
-                const bool near = gx * gx + gz * gz <= 1.0001 * (double)ts.thr_max + 1e-300;
-                const float txd = (float)(xi - ts.cx);
-                const float tzd = (float)(zi - ts.cz);
-                const int64_t jb = base + (int64_t)sti * kBT;
-                const int cnt = n - (int64_t)sti * kBT < kBT ? (int)(n - (int64_t)sti * kBT) : kBT;
-                const float4* __restrict__ sp = s32 + jb;
-                float au = 0.0f, aw = 0.0f;
-                if (!near) {
-                    if (cnt == kBT) {
-#pragma unroll 16
-                        for (int k = 0; k < kBT; ++k) {
-                            const float4 sj = sp[k];
-                            const float dx = txd - sj.x;
-                            const float dz = tzd - sj.y;
-                            const float r2 = __builtin_fmaf(dx, dx, dz * dz);
-                            const float kk = __builtin_amdgcn_rcpf(r2) * sj.z;
-                            au = __builtin_fmaf(kk, dz, au);
-                            aw = __builtin_fmaf(kk, dx, aw);
-                        }
-                    } else {
-                        for (int k = 0; k < cnt; ++k) {
-                            const float4 sj = sp[k];
-                            const float dx = txd - sj.x;
-                            const float dz = tzd - sj.y;
-                            const float r2 = __builtin_fmaf(dx, dx, dz * dz);
-                            const float kk = __builtin_amdgcn_rcpf(r2) * sj.z;
-                            au = __builtin_fmaf(kk, dz, au);
-                            aw = __builtin_fmaf(kk, dx, aw);
-                        }
-                    }
-                } else {
-                    const float2* __restrict__ np = near32 + jb;
-                    for (int k = 0; k < cnt; ++k) {
-                        const float4 sj = sp[k];
-                        const float2 nj = np[k];
-                        const float dx = (float)(xi - b.x[jb + k]);
-                        const float dz = (float)(zi - b.z[jb + k]);
-                        const float r2 = __builtin_fmaf(dx, dx, dz * dz);
-                        const float inv = __builtin_amdgcn_rcpf(r2);
-                        float K = inv;
-                        if (r2 < nj.x) K = -expm1f(-r2 * nj.y) * inv;
-                        const bool take = ((int64_t)sti * kBT + k) != i;
-                        const float kk = take ? K * sj.z : 0.0f;
-                        au = __builtin_fmaf(kk, dz, au);
-                        aw = __builtin_fmaf(kk, dx, aw);
-                    }
-                }
-                kahan_add(su, cu, au);
-                kahan_add(sw, cw, aw);
-            }
-            if (i < n) {
-                b.u[base + i] = -((double)su - (double)cu) * kRpi2;
-                b.w[base + i] = ((double)sw - (double)cw) * kRpi2;
-            }
-        } else {
-            double ui = 0.0, wi = 0.0;
-            for (int64_t j = 0; j < n; ++j) {
-                if (j == i) continue;
-                bs_pair_f64(xi, zi, b.x[base + j], b.z[base + j], b.c[base + j], b.s[base + j], ui, wi);
-            }
-            if (i < n) {
-                b.u[base + i] = ui * kRpi2;
-                b.w[base + i] = wi * kRpi2;
-            }
+        double ui = 0.0, wi = 0.0;
+        for (int64_t j = 0; j < n; ++j) {
+            if (j == i) continue;
+            bs_pair_f64(xi, zi, b.x[base + j], b.z[base + j], b.c[base + j], b.s[base + j], ui, wi);
+        }
+        if (i < n) {
+            b.u[base + i] = ui * kRpi2;
+            b.w[base + i] = wi * kRpi2;
         }
     }
 }
@@ -768,9 +937,28 @@ size_t batch_bytes(const dvm_batch_config& c, BatchDev* d, char* mem)
     carve_blob_set(cv, o.alt, nb);
     o.q1 = cv.take<double>(nb);
     o.q2 = cv.take<double>(nb);
-    o.s32 = cv.take<float4>(nb);
-    o.near32 = cv.take<float2>(nb);
+    const bool f32c = c.prec == DVM_F32C;
+    const size_t nf = f32c ? nb : 0;  // FP32C-only arrays
+    const size_t maxtb = (cap + kTB - 1) / kTB;
+    o.s32 = cv.take<float4>(nf);
+    o.near32 = cv.take<float4>(nf);
+    o.p64 = cv.take<double2>(nf);
     o.tiles = cv.take<TileInfo>(S * maxt);
+    o.item_pre = cv.take<int>(S + 1);
+    o.sbox = cv.take<double4>(S);
+    o.keys = cv.take<uint32_t>(nf);
+    o.keys2 = cv.take<uint32_t>(nf);
+    o.vals = cv.take<uint32_t>(nf);
+    o.perm = cv.take<uint32_t>(nf);
+    o.seg_begin = cv.take<int>(S);
+    o.seg_end = cv.take<int>(S);
+    o.tboxes = cv.take<TileInfo>(f32c ? S * maxtb : 0);
+    o.max_tboxes = (int)maxtb;
+    o.part = cv.take<double>(2 * kMaxChunks * nf);
+    // segmented-sort scratch: two key/value buffers' worth plus 1 MiB (the
+    // size the library asks for is checked against it at creation)
+    o.sort_tmp_bytes = f32c ? 8 * nb + (1u << 20) : 0;
+    o.sort_tmp = cv.take<char>(o.sort_tmp_bytes);
     o.flags = cv.take<int>(nb);
     o.bx = cv.take<double>(np1);
     o.bz = cv.take<double>(np1);
@@ -803,15 +991,24 @@ void record_convect(dvm_batch_t B)
     const double bb[3] = {1. / 4, 8. / 9, 3. / 4};
     const int stages = rk ? 3 : 1;
     for (int k = 0; k < stages; ++k) {
-        bt_prefix_kernel<<<1, 1024, 0, st>>>(S, d.st, d.wl, d.tile_pre, d.order, d.otile_pre, d.blob_pre);
+        bt_prefix_kernel<<<1, 1024, 0, st>>>(S, cap, d.st, d.wl, d.tile_pre, d.order, d.otile_pre,
+                                             d.blob_pre, d.item_pre, d.seg_begin, d.seg_end);
         if (rk && k == 0) bt_zero_q_kernel<<<gs, 256, 0, st>>>(S, cap, d.wl, d.blob_pre, d.q1, d.q2);
         if (f32c) {
-            bt_tiles_kernel<<<gs, kBT, 0, st>>>(S, cap, d.wl, d.tile_pre, d.st, d.b, d.s32, d.near32, d.tiles);
-            bt_bs_kernel<true><<<persistent, kBT, 0, st>>>(S, cap, d.wl, d.tile_pre, d.order, d.otile_pre,
-                                                           d.st, d.b, d.s32, d.near32, d.tiles);
+            bt_bbox_kernel<<<S, 256, 0, st>>>(cap, d.st, d.b, d.sbox);
+            bt_morton_kernel<<<gs, 256, 0, st>>>(S, cap, d.wl, d.blob_pre, d.sbox, d.b, d.keys, d.vals);
+            size_t tmp_bytes = d.sort_tmp_bytes;
+            DVM_HIP_CHECK(hipcub::DeviceSegmentedRadixSort::SortPairs(
+                d.sort_tmp, tmp_bytes, d.keys, d.keys2, d.vals, d.perm, (int)(S * cap), S,
+                d.seg_begin, d.seg_end, 0, kSortBits, st));
+            bt_tiles_kernel<<<gs, kBT, 0, st>>>(S, cap, d.max_tboxes, d.wl, d.tile_pre, d.st, d.perm,
+                                                d.b, d.s32, d.near32, d.p64, d.tiles, d.tboxes);
+            bt_bs32_kernel<<<persistent, 256, 0, st>>>(S, cap, d.max_tboxes, d.wl, d.tile_pre, d.order,
+                                                       d.item_pre, d.st, d.s32, d.near32, d.p64,
+                                                       d.tiles, d.tboxes, d.part);
+            bt_reduce32_kernel<<<gs, 256, 0, st>>>(S, cap, d.wl, d.blob_pre, d.st, d.perm, d.part, d.b);
         } else {
-            bt_bs_kernel<false><<<persistent, kBT, 0, st>>>(S, cap, d.wl, d.tile_pre, d.order, d.otile_pre,
-                                                            d.st, d.b, d.s32, d.near32, d.tiles);
+            bt_bs64_kernel<<<persistent, kBT, 0, st>>>(S, cap, d.wl, d.order, d.otile_pre, d.st, d.b);
         }
         bt_colloc_kernel<<<dim3(kColChunks, S), 256, 0, st>>>(cap, mp, d.prm, d.st, d.b, d.panel, d.col);
         bt_gamma_kernel<<<S, 256, sizeof(double) * (mp + 1), st>>>(mp, d.prm, d.st, d.enx, d.enz, d.P,
@@ -831,14 +1028,16 @@ void record_rest(dvm_batch_t B)
     const int64_t cap = d.cap;
     const int mp = d.mp;
     const unsigned gs = 4096;
-    bt_prefix_kernel<<<1, 1024, 0, st>>>(S, d.st, d.wl, d.tile_pre, d.order, d.otile_pre, d.blob_pre);
+    bt_prefix_kernel<<<1, 1024, 0, st>>>(S, cap, d.st, d.wl, d.tile_pre, d.order, d.otile_pre,
+                                         d.blob_pre, d.item_pre, d.seg_begin, d.seg_end);
     bt_diffuse_kernel<<<gs, 256, 0, st>>>(S, cap, d.wl, d.blob_pre, d.prm, d.st, d.b);
     bt_release_count_kernel<<<S, 1024, 0, st>>>(cap, mp, d.prm, d.st, d.gamma, d.ds, d.cnt, d.off, d.cnew);
     bt_release_emit_kernel<<<dim3(16, S), 256, 0, st>>>(cap, mp, d.prm, d.st, d.bx, d.bz, d.ds, d.etx,
                                                          d.etz, d.enx, d.enz, d.cnt, d.off, d.cnew,
                                                          d.panel, d.b);
     bt_commit_release_kernel<<<(S + 255) / 256, 256, 0, st>>>(S, d.st);
-    bt_prefix_kernel<<<1, 1024, 0, st>>>(S, d.st, d.wl, d.tile_pre, d.order, d.otile_pre, d.blob_pre);
+    bt_prefix_kernel<<<1, 1024, 0, st>>>(S, cap, d.st, d.wl, d.tile_pre, d.order, d.otile_pre,
+                                         d.blob_pre, d.item_pre, d.seg_begin, d.seg_end);
     const int reflect = B->cfg.crossing != DVM_DELETE ? 1 : 0;
     bt_inside_kernel<<<gs, 256, 0, st>>>(S, cap, mp, d.wl, d.blob_pre, d.prm, d.bx, d.bz, d.enx, d.enz,
                                          reflect, d.b, d.flags);
@@ -946,9 +1145,20 @@ dvm_status dvm_batch_create(dvm_ctx_t ctx, const dvm_batch_config* cfg, dvm_batc
         B->d.cap = c.slot_capacity;
         B->d.mp = c.max_panels;
         B->d.max_tiles = (int)((c.slot_capacity + kBT - 1) / kBT);
+        if (c.prec == DVM_F32C)
+            require((int64_t)c.slots * c.slot_capacity < (int64_t(1) << 31), DVM_E_INVALID,
+                    "slots x slot capacity must stay below 2^31 blobs");
         const size_t bytes = batch_bytes(c, nullptr, nullptr);
         DVM_HIP_CHECK(hipMalloc(&B->mem, bytes));
         batch_bytes(c, &B->d, B->mem);
+        if (c.prec == DVM_F32C) {
+            size_t need = 0;
+            DVM_HIP_CHECK(hipcub::DeviceSegmentedRadixSort::SortPairs(
+                nullptr, need, B->d.keys, B->d.keys2, B->d.vals, B->d.perm,
+                (int)(c.slots * c.slot_capacity), c.slots, B->d.seg_begin, B->d.seg_end, 0,
+                kSortBits, ctx->stream));
+            require(need <= B->d.sort_tmp_bytes, DVM_E_STATE, "segmented sort scratch too small");
+        }
         DVM_HIP_CHECK(hipMemsetAsync(B->d.st, 0, sizeof(SlotState) * c.slots, ctx->stream));
         DVM_HIP_CHECK(hipMemsetAsync(B->d.prm, 0, sizeof(SlotParam) * c.slots, ctx->stream));
         DVM_HIP_CHECK(hipStreamSynchronize(ctx->stream));

@@ -19,8 +19,9 @@
 //           bounding boxes (source tile, target block of 64 or 256) are
 //           farther apart than sqrt(10)*max(sigma) takes the far-field loop
 //           (1/r^2 only: 7 VALU ops per pair and one reciprocal per 1, 2 or
-//           4 pairs); the others take the full regularised loop with dx in
-//           FP64.  Partial sums within a tile accumulate in FP32 with FMA;
+//           4 pairs); the others take the full regularised loop (near_k:
+//           Taylor series or hardware exponential for 1 - e^-x).  Partial
+//           sums within a tile accumulate in FP32 with FMA;
 //           tile partials are added with Kahan compensation and the result
 //           is returned in FP64.  The variant (tile, block, sources per
 //           reciprocal, occupancy target) comes from a size table or is
@@ -171,16 +172,6 @@ __global__ void bbox_partial_kernel(int64_t n, const double* __restrict__ x,
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-__device__ inline uint32_t spread16(uint32_t v)
-{
-    v &= 0xffffu;
-    v = (v | (v << 8)) & 0x00ff00ffu;
-    v = (v | (v << 4)) & 0x0f0f0f0fu;
-    v = (v | (v << 2)) & 0x33333333u;
-    v = (v | (v << 1)) & 0x55555555u;
-    return v;
-}
-
 // Morton key on a square 65536^2 grid spanning the bounding box.
 __global__ void morton_kernel(int64_t n, const double* __restrict__ x, const double* __restrict__ z,
                               int nparts, const double4* __restrict__ part,
@@ -207,12 +198,12 @@ __global__ void morton_kernel(int64_t n, const double* __restrict__ x, const dou
 // One block per tile of T consecutive (possibly permuted) points: FP64 centre
 // and half extents; for sources also the FP32 offsets from the centre with
 // circulation, the FP32 near-field constants and an FP64 sorted copy of the
-// positions (read by the near-field loop only).
+// positions (the blob-on-blob targets).
 template <int T>
 __global__ __launch_bounds__(T) void tile_build_kernel(
     int64_t n, const uint32_t* __restrict__ perm, const double* __restrict__ x,
     const double* __restrict__ z, const double* __restrict__ c, const double* __restrict__ s,
-    float4* __restrict__ p32, float2* __restrict__ near32, double2* __restrict__ p64,
+    float4* __restrict__ p32, float4* __restrict__ near32, double2* __restrict__ p64,
     TileInfo* __restrict__ tiles)
 {
     __shared__ double red[5][T];
@@ -255,88 +246,10 @@ __global__ __launch_bounds__(T) void tile_build_kernel(
     const double ci = c[src];
     p32[k] = make_float4((float)(xi - cx), (float)(zi - cz), (float)ci, 0.0f);
     const double s2 = si * si;
-    near32[k] = make_float2((float)(10.0 * s2), (float)(1.0 / s2));
+    const double ox = xi - cx, oz = zi - cz;
+    near32[k] = make_float4((float)(10.0 * s2), (float)(1.0 / s2), (float)(ox - (double)(float)ox),
+                            (float)(oz - (double)(float)oz));
     p64[k] = make_double2(xi, zi);
-}
-
-// Far-field loops over one source tile: sum of circ_j * (dz, dx) / r^2.
-// The shared forms take one reciprocal for 2 (or 4) sources:
-// 1/ra = rb * (1/(ra rb)); they are used only when every r^2 of the tile pair
-// lies in a range where the product stays a normal FP32 number.
-
-__device__ __forceinline__ void far_pair(float4 sj, float txd, float tzd, float& au, float& aw)
-{
-    const float dx = txd - sj.x;
-    const float dz = tzd - sj.y;
-    const float r2 = __builtin_fmaf(dx, dx, dz * dz);
-    const float kk = __builtin_amdgcn_rcpf(r2) * sj.z;
-    au = __builtin_fmaf(kk, dz, au);
-    aw = __builtin_fmaf(kk, dx, aw);
-}
-
-template <int T>
-__device__ __forceinline__ void far_tile(const float4* __restrict__ sp, float txd, float tzd,
-                                         float& au, float& aw)
-{
-#pragma unroll 16
-    for (int k = 0; k < T; ++k) far_pair(sp[k], txd, tzd, au, aw);
-}
-
-template <int T>
-__device__ __forceinline__ void far_tile_share2(const float4* __restrict__ sp, float txd,
-                                                float tzd, float& au, float& aw)
-{
-#pragma unroll 8
-    for (int k = 0; k < T; k += 2) {
-        const float4 sa = sp[k];
-        const float4 sb = sp[k + 1];
-        const float dxa = txd - sa.x, dza = tzd - sa.y;
-        const float dxb = txd - sb.x, dzb = tzd - sb.y;
-        const float ra = __builtin_fmaf(dxa, dxa, dza * dza);
-        const float rb = __builtin_fmaf(dxb, dxb, dzb * dzb);
-        const float inv = __builtin_amdgcn_rcpf(ra * rb);
-        const float ka = (rb * inv) * sa.z;
-        const float kb = (ra * inv) * sb.z;
-        au = __builtin_fmaf(ka, dza, au);
-        aw = __builtin_fmaf(ka, dxa, aw);
-        au = __builtin_fmaf(kb, dzb, au);
-        aw = __builtin_fmaf(kb, dxb, aw);
-    }
-}
-
-template <int T>
-__device__ __forceinline__ void far_tile_share4(const float4* __restrict__ sp, float txd,
-                                                float tzd, float& au, float& aw)
-{
-#pragma unroll 4
-    for (int k = 0; k < T; k += 4) {
-        const float4 sa = sp[k], sb = sp[k + 1], sc = sp[k + 2], sd = sp[k + 3];
-        const float dxa = txd - sa.x, dza = tzd - sa.y;
-        const float dxb = txd - sb.x, dzb = tzd - sb.y;
-        const float dxc = txd - sc.x, dzc = tzd - sc.y;
-        const float dxd = txd - sd.x, dzd = tzd - sd.y;
-        const float ra = __builtin_fmaf(dxa, dxa, dza * dza);
-        const float rb = __builtin_fmaf(dxb, dxb, dzb * dzb);
-        const float rc = __builtin_fmaf(dxc, dxc, dzc * dzc);
-        const float rd = __builtin_fmaf(dxd, dxd, dzd * dzd);
-        const float pab = ra * rb;
-        const float pcd = rc * rd;
-        const float inv = __builtin_amdgcn_rcpf(pab * pcd);
-        const float iab = inv * pcd;
-        const float icd = inv * pab;
-        const float ka = (rb * iab) * sa.z;
-        const float kb = (ra * iab) * sb.z;
-        const float kc = (rd * icd) * sc.z;
-        const float kd = (rc * icd) * sd.z;
-        au = __builtin_fmaf(ka, dza, au);
-        aw = __builtin_fmaf(ka, dxa, aw);
-        au = __builtin_fmaf(kb, dzb, au);
-        aw = __builtin_fmaf(kb, dxb, aw);
-        au = __builtin_fmaf(kc, dzc, au);
-        aw = __builtin_fmaf(kc, dxc, aw);
-        au = __builtin_fmaf(kd, dzd, au);
-        aw = __builtin_fmaf(kd, dxd, aw);
-    }
 }
 
 // TILE sources per source tile, BLOCK targets per block (one per lane),
@@ -344,13 +257,13 @@ __device__ __forceinline__ void far_tile_share4(const float4* __restrict__ sp, f
 // copy p64 (target i is source i); otherwise tx, tz in the caller's order.
 // tblk: bounding box of each target block (the near/far test).  Per source
 // tile each target is recentred on the tile centre in FP64 and rounded once,
-// so a far pair's dx = (x_i - c) - (x_j - c) carries only FP32 rounding
-// relative to |x - c|; near pairs form dx in FP64.
+// so a pair's dx = (x_i - c) - (x_j - c) carries only FP32 rounding relative
+// to |x - c| (a few 1e-8 absolute for near pairs, whose tiles span tenths).
 template <int TILE, int BLOCK, int SHARE, bool SELF, bool GUARD>
 __global__ __launch_bounds__(BLOCK) void bs_f32c_kernel(
     int64_t nt, const double* __restrict__ tx, const double* __restrict__ tz,
     const TileInfo* __restrict__ tblk, int64_t ns, const float4* __restrict__ s32,
-    const float2* __restrict__ near32, const double2* __restrict__ p64,
+    const float4* __restrict__ near32, const double2* __restrict__ p64,
     const TileInfo* __restrict__ stile, int ntiles, int tiles_per_chunk, float r2_skip,
     const uint32_t* __restrict__ out_index, double* __restrict__ u, double* __restrict__ w,
     double* __restrict__ part)
@@ -410,25 +323,10 @@ __global__ __launch_bounds__(BLOCK) void bs_f32c_kernel(
                 for (int k = 0; k < cnt; ++k) far_pair(sp[k], txd, tzd, au, aw);
             }
         } else {
-            const float2* __restrict__ np = near32 + jb;
-            const double2* __restrict__ pp = p64 + jb;
-            for (int k = 0; k < cnt; ++k) {
-                const float4 sj = sp[k];
-                const float2 nj = np[k];
-                const double2 pj = pp[k];
-                const float dx = (float)(xi - pj.x);
-                const float dz = (float)(zi - pj.y);
-                const float r2 = __builtin_fmaf(dx, dx, dz * dz);
-                const float inv = __builtin_amdgcn_rcpf(r2);
-                float K = inv;
-                if (r2 < nj.x) K = -expm1f(-r2 * nj.y) * inv;
-                bool take = true;
-                if (SELF) take = (jb + k) != i;
-                if (GUARD) take = take && (r2 > r2_skip);
-                const float kk = take ? K * sj.z : 0.0f;
-                au = __builtin_fmaf(kk, dz, au);
-                aw = __builtin_fmaf(kk, dx, aw);
-            }
+            const float txl = (float)((xi - ts.cx) - (double)txd);
+            const float tzl = (float)((zi - ts.cz) - (double)tzd);
+            near_range<SELF, GUARD>(sp, near32 + jb, 0, cnt, txd, tzd, txl, tzl, jb, i, r2_skip,
+                                    au, aw);
         }
         kahan_add(su, cu, au);
         kahan_add(sw, cw, aw);
@@ -485,7 +383,7 @@ template <int TILE, bool SELF, bool GUARD>
 __global__ __launch_bounds__(256) void bs_f32m_kernel(
     int64_t nt, const double* __restrict__ tx, const double* __restrict__ tz,
     const TileInfo* __restrict__ tgrp, int64_t ns, const float4* __restrict__ s32,
-    const float2* __restrict__ near32, const double2* __restrict__ p64,
+    const float4* __restrict__ near32, const double2* __restrict__ p64,
     const TileInfo* __restrict__ stile, int ntiles, int tiles_per_chunk, float r2_skip,
     float kappa2, const uint32_t* __restrict__ out_index, double* __restrict__ u,
     double* __restrict__ w, double* __restrict__ part)
@@ -626,25 +524,12 @@ __global__ __launch_bounds__(256) void bs_f32m_kernel(
                 const float tzd = (float)(zi - ts.cz);
                 for (int k = k0; k < k1; ++k) far_pair(sp[k], txd, tzd, au, aw);
             } else {
-                const float2* __restrict__ np = near32 + jb;
-                const double2* __restrict__ pp = p64 + jb;
-                for (int k = k0; k < k1; ++k) {
-                    const float4 sj = sp[k];
-                    const float2 nj = np[k];
-                    const double2 pj = pp[k];
-                    const float dx = (float)(xi - pj.x);
-                    const float dz = (float)(zi - pj.y);
-                    const float r2 = __builtin_fmaf(dx, dx, dz * dz);
-                    const float inv = __builtin_amdgcn_rcpf(r2);
-                    float K = inv;
-                    if (r2 < nj.x) K = -expm1f(-r2 * nj.y) * inv;
-                    bool take = true;
-                    if (SELF) take = (jb + k) != i;
-                    if (GUARD) take = take && (r2 > r2_skip);
-                    const float kk = take ? K * sj.z : 0.0f;
-                    au = __builtin_fmaf(kk, dz, au);
-                    aw = __builtin_fmaf(kk, dx, aw);
-                }
+                const float txd = (float)(xi - ts.cx);
+                const float tzd = (float)(zi - ts.cz);
+                const float txl = (float)((xi - ts.cx) - (double)txd);
+                const float tzl = (float)((zi - ts.cz) - (double)tzd);
+                near_range<SELF, GUARD>(sp, near32 + jb, k0, k1, txd, tzd, txl, tzl, jb, i, r2_skip,
+                                        au, aw);
             }
             kahan_add(su, cu, au);
             kahan_add(sw, cw, aw);
@@ -744,7 +629,7 @@ struct F32cArgs {
     const TileInfo* tblk;
     int64_t ns;
     const float4* s32;
-    const float2* near32;
+    const float4* near32;
     const double2* p64;
     const TileInfo* stile;
     int ntiles;
@@ -792,7 +677,7 @@ static void f32m_dispatch(int kind, dim3 grid, hipStream_t st, const F32cArgs& a
 template <int T>
 static void tile_build(hipStream_t st, int64_t n, const uint32_t* perm, const double* x,
                        const double* z, const double* c, const double* s, float4* p32,
-                       float2* near32, double2* p64, TileInfo* tiles)
+                       float4* near32, double2* p64, TileInfo* tiles)
 {
     tile_build_kernel<T><<<(unsigned)cdiv(n, T), T, 0, st>>>(n, perm, x, z, c, s, p32, near32,
                                                              p64, tiles);
@@ -800,7 +685,7 @@ static void tile_build(hipStream_t st, int64_t n, const uint32_t* perm, const do
 
 using F32cFn = void (*)(int, dim3, hipStream_t, const F32cArgs&);
 using TileFn = void (*)(hipStream_t, int64_t, const uint32_t*, const double*, const double*,
-                        const double*, const double*, float4*, float2*, double2*, TileInfo*);
+                        const double*, const double*, float4*, float4*, double2*, TileInfo*);
 
 struct F32cKernel {
     int tile, block, share;
@@ -873,7 +758,7 @@ static void f32c_run(dvm_ctx_t ctx, int kind, int v, int occ, int64_t ns, const 
                                                      ctx->stream));
     const size_t need = Carver::need({
         sizeof(uint32_t) * ns * 4, sizeof(double4) * nbb, sort_tmp, sizeof(float4) * ns,
-        sizeof(float2) * ns, sizeof(double2) * ns, sizeof(TileInfo) * ntiles,
+        sizeof(float4) * ns, sizeof(double2) * ns, sizeof(TileInfo) * ntiles,
         sizeof(TileInfo) * bx, nch > 1 ? sizeof(double) * 2 * nt * nch : 0});
     Carver cv(ws_get(ctx, need));
     uint32_t* keys = cv.take<uint32_t>(ns);
@@ -883,7 +768,7 @@ static void f32c_run(dvm_ctx_t ctx, int kind, int v, int occ, int64_t ns, const 
     double4* bb = cv.take<double4>(nbb);
     void* tmp = cv.take<char>(sort_tmp);
     float4* s32 = cv.take<float4>(ns);
-    float2* near32 = cv.take<float2>(ns);
+    float4* near32 = cv.take<float4>(ns);
     double2* p64 = cv.take<double2>(ns);
     TileInfo* stile = cv.take<TileInfo>(ntiles);
     TileInfo* tblk = cv.take<TileInfo>(bx);

@@ -104,4 +104,150 @@ __device__ __forceinline__ void bs_pair_f64(double xi, double zi, double xj, dou
     wi += K * dx * cj;
 }
 
+// Spreads the low 16 bits of v to the even bits (Morton interleave).
+__device__ inline uint32_t spread16(uint32_t v)
+{
+    v &= 0xffffu;
+    v = (v | (v << 8)) & 0x00ff00ffu;
+    v = (v | (v << 4)) & 0x0f0f0f0fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+// Far-field loops over one source tile: sum of circ_j * (dz, dx) / r^2.
+// The shared forms take one reciprocal for 2 (or 4) sources:
+// 1/ra = rb * (1/(ra rb)); they are used only when every r^2 of the tile pair
+// lies in a range where the product stays a normal FP32 number.
+
+__device__ __forceinline__ void far_pair(float4 sj, float txd, float tzd, float& au, float& aw)
+{
+    const float dx = txd - sj.x;
+    const float dz = tzd - sj.y;
+    const float r2 = __builtin_fmaf(dx, dx, dz * dz);
+    const float kk = __builtin_amdgcn_rcpf(r2) * sj.z;
+    au = __builtin_fmaf(kk, dz, au);
+    aw = __builtin_fmaf(kk, dx, aw);
+}
+
+template <int T>
+__device__ __forceinline__ void far_tile(const float4* __restrict__ sp, float txd, float tzd,
+                                         float& au, float& aw)
+{
+#pragma unroll 16
+    for (int k = 0; k < T; ++k) far_pair(sp[k], txd, tzd, au, aw);
+}
+
+template <int T>
+__device__ __forceinline__ void far_tile_share2(const float4* __restrict__ sp, float txd,
+                                                float tzd, float& au, float& aw)
+{
+#pragma unroll 8
+    for (int k = 0; k < T; k += 2) {
+        const float4 sa = sp[k];
+        const float4 sb = sp[k + 1];
+        const float dxa = txd - sa.x, dza = tzd - sa.y;
+        const float dxb = txd - sb.x, dzb = tzd - sb.y;
+        const float ra = __builtin_fmaf(dxa, dxa, dza * dza);
+        const float rb = __builtin_fmaf(dxb, dxb, dzb * dzb);
+        const float inv = __builtin_amdgcn_rcpf(ra * rb);
+        const float ka = (rb * inv) * sa.z;
+        const float kb = (ra * inv) * sb.z;
+        au = __builtin_fmaf(ka, dza, au);
+        aw = __builtin_fmaf(ka, dxa, aw);
+        au = __builtin_fmaf(kb, dzb, au);
+        aw = __builtin_fmaf(kb, dxb, aw);
+    }
+}
+
+template <int T>
+__device__ __forceinline__ void far_tile_share4(const float4* __restrict__ sp, float txd,
+                                                float tzd, float& au, float& aw)
+{
+#pragma unroll 4
+    for (int k = 0; k < T; k += 4) {
+        const float4 sa = sp[k], sb = sp[k + 1], sc = sp[k + 2], sd = sp[k + 3];
+        const float dxa = txd - sa.x, dza = tzd - sa.y;
+        const float dxb = txd - sb.x, dzb = tzd - sb.y;
+        const float dxc = txd - sc.x, dzc = tzd - sc.y;
+        const float dxd = txd - sd.x, dzd = tzd - sd.y;
+        const float ra = __builtin_fmaf(dxa, dxa, dza * dza);
+        const float rb = __builtin_fmaf(dxb, dxb, dzb * dzb);
+        const float rc = __builtin_fmaf(dxc, dxc, dzc * dzc);
+        const float rd = __builtin_fmaf(dxd, dxd, dzd * dzd);
+        const float pab = ra * rb;
+        const float pcd = rc * rd;
+        const float inv = __builtin_amdgcn_rcpf(pab * pcd);
+        const float iab = inv * pcd;
+        const float icd = inv * pab;
+        const float ka = (rb * iab) * sa.z;
+        const float kb = (ra * iab) * sb.z;
+        const float kc = (rd * icd) * sc.z;
+        const float kd = (rc * icd) * sd.z;
+        au = __builtin_fmaf(ka, dza, au);
+        aw = __builtin_fmaf(ka, dxa, aw);
+        au = __builtin_fmaf(kb, dzb, au);
+        aw = __builtin_fmaf(kb, dxb, aw);
+        au = __builtin_fmaf(kc, dzc, au);
+        aw = __builtin_fmaf(kc, dxc, aw);
+        au = __builtin_fmaf(kd, dzd, au);
+        aw = __builtin_fmaf(kd, dxd, aw);
+    }
+}
+
+
+// Kernel factor of a near-tile pair (src/VelocityKernel.cpp:48-55):
+// K = (1 - exp(-r2/s^2)) / r2 when r2 < 10 s^2, else 1/r2, with
+// nj = {10 s^2, 1/s^2}.  With x = r2/s^2, (1 - e^-x)/r2 = f(x)/s^2 where
+// f(x) = (1 - e^-x)/x is summed from its Taylor series for x < 0.5
+// (truncation below 1.1e-8 relative); above, 1 - e^-x uses the hardware
+// exponential (v_exp_f32 on x log2 e; relative error below 1e-6).
+// Coincident distinct points (r2 = 0) give NaN, as the reference does.
+__device__ __forceinline__ float near_k(float r2, float2 nj)
+{
+    const float inv = __builtin_amdgcn_rcpf(r2);
+    const float y = -r2 * nj.y;
+    float f = __builtin_fmaf(y, 1.0f / 40320.0f, 1.0f / 5040.0f);
+    f = __builtin_fmaf(y, f, 1.0f / 720.0f);
+    f = __builtin_fmaf(y, f, 1.0f / 120.0f);
+    f = __builtin_fmaf(y, f, 1.0f / 24.0f);
+    f = __builtin_fmaf(y, f, 1.0f / 6.0f);
+    f = __builtin_fmaf(y, f, 0.5f);
+    f = __builtin_fmaf(y, f, 1.0f);
+    const float small = f * nj.y;
+    const float mid = (1.0f - __builtin_amdgcn_exp2f(y * 1.44269504088896341f)) * inv;
+    return r2 < nj.x ? ((y > -0.5f && r2 > 0.0f) ? small : mid) : inv;
+}
+
+// Near-tile pairs k0 <= k < k1 of one source tile.  Offsets from the tile
+// centre are carried as two FP32 words (rounded value + rounding residual:
+// sj.xy + nj.zw for the source, txd + txl for the target), so dx is exact to
+// FP32 rounding of dx itself however far the pair sits from the centre.
+// nj = {10 sigma^2, 1/sigma^2, x residual, z residual}.  SELF skips source
+// j0 + k == i, GUARD skips r2 <= r2_skip.
+template <bool SELF, bool GUARD>
+__device__ __forceinline__ void near_range(const float4* __restrict__ sp,
+                                           const float4* __restrict__ np, int k0, int k1,
+                                           float txd, float tzd, float txl, float tzl,
+                                           int64_t j0, int64_t i, float r2_skip, float& au,
+                                           float& aw)
+{
+#pragma unroll 4
+    for (int k = k0; k < k1; ++k) {
+        const float4 sj = sp[k];
+        const float4 nj4 = np[k];
+        const float2 nj = make_float2(nj4.x, nj4.y);
+        const float dx = (txd - sj.x) + (txl - nj4.z);
+        const float dz = (tzd - sj.y) + (tzl - nj4.w);
+        const float r2 = __builtin_fmaf(dx, dx, dz * dz);
+        const float K = near_k(r2, nj);
+        bool take = true;
+        if (SELF) take = (j0 + k) != i;
+        if (GUARD) take = take && (r2 > r2_skip);
+        const float kk = take ? K * sj.z : 0.0f;
+        au = __builtin_fmaf(kk, dz, au);
+        aw = __builtin_fmaf(kk, dx, aw);
+    }
+}
+
 }  // namespace dvmi

@@ -14,7 +14,10 @@ from dataclasses import dataclass, field
 from .batch import Batch, footprint
 from .workloads import projected_blobs
 
-BYTES_PER_BLOB = 172  # SoA fields, alternate set, RK stores, FP32 tile copies, flags
+# Per blob of slot capacity in an FP32C batch: two SoA sets (136), RK stores
+# (16), sorted FP32 tile copies and FP64 positions (48), sort keys/values and
+# permutation (16) plus sort scratch (8), chunk partials (64), flags (4).
+BYTES_PER_BLOB = 292
 
 
 @dataclass

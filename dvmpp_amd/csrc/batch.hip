@@ -8,10 +8,10 @@
 // a hipGraph and replayed.  Every launch covers all slots: per-blob kernels
 // stride over the concatenated live ranges (slot found by binary search of a
 // device prefix), per-slot kernels use one workgroup per slot, and the
-// pair sum is one persistent launch whose workgroups pull (slot, target tile)
-// items from a device work counter, largest slots first.  Each item's sum
-// order is fixed, so results do not depend on which workgroup ran it or on
-// what else is in the batch.
+// pair sum is one persistent launch whose waves (FP32C) or workgroups (FP64)
+// pull work items from a device work counter, largest slots first.  Each
+// item's sum order is fixed, so results do not depend on which wave ran it or
+// on what else is in the batch.
 #include "dvm_device.hpp"
 #include "dvm_internal.hpp"
 

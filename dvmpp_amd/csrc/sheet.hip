@@ -61,97 +61,92 @@ __global__ void influence_kernel(int nl, const double* __restrict__ x, const dou
     A[(int64_t)i * nl + j] = -kRpi2 * (p + q);
 }
 
-// M = A^T A (nl x nl), sequential over the nl+1 rows of A.
-__global__ void ata_kernel(int nl, const double* __restrict__ A, double* __restrict__ M)
+// The least-squares operator P = (A^T A)^-1 A^T (the solve of
+// src/VortexSheet.cpp:214 for every right-hand side at once), by
+// Gauss-Jordan elimination with partial pivoting on G = [A^T A | A^T]
+// (n rows, 2n+1 columns, row-major).  The pivot of column k is the row not
+// yet used as a pivot with the largest |G[r][k]| (the first such row on
+// ties) — the rows LU with partial pivoting would choose.  Rows are not
+// swapped and the pivot row is not scaled; column k is eliminated from every
+// other row, so at the end pivot row p of column i holds G[p][i] x_i.
+
+// G = [A^T A | A^T]; A^T A summed over the n+1 rows of A in order.
+__global__ void gj_build_kernel(int n, const double* __restrict__ A, double* __restrict__ G)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y;
-    if (j >= nl) return;
-    double s = 0.0;
-    for (int k = 0; k <= nl; ++k) s += A[(int64_t)k * nl + i] * A[(int64_t)k * nl + j];
-    M[(int64_t)i * nl + j] = s;
-}
-
-constexpr int kLuThreads = 1024;
-
-// In-place LU with partial pivoting of the nl x nl matrix M (one workgroup).
-// Pivot: first row with the largest |M[i][k]|.
-__global__ __launch_bounds__(kLuThreads) void lu_kernel(int n, double* __restrict__ M,
-                                                        int* __restrict__ piv)
-{
-    __shared__ double bv[kLuThreads];
-    __shared__ int bi[kLuThreads];
-    const int t = threadIdx.x;
-    for (int k = 0; k < n; ++k) {
-        double best = -1.0;
-        int bidx = n;
-        for (int i = k + t; i < n; i += kLuThreads) {
-            const double v = fabs(M[(int64_t)i * n + k]);
-            if (v > best) { best = v; bidx = i; }
-        }
-        bv[t] = best;
-        bi[t] = bidx;
-        __syncthreads();
-        for (int s = kLuThreads / 2; s > 0; s >>= 1) {
-            if (t < s) {
-                const double ov = bv[t + s];
-                const int oi = bi[t + s];
-                if (ov > bv[t] || (ov == bv[t] && oi < bi[t])) { bv[t] = ov; bi[t] = oi; }
-            }
-            __syncthreads();
-        }
-        const int p = bi[0];
-        __syncthreads();
-        if (t == 0) piv[k] = p;
-        if (p != k) {
-            for (int j = t; j < n; j += kLuThreads) {
-                const double a = M[(int64_t)k * n + j];
-                M[(int64_t)k * n + j] = M[(int64_t)p * n + j];
-                M[(int64_t)p * n + j] = a;
-            }
-        }
-        __syncthreads();
-        const double d = M[(int64_t)k * n + k];
-        for (int i = k + 1 + t; i < n; i += kLuThreads) M[(int64_t)i * n + k] /= d;
-        __syncthreads();
-        const int m = n - k - 1;
-        for (int e = t; e < m * m; e += kLuThreads) {
-            const int i = k + 1 + e / m;
-            const int j = k + 1 + e % m;
-            M[(int64_t)i * n + j] -= M[(int64_t)i * n + k] * M[(int64_t)k * n + j];
-        }
-        __syncthreads();
+    const int W = 2 * n + 1;
+    if (j >= W) return;
+    if (j < n) {
+        double s = 0.0;
+        for (int k = 0; k <= n; ++k) s += A[(int64_t)k * n + i] * A[(int64_t)k * n + j];
+        G[(int64_t)i * W + j] = s;
+    } else {
+        G[(int64_t)i * W + j] = A[(int64_t)(j - n) * n + i];
     }
 }
 
-// P = M^-1 A^T using the LU factors: column c of A^T is row c of A.
-// Thread c solves for column c; X is stored row-major nl x (nl+1).
-__global__ void lu_solve_columns_kernel(int n, const double* __restrict__ LU,
-                                        const int* __restrict__ piv, const double* __restrict__ A,
-                                        double* __restrict__ X)
+constexpr int kGjThreads = 256;
+constexpr int kGjMaxRows = 4096;  // at most 4096 panels per body
+
+// Step k: every workgroup finds the same pivot row p, workgroup 0 records
+// it, and each workgroup eliminates column k from its rows (r = blockIdx.x +
+// m gridDim.x, r != p) over columns k+1 .. 2n.
+__global__ __launch_bounds__(kGjThreads) void gj_step_kernel(int n, int k, double* __restrict__ G,
+                                                            int* __restrict__ pivrow)
+{
+    __shared__ unsigned char used[kGjMaxRows];
+    __shared__ double bv[kGjThreads];
+    __shared__ int bi[kGjThreads];
+    const int t = threadIdx.x;
+    const int W = 2 * n + 1;
+    for (int r = t; r < n; r += kGjThreads) used[r] = 0;
+    __syncthreads();
+    for (int m = t; m < k; m += kGjThreads) used[pivrow[m]] = 1;
+    __syncthreads();
+    double best = -1.0;
+    int bidx = n;
+    for (int r = t; r < n; r += kGjThreads) {
+        if (used[r]) continue;
+        const double v = fabs(G[(int64_t)r * W + k]);
+        if (bidx == n || v > best) { best = v; bidx = r; }  // any unused row beats none
+    }
+    bv[t] = best;
+    bi[t] = bidx;
+    __syncthreads();
+    for (int s = kGjThreads / 2; s > 0; s >>= 1) {
+        if (t < s) {
+            const double ov = bv[t + s];
+            const int oi = bi[t + s];
+            if (oi < n && (bi[t] == n || ov > bv[t] || (ov == bv[t] && oi < bi[t]))) {
+                bv[t] = ov;
+                bi[t] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    const int p = bi[0];
+    if (blockIdx.x == 0 && t == 0) pivrow[k] = p;
+    const double* __restrict__ gp = G + (int64_t)p * W;
+    const double pk = gp[k];
+    for (int r = blockIdx.x; r < n; r += gridDim.x) {
+        if (r == p) continue;
+        double* __restrict__ gr = G + (int64_t)r * W;
+        const double f = gr[k] / pk;
+        for (int j = k + 1 + t; j < W; j += kGjThreads) gr[j] -= f * gp[j];
+    }
+}
+
+// P[i][c] = G[p][n + c] / G[p][i] with p the pivot row of column i.
+__global__ void gj_extract_kernel(int n, const double* __restrict__ G, const int* __restrict__ pivrow,
+                                  double* __restrict__ P)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ncol = n + 1;
-    if (c >= ncol) return;
-    for (int i = 0; i < n; ++i) X[(int64_t)i * ncol + c] = A[(int64_t)c * n + i];
-    for (int k = 0; k < n; ++k) {
-        const int p = piv[k];
-        if (p != k) {
-            const double a = X[(int64_t)k * ncol + c];
-            X[(int64_t)k * ncol + c] = X[(int64_t)p * ncol + c];
-            X[(int64_t)p * ncol + c] = a;
-        }
-    }
-    for (int i = 1; i < n; ++i) {
-        double s = X[(int64_t)i * ncol + c];
-        for (int j = 0; j < i; ++j) s -= LU[(int64_t)i * n + j] * X[(int64_t)j * ncol + c];
-        X[(int64_t)i * ncol + c] = s;
-    }
-    for (int i = n - 1; i >= 0; --i) {
-        double s = X[(int64_t)i * ncol + c];
-        for (int j = i + 1; j < n; ++j) s -= LU[(int64_t)i * n + j] * X[(int64_t)j * ncol + c];
-        X[(int64_t)i * ncol + c] = s / LU[(int64_t)i * n + i];
-    }
+    const int i = blockIdx.y;
+    if (c > n) return;
+    const int W = 2 * n + 1;
+    const int p = pivrow[i];
+    P[(int64_t)i * (n + 1) + c] = G[(int64_t)p * W + n + c] / G[(int64_t)p * W + i];
 }
 
 void sheet_free(SheetDev& sh)
@@ -242,23 +237,29 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
     DVM_HIP_CHECK(hipMemsetAsync(sh.scal, 0, d * 8, st));
 
     // compute_influence_matrix, then the least-squares operator (A^T A)^-1 A^T.
-    double* M = nullptr;
-    int* piv = nullptr;
-    DVM_HIP_CHECK(hipMallocAsync((void**)&M, d * nl * nl + sizeof(int) * nl, st));
-    piv = reinterpret_cast<int*>(M + (size_t)nl * nl);
+    const int W = 2 * nl + 1;
+    double* G = nullptr;
+    DVM_HIP_CHECK(hipMallocAsync((void**)&G, d * nl * W + sizeof(int) * nl, st));
+    int* pivrow = reinterpret_cast<int*>(G + (size_t)nl * W);
     {
         ProfScope ps(ctx, "panel_influence");
         influence_kernel<<<dim3(cdiv(nl, 128), nl + 1), 128, 0, st>>>(nl, sh.x, sh.z, sh.xc, sh.zc,
                                                                       sh.theta, sh.ds, sh.A);
         DVM_LAUNCH_CHECK();
     }
-    ata_kernel<<<dim3(cdiv(nl, 128), nl), 128, 0, st>>>(nl, sh.A, M);
-    DVM_LAUNCH_CHECK();
-    lu_kernel<<<1, kLuThreads, 0, st>>>(nl, M, piv);
-    DVM_LAUNCH_CHECK();
-    lu_solve_columns_kernel<<<cdiv(nl + 1, 64), 64, 0, st>>>(nl, M, piv, sh.A, sh.P);
-    DVM_LAUNCH_CHECK();
-    DVM_HIP_CHECK(hipFreeAsync(M, st));
+    {
+        ProfScope ps(ctx, "sheet_operator");
+        gj_build_kernel<<<dim3(cdiv(W, 128), nl), 128, 0, st>>>(nl, sh.A, G);
+        DVM_LAUNCH_CHECK();
+        const unsigned blocks = (unsigned)(nl < ctx->num_cus ? nl : ctx->num_cus);
+        for (int k = 0; k < nl; ++k) {
+            gj_step_kernel<<<blocks, kGjThreads, 0, st>>>(nl, k, G, pivrow);
+            DVM_LAUNCH_CHECK();
+        }
+        gj_extract_kernel<<<dim3(cdiv(nl + 1, 128), nl), 128, 0, st>>>(nl, G, pivrow, sh.P);
+        DVM_LAUNCH_CHECK();
+    }
+    DVM_HIP_CHECK(hipFreeAsync(G, st));
     DVM_HIP_CHECK(hipStreamSynchronize(st));
 }
 

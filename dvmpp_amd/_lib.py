@@ -133,6 +133,18 @@ SIGNATURES.update({
     "dvm_sim_set_gamma": (_st, [_vp, _vp]),
     "dvm_sim_loads": (_st, [_vp, _vp]),
     "dvm_sim_blobs": (_st, [_vp] + [ctypes.POINTER(_vp)] * 4),
+    "dvm_sim_snapshot_size": (_st, [_vp, ctypes.POINTER(_sz)]),
+    "dvm_sim_snapshot": (_st, [_vp, _vp, _sz]),
+    "dvm_sim_restore": (_st, [_vp, _vp, _sz]),
+    "dvm_sim_copy_out": (_st, [_vp, _i64] + [_vp] * 5 + [ctypes.POINTER(_i64)]),
+    "dvm_sim_probe_quirk": (_st, [_vp, _int, _i64] + [_vp] * 6),
+    "dvm_event_create": (_st, [_vp, ctypes.POINTER(_vp)]),
+    "dvm_event_record": (_st, [_vp, _vp]),
+    "dvm_event_wait": (_st, [_vp]),
+    "dvm_event_query": (_st, [_vp, ctypes.POINTER(_int)]),
+    "dvm_event_destroy": (_st, [_vp]),
+    "dvm_host_alloc": (_st, [_sz, ctypes.POINTER(_vp)]),
+    "dvm_host_free": (_st, [_vp]),
     "dvm_batch_footprint": (_st, [ctypes.POINTER(BatchConfig), ctypes.POINTER(_sz)]),
     "dvm_batch_create": (_st, [_vp, ctypes.POINTER(BatchConfig), ctypes.POINTER(_vp)]),
     "dvm_batch_destroy": (_st, [_vp]),

@@ -246,4 +246,78 @@ dvm_status dvm_memcpy_d2d(dvm_ctx_t ctx, void* dst, const void* src, size_t byte
     });
 }
 
+
+struct dvm_event_s {
+    hipEvent_t ev = nullptr;
+};
+
+dvm_status dvm_event_create(dvm_ctx_t ctx, dvm_event_t* out)
+{
+    if (!ctx || !out) return DVM_E_INVALID;
+    *out = nullptr;
+    dvm_event_t e = new (std::nothrow) dvm_event_s();
+    if (!e) return DVM_E_OOM;
+    dvm_status st = guard(ctx, [&] {
+        DVM_HIP_CHECK(hipSetDevice(ctx->device));
+        DVM_HIP_CHECK(hipEventCreateWithFlags(&e->ev, hipEventDisableTiming));
+    });
+    if (st != DVM_OK) {
+        delete e;
+        return st;
+    }
+    *out = e;
+    return DVM_OK;
+}
+
+dvm_status dvm_event_record(dvm_ctx_t ctx, dvm_event_t e)
+{
+    if (!ctx || !e) return DVM_E_INVALID;
+    return guard(ctx, [&] { DVM_HIP_CHECK(hipEventRecord(e->ev, ctx->stream)); });
+}
+
+dvm_status dvm_event_wait(dvm_event_t e)
+{
+    if (!e) return DVM_E_INVALID;
+    return hipEventSynchronize(e->ev) == hipSuccess ? DVM_OK : DVM_E_HIP;
+}
+
+dvm_status dvm_event_query(dvm_event_t e, int* done)
+{
+    if (!e || !done) return DVM_E_INVALID;
+    const hipError_t r = hipEventQuery(e->ev);
+    if (r == hipSuccess) {
+        *done = 1;
+        return DVM_OK;
+    }
+    if (r == hipErrorNotReady) {
+        *done = 0;
+        return DVM_OK;
+    }
+    return DVM_E_HIP;
+}
+
+dvm_status dvm_event_destroy(dvm_event_t e)
+{
+    if (!e) return DVM_OK;
+    const hipError_t r = e->ev ? hipEventDestroy(e->ev) : hipSuccess;
+    delete e;
+    return r == hipSuccess ? DVM_OK : DVM_E_HIP;
+}
+
+dvm_status dvm_host_alloc(size_t bytes, void** ptr)
+{
+    if (!ptr) return DVM_E_INVALID;
+    *ptr = nullptr;
+    if (bytes == 0) return DVM_OK;
+    const hipError_t r = hipHostMalloc(ptr, bytes, hipHostMallocDefault);
+    if (r == hipErrorOutOfMemory) return DVM_E_OOM;
+    return r == hipSuccess ? DVM_OK : DVM_E_HIP;
+}
+
+dvm_status dvm_host_free(void* ptr)
+{
+    if (!ptr) return DVM_OK;
+    return hipHostFree(ptr) == hipSuccess ? DVM_OK : DVM_E_HIP;
+}
+
 }  // extern "C"

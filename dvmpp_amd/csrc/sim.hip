@@ -15,6 +15,8 @@
 #include <cmath>
 #include <cstring>
 #include <random>
+#include <sstream>
+#include <string>
 
 namespace dvmi {
 
@@ -143,6 +145,50 @@ struct dvm_sim_s {
 };
 
 namespace {
+
+// Snapshot layout (host byte order): header, mt19937 state as text, body
+// points x then z, gamma, ids, then x z circ sigma u w uvs wvs.
+constexpr char kSnapMagic[8] = {'D', 'V', 'M', 'S', 'N', 'A', 'P', '1'};
+constexpr uint32_t kSnapVersion = 1;
+struct SnapHeader {
+    char magic[8];
+    uint32_t version;
+    int32_t npts;
+    int32_t nl;
+    int32_t step;
+    int64_t n;
+    uint64_t draws;
+    uint32_t philox_seed;
+    uint32_t rng_bytes;
+    dvm_sim_params p;
+    double loads[4];
+};
+
+std::string rng_text(const std::mt19937& re)
+{
+    std::ostringstream o;
+    o << re;
+    return o.str();
+}
+
+size_t snap_bytes(dvm_sim_t s, size_t rng_len)
+{
+    return sizeof(SnapHeader) + rng_len + sizeof(double) * 2 * s->sh.hx.size() +
+           sizeof(double) * s->sh.nl + (sizeof(uint32_t) + 8 * sizeof(double)) * (size_t)s->n;
+}
+
+bool params_equal(const dvm_sim_params& a, const dvm_sim_params& b)
+{
+    const dvm_sheet_params &x = a.sheet, &y = b.sheet;
+    const dvm_remesh_params &r = a.remesh, &q = b.remesh;
+    return x.rho == y.rho && x.nu == y.nu && x.dt == y.dt && x.ux == y.ux && x.uz == y.uz &&
+           x.cutoff_exp == y.cutoff_exp && x.max_npv == y.max_npv && a.scheme == b.scheme &&
+           a.crossing == b.crossing && a.seed == b.seed && a.prec == b.prec &&
+           a.rng_mode == b.rng_mode && r.frequency == q.frequency && r.kernel == q.kernel &&
+           r.xmin == q.xmin && r.xmax == q.xmax && r.zmin == q.zmin && r.zmax == q.zmax &&
+           r.dx == q.dx && r.dz == q.dz && r.sigma == q.sigma &&
+           r.circ_threshold == q.circ_threshold;
+}
 
 void carve_blobs(Carver& cv, BlobBuf& b, int64_t cap)
 {
@@ -686,4 +732,146 @@ dvm_status dvm_sim_blobs(dvm_sim_t s, double** x, double** z, double** circ, dou
     return DVM_OK;
 }
 
-}  // extern "C"
+dvm_status dvm_sim_snapshot_size(dvm_sim_t s, size_t* bytes)
+{
+    if (!s || !bytes) return DVM_E_INVALID;
+    return guard(s->ctx, [&] { *bytes = snap_bytes(s, rng_text(s->re).size()); });
+}
+
+dvm_status dvm_sim_snapshot(dvm_sim_t s, void* buf, size_t bytes)
+{
+    if (!s || !buf) return DVM_E_INVALID;
+    return guard(s->ctx, [&] {
+        const std::string rt = rng_text(s->re);
+        require(bytes >= snap_bytes(s, rt.size()), DVM_E_SIZE, "snapshot buffer too small");
+        const int64_t n = s->n;
+        const int nl = s->sh.nl;
+        const size_t npts = s->sh.hx.size();
+        SnapHeader h{};
+        std::memcpy(h.magic, kSnapMagic, 8);
+        h.version = kSnapVersion;
+        h.npts = (int32_t)npts;
+        h.nl = nl;
+        h.step = s->step;
+        h.n = n;
+        h.draws = s->draws;
+        h.philox_seed = s->philox_seed;
+        h.rng_bytes = (uint32_t)rt.size();
+        h.p = s->p;
+        hipStream_t st = s->ctx->stream;
+        DVM_HIP_CHECK(hipMemcpyAsync(h.loads, s->sh.scal + 1, sizeof(double) * 4,
+                                     hipMemcpyDeviceToHost, st));
+        char* o = static_cast<char*>(buf) + sizeof(SnapHeader);
+        std::memcpy(o, rt.data(), rt.size());
+        o += rt.size();
+        std::memcpy(o, s->sh.hx.data(), sizeof(double) * npts);
+        o += sizeof(double) * npts;
+        std::memcpy(o, s->sh.hz.data(), sizeof(double) * npts);
+        o += sizeof(double) * npts;
+        DVM_HIP_CHECK(hipMemcpyAsync(o, s->sh.gamma, sizeof(double) * nl, hipMemcpyDeviceToHost, st));
+        o += sizeof(double) * nl;
+        if (n > 0) {
+            DVM_HIP_CHECK(hipMemcpyAsync(o, s->b.id, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, st));
+            o += sizeof(uint32_t) * n;
+            const double* srcs[8] = {s->b.x, s->b.z, s->b.c, s->b.s, s->b.u, s->b.w, s->b.uvs, s->b.wvs};
+            for (int k = 0; k < 8; ++k) {
+                DVM_HIP_CHECK(hipMemcpyAsync(o, srcs[k], sizeof(double) * n, hipMemcpyDeviceToHost, st));
+                o += sizeof(double) * n;
+            }
+        }
+        DVM_HIP_CHECK(hipStreamSynchronize(st));
+        std::memcpy(buf, &h, sizeof(SnapHeader));
+    });
+}
+
+dvm_status dvm_sim_restore(dvm_sim_t s, const void* buf, size_t bytes)
+{
+    if (!s || !buf) return DVM_E_INVALID;
+    return guard(s->ctx, [&] {
+        require(bytes >= sizeof(SnapHeader), DVM_E_SIZE, "snapshot truncated");
+        SnapHeader h;
+        std::memcpy(&h, buf, sizeof(SnapHeader));
+        require(std::memcmp(h.magic, kSnapMagic, 8) == 0 && h.version == kSnapVersion,
+                DVM_E_INVALID, "not a simulation snapshot of this version");
+        const size_t npts = s->sh.hx.size();
+        require(h.npts == (int32_t)npts && h.nl == s->sh.nl, DVM_E_INVALID,
+                "snapshot of a different body");
+        require(h.n >= 0 && h.n < (int64_t(1) << 31), DVM_E_INVALID, "snapshot blob count out of range");
+        const int64_t n = h.n;
+        const size_t need = sizeof(SnapHeader) + h.rng_bytes + sizeof(double) * 2 * npts +
+                            sizeof(double) * h.nl + (sizeof(uint32_t) + 8 * sizeof(double)) * (size_t)n;
+        require(bytes >= need, DVM_E_SIZE, "snapshot truncated");
+        require(params_equal(h.p, s->p), DVM_E_INVALID, "snapshot of different simulation parameters");
+        const char* o = static_cast<const char*>(buf) + sizeof(SnapHeader);
+        std::mt19937 re;
+        {
+            std::istringstream in(std::string(o, h.rng_bytes));
+            in >> re;
+            require(!in.fail(), DVM_E_INVALID, "corrupt random-stream state in snapshot");
+        }
+        o += h.rng_bytes;
+        require(std::memcmp(o, s->sh.hx.data(), sizeof(double) * npts) == 0 &&
+                    std::memcmp(o + sizeof(double) * npts, s->sh.hz.data(), sizeof(double) * npts) == 0,
+                DVM_E_INVALID, "snapshot of a different body");
+        o += sizeof(double) * 2 * npts;
+        ensure_cap(s, n);
+        hipStream_t st = s->ctx->stream;
+        DVM_HIP_CHECK(hipMemcpyAsync(s->sh.gamma, o, sizeof(double) * h.nl, hipMemcpyHostToDevice, st));
+        o += sizeof(double) * h.nl;
+        DVM_HIP_CHECK(hipMemcpyAsync(s->sh.scal + 1, h.loads, sizeof(double) * 4, hipMemcpyHostToDevice, st));
+        if (n > 0) {
+            DVM_HIP_CHECK(hipMemcpyAsync(s->b.id, o, sizeof(uint32_t) * n, hipMemcpyHostToDevice, st));
+            o += sizeof(uint32_t) * n;
+            double* dsts[8] = {s->b.x, s->b.z, s->b.c, s->b.s, s->b.u, s->b.w, s->b.uvs, s->b.wvs};
+            for (int k = 0; k < 8; ++k) {
+                DVM_HIP_CHECK(hipMemcpyAsync(dsts[k], o, sizeof(double) * n, hipMemcpyHostToDevice, st));
+                o += sizeof(double) * n;
+            }
+        }
+        DVM_HIP_CHECK(hipStreamSynchronize(st));
+        s->n = n;
+        s->step = h.step;
+        s->draws = h.draws;
+        s->philox_seed = h.philox_seed;
+        s->re = re;
+        s->dist.reset();
+    });
+}
+
+dvm_status dvm_sim_copy_out(dvm_sim_t s, int64_t cap, double* x, double* z, double* circ,
+                            double* gamma, double* loads4, int64_t* n)
+{
+    if (!s) return DVM_E_INVALID;
+    return guard(s->ctx, [&] {
+        const int64_t m = cap < 0 ? 0 : (s->n < cap ? s->n : cap);
+        hipStream_t st = s->ctx->stream;
+        const size_t b = sizeof(double) * m;
+        if (m > 0) {
+            if (x) DVM_HIP_CHECK(hipMemcpyAsync(x, s->b.x, b, hipMemcpyDeviceToHost, st));
+            if (z) DVM_HIP_CHECK(hipMemcpyAsync(z, s->b.z, b, hipMemcpyDeviceToHost, st));
+            if (circ) DVM_HIP_CHECK(hipMemcpyAsync(circ, s->b.c, b, hipMemcpyDeviceToHost, st));
+        }
+        if (gamma)
+            DVM_HIP_CHECK(hipMemcpyAsync(gamma, s->sh.gamma, sizeof(double) * s->sh.nl,
+                                         hipMemcpyDeviceToHost, st));
+        if (loads4)
+            DVM_HIP_CHECK(hipMemcpyAsync(loads4, s->sh.scal + 1, sizeof(double) * 4,
+                                         hipMemcpyDeviceToHost, st));
+        if (n) *n = s->n;
+    });
+}
+
+dvm_status dvm_sim_probe_quirk(dvm_sim_t s, int prec, int64_t np, const double* px,
+                               const double* pz, double* u, double* w, double* uvs, double* wvs)
+{
+    if (!s) return DVM_E_INVALID;
+    return guard(s->ctx, [&] {
+        require(prec == DVM_F64 || prec == DVM_F32C, DVM_E_INVALID, "unknown precision");
+        require(np >= 0, DVM_E_SIZE, "negative probe count");
+        require(np == 0 || (px && pz && u && w && uvs && wvs), DVM_E_INVALID, "null probe array");
+        sheet_probe_quirk(s->ctx, s->sh, prec, s->n, s->b.x, s->b.z, s->b.c, s->b.s, np, px, pz, u, w,
+                          uvs, wvs);
+    });
+}
+
+}  // extern "C""

@@ -24,7 +24,28 @@ public:
 class FileIOException : public DVMException {
 public:
     explicit FileIOException(const std::string& m) : DVMException(m) {}
+    static FileIOException cannotOpen(const std::string& f) { return FileIOException("Cannot open file: " + f); }
+    static FileIOException cannotWrite(const std::string& f) { return FileIOException("Cannot write to file: " + f); }
     static FileIOException cannotRead(const std::string& f) { return FileIOException("Cannot read file: " + f); }
+};
+
+class XMLParseException : public DVMException {
+public:
+    explicit XMLParseException(const std::string& m) : DVMException(m) {}
+    static XMLParseException cannotLoad(const std::string& f, const std::string& why)
+    {
+        return XMLParseException("Cannot load XML file '" + f + "': " + why);
+    }
+    static XMLParseException missingParameter(const std::string& level, const std::string& param)
+    {
+        return XMLParseException("Missing XML parameter: <" + level + "><" + param + ">");
+    }
+    static XMLParseException invalidOption(const std::string& level, const std::string& param,
+                                           const std::string& options)
+    {
+        return XMLParseException("Invalid option for <" + level + "><" + param +
+                                 ">. Valid options: " + options);
+    }
 };
 
 class AlgorithmException : public DVMException {

@@ -1,7 +1,11 @@
 // Drop-in C++17 classes over the C-ABI (include/dvm_hip.h).
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <filesystem>
 #include <fstream>
+#include <iostream>
+#include <iterator>
 #include <mutex>
 #include <sstream>
 
@@ -9,6 +13,7 @@
 #include "DVMBase.hpp"
 #include "Device.hpp"
 #include "Exceptions.hpp"
+#include "StepWriter.hpp"
 #include "VelocityKernel.hpp"
 #include "VortexBlobs.hpp"
 #include "VortexSheet.hpp"
@@ -512,12 +517,179 @@ void DVMBase::init(const std::vector<double>& x, const std::vector<double>& z)
     dvm::check(dvm_sim_create(ctx, (int)x.size(), x.data(), z.data(), &m_p, &m_sim), "DVMBase", ctx);
 }
 
+DVMBase::DVMBase(XmlHandler& xml, const std::string& timestamp) : m_steps(0)
+{
+    from_xml(xml, timestamp, RunOptions{});
+}
+
+DVMBase::DVMBase(XmlHandler& xml, const std::string& timestamp, const RunOptions& opts) : m_steps(0)
+{
+    from_xml(xml, timestamp, opts);
+}
+
+// The reference's constructor (src/DVMBase.cpp:8-66) and those of its members
+// (src/VortexBlobs.cpp:16-42, src/VortexSheet.cpp:12-54, src/Probe.cpp:12-34):
+// the same entries, the same checks, the same files.
+void DVMBase::from_xml(XmlHandler& xml, const std::string& timestamp, const RunOptions& opts)
+{
+    m_opts = opts;
+    if (m_opts.output_every == 0) m_opts.output_every = 1;
+    auto val = [&xml](const char* l, const char* p) { return xml.getValueAttribute(l, p); };
+    auto str = [&xml](const char* l, const char* p) { return xml.getStringAttribute(l, p); };
+
+    m_p = dvm_sim_params{};
+    dvm_sheet_params& sp = m_p.sheet;
+    sp.rho = val("constants", "density");
+    sp.nu = val("constants", "nu");
+    sp.dt = val("time", "dt");
+    const double steps = val("time", "steps");
+    m_steps = (unsigned)steps;
+    sp.ux = val("flow", "ux");
+    sp.uz = val("flow", "uz");
+    sp.max_npv = (int)(unsigned)val("constants", "max_NumPanelVort");
+    sp.cutoff_exp = val("constants", "cutoff_exp");
+    m_p.seed = xml.getIntAttribute("constants", "seed");
+
+    const std::string scheme = str("time", "scheme");
+    if (scheme == "euler") m_p.scheme = DVM_EULER;
+    else if (scheme == "RK3") m_p.scheme = DVM_RK3;
+    else throw dvm::XMLParseException::invalidOption("time", "scheme", "euler RK3");
+    const std::string cross = str("algorithms", "surface_crossing");
+    if (cross == "DELETE") m_p.crossing = DVM_DELETE;
+    else if (cross == "ABSORB") m_p.crossing = DVM_ABSORB;
+    else if (cross == "REFLECT") m_p.crossing = DVM_REFLECT;
+    else throw dvm::XMLParseException::invalidOption("algorithms", "surface_crossing",
+                                                     "DELETE ABSORB REFLECT");
+
+    // Vortex merging is optional in the input (src/DVMBase.cpp:49-60).
+    double merge_threshold = 0.0;
+    int merge_frequency = 0;
+    try {
+        merge_threshold = xml.getValueAttribute("algorithms", "merge_threshold");
+        merge_frequency = xml.getIntAttribute("algorithms", "merge_frequency");
+    } catch (const dvm::XMLParseException&) {
+        merge_threshold = 0.0;
+        merge_frequency = 0;
+    }
+    if (merge_frequency > 0 && merge_threshold > 0)
+        throw dvm::AlgorithmException(
+            "vortex merging (<algorithms><merge_threshold>, <merge_frequency>) is not "
+            "available in this build; remove the entries to run without it");
+
+    // Remeshing is optional: no <remesh><frequency> means none.
+    m_p.remesh = dvm_remesh_params{};
+    if (xml.has("remesh", "frequency")) {
+        dvm_remesh_params& r = m_p.remesh;
+        r.frequency = xml.getIntAttribute("remesh", "frequency");
+        if (r.frequency > 0) {
+            r.xmin = val("remesh", "xmin");
+            r.xmax = val("remesh", "xmax");
+            r.zmin = val("remesh", "zmin");
+            r.zmax = val("remesh", "zmax");
+            r.dx = val("remesh", "dx");
+            r.dz = val("remesh", "dz");
+            r.sigma = val("remesh", "sigma");
+            r.circ_threshold = val("remesh", "circ_threshold");
+            const std::string k = xml.has("remesh", "kernel") ? str("remesh", "kernel") : "M4prime";
+            if (k == "M4prime") r.kernel = 0;
+            else if (k == "gaussian") r.kernel = 1;
+            else if (k == "M6") r.kernel = 2;
+            else throw dvm::XMLParseException::invalidOption("remesh", "kernel", "M4prime gaussian M6");
+        } else {
+            r.frequency = 0;
+        }
+    }
+    m_p.prec = m_opts.precision;
+    m_p.rng_mode = m_opts.rng_mode;
+
+    // Output directory first: the blob files are created before the body is read.
+    dvm::OutputSpec spec;
+    spec.dir = xml.getStringAttribute("io", "output_dir", true);
+    spec.stamp = timestamp;
+    spec.dt = sp.dt;
+    spec.steps = steps;
+    spec.write_blobs = m_opts.write_blobs;
+
+    const std::filesystem::path domain = std::filesystem::path(str("io", "input_dir")) /
+                                         str("io", "domain_file");
+    std::vector<double> x, z;
+    read_body(domain.string(), x, z);
+    if (m_opts.screen) {
+        std::cout << "Successfully loaded coordinate file with " << x.size() << " points." << std::endl;
+        std::cout << "A = " << *std::max_element(x.begin(), x.end()) - *std::min_element(x.begin(), x.end())
+                  << " (assuming cylinder)\n";
+    }
+    const size_t nl = x.size() - 1;
+    spec.xc.resize(nl);
+    spec.zc.resize(nl);
+    spec.ds.resize(nl);
+    for (size_t i = 0; i < nl; ++i) {
+        spec.xc[i] = 0.5 * (x[i] + x[i + 1]);
+        spec.zc[i] = 0.5 * (z[i] + z[i + 1]);
+        const double dx = x[i + 1] - x[i], dz = z[i + 1] - z[i];
+        spec.ds[i] = std::sqrt(std::pow(dx, 2) + std::pow(dz, 2));
+    }
+    spec.probe_x = xml.getList("probe", "x");
+    spec.probe_z = xml.getList("probe", "z");
+    if (spec.probe_x.size() != spec.probe_z.size()) throw dvm::SizeMismatchException("probe x and z lists");
+
+    init(x, z);
+    m_writer = std::make_unique<dvm::StepWriter>(dvm::device_context(), m_sim, std::move(spec));
+}
+
 DVMBase::~DVMBase()
 {
+    try {
+        m_writer.reset();
+    } catch (...) {
+    }
     if (m_sim) (void)dvm_sim_destroy(m_sim);
 }
 
-void DVMBase::solve() { dvm::check(dvm_sim_run(m_sim, (int)m_steps), "DVMBase::solve", dvm::device_context()); }
+void DVMBase::solve() { solve_until(m_steps); }
+
+void DVMBase::solve_until(unsigned last)
+{
+    dvm_ctx_t ctx = dvm::device_context();
+    if (last > m_steps) last = m_steps;
+    if (!m_writer) {
+        const unsigned done = current_step();
+        if (done < last) dvm::check(dvm_sim_run(m_sim, (int)(last - done)), "DVMBase::solve", ctx);
+        return;
+    }
+    for (unsigned j = current_step() + 1; j <= last; ++j) {
+        dvm::check(dvm_sim_step(m_sim), "DVMBase::solve", ctx);
+        const double t = j * m_p.sheet.dt;
+        if (j % m_opts.output_every == 0) m_writer->push(j, t);
+        if (m_opts.screen) {
+            std::cout << "Simulation time          = " << t << "\tStep " << j << "/" << m_steps << std::endl;
+            std::cout << "Number of vortex blobs   = " << num_blobs() << std::endl;
+        }
+    }
+    m_writer->flush();
+}
+
+void DVMBase::save_snapshot(const std::string& file) const
+{
+    dvm_ctx_t ctx = dvm::device_context();
+    size_t bytes = 0;
+    dvm::check(dvm_sim_snapshot_size(m_sim, &bytes), "snapshot", ctx);
+    std::vector<char> buf(bytes);
+    dvm::check(dvm_sim_snapshot(m_sim, buf.data(), bytes), "snapshot", ctx);
+    std::ofstream out(file, std::ios::binary | std::ios::trunc);
+    if (!out) throw dvm::FileIOException::cannotOpen(file);
+    out.write(buf.data(), (std::streamsize)bytes);
+    if (!out) throw dvm::FileIOException::cannotWrite(file);
+}
+
+void DVMBase::load_snapshot(const std::string& file)
+{
+    std::ifstream in(file, std::ios::binary);
+    if (!in) throw dvm::FileIOException::cannotRead(file);
+    std::vector<char> buf((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    dvm_ctx_t ctx = dvm::device_context();
+    dvm::check(dvm_sim_restore(m_sim, buf.data(), buf.size()), "restore", ctx);
+}
 
 void DVMBase::step() { dvm::check(dvm_sim_step(m_sim), "DVMBase::step", dvm::device_context()); }
 

@@ -233,3 +233,92 @@ class Sim:
         out = np.empty(4)
         check(self.ctx.h, load().dvm_sim_loads(self.h, _hp(out)))
         return tuple(out)
+
+    def snapshot(self):
+        """The whole simulation state as bytes (dvm_sim_snapshot)."""
+        nb = ctypes.c_size_t()
+        check(self.ctx.h, load().dvm_sim_snapshot_size(self.h, ctypes.byref(nb)))
+        buf = np.empty(nb.value, dtype=np.uint8)
+        check(self.ctx.h, load().dvm_sim_snapshot(self.h, _hp(buf), nb.value))
+        return buf.tobytes()
+
+    def restore(self, blob):
+        """Loads a snapshot taken from a simulation of the same body and parameters."""
+        buf = np.frombuffer(blob, dtype=np.uint8).copy()
+        check(self.ctx.h, load().dvm_sim_restore(self.h, _hp(buf), buf.size))
+
+    def copy_out(self, x, z, circ, gamma, loads4):
+        """Enqueues copies of positions, circulations, gamma and loads into
+        host buffers (numpy views of dvm_host_alloc memory for asynchronous
+        copies); returns the blob count.  Wait on an Event recorded after it."""
+        n = ctypes.c_int64()
+        cap = x.size if x is not None else 0
+        check(self.ctx.h, load().dvm_sim_copy_out(self.h, cap, _hp(x), _hp(z), _hp(circ),
+                                                  _hp(gamma), _hp(loads4), ctypes.byref(n)))
+        return n.value
+
+    def probe_quirk(self, px, pz, prec=F64):
+        """Probe velocities (u, w, uvs, wvs) for the current state, in the
+        reference's probe_velocities form (device tensors in and out)."""
+        import torch
+        npr = px.numel()
+        outs = [torch.empty(npr, dtype=torch.float64, device=px.device) for _ in range(4)]
+        ptr = (lambda t: ctypes.c_void_p(t.data_ptr()))
+        check(self.ctx.h, load().dvm_sim_probe_quirk(self.h, prec, npr, ptr(px), ptr(pz),
+                                                     *[ptr(o) for o in outs]))
+        return tuple(outs)
+
+
+class Event:
+    """dvm_event_t: a marker on a context's stream."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        check(ctx.h, load().dvm_event_create(ctx.h, ctypes.byref(h)))
+        self.h = h
+
+    def record(self):
+        check(self.ctx.h, load().dvm_event_record(self.ctx.h, self.h))
+
+    def wait(self):
+        check(None, load().dvm_event_wait(self.h))
+
+    def done(self):
+        d = ctypes.c_int()
+        check(None, load().dvm_event_query(self.h, ctypes.byref(d)))
+        return bool(d.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().dvm_event_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HostBuffer:
+    """Pinned host memory (dvm_host_alloc) viewed as a float64 numpy array."""
+
+    def __init__(self, count):
+        p = ctypes.c_void_p()
+        check(None, load().dvm_host_alloc(8 * max(count, 1), ctypes.byref(p)))
+        self.p = p
+        self.array = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_double)),
+                                           shape=(max(count, 1),))[:count]
+
+    def close(self):
+        if getattr(self, "p", None):
+            self.array = None
+            load().dvm_host_free(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -326,6 +326,50 @@ dvm_status dvm_sim_loads(dvm_sim_t sim, double* out4);
 /* Device pointers to the blob SoA (valid until the next call that changes N). */
 dvm_status dvm_sim_blobs(dvm_sim_t sim, double** x, double** z, double** circ, double** sigma);
 
+/* Snapshot / restore of a whole simulation: parameters, body points, step
+ * index, blobs (all VortexBlobs fields), gamma, the last loads, and both
+ * random streams (the mt19937 state and draw count of the reference stream,
+ * the counter-based stream's seed).  A restored simulation continues exactly
+ * as the snapshotted one would have, bit for bit.  The reference has no
+ * checkpoint; its output is per-step text only (src/DVMBase.cpp:151-158).
+ * dvm_sim_snapshot_size gives the bytes dvm_sim_snapshot writes into HOST
+ * memory; dvm_sim_restore reads such a buffer into a simulation created with
+ * the same body and parameters (DVM_E_INVALID otherwise, DVM_E_SIZE for a
+ * truncated buffer).  Both synchronise the context's stream. */
+dvm_status dvm_sim_snapshot_size(dvm_sim_t sim, size_t* bytes);
+dvm_status dvm_sim_snapshot(dvm_sim_t sim, void* buf, size_t bytes);
+dvm_status dvm_sim_restore(dvm_sim_t sim, const void* buf, size_t bytes);
+
+/* Output staging without a stall.  dvm_sim_copy_out enqueues, on the
+ * context's stream, copies of the current blob positions and circulations
+ * (the first min(n, cap)), gamma (Nl) and the last loads {fx, fz, cd, cl}
+ * into HOST buffers — pinned (dvm_host_alloc) for the copies to run
+ * asynchronously — and returns the blob count n at once; NULL buffers are
+ * skipped.  Record an event after it (dvm_event_record) and wait on the event
+ * before reading the buffers; the simulation can step meanwhile.  This is
+ * the state the reference writes every step (VortexBlobs::write_step,
+ * VortexSheet::write_step, src/VortexBlobs.cpp:310-331,
+ * src/VortexSheet.cpp:335-343). */
+dvm_status dvm_sim_copy_out(dvm_sim_t sim, int64_t cap, double* x, double* z, double* circ,
+                            double* gamma, double* loads4, int64_t* n);
+/* Probe velocities for the simulation's current state, in the reference's
+ * probe_velocities form (dvm_sheet_probe_quirk); device arrays of np probes. */
+dvm_status dvm_sim_probe_quirk(dvm_sim_t sim, int prec, int64_t np, const double* px,
+                               const double* pz, double* u, double* w, double* uvs, double* wvs);
+
+/* Events and pinned host memory for the staging above.  dvm_event_record
+ * marks the context stream's current position; dvm_event_wait blocks until
+ * the work before it has finished and may be called from any host thread;
+ * dvm_event_query sets *done to 1 once it has. */
+typedef struct dvm_event_s* dvm_event_t;
+dvm_status dvm_event_create(dvm_ctx_t ctx, dvm_event_t* ev);
+dvm_status dvm_event_record(dvm_ctx_t ctx, dvm_event_t ev);
+dvm_status dvm_event_wait(dvm_event_t ev);
+dvm_status dvm_event_query(dvm_event_t ev, int* done);
+dvm_status dvm_event_destroy(dvm_event_t ev);
+dvm_status dvm_host_alloc(size_t bytes, void** ptr);
+dvm_status dvm_host_free(void* ptr);
+
 /* ------------------------------------------------------------------ batched ensemble
  * Many independent simulations (different bodies, viscosities, time steps,
  * seeds and step counts; ragged blob counts) stepped together: each launch

@@ -80,7 +80,8 @@ class RemeshParams(ctypes.Structure):
 class SimParams(ctypes.Structure):
     """dvm_sim_params."""
     _fields_ = [("sheet", SheetParams), ("scheme", _int), ("crossing", _int), ("seed", _int),
-                ("prec", _int), ("rng_mode", _int), ("remesh", RemeshParams)]
+                ("prec", _int), ("rng_mode", _int), ("remesh", RemeshParams),
+                ("merge_threshold", _dbl), ("merge_frequency", _int)]
 
 
 class BatchConfig(ctypes.Structure):
@@ -133,6 +134,9 @@ SIGNATURES.update({
     "dvm_sim_set_gamma": (_st, [_vp, _vp]),
     "dvm_sim_loads": (_st, [_vp, _vp]),
     "dvm_sim_blobs": (_st, [_vp] + [ctypes.POINTER(_vp)] * 4),
+    "dvm_sim_merged": (_st, [_vp, ctypes.POINTER(_i64)]),
+    "dvm_merge_blobs": (_st, [_vp, _dbl, _i64] + [_vp] * 9 + [ctypes.POINTER(_i64)]),
+    "dvm_merge_stats": (_st, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "dvm_sim_snapshot_size": (_st, [_vp, ctypes.POINTER(_sz)]),
     "dvm_sim_snapshot": (_st, [_vp, _vp, _sz]),
     "dvm_sim_restore": (_st, [_vp, _vp, _sz]),

@@ -34,6 +34,8 @@ struct dvm_ctx_s {
     int f32c_forced[4] = {0, 0, 0, 0};
     int f32c_last[4] = {0, 0, 0, 0};
     std::map<std::tuple<int, int, int>, std::pair<int, int>> f32c_tune;
+    // Last merge: candidate blobs, and speculative sequences recomputed.
+    int64_t merge_stats[2] = {0, 0};
     // Kernel timing (dvm_prof_*): pending event pairs and totals per kernel.
     bool prof = false;
     struct ProfEntry {
@@ -211,6 +213,14 @@ int64_t remesh_count(dvm_ctx_t ctx, const dvm_remesh_params& rp, int64_t n, cons
                      RemeshPlan& plan);
 void remesh_finish(dvm_ctx_t ctx, const dvm_remesh_params& rp, const BlobBuf& in,
                    RemeshPlan& plan, BlobBuf& out);
+
+// VortexBlobs::merge_blobs (see merge.hip): merges in place in `b`, and when
+// anything merged writes the survivors compacted into `out` (IDs 0..m-1);
+// returns the number of merges (host value; synchronizes).  `scratch` holds
+// merge_scratch_bytes(n) bytes of device memory.
+size_t merge_scratch_bytes(int64_t n);
+int64_t merge_run(dvm_ctx_t ctx, double threshold, int64_t n, const BlobBuf& b, const BlobBuf& out,
+                  void* scratch);
 
 // Biot-Savart entry points shared with the sheet and simulation code.
 void bs_self_launch(dvm_ctx_t ctx, int prec, int64_t n, const double* x, const double* z,

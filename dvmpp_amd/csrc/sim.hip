@@ -126,6 +126,7 @@ struct dvm_sim_s {
     double *q1 = nullptr, *q2 = nullptr, *tape = nullptr;
     int *flags = nullptr, *keep = nullptr, *pos = nullptr;
     double* loads = nullptr;  // last step's fx fz cd cl (device, inside sh.scal)
+    int64_t last_merged = 0;  // merges of the last merge sub-step
     // reference random stream (src/Random.hpp:13-15)
     std::uniform_real_distribution<> dist;
     std::mt19937 re;
@@ -187,7 +188,8 @@ bool params_equal(const dvm_sim_params& a, const dvm_sim_params& b)
            a.rng_mode == b.rng_mode && r.frequency == q.frequency && r.kernel == q.kernel &&
            r.xmin == q.xmin && r.xmax == q.xmax && r.zmin == q.zmin && r.zmax == q.zmax &&
            r.dx == q.dx && r.dz == q.dz && r.sigma == q.sigma &&
-           r.circ_threshold == q.circ_threshold;
+           r.circ_threshold == q.circ_threshold && a.merge_threshold == b.merge_threshold &&
+           a.merge_frequency == b.merge_frequency;
 }
 
 void carve_blobs(Carver& cv, BlobBuf& b, int64_t cap)
@@ -410,6 +412,21 @@ void crossing(dvm_sim_t s)
 
 void loads(dvm_sim_t s) { sheet_loads(s->ctx, s->sh, s->ur); }
 
+// src/DVMBase.cpp:126-132
+void merge(dvm_sim_t s)
+{
+    s->last_merged = 0;
+    const double thr = s->p.merge_threshold;
+    const int f = s->p.merge_frequency;
+    if (!(f > 0 && thr > 0 && s->step % f == 0) || s->n < 2) return;
+    void* scratch = ws_get(s->ctx, merge_scratch_bytes(s->n));
+    const int64_t k = merge_run(s->ctx, thr, s->n, s->b, s->alt, scratch);
+    if (k == 0) return;
+    std::swap(s->b, s->alt);
+    s->n -= k;
+    s->last_merged = k;
+}
+
 void substep(dvm_sim_t s, int which)
 {
     switch (which) {
@@ -422,6 +439,7 @@ void substep(dvm_sim_t s, int which)
     case 3: release(s); break;
     case 4: crossing(s); break;
     case 5: loads(s); break;
+    case 6: merge(s); break;
     default: fail(DVM_E_INVALID, "unknown sub-step");
     }
 }
@@ -491,7 +509,7 @@ dvm_status dvm_sim_step(dvm_sim_t s)
 {
     if (!s) return DVM_E_INVALID;
     return guard(s->ctx, [&] {
-        for (int k = 0; k <= 5; ++k) substep(s, k);
+        for (int k = 0; k <= 6; ++k) substep(s, k);
     });
 }
 
@@ -500,7 +518,7 @@ dvm_status dvm_sim_run(dvm_sim_t s, int steps)
     if (!s) return DVM_E_INVALID;
     return guard(s->ctx, [&] {
         for (int j = 0; j < steps; ++j)
-            for (int k = 0; k <= 5; ++k) substep(s, k);
+            for (int k = 0; k <= 6; ++k) substep(s, k);
     });
 }
 
@@ -521,6 +539,13 @@ dvm_status dvm_sim_step_index(dvm_sim_t s, int* step)
 {
     if (!s || !step) return DVM_E_INVALID;
     *step = s->step;
+    return DVM_OK;
+}
+
+dvm_status dvm_sim_merged(dvm_sim_t s, int64_t* merged)
+{
+    if (!s || !merged) return DVM_E_INVALID;
+    *merged = s->last_merged;
     return DVM_OK;
 }
 

@@ -44,6 +44,9 @@ public:
     /// reference's order; the displacements are applied on the GPU.
     void diffusion_random_walk(Random& rand, double nu, double dt);
     [[nodiscard]] double totalcirc() const;
+    /// src/VortexBlobs.cpp:110-209 on the GPU (dvm_merge_blobs): the same
+    /// merges and values; returns the number of merged pairs.
+    unsigned merge_blobs(double threshold);
     /// Throws SizeMismatchException if the fields disagree (src/VortexBlobs.cpp:272-285).
     [[nodiscard]] unsigned size() const;
     void setKernelType(const std::string& type);

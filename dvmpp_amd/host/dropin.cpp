@@ -216,6 +216,34 @@ void VortexBlobs::diffusion_random_walk(Random& rand, double nu, double dt)
     buf.download(m_z.memptr(), b, 3 * b);
 }
 
+unsigned VortexBlobs::merge_blobs(double threshold)
+{
+    const unsigned N = size();
+    if (N < 2) return 0;
+    const size_t b = sizeof(double) * N;
+    dvm::DeviceBuffer buf(8 * b + sizeof(uint32_t) * N);
+    Vector* f[8] = {&m_x, &m_z, &m_circ, &m_sigma, &m_u, &m_w, &m_uvs, &m_wvs};
+    for (int k = 0; k < 8; ++k) buf.upload(f[k]->memptr(), b, k * b);
+    double* d = buf.as<double>();
+    uint32_t* id = reinterpret_cast<uint32_t*>(buf.as<char>() + 8 * b);
+    dvm_ctx_t ctx = dvm::device_context();
+    int64_t merged = 0;
+    dvm::check(dvm_merge_blobs(ctx, threshold, N, id, d, d + N, d + 2 * (size_t)N, d + 3 * (size_t)N,
+                               d + 4 * (size_t)N, d + 5 * (size_t)N, d + 6 * (size_t)N,
+                               d + 7 * (size_t)N, &merged),
+               "merge_blobs", ctx);
+    if (merged == 0) return 0;
+    const unsigned m = N - (unsigned)merged;
+    for (int k = 0; k < 8; ++k) {
+        f[k]->set_size(m);
+        buf.download(f[k]->memptr(), sizeof(double) * m, k * b);
+    }
+    std::vector<uint32_t> ids(m);
+    buf.download(ids.data(), sizeof(uint32_t) * m, 8 * b);
+    m_ID.assign(ids.begin(), ids.end());
+    return (unsigned)merged;
+}
+
 double VortexBlobs::totalcirc() const
 {
     // Armadillo's accumulate order: two interleaved partial sums.
@@ -571,10 +599,13 @@ void DVMBase::from_xml(XmlHandler& xml, const std::string& timestamp, const RunO
         merge_threshold = 0.0;
         merge_frequency = 0;
     }
-    if (merge_frequency > 0 && merge_threshold > 0)
-        throw dvm::AlgorithmException(
-            "vortex merging (<algorithms><merge_threshold>, <merge_frequency>) is not "
-            "available in this build; remove the entries to run without it");
+    if (merge_frequency > 0 && merge_threshold > 0) {
+        m_p.merge_threshold = merge_threshold;
+        m_p.merge_frequency = merge_frequency;
+        if (m_opts.screen)
+            std::cout << "Vortex merging enabled: threshold = " << merge_threshold
+                      << ", frequency = " << merge_frequency << std::endl;
+    }
 
     // Remeshing is optional: no <remesh><frequency> means none.
     m_p.remesh = dvm_remesh_params{};
@@ -659,6 +690,9 @@ void DVMBase::solve_until(unsigned last)
     }
     for (unsigned j = current_step() + 1; j <= last; ++j) {
         dvm::check(dvm_sim_step(m_sim), "DVMBase::solve", ctx);
+        int64_t merged = 0;
+        dvm::check(dvm_sim_merged(m_sim, &merged), "DVMBase::solve", ctx);
+        if (merged > 0 && m_opts.screen) std::cout << "Merged " << merged << " vortex pairs" << std::endl;
         const double t = j * m_p.sheet.dt;
         if (j % m_opts.output_every == 0) m_writer->push(j, t);
         if (m_opts.screen) {

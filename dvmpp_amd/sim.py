@@ -32,9 +32,10 @@ def remesh_params(frequency=0, kernel=M4PRIME, xmin=-5.0, xmax=25.0, zmin=-10.0,
 
 
 def sim_params(scheme=EULER, crossing=REFLECT, seed=0, prec=0, rng_mode=RNG_REFERENCE,
-               remesh=None, **sheet_kw):
+               remesh=None, merge_threshold=0.0, merge_frequency=0, **sheet_kw):
     return SimParams(sheet_params(**sheet_kw), scheme, crossing, seed, prec, rng_mode,
-                     remesh if remesh is not None else remesh_params())
+                     remesh if remesh is not None else remesh_params(), merge_threshold,
+                     merge_frequency)
 
 
 class Sheet:
@@ -138,6 +139,17 @@ class Sheet:
         check(self.ctx.h, load().dvm_sheet_reflect(self.h, x.numel(), ptr(x), ptr(z)))
 
 
+def merge_blobs(ctx, threshold, x, z, circ, sigma, u, w, uvs, wvs, ids=None):
+    """dvm_merge_blobs in place on torch device tensors; returns the number of
+    merges (the survivors are the first n - merges entries)."""
+    ptr = (lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None)
+    k = ctypes.c_int64()
+    check(ctx.h, load().dvm_merge_blobs(ctx.h, float(threshold), x.numel(), ptr(ids), ptr(x),
+                                        ptr(z), ptr(circ), ptr(sigma), ptr(u), ptr(w), ptr(uvs),
+                                        ptr(wvs), ctypes.byref(k)))
+    return k.value
+
+
 def remesh(ctx, params, x, z, circ, sigma):
     """dvm_remesh on torch device tensors; returns four new device tensors."""
     import torch
@@ -233,6 +245,11 @@ class Sim:
         out = np.empty(4)
         check(self.ctx.h, load().dvm_sim_loads(self.h, _hp(out)))
         return tuple(out)
+
+    def merged(self):
+        k = ctypes.c_int64()
+        check(self.ctx.h, load().dvm_sim_merged(self.h, ctypes.byref(k)))
+        return k.value
 
     def snapshot(self):
         """The whole simulation state as bytes (dvm_sim_snapshot)."""

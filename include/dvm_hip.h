@@ -289,6 +289,8 @@ typedef struct dvm_sim_params {
     int prec;          /* dvm_precision of the pair sums */
     int rng_mode;      /* dvm_rng_mode */
     dvm_remesh_params remesh;
+    double merge_threshold; /* algorithms/merge_threshold (src/DVMBase.cpp:49-60) */
+    int merge_frequency;    /* algorithms/merge_frequency; merging runs when both are > 0 */
 } dvm_sim_params;
 
 /* DVMBase(XmlHandler&, stamp) (src/DVMBase.cpp:8-65): body points px, pz on
@@ -297,15 +299,19 @@ dvm_status dvm_sim_create(dvm_ctx_t ctx, int npts, const double* px, const doubl
                           const dvm_sim_params* params, dvm_sim_t* out);
 dvm_status dvm_sim_destroy(dvm_sim_t sim);
 /* increment_step + compute_step (src/DVMBase.cpp:70-71, 86-133): convect
- * (Euler or RK3), remesh, diffusion, release + append, crossing, loads.
+ * (Euler or RK3), remesh, diffusion, release + append, crossing, loads, and
+ * merge_blobs every merge_frequency steps when configured.
  * DVM_E_NAN on the VortexException path (non-finite release counts). */
 dvm_status dvm_sim_step(dvm_sim_t sim);
 /* `steps` calls of dvm_sim_step (DVMBase::solve without output). */
 dvm_status dvm_sim_run(dvm_sim_t sim, int steps);
 /* Sub-steps of compute_step, in its order, for lock-step checking:
  * 0 = increment_step + convect, 1 = remesh, 2 = diffusion,
- * 3 = release + append, 4 = crossing, 5 = loads. */
+ * 3 = release + append, 4 = crossing, 5 = loads, 6 = merge (when due). */
 dvm_status dvm_sim_substep(dvm_sim_t sim, int which);
+/* Merges made by the last merge sub-step (0 when it was not due); the
+ * reference prints this count (src/DVMBase.cpp:128-131). */
+dvm_status dvm_sim_merged(dvm_sim_t sim, int64_t* merged);
 dvm_status dvm_sim_size(dvm_sim_t sim, int64_t* n);
 dvm_status dvm_sim_step_index(dvm_sim_t sim, int* step);
 /* Uniform draws consumed so far from the reference stream. */
@@ -369,6 +375,23 @@ dvm_status dvm_event_query(dvm_event_t ev, int* done);
 dvm_status dvm_event_destroy(dvm_event_t ev);
 dvm_status dvm_host_alloc(size_t bytes, void** ptr);
 dvm_status dvm_host_free(void* ptr);
+
+/* VortexBlobs::merge_blobs (src/VortexBlobs.cpp:110-209) on n device blobs,
+ * in place: the reference's greedy pass in index order (blob i absorbs each
+ * later unabsorbed j with |Gi Gj / (Gi + Gj)| * |xi - xj| < threshold against
+ * i's current state; moments conserved; u w uvs wvs of an absorbing blob set
+ * to 0), with the same decisions and values.  When anything merged, the
+ * survivors are compacted to the front in order and, if `id` is not NULL,
+ * their IDs set to 0..m-1.  *merged = n - m (the reference's return value).
+ * Synchronizes the context's stream. */
+dvm_status dvm_merge_blobs(dvm_ctx_t ctx, double threshold, int64_t n, uint32_t* id, double* x,
+                           double* z, double* circ, double* sigma, double* u, double* w,
+                           double* uvs, double* wvs, int64_t* merged);
+/* Work counts of the context's last merge: blobs that had a partner in the
+ * input, and how many of their speculative absorption sequences had to be
+ * recomputed because an earlier blob took one of their partners.  No
+ * counterpart in the reference. */
+dvm_status dvm_merge_stats(dvm_ctx_t ctx, int64_t* candidates, int64_t* recomputed);
 
 /* ------------------------------------------------------------------ batched ensemble
  * Many independent simulations (different bodies, viscosities, time steps,

@@ -28,6 +28,7 @@ class Params(ctypes.Structure):
         ("remesh_zmin", ctypes.c_double), ("remesh_zmax", ctypes.c_double),
         ("remesh_dx", ctypes.c_double), ("remesh_dz", ctypes.c_double),
         ("remesh_sigma", ctypes.c_double), ("remesh_circ_threshold", ctypes.c_double),
+        ("merge_threshold", ctypes.c_double), ("merge_frequency", ctypes.c_int),
     ]
 
 
@@ -36,7 +37,8 @@ def make_params(**kw):
     defaults = dict(rho=1.0, nu=1e-3, dt=0.005, ux=1.0, uz=0.0, cutoff_exp=0.675, max_npv=3,
                     scheme=0, crossing=2, seed=0, remesh_frequency=0, remesh_kernel=0,
                     remesh_xmin=-5.0, remesh_xmax=25.0, remesh_zmin=-10.0, remesh_zmax=10.0,
-                    remesh_dx=0.01, remesh_dz=0.01, remesh_sigma=0.05, remesh_circ_threshold=1e-10)
+                    remesh_dx=0.01, remesh_dz=0.01, remesh_sigma=0.05, remesh_circ_threshold=1e-10,
+                    merge_threshold=0.0, merge_frequency=0)
     defaults.update(kw)
     for k, v in defaults.items():
         setattr(p, k, v)
@@ -88,6 +90,8 @@ def _load():
         "orc_sim_release": (ctypes.c_int, [vp]),
         "orc_sim_crossing": (None, [vp]),
         "orc_sim_loads": (None, [vp]),
+        "orc_sim_merge": (_i64, [vp]),
+        "orc_merge_blobs": (_i64, [ctypes.c_double, _i64, ctypes.POINTER(ctypes.c_uint32)] + [_dp] * 8),
         "orc_remesh": (_i64, [ctypes.POINTER(Params), _i64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "orc_remesh_weight": (ctypes.c_double, [ctypes.c_int, ctypes.c_double]),
     }
@@ -310,6 +314,25 @@ class Sim:
 
     def compute_loads(self):
         lib().orc_sim_loads(self.h)
+
+    def merge(self):
+        return lib().orc_sim_merge(self.h)
+
+
+def merge_blobs(threshold, st):
+    """VortexBlobs::merge_blobs on a state dict (id, x, z, circ, sigma, u, w,
+    uvs, wvs); returns (merges, new state)."""
+    n = st["x"].size
+    ids = np.ascontiguousarray(st["id"], dtype=np.uint32).copy()
+    f = {k: _f64(st[k]).copy() for k in ("x", "z", "circ", "sigma", "u", "w", "uvs", "wvs")}
+    merged = lib().orc_merge_blobs(float(threshold), n,
+                                   ids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                   *[_p(f[k]) for k in ("x", "z", "circ", "sigma", "u", "w",
+                                                        "uvs", "wvs")])
+    m = n - merged
+    out = {k: v[:m].copy() for k, v in f.items()}
+    out["id"] = ids[:m].copy()
+    return merged, out
 
 
 def remesh(params, x, z, circ, sigma):

@@ -82,6 +82,59 @@ struct Blobs {
     }
     // src/VortexBlobs.cpp:267-270
     double totalcirc() const { return arma_sum(circ.data(), circ.size()); }
+
+    // src/VortexBlobs.cpp:110-209 (merge_blobs): greedy in index order; blob i
+    // absorbs every later unmerged j whose criterion against i's current state
+    // is below the threshold; survivors keep their order, IDs become 0..m-1.
+    size_t merge(double threshold)
+    {
+        const size_t n = size();
+        if (n < 2) return 0;
+        std::vector<bool> merged(n, false);
+        size_t count = 0;
+        for (size_t i = 0; i < n; ++i) {
+            if (merged[i]) continue;
+            for (size_t j = i + 1; j < n; ++j) {
+                if (merged[j]) continue;
+                const double gi = circ[i];
+                const double gj = circ[j];
+                const double gs = gi + gj;
+                if (std::abs(gs) < 1e-15) continue;
+                const double dx = x[i] - x[j];
+                const double dz = z[i] - z[j];
+                const double dist = std::sqrt(dx * dx + dz * dz);
+                const double crit = std::abs(gi * gj / gs) * dist;
+                if (crit < threshold) {
+                    const double nx = (gi * x[i] + gj * x[j]) / gs;
+                    const double nz = (gi * z[i] + gj * z[j]) / gs;
+                    const double ns = (std::abs(gi) * sigma[i] + std::abs(gj) * sigma[j]) /
+                                      (std::abs(gi) + std::abs(gj));
+                    x[i] = nx;
+                    z[i] = nz;
+                    circ[i] = gs;
+                    sigma[i] = ns;
+                    u[i] = 0.0;
+                    w[i] = 0.0;
+                    uvs[i] = 0.0;
+                    wvs[i] = 0.0;
+                    merged[j] = true;
+                    ++count;
+                }
+            }
+        }
+        if (count == 0) return 0;
+        size_t o = 0;
+        for (size_t i = 0; i < n; ++i) {
+            if (merged[i]) continue;
+            id[o] = (uint32_t)o;
+            x[o] = x[i]; z[o] = z[i]; circ[o] = circ[i]; sigma[o] = sigma[i];
+            u[o] = u[i]; w[o] = w[i]; uvs[o] = uvs[i]; wvs[o] = wvs[i];
+            ++o;
+        }
+        id.resize(o);
+        for (Vec* v : {&x, &z, &circ, &sigma, &u, &w, &uvs, &wvs}) v->resize(o);
+        return count;
+    }
 };
 
 // ---------------------------------------------------------------- Biot-Savart
@@ -760,6 +813,15 @@ struct Sim {
         if (!release()) return 3;
         crossing();
         sheet.loads(Ur);
+        do_merge();
+        return 0;
+    }
+
+    // src/DVMBase.cpp:126-132
+    size_t do_merge()
+    {
+        if (p.merge_frequency > 0 && p.merge_threshold > 0 && step % p.merge_frequency == 0)
+            return v.merge(p.merge_threshold);
         return 0;
     }
 };
@@ -954,6 +1016,25 @@ void orc_sim_diffuse(void* sim) { static_cast<Sim*>(sim)->diffuse(); }
 int orc_sim_release(void* sim) { return static_cast<Sim*>(sim)->release() ? 0 : 3; }
 void orc_sim_crossing(void* sim) { static_cast<Sim*>(sim)->crossing(); }
 void orc_sim_loads(void* sim) { Sim* s = static_cast<Sim*>(sim); s->sheet.loads(s->Ur); }
+int64_t orc_sim_merge(void* sim) { return (int64_t)static_cast<Sim*>(sim)->do_merge(); }
+
+int64_t orc_merge_blobs(double threshold, int64_t n, uint32_t* id, double* x, double* z,
+                        double* circ, double* sigma, double* u, double* w, double* uvs,
+                        double* wvs)
+{
+    Blobs b;
+    b.id.assign(id, id + n);
+    b.x.assign(x, x + n); b.z.assign(z, z + n); b.circ.assign(circ, circ + n);
+    b.sigma.assign(sigma, sigma + n); b.u.assign(u, u + n); b.w.assign(w, w + n);
+    b.uvs.assign(uvs, uvs + n); b.wvs.assign(wvs, wvs + n);
+    const size_t merged = b.merge(threshold);
+    const size_t m = b.size();
+    std::copy(b.id.begin(), b.id.end(), id);
+    double* outs[8] = {x, z, circ, sigma, u, w, uvs, wvs};
+    const Vec* ins[8] = {&b.x, &b.z, &b.circ, &b.sigma, &b.u, &b.w, &b.uvs, &b.wvs};
+    for (int k = 0; k < 8; ++k) std::copy(ins[k]->begin(), ins[k]->begin() + m, outs[k]);
+    return (int64_t)merged;
+}
 
 int64_t orc_remesh(const orc_params* p, int64_t n, const double* x, const double* z,
                    const double* circ, const double* sigma, double* ox, double* oz,

@@ -43,6 +43,9 @@ typedef struct orc_params {
     int remesh_kernel;   /* 0 = M4prime, 1 = gaussian, 2 = M6 */
     double remesh_xmin, remesh_xmax, remesh_zmin, remesh_zmax;
     double remesh_dx, remesh_dz, remesh_sigma, remesh_circ_threshold;
+    /* algorithms/merge_threshold, merge_frequency (src/DVMBase.cpp:49-60); 0 = off */
+    double merge_threshold;
+    int merge_frequency;
 } orc_params;
 
 /* ---- Random (src/Random.cpp:3-18) ---- */
@@ -111,6 +114,15 @@ void    orc_sim_diffuse(void* sim);
 int     orc_sim_release(void* sim);
 void    orc_sim_crossing(void* sim);
 void    orc_sim_loads(void* sim);
+/* merge_blobs on the sim's state when due (src/DVMBase.cpp:126-132); returns merges */
+int64_t orc_sim_merge(void* sim);
+
+/* ---- VortexBlobs::merge_blobs (src/VortexBlobs.cpp:110-209) ---- */
+/* In place on n blobs: survivors compacted to the front, IDs renumbered
+ * 0..m-1 when anything merged; returns the number of merges (n - m). */
+int64_t orc_merge_blobs(double threshold, int64_t n, uint32_t* id, double* x, double* z,
+                        double* circ, double* sigma, double* u, double* w, double* uvs,
+                        double* wvs);
 
 /* ---- Remesh (BackgroundMesh spec; reference source missing) ---- */
 /* Returns the output count; out arrays must hold the returned size (call with

@@ -7,9 +7,9 @@ this module only moves torch device tensors across that boundary.
 import ctypes
 import weakref
 
-from ._lib import F32C, F64, TUNE_MEASURE, TUNE_TABLE, DvmError, check, load  # noqa: F401
+from ._lib import F32C, F64, FMM, TUNE_MEASURE, TUNE_TABLE, DvmError, check, load  # noqa: F401
 
-__all__ = ["F64", "F32C", "TUNE_TABLE", "TUNE_MEASURE", "DvmError", "Context", "bs_self",
+__all__ = ["F64", "F32C", "FMM", "TUNE_TABLE", "TUNE_MEASURE", "DvmError", "Context", "bs_self",
            "bs_targets", "load"]
 
 # Compiled F32C variants (source tile, target block, sources per reciprocal;
@@ -110,6 +110,16 @@ class Context:
         out = (ctypes.c_int32 * 4)()
         check(self.h, load().dvm_get_f32c_variant(self.h, out))
         return tuple(out)
+
+    def set_fmm(self, expansion_order=10, theta=0.5):
+        """Parameters of FMM sums (the reference FMMKernel's defaults)."""
+        check(self.h, load().dvm_set_fmm(self.h, expansion_order, theta))
+
+    def fmm_last_level(self):
+        """Leaf level of the last FMM sum (0: it ran the direct sum)."""
+        out = ctypes.c_int()
+        check(self.h, load().dvm_fmm_last_level(self.h, ctypes.byref(out)))
+        return out.value
 
 
 def bs_self(ctx, x, z, circ, sigma, prec=F64, out=None):

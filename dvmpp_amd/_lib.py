@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libdvm_hip.so")
 
 F64 = 0
 F32C = 1
+FMM = 2
 
 OK = 0
 _STATUS = {0: "ok", 1: "invalid argument", 2: "size mismatch", 3: "non-finite release counts",
@@ -137,6 +138,8 @@ SIGNATURES.update({
     "dvm_sim_merged": (_st, [_vp, ctypes.POINTER(_i64)]),
     "dvm_merge_blobs": (_st, [_vp, _dbl, _i64] + [_vp] * 9 + [ctypes.POINTER(_i64)]),
     "dvm_merge_stats": (_st, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "dvm_set_fmm": (_st, [_vp, _int, _dbl]),
+    "dvm_fmm_last_level": (_st, [_vp, ctypes.POINTER(_int)]),
     "dvm_sim_snapshot_size": (_st, [_vp, ctypes.POINTER(_sz)]),
     "dvm_sim_snapshot": (_st, [_vp, _vp, _sz]),
     "dvm_sim_restore": (_st, [_vp, _vp, _sz]),

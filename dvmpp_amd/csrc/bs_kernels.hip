@@ -928,11 +928,14 @@ void f32c_set_variant(dvm_ctx_t ctx, int tile, int block, int share, int occupan
 void bs_self_launch(dvm_ctx_t ctx, int prec, int64_t n, const double* x, const double* z,
                     const double* c, const double* s, double* u, double* w)
 {
-    require(prec == DVM_F64 || prec == DVM_F32C, DVM_E_INVALID, "unknown precision");
+    require(prec == DVM_F64 || prec == DVM_F32C || prec == DVM_FMM, DVM_E_INVALID,
+            "unknown precision");
     require(n >= 0 && n < (int64_t(1) << 31), DVM_E_SIZE, "blob count out of range");
     if (n == 0) return;
     require(x && z && c && s && u && w, DVM_E_INVALID, "null array");
-    if (prec == DVM_F64)
+    if (prec == DVM_FMM) {
+        if (!fmm_self(ctx, n, x, z, c, s, u, w)) launch_f64(ctx, true, n, x, z, c, s, n, x, z, 0, 0.0, u, w);
+    } else if (prec == DVM_F64)
         launch_f64(ctx, true, n, x, z, c, s, n, x, z, 0, 0.0, u, w);
     else
         launch_f32c(ctx, true, n, x, z, c, s, n, x, z, 0, 0.0, u, w);
@@ -942,7 +945,9 @@ void bs_targets_launch(dvm_ctx_t ctx, int prec, int64_t ns, const double* sx, co
                        const double* sc, const double* ss, int64_t nt, const double* tx,
                        const double* tz, int guard, double r2_skip, double* u, double* w)
 {
-    require(prec == DVM_F64 || prec == DVM_F32C, DVM_E_INVALID, "unknown precision");
+    require(prec == DVM_F64 || prec == DVM_F32C || prec == DVM_FMM, DVM_E_INVALID,
+            "unknown precision");
+    if (prec == DVM_FMM) prec = DVM_F64;  // FMMKernel::computeInducedAt is the direct sum
     require(ns >= 0 && nt >= 0 && ns < (int64_t(1) << 31) && nt < (int64_t(1) << 31), DVM_E_SIZE,
             "count out of range");
     if (nt == 0) return;

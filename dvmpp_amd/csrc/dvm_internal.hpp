@@ -36,6 +36,11 @@ struct dvm_ctx_s {
     std::map<std::tuple<int, int, int>, std::pair<int, int>> f32c_tune;
     // Last merge: candidate blobs, and speculative sequences recomputed.
     int64_t merge_stats[2] = {0, 0};
+    // DVM_FMM parameters (the reference FMMKernel's defaults) and the leaf
+    // level of the last DVM_FMM sum (0: direct).
+    int fmm_order = 10;
+    double fmm_theta = 0.5;
+    int fmm_last_level = 0;
     // Kernel timing (dvm_prof_*): pending event pairs and totals per kernel.
     bool prof = false;
     struct ProfEntry {
@@ -221,6 +226,11 @@ void remesh_finish(dvm_ctx_t ctx, const dvm_remesh_params& rp, const BlobBuf& in
 size_t merge_scratch_bytes(int64_t n);
 int64_t merge_run(dvm_ctx_t ctx, double threshold, int64_t n, const BlobBuf& b, const BlobBuf& out,
                   void* scratch);
+
+// DVM_FMM blob-on-blob sum (fmm.hip); false when it declined (the caller then
+// runs the direct FP64 sum).
+bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const double* c,
+              const double* s, double* u, double* w);
 
 // Biot-Savart entry points shared with the sheet and simulation code.
 void bs_self_launch(dvm_ctx_t ctx, int prec, int64_t n, const double* x, const double* z,

@@ -1,0 +1,612 @@
+// Fast multipole form of the blob-on-blob sum: the reference's FMMKernel
+// (src/VelocityKernel.hpp:84-110, parameters expansion_order and theta),
+// whose body there is a stub that calls the direct sum
+// (src/VelocityKernel.cpp:113-147).  Here it is a 2-D FMM on the device.
+//
+// Far field.  Beyond r^2 >= 10 sigma_j^2 the reference kernel is exactly the
+// point vortex 1/r^2 (src/VelocityKernel.cpp:47-55), so with
+// phi(z) = sum_j G_j / (z - z_j) over the far sources (z = x + i z),
+// u = Im(phi) / 2pi and w = Re(phi) / 2pi.  phi is carried by complex
+// multipole and local expansions of `p` terms on a uniform quadtree over the
+// square bounding box, scaled by each level's box width so every coefficient
+// is O(1):
+//   P2M  A_k  = sum_j G_j Z_j^k,                      Z_j = (z_j - c) / s
+//   M2M  A'_k = sum_{m<=k} C(k,m) 2^-m A_m D^(k-m),   D = (c_child - c) / s
+//   M2L  B_l  = (-1)^l sum_k C(k+l,l) A_k T^-(k+l+1), T = (c_tgt - c_src) / s
+//   L2L  B'_m = 2^-(m+1) sum_{l>=m} C(l,m) B_l E^(l-m), E = (c_child - c) / s
+//   L2P  phi  = (1/s) sum_l B_l W^l,                   W = (z - c) / s
+// Two boxes of one level are well separated when their centres are farther
+// apart than sqrt(2) s / theta (the half-diagonals sum to theta times the
+// distance: theta is the reference's opening criterion); the interaction
+// list of a box is the children of its parent's near boxes that are not
+// near it.  The truncation error falls roughly as theta^p.
+//
+// Near field.  Sources in the near leaves are summed directly with the
+// reference's regularised kernel.  The leaf width is kept large enough that
+// every pair closer than sqrt(10) * max(sigma) lies in near leaves, so the
+// only difference from the direct sum is the truncation of the far field.
+//
+// Layout: particles Morton-sorted by leaf (radix sort); boxes of level l are
+// indexed by their Morton code after an offset (4^l - 1) / 3; expansions are
+// complex FP64, `P` per box.  Kernels: key, sort, gather, leaf bounds, P2M
+// (one thread per leaf), M2M per level, M2L + L2L per level (one thread per
+// occupied box), and L2P + P2P (one thread per particle).
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <vector>
+
+#include "dvm_internal.hpp"
+
+namespace dvmi {
+
+namespace {
+
+constexpr int kFmmMaxLevel = 10;
+constexpr int kBinom = 64;  // binomial table side (2P <= 64)
+
+struct cplx {
+    double re, im;
+};
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cplx cscale(cplx a, double f) { return {a.re * f, a.im * f}; }
+__device__ __forceinline__ cplx cinv(cplx a)
+{
+    const double d = a.re * a.re + a.im * a.im;
+    return {a.re / d, -a.im / d};
+}
+
+__host__ __device__ __forceinline__ uint32_t spread_bits(uint32_t v)
+{
+    v &= 0xffffu;
+    v = (v | (v << 8)) & 0x00ff00ffu;
+    v = (v | (v << 4)) & 0x0f0f0f0fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+__host__ __device__ __forceinline__ uint32_t morton(uint32_t ix, uint32_t iz)
+{
+    return spread_bits(ix) | (spread_bits(iz) << 1);
+}
+__device__ __forceinline__ uint32_t compact_bits(uint32_t v)
+{
+    v &= 0x55555555u;
+    v = (v | (v >> 1)) & 0x33333333u;
+    v = (v | (v >> 2)) & 0x0f0f0f0fu;
+    v = (v | (v >> 4)) & 0x00ff00ffu;
+    v = (v | (v >> 8)) & 0x0000ffffu;
+    return v;
+}
+
+struct FmmTree {
+    double x0, z0, S;  // square domain origin and side
+    int L;             // leaf level
+    int p;             // terms used (<= P)
+    const double* binom;  // C(n, k) at [n * kBinom + k]
+    const int* il;        // interaction-list offsets (dx, dz) per child position, il_n each
+    int il_n[4];
+    int il_stride;
+    const int* nearo;     // near offsets (dx, dz)
+    int near_n;
+    const int* lbeg;      // leaf particle bounds (4^L + 1)
+    int* cnt;             // particles per box (all levels)
+    cplx* A;              // multipole coefficients per box
+    cplx* B;              // local coefficients per box
+};
+
+__host__ __device__ __forceinline__ int64_t level_off(int l) { return ((int64_t(1) << (2 * l)) - 1) / 3; }
+
+__global__ void fmm_binom_kernel(double* binom)
+{
+    // one thread per row n: C(n, k) for k <= n (Pascal, exact up to 2^53)
+    const int n = threadIdx.x;
+    if (n >= kBinom) return;
+    double v = 1.0;
+    for (int k = 0; k < kBinom; ++k) {
+        binom[n * kBinom + k] = k <= n ? v : 0.0;
+        v = v * (double)(n - k) / (double)(k + 1);
+    }
+}
+
+__global__ void fmm_key_kernel(int n, const double* __restrict__ x, const double* __restrict__ z,
+                               double x0, double z0, double sl, int nb, uint32_t* __restrict__ key,
+                               int* __restrict__ idx)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int ix = min(max((int)floor((x[i] - x0) / sl), 0), nb - 1);
+    const int iz = min(max((int)floor((z[i] - z0) / sl), 0), nb - 1);
+    key[i] = morton((uint32_t)ix, (uint32_t)iz);
+    idx[i] = i;
+}
+
+__global__ void fmm_gather_kernel(int n, const int* __restrict__ perm, const double* __restrict__ x,
+                                  const double* __restrict__ z, const double* __restrict__ c,
+                                  const double* __restrict__ s, double4* __restrict__ sp)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int i = perm[k];
+    sp[k] = make_double4(x[i], z[i], c[i], s[i]);
+}
+
+__global__ void fmm_lbeg_kernel(int n, const uint32_t* __restrict__ key, int nleaf, int* __restrict__ lbeg)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > n) return;
+    const int lo = p == 0 ? -1 : (int)key[p - 1];
+    const int hi = p == n ? nleaf : (int)key[p];
+    for (int q = lo + 1; q <= hi; ++q) lbeg[q] = p;
+}
+
+__device__ __forceinline__ cplx box_centre(const FmmTree& t, int l, uint32_t m)
+{
+    const double s = t.S / (double)(1 << l);
+    return {t.x0 + (compact_bits(m) + 0.5) * s, t.z0 + (compact_bits(m >> 1) + 0.5) * s};
+}
+
+// P2M: one thread per leaf.
+template <int P>
+__global__ __launch_bounds__(64) void fmm_p2m_kernel(FmmTree t, const double4* __restrict__ sp)
+{
+    const int64_t nleaf = int64_t(1) << (2 * t.L);
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nleaf) return;
+    const int64_t bi = level_off(t.L) + m;
+    const int b = t.lbeg[m], e = t.lbeg[m + 1];
+    t.cnt[bi] = e - b;
+    cplx a[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) a[k] = {0.0, 0.0};
+    const double s = t.S / (double)(1 << t.L), rs = 1.0 / s;
+    const cplx c = box_centre(t, t.L, (uint32_t)m);
+    for (int j = b; j < e; ++j) {
+        const double4 q = sp[j];
+        const cplx zj = {(q.x - c.re) * rs, (q.y - c.im) * rs};
+        cplx pw = {q.z, 0.0};
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            if (k < t.p) a[k] = cadd(a[k], pw);
+            pw = cmul(pw, zj);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k) t.A[bi * P + k] = a[k];
+}
+
+// M2M into level l from its children at l + 1: one thread per box.
+template <int P>
+__global__ __launch_bounds__(64) void fmm_m2m_kernel(FmmTree t, int l)
+{
+    const int64_t nbox = int64_t(1) << (2 * l);
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nbox) return;
+    const int64_t bi = level_off(l) + m;
+    cplx a[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) a[k] = {0.0, 0.0};
+    int total = 0;
+    for (int ch = 0; ch < 4; ++ch) {
+        const int64_t ci = level_off(l + 1) + 4 * m + ch;
+        const int cn = t.cnt[ci];
+        if (cn == 0) continue;
+        total += cn;
+        // D = (c_child - c) / s = (+-1/4, +-1/4)
+        const cplx d = {(ch & 1) ? 0.25 : -0.25, (ch & 2) ? 0.25 : -0.25};
+        cplx h[P];
+        double f = 1.0;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            h[k] = cscale(t.A[ci * P + k], f);
+            f *= 0.5;
+        }
+        // a_k += sum_m C(k,m) h_m d^(k-m)
+        cplx dp[P];
+        dp[0] = {1.0, 0.0};
+#pragma unroll
+        for (int k = 1; k < P; ++k) dp[k] = cmul(dp[k - 1], d);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            if (k < t.p) {
+                cplx acc = {0.0, 0.0};
+#pragma unroll
+                for (int mm = 0; mm <= k; ++mm)
+                    acc = cadd(acc, cscale(cmul(h[mm], dp[k - mm]), t.binom[k * kBinom + mm]));
+                a[k] = cadd(a[k], acc);
+            }
+        }
+    }
+    t.cnt[bi] = total;
+#pragma unroll
+    for (int k = 0; k < P; ++k) t.A[bi * P + k] = a[k];
+}
+
+// Level l locals: L2L from the parent (l > 2) plus M2L from the interaction
+// list.  One thread per box with particles.
+template <int P>
+__global__ __launch_bounds__(64) void fmm_down_kernel(FmmTree t, int l)
+{
+    const int64_t nbox = int64_t(1) << (2 * l);
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nbox) return;
+    const int64_t bi = level_off(l) + m;
+    if (t.cnt[bi] == 0) return;
+    cplx b[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) b[k] = {0.0, 0.0};
+    const int ix = (int)compact_bits((uint32_t)m), iz = (int)compact_bits((uint32_t)m >> 1);
+    if (l > 2) {
+        const int64_t pi = level_off(l - 1) + (m >> 2);
+        const int ch = (int)(m & 3);
+        const cplx e = {(ch & 1) ? 0.25 : -0.25, (ch & 2) ? 0.25 : -0.25};
+        cplx pb[P];
+#pragma unroll
+        for (int k = 0; k < P; ++k) pb[k] = t.B[pi * P + k];
+        cplx ep[P];
+        ep[0] = {1.0, 0.0};
+#pragma unroll
+        for (int k = 1; k < P; ++k) ep[k] = cmul(ep[k - 1], e);
+        double f = 0.5;
+#pragma unroll
+        for (int mm = 0; mm < P; ++mm) {
+            if (mm < t.p) {
+                cplx acc = {0.0, 0.0};
+#pragma unroll
+                for (int ll = mm; ll < P; ++ll)
+                    if (ll < t.p) acc = cadd(acc, cscale(cmul(pb[ll], ep[ll - mm]), t.binom[ll * kBinom + mm]));
+                b[mm] = cscale(acc, f);
+            }
+            f *= 0.5;
+        }
+    }
+    const int side = 1 << l;
+    const int pos = (ix & 1) | ((iz & 1) << 1);
+    const int* il = t.il + pos * t.il_stride;
+    for (int q = 0; q < t.il_n[pos]; ++q) {
+        const int sx = ix + il[2 * q], sz = iz + il[2 * q + 1];
+        if (sx < 0 || sz < 0 || sx >= side || sz >= side) continue;
+        const int64_t si = level_off(l) + morton((uint32_t)sx, (uint32_t)sz);
+        if (t.cnt[si] == 0) continue;
+        // T = (c_tgt - c_src) / s; a'_k = A_k t^(k+1) with t = 1/T;
+        // b_l += (-1)^l t^l sum_k C(k+l, l) a'_k
+        const cplx tv = cinv({(double)(ix - sx), (double)(iz - sz)});
+        cplx ap[P];
+        cplx pw = tv;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            ap[k] = cmul(t.A[si * P + k], pw);
+            pw = cmul(pw, tv);
+        }
+        cplx tl = {1.0, 0.0};
+#pragma unroll
+        for (int ll = 0; ll < P; ++ll) {
+            if (ll < t.p) {
+                cplx acc = {0.0, 0.0};
+#pragma unroll
+                for (int k = 0; k < P; ++k)
+                    if (k < t.p) acc = cadd(acc, cscale(ap[k], t.binom[(k + ll) * kBinom + ll]));
+                acc = cmul(acc, tl);
+                b[ll] = (ll & 1) ? cadd(b[ll], cscale(acc, -1.0)) : cadd(b[ll], acc);
+            }
+            tl = cmul(tl, tv);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k) t.B[bi * P + k] = b[k];
+}
+
+// L2P + P2P: one thread per particle in sorted order.
+template <int P>
+__global__ void fmm_eval_kernel(FmmTree t, int n, const double4* __restrict__ sp,
+                                const uint32_t* __restrict__ skey, const int* __restrict__ perm,
+                                double* __restrict__ u, double* __restrict__ w)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double4 me = sp[k];
+    const uint32_t m = skey[k];
+    const int L = t.L;
+    const double s = t.S / (double)(1 << L);
+    double fu = 0.0, fw = 0.0;
+    if (L >= 2) {
+        const int64_t bi = level_off(L) + m;
+        const cplx c = box_centre(t, L, m);
+        const cplx wv = {(me.x - c.re) / s, (me.y - c.im) / s};
+        cplx phi = {0.0, 0.0};
+#pragma unroll
+        for (int ll = P - 1; ll >= 0; --ll)
+            if (ll < t.p) phi = cadd(cmul(phi, wv), t.B[bi * P + ll]);
+        fu = phi.im / s;
+        fw = phi.re / s;
+    }
+    // near field with the reference's regularised kernel
+    const int ix = (int)compact_bits(m), iz = (int)compact_bits(m >> 1);
+    const int side = 1 << L;
+    double nu = 0.0, nw = 0.0;
+    for (int q = 0; q < t.near_n; ++q) {
+        const int sx = ix + t.nearo[2 * q], sz = iz + t.nearo[2 * q + 1];
+        if (sx < 0 || sz < 0 || sx >= side || sz >= side) continue;
+        const uint32_t lm = morton((uint32_t)sx, (uint32_t)sz);
+        const int b = t.lbeg[lm], e = t.lbeg[lm + 1];
+        for (int j = b; j < e; ++j) {
+            if (j == k) continue;
+            const double4 q4 = sp[j];
+            const double dx = me.x - q4.x;
+            const double dz = me.y - q4.y;
+            const double r2 = dx * dx + dz * dz;
+            const double s2 = q4.w * q4.w;
+            const double dk = r2 < 10.0 * s2 ? (1.0 - exp(-r2 * (1.0 / s2))) / r2 : 1.0 / r2;
+            nu -= dk * dz * q4.z;
+            nw += dk * dx * q4.z;
+        }
+    }
+    const int i = perm[k];
+    u[i] = (fu + nu) * kRpi2;
+    w[i] = (fw + nw) * kRpi2;
+}
+
+template <int P>
+void fmm_levels(dvm_ctx_t ctx, const FmmTree& t, int n, const double4* sp, const uint32_t* skey,
+                const int* perm, double* u, double* w)
+{
+    hipStream_t st = ctx->stream;
+    const int64_t nleaf = int64_t(1) << (2 * t.L);
+    {
+        ProfScope ps(ctx, "fmm_p2m_kernel");
+        fmm_p2m_kernel<P><<<cdiv(nleaf, 64), 64, 0, st>>>(t, sp);
+        DVM_LAUNCH_CHECK();
+    }
+    {
+        ProfScope ps(ctx, "fmm_m2m_kernel");
+        for (int l = t.L - 1; l >= 2; --l) {
+            fmm_m2m_kernel<P><<<cdiv(int64_t(1) << (2 * l), 64), 64, 0, st>>>(t, l);
+            DVM_LAUNCH_CHECK();
+        }
+    }
+    {
+        ProfScope ps(ctx, "fmm_down_kernel");
+        for (int l = 2; l <= t.L; ++l) {
+            fmm_down_kernel<P><<<cdiv(int64_t(1) << (2 * l), 64), 64, 0, st>>>(t, l);
+            DVM_LAUNCH_CHECK();
+        }
+    }
+    {
+        ProfScope ps(ctx, "fmm_eval_kernel");
+        fmm_eval_kernel<P><<<cdiv(n, 128), 128, 0, st>>>(t, n, sp, skey, perm, u, w);
+        DVM_LAUNCH_CHECK();
+    }
+}
+
+constexpr int kOrders[] = {8, 12, 16, 20, 24, 32};
+constexpr int kRedBlocks = 256;
+
+// Box of the finite positions, largest core size and the count of
+// non-finite entries: one partial per block, [xmin, xmax, zmin, zmax, smax, bad].
+__global__ void fmm_bbox_kernel(int n, const double* __restrict__ x, const double* __restrict__ z,
+                                const double* __restrict__ s, const double* __restrict__ c,
+                                double* __restrict__ part)
+{
+    __shared__ double r[6][256];
+    double a0 = INFINITY, a1 = -INFINITY, a2 = INFINITY, a3 = -INFINITY, a4 = 0.0, a5 = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double xi = x[i], zi = z[i], si = s[i], ci = c[i];
+        if (isfinite(xi) && isfinite(zi) && isfinite(si) && isfinite(ci)) {
+            a0 = fmin(a0, xi);
+            a1 = fmax(a1, xi);
+            a2 = fmin(a2, zi);
+            a3 = fmax(a3, zi);
+            a4 = fmax(a4, fabs(si));
+        } else {
+            a5 += 1.0;
+        }
+    }
+    const int t = threadIdx.x;
+    r[0][t] = a0; r[1][t] = a1; r[2][t] = a2; r[3][t] = a3; r[4][t] = a4; r[5][t] = a5;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) {
+            r[0][t] = fmin(r[0][t], r[0][t + h]);
+            r[1][t] = fmax(r[1][t], r[1][t + h]);
+            r[2][t] = fmin(r[2][t], r[2][t + h]);
+            r[3][t] = fmax(r[3][t], r[3][t + h]);
+            r[4][t] = fmax(r[4][t], r[4][t + h]);
+            r[5][t] += r[5][t + h];
+        }
+        __syncthreads();
+    }
+    if (t < 6) part[blockIdx.x * 6 + t] = r[t][0];
+}
+
+}  // namespace
+
+// The expansion order used for a requested one: the next supported order.
+int fmm_terms(int order)
+{
+    for (int p : kOrders)
+        if (order <= p) return p;
+    return 0;
+}
+
+bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const double* c,
+              const double* s, double* u, double* w)
+{
+    const int order = ctx->fmm_order;
+    const double theta = ctx->fmm_theta;
+    require(n >= 0 && n < (int64_t(1) << 31), DVM_E_SIZE, "blob count out of range");
+    const int P = fmm_terms(order);
+    require(order >= 1 && P > 0, DVM_E_INVALID, "FMM expansion order must be 1..32");
+    require(theta > 0.2 && theta <= 1.0, DVM_E_INVALID, "FMM theta must be in (0.2, 1]");
+    ctx->fmm_last_level = 0;
+    if (n < 2) return false;
+    hipStream_t st = ctx->stream;
+
+    // near offsets: centre distance <= sqrt(2)/theta leaf widths; interaction
+    // list per child position: children of the parent's near boxes not near
+    const double r2n = 2.0 / (theta * theta) * (1.0 + 1e-12);
+    const int rmax = (int)std::ceil(std::sqrt(r2n)) + 1;
+    std::vector<int> nearo;
+    for (int dz = -rmax; dz <= rmax; ++dz)
+        for (int dx = -rmax; dx <= rmax; ++dx)
+            if ((double)(dx * dx + dz * dz) <= r2n) {
+                nearo.push_back(dx);
+                nearo.push_back(dz);
+            }
+    std::vector<int> il[4];
+    for (int pos = 0; pos < 4; ++pos) {
+        const int bx = pos & 1, bz = pos >> 1;
+        for (size_t q = 0; q < nearo.size(); q += 2)
+            for (int a = 0; a < 2; ++a)
+                for (int cc = 0; cc < 2; ++cc) {
+                    const int ox = 2 * nearo[q] + a - bx, oz = 2 * nearo[q + 1] + cc - bz;
+                    if ((double)(ox * ox + oz * oz) <= r2n) continue;
+                    il[pos].push_back(ox);
+                    il[pos].push_back(oz);
+                }
+    }
+    // smallest distance between points of two leaves that are not near (leaf widths)
+    double dmin = 1e300;
+    for (int dz = -2 * rmax; dz <= 2 * rmax; ++dz)
+        for (int dx = -2 * rmax; dx <= 2 * rmax; ++dx) {
+            if ((double)(dx * dx + dz * dz) <= r2n) continue;
+            const double ax = std::max(std::abs(dx) - 1, 0), az = std::max(std::abs(dz) - 1, 0);
+            dmin = std::min(dmin, std::sqrt(ax * ax + az * az));
+        }
+
+    // extent, largest core, non-finite entries
+    double* red = static_cast<double*>(ws_get(ctx, sizeof(double) * 6 * kRedBlocks));
+    fmm_bbox_kernel<<<kRedBlocks, 256, 0, st>>>((int)n, x, z, s, c, red);
+    DVM_LAUNCH_CHECK();
+    double* hr = static_cast<double*>(pinned_get(ctx, sizeof(double) * 6 * kRedBlocks));
+    DVM_HIP_CHECK(hipMemcpyAsync(hr, red, sizeof(double) * 6 * kRedBlocks, hipMemcpyDeviceToHost, st));
+    DVM_HIP_CHECK(hipStreamSynchronize(st));
+    double x0 = INFINITY, x1 = -INFINITY, z0 = INFINITY, z1 = -INFINITY, smax = 0.0, bad = 0.0;
+    for (int k = 0; k < kRedBlocks; ++k) {
+        x0 = std::fmin(x0, hr[6 * k]);
+        x1 = std::fmax(x1, hr[6 * k + 1]);
+        z0 = std::fmin(z0, hr[6 * k + 2]);
+        z1 = std::fmax(z1, hr[6 * k + 3]);
+        smax = std::fmax(smax, hr[6 * k + 4]);
+        bad += hr[6 * k + 5];
+    }
+    if (bad > 0.0) return false;  // non-finite input: the direct sum reproduces it
+    double S = std::fmax(x1 - x0, z1 - z0);
+    if (!(S > 0.0)) return false;
+    S *= 1.0 + 1e-9;
+    // depth: about 16 blobs per leaf over the box, leaves no narrower than
+    // sqrt(10) * max(sigma) / dmin (every regularised pair in the near field)
+    int L = 0;
+    while (L < kFmmMaxLevel && (double)n / (double)(int64_t(1) << (2 * (L + 1))) >= 16.0) ++L;
+    const double smin = std::sqrt(10.0) * smax * (1.0 + 1e-9) / dmin;
+    while (L > 0 && S / (double)(1 << L) < smin) --L;
+    if (L < 2) return false;  // no well-separated boxes: direct sum
+    ctx->fmm_last_level = L;
+
+    const int64_t nbox = level_off(L + 1);
+    const int64_t nleaf = int64_t(1) << (2 * L);
+    size_t sort_bytes = 0;
+    DVM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint32_t*)nullptr,
+                                                     (uint32_t*)nullptr, (int*)nullptr, (int*)nullptr,
+                                                     (int)n, 0, 2 * L, st));
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t il_stride = std::max({il[0].size(), il[1].size(), il[2].size(), il[3].size()}) / 2;
+    const size_t need = al(sizeof(double) * kBinom * kBinom) + al(sizeof(int) * 8 * il_stride) +
+                        al(sizeof(int) * nearo.size()) + 2 * al(sizeof(uint32_t) * n) +
+                        2 * al(sizeof(int) * n) + al(sizeof(double4) * n) + al(sizeof(int) * (nleaf + 1)) +
+                        al(sizeof(int) * nbox) + 2 * al(sizeof(cplx) * P * nbox) + al(sort_bytes);
+    char* p = static_cast<char*>(ws_get(ctx, need));
+    auto take = [&](size_t bytes) {
+        char* q = p;
+        p += al(bytes);
+        return q;
+    };
+    double* binom = reinterpret_cast<double*>(take(sizeof(double) * kBinom * kBinom));
+    int* ild = reinterpret_cast<int*>(take(sizeof(int) * 8 * il_stride));
+    int* neard = reinterpret_cast<int*>(take(sizeof(int) * nearo.size()));
+    uint32_t* key = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * n));
+    uint32_t* skey = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * n));
+    int* idx = reinterpret_cast<int*>(take(sizeof(int) * n));
+    int* perm = reinterpret_cast<int*>(take(sizeof(int) * n));
+    double4* sp = reinterpret_cast<double4*>(take(sizeof(double4) * n));
+    int* lbeg = reinterpret_cast<int*>(take(sizeof(int) * (nleaf + 1)));
+    int* cnt = reinterpret_cast<int*>(take(sizeof(int) * nbox));
+    cplx* A = reinterpret_cast<cplx*>(take(sizeof(cplx) * P * nbox));
+    cplx* B = reinterpret_cast<cplx*>(take(sizeof(cplx) * P * nbox));
+    void* sort_tmp = take(sort_bytes);
+
+    // tables: interaction lists padded to il_stride entries per child position
+    std::vector<int> ilh(8 * il_stride, 0);
+    FmmTree t{};
+    for (int pos = 0; pos < 4; ++pos) {
+        std::copy(il[pos].begin(), il[pos].end(), ilh.begin() + pos * 2 * il_stride);
+        t.il_n[pos] = (int)(il[pos].size() / 2);
+    }
+    DVM_HIP_CHECK(hipMemcpyAsync(ild, ilh.data(), sizeof(int) * ilh.size(), hipMemcpyHostToDevice, st));
+    DVM_HIP_CHECK(hipMemcpyAsync(neard, nearo.data(), sizeof(int) * nearo.size(), hipMemcpyHostToDevice, st));
+    fmm_binom_kernel<<<1, kBinom, 0, st>>>(binom);
+    DVM_LAUNCH_CHECK();
+
+    const double sl = S / (double)(1 << L);
+    fmm_key_kernel<<<cdiv(n, 256), 256, 0, st>>>((int)n, x, z, x0, z0, sl, 1 << L, key, idx);
+    DVM_LAUNCH_CHECK();
+    DVM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(sort_tmp, sort_bytes, key, skey, idx, perm, (int)n,
+                                                     0, 2 * L, st));
+    fmm_gather_kernel<<<cdiv(n, 256), 256, 0, st>>>((int)n, perm, x, z, c, s, sp);
+    DVM_LAUNCH_CHECK();
+    fmm_lbeg_kernel<<<cdiv(n + 1, 256), 256, 0, st>>>((int)n, skey, (int)nleaf, lbeg);
+    DVM_LAUNCH_CHECK();
+
+    t.x0 = x0;
+    t.z0 = z0;
+    t.S = S;
+    t.L = L;
+    t.p = order;
+    t.binom = binom;
+    t.il = ild;
+    t.il_stride = (int)(2 * il_stride);
+    t.nearo = neard;
+    t.near_n = (int)(nearo.size() / 2);
+    t.lbeg = lbeg;
+    t.cnt = cnt;
+    t.A = A;
+    t.B = B;
+    switch (P) {
+    case 8: fmm_levels<8>(ctx, t, (int)n, sp, skey, perm, u, w); break;
+    case 12: fmm_levels<12>(ctx, t, (int)n, sp, skey, perm, u, w); break;
+    case 16: fmm_levels<16>(ctx, t, (int)n, sp, skey, perm, u, w); break;
+    case 20: fmm_levels<20>(ctx, t, (int)n, sp, skey, perm, u, w); break;
+    case 24: fmm_levels<24>(ctx, t, (int)n, sp, skey, perm, u, w); break;
+    default: fmm_levels<32>(ctx, t, (int)n, sp, skey, perm, u, w); break;
+    }
+    // the host tables must outlive the copies that read them
+    DVM_HIP_CHECK(hipStreamSynchronize(st));
+    return true;
+}
+
+}  // namespace dvmi
+
+using namespace dvmi;
+
+extern "C" {
+
+dvm_status dvm_set_fmm(dvm_ctx_t ctx, int expansion_order, double theta)
+{
+    if (!ctx) return DVM_E_INVALID;
+    return guard(ctx, [&] {
+        require(expansion_order >= 1 && fmm_terms(expansion_order) > 0, DVM_E_INVALID,
+                "FMM expansion order must be 1..32");
+        require(theta > 0.2 && theta <= 1.0, DVM_E_INVALID, "FMM theta must be in (0.2, 1]");
+        ctx->fmm_order = expansion_order;
+        ctx->fmm_theta = theta;
+    });
+}
+
+dvm_status dvm_fmm_last_level(dvm_ctx_t ctx, int* level)
+{
+    if (!ctx || !level) return DVM_E_INVALID;
+    *level = ctx->fmm_last_level;
+    return DVM_OK;
+}
+
+}  // extern "C"

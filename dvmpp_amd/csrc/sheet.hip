@@ -835,7 +835,8 @@ dvm_status dvm_sheet_probe_quirk(dvm_sheet_t s, int prec, int64_t n, const doubl
 {
     if (!s) return DVM_E_INVALID;
     return guard(s->ctx, [&] {
-        require(prec == DVM_F64 || prec == DVM_F32C, DVM_E_INVALID, "unknown precision");
+        require(prec == DVM_F64 || prec == DVM_F32C || prec == DVM_FMM, DVM_E_INVALID,
+                "unknown precision");
         require(n >= 0 && np >= 0, DVM_E_SIZE, "negative count");
         require(n < 2 || (x && z && circ && sigma), DVM_E_INVALID, "null blob array");
         require(np == 0 || (px && pz && u && w && uvs && wvs), DVM_E_INVALID,

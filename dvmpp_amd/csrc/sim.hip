@@ -461,7 +461,8 @@ dvm_status dvm_sim_create(dvm_ctx_t ctx, int npts, const double* px, const doubl
         require(p.scheme == DVM_EULER || p.scheme == DVM_RK3, DVM_E_INVALID, "unknown scheme");
         require(p.crossing >= DVM_DELETE && p.crossing <= DVM_REFLECT, DVM_E_INVALID,
                 "unknown surface crossing");
-        require(p.prec == DVM_F64 || p.prec == DVM_F32C, DVM_E_INVALID, "unknown precision");
+        require(p.prec == DVM_F64 || p.prec == DVM_F32C || p.prec == DVM_FMM, DVM_E_INVALID,
+                "unknown precision");
         require(p.rng_mode == DVM_RNG_REFERENCE || p.rng_mode == DVM_RNG_DEVICE, DVM_E_INVALID,
                 "unknown rng mode");
         if (p.remesh.frequency > 0) {
@@ -891,7 +892,8 @@ dvm_status dvm_sim_probe_quirk(dvm_sim_t s, int prec, int64_t np, const double* 
 {
     if (!s) return DVM_E_INVALID;
     return guard(s->ctx, [&] {
-        require(prec == DVM_F64 || prec == DVM_F32C, DVM_E_INVALID, "unknown precision");
+        require(prec == DVM_F64 || prec == DVM_F32C || prec == DVM_FMM, DVM_E_INVALID,
+                "unknown precision");
         require(np >= 0, DVM_E_SIZE, "negative probe count");
         require(np == 0 || (px && pz && u && w && uvs && wvs), DVM_E_INVALID, "null probe array");
         sheet_probe_quirk(s->ctx, s->sh, prec, s->n, s->b.x, s->b.z, s->b.c, s->b.s, np, px, pz, u, w,

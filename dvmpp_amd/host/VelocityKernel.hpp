@@ -4,8 +4,10 @@
 //   create("direct")      -> DirectKernel: FP64 direct sum on the GPU, the
 //                            reference's pair arithmetic and order
 //   create("direct_f32c") -> DirectF32CKernel: FP32-compensated direct sum
-//   anything else         -> AlgorithmException (the reference's FMM kernel is
-//                            a stub that is compiled out by default)
+//   create("fmm")         -> FMMKernel(10, 0.5): 2-D fast multipole sum on the
+//                            GPU (the reference's FMMKernel, a stub there that
+//                            calls the direct sum); targets use the direct sum
+//   anything else         -> AlgorithmException
 #ifndef DVMPP_AMD_VELOCITYKERNEL_HPP
 #define DVMPP_AMD_VELOCITYKERNEL_HPP
 
@@ -62,6 +64,23 @@ class DirectF32CKernel : public DirectKernel {
 public:
     [[nodiscard]] std::string name() const override { return "direct_f32c"; }
     [[nodiscard]] int precision() const override;
+};
+
+/// src/VelocityKernel.hpp:84-110: expansion order (terms, 1..32) and opening
+/// criterion theta in (0.2, 1]; the error falls roughly as theta^order.
+class FMMKernel : public DirectKernel {
+public:
+    explicit FMMKernel(int expansion_order = 10, double theta = 0.5);
+    void computeSelfInduced(const std::vector<VortexParticle>& particles,
+                            std::vector<VelocityResult>& velocities) override;
+    [[nodiscard]] std::string name() const override { return "fmm"; }
+    [[nodiscard]] int precision() const override;
+    [[nodiscard]] int expansion_order() const { return m_expansion_order; }
+    [[nodiscard]] double theta() const { return m_theta; }
+
+private:
+    int m_expansion_order;
+    double m_theta;
 };
 
 }  // namespace dvm

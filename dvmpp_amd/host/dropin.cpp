@@ -85,12 +85,29 @@ std::unique_ptr<VelocityKernel> VelocityKernel::create(const std::string& type)
 {
     if (type == "direct") return std::make_unique<DirectKernel>();
     if (type == "direct_f32c") return std::make_unique<DirectF32CKernel>();
+    if (type == "fmm") return std::make_unique<FMMKernel>();
     throw AlgorithmException("Unknown velocity kernel type: " + type +
-                             ". Available: direct, direct_f32c");
+                             ". Available: direct, direct_f32c, fmm");
 }
 
 int DirectKernel::precision() const { return DVM_F64; }
 int DirectF32CKernel::precision() const { return DVM_F32C; }
+int FMMKernel::precision() const { return DVM_FMM; }
+
+FMMKernel::FMMKernel(int expansion_order, double theta)
+    : m_expansion_order(expansion_order), m_theta(theta)
+{
+    if (expansion_order < 1 || expansion_order > 32 || !(theta > 0.2 && theta <= 1.0))
+        throw AlgorithmException("FMMKernel: expansion order must be 1..32 and theta in (0.2, 1]");
+}
+
+void FMMKernel::computeSelfInduced(const std::vector<VortexParticle>& particles,
+                                   std::vector<VelocityResult>& velocities)
+{
+    dvm_ctx_t ctx = device_context();
+    check(dvm_set_fmm(ctx, m_expansion_order, m_theta), "FMMKernel", ctx);
+    DirectKernel::computeSelfInduced(particles, velocities);
+}
 
 static_assert(sizeof(VortexParticle) == 4 * sizeof(double), "AoS particle layout");
 static_assert(sizeof(VelocityResult) == 2 * sizeof(double), "AoS result layout");
@@ -632,6 +649,17 @@ void DVMBase::from_xml(XmlHandler& xml, const std::string& timestamp, const RunO
     }
     m_p.prec = m_opts.precision;
     m_p.rng_mode = m_opts.rng_mode;
+    // <algorithms><velocity_kernel> (optional; the reference lists it but its
+    // blobs always use "direct"): "fmm" selects the fast multipole sum with
+    // <fmm_expansion_order> and <fmm_theta> (defaults 10 and 0.5)
+    if (xml.has("algorithms", "velocity_kernel") && str("algorithms", "velocity_kernel") == "fmm") {
+        const int order = xml.has("algorithms", "fmm_expansion_order")
+                              ? (int)val("algorithms", "fmm_expansion_order") : 10;
+        const double theta = xml.has("algorithms", "fmm_theta") ? val("algorithms", "fmm_theta") : 0.5;
+        dvm_ctx_t ctx = dvm::device_context();
+        dvm::check(dvm_set_fmm(ctx, order, theta), "velocity_kernel fmm", ctx);
+        m_p.prec = DVM_FMM;
+    }
 
     // Output directory first: the blob files are created before the body is read.
     dvm::OutputSpec spec;

@@ -43,8 +43,15 @@ typedef enum dvm_status {
  *              (src/VelocityKernel.cpp:30-108); the parity form.
  *   DVM_F32C : FP32 pair terms on spatially sorted, tile-recentred
  *              coordinates with compensated (Kahan) accumulation across
- *              source tiles; results returned in FP64. */
-typedef enum dvm_precision { DVM_F64 = 0, DVM_F32C = 1 } dvm_precision;
+ *              source tiles; results returned in FP64.
+ *   DVM_FMM  : blob-on-blob sums by a 2-D fast multipole method in FP64
+ *              (the reference's FMMKernel, src/VelocityKernel.hpp:84-110,
+ *              whose body there falls back to the direct sum): exact
+ *              regularised near field, far field truncated at the context's
+ *              expansion order and opening criterion (dvm_set_fmm).  Sums at
+ *              separate targets use DVM_F64, as the reference's
+ *              FMMKernel::computeInducedAt does.  Not for the batched engine. */
+typedef enum dvm_precision { DVM_F64 = 0, DVM_F32C = 1, DVM_FMM = 2 } dvm_precision;
 
 typedef struct dvm_ctx_s* dvm_ctx_t;
 
@@ -375,6 +382,17 @@ dvm_status dvm_event_query(dvm_event_t ev, int* done);
 dvm_status dvm_event_destroy(dvm_event_t ev);
 dvm_status dvm_host_alloc(size_t bytes, void** ptr);
 dvm_status dvm_host_free(void* ptr);
+
+/* FMMKernel(expansion_order, theta) (src/VelocityKernel.hpp:84-90; the
+ * reference's defaults are 10 and 0.5): the parameters of DVM_FMM sums on
+ * this context.  The order (1..32) is the number of expansion terms; theta in
+ * (0.2, 1] separates boxes whose centre distance exceeds sqrt(2) * width /
+ * theta.  The error falls roughly as theta^order.  Inputs with non-finite
+ * values, fewer than two blobs, or too few for a tree of three levels use the
+ * direct FP64 sum; dvm_fmm_last_level reports the leaf level of the last
+ * DVM_FMM sum (0: it was direct). */
+dvm_status dvm_set_fmm(dvm_ctx_t ctx, int expansion_order, double theta);
+dvm_status dvm_fmm_last_level(dvm_ctx_t ctx, int* level);
 
 /* VortexBlobs::merge_blobs (src/VortexBlobs.cpp:110-209) on n device blobs,
  * in place: the reference's greedy pass in index order (blob i absorbs each

@@ -2,15 +2,18 @@
 // GoogleTest suites case by case (tests/unit, tests/integration,
 // tests/regression) with a minimal runner, since GoogleTest is not available
 // offline.  Needs a GPU: the classes run the HIP kernels.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <functional>
+#include <random>
 #include <set>
 #include <string>
 #include <vector>
 
 #include "BackgroundMesh.hpp"
 #include "DVMBase.hpp"
+#include "Device.hpp"
 #include "Exceptions.hpp"
 #include "Random.hpp"
 #include "VelocityKernel.hpp"
@@ -70,6 +73,60 @@ TEST(CreateDirectKernel)
 }
 
 TEST(CreateInvalidKernelThrows) { EXPECT_THROW((void)dvm::VelocityKernel::create("invalid"), dvm::AlgorithmException); }
+
+// tests/unit/test_VelocityKernel.cpp:134-165 (the USE_FMM cases)
+TEST(CreateFMMKernel)
+{
+    auto k = dvm::VelocityKernel::create("fmm");
+    EXPECT(k != nullptr);
+    EXPECT(k->name() == "fmm");
+    EXPECT_THROW(dvm::FMMKernel(0, 0.5), dvm::AlgorithmException);
+    EXPECT_THROW(dvm::FMMKernel(10, 1.5), dvm::AlgorithmException);
+}
+
+static std::vector<dvm::VortexParticle> random_particles(int n, double lo, double hi, double sigma)
+{
+    std::mt19937 rng(42);
+    std::uniform_real_distribution<> pos(lo, hi);
+    std::uniform_real_distribution<> circ(-1.0, 1.0);
+    std::vector<dvm::VortexParticle> p(n);
+    for (int i = 0; i < n; ++i) p[i] = {pos(rng), pos(rng), circ(rng), sigma};
+    return p;
+}
+
+TEST(FMMKernelMatchesDirectForSmallN)
+{
+    auto direct = dvm::VelocityKernel::create("direct");
+    auto fmm = dvm::VelocityKernel::create("fmm");
+    const auto p = random_particles(100, -10.0, 10.0, 0.1);
+    std::vector<dvm::VelocityResult> vd, vf;
+    direct->computeSelfInduced(p, vd);
+    fmm->computeSelfInduced(p, vf);
+    for (int i = 0; i < 100; ++i) {
+        EXPECT_NEAR(vf[i].u, vd[i].u, 1e-6);
+        EXPECT_NEAR(vf[i].w, vd[i].w, 1e-6);
+    }
+}
+
+TEST(FMMKernelMatchesDirectWithATree)
+{
+    // large enough for a multi-level tree; order 20 reaches FP64 rounding
+    dvm::FMMKernel fmm(20, 0.5);
+    auto direct = dvm::VelocityKernel::create("direct");
+    const auto p = random_particles(20000, -10.0, 10.0, 0.02);
+    std::vector<dvm::VelocityResult> vd, vf;
+    direct->computeSelfInduced(p, vd);
+    fmm.computeSelfInduced(p, vf);
+    int level = 0;
+    dvm::check(dvm_fmm_last_level(dvm::device_context(), &level), "level", nullptr);
+    EXPECT(level >= 4);
+    double scale = 0.0, err = 0.0;
+    for (size_t i = 0; i < p.size(); ++i) {
+        scale = std::max(scale, std::fabs(vd[i].u) + std::fabs(vd[i].w));
+        err = std::max(err, std::fabs(vf[i].u - vd[i].u) + std::fabs(vf[i].w - vd[i].w));
+    }
+    EXPECT(err <= 1e-10 * scale);
+}
 
 TEST(DirectKernelComputesSelfInducedVelocity)
 {

@@ -19,12 +19,12 @@ CLI = os.path.join(ROOT, "dvmpp_amd", "dvmpp")
 STEPS = 10
 
 
-def _xml(path, out_dir, steps=STEPS, probes=True, scheme="euler", crossing="REFLECT"):
+def _xml(path, out_dir, steps=STEPS, probes=True, scheme="euler", crossing="REFLECT", extra=""):
     probe = ('<x string="2.0 1.5,0.6" />\n<z string="0.1 -0.3 0.05" />' if probes
              else '<x val="20" />\n<z val="1" />')
     path.write_text(f"""<?xml version="1.0"?>
 <DVMpp>
-  <algorithms><surface_crossing string="{crossing}" /></algorithms>
+  <algorithms><surface_crossing string="{crossing}" />{extra}</algorithms>
   <constants>
     <density val="1.0" /><nu val="0.001" /><max_NumPanelVort val="3" />
     <cutoff_exp val="0.675" /><seed val="42" />
@@ -143,3 +143,27 @@ def test_cli_restore_continues_run(tmp_path):
     c = _rows(out + "c_vortex_num.dat", 1)[1]
     assert d == [c[4], c[9]]
     assert _rows(out + "d_vortex.dat", 4)[1] == []
+
+
+def test_cli_fmm_velocity_kernel_and_merging(tmp_path):
+    # <velocity_kernel string="fmm"> with order 20 (error at FP64 rounding):
+    # the run follows the direct one; merging on every 2nd step prints the
+    # reference's lines and lowers the blob count
+    out = str(tmp_path) + "/"
+    direct, fmm, merge = tmp_path / "d.xml", tmp_path / "f.xml", tmp_path / "m.xml"
+    _xml(direct, out, probes=False)
+    _xml(fmm, out, probes=False, extra='<velocity_kernel string="fmm" />'
+         '<fmm_expansion_order val="20" /><fmm_theta val="0.5" />')
+    _xml(merge, out, probes=False, extra='<merge_threshold val="1e-4" /><merge_frequency val="2" />')
+    _run(direct, "--stamp", "d", "--quiet")
+    _run(fmm, "--stamp", "f", "--quiet")
+    assert _rows(out + "f_vortex_num.dat", 1)[1] == _rows(out + "d_vortex_num.dat", 1)[1]
+    ld = np.array([r.split("\t") for r in _rows(out + "d_loads.dat", 1)[1]], dtype=float)
+    lf = np.array([r.split("\t") for r in _rows(out + "f_loads.dat", 1)[1]], dtype=float)
+    assert _close(lf, ld)
+    log = _run(merge, "--stamp", "m")
+    assert "Vortex merging enabled: threshold = 0.0001, frequency = 2" in log
+    assert "vortex pairs" in log
+    nd = [int(r.split("\t")[1]) for r in _rows(out + "d_vortex_num.dat", 1)[1]]
+    nm = [int(r.split("\t")[1]) for r in _rows(out + "m_vortex_num.dat", 1)[1]]
+    assert nm[0] == nd[0] and nm[-1] < nd[-1]

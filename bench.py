@@ -91,6 +91,32 @@ def cpu_baseline(x, z, c, s, budget_s):
                       f"loop order, OpenMP, {dt:.1f} s"}
 
 
+def fmm_rate(ctx, dvm, dev, u64, w64, pairs_self, reps=10):
+    """The blob-on-blob sum of the same blobs by the fast multipole form at
+    two orders: time per sum, error against the FP64 direct result, and the
+    direct-sum pair rate that time corresponds to."""
+    import torch
+    out = {}
+    for order, theta in ((10, 0.5), (20, 0.5)):
+        ctx.set_fmm(order, theta)
+        u, w = dvm.bs_self(ctx, *dev[:4], prec=dvm.FMM)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            u, w = dvm.bs_self(ctx, *dev[:4], prec=dvm.FMM)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        uu, ww = u.cpu().numpy(), w.cpu().numpy()
+        err = max(np.abs(uu - u64).max() / np.abs(u64).max(), np.abs(ww - w64).max() / np.abs(w64).max())
+        out[f"order{order}_theta{theta}"] = {"ms": ms, "normwise_max_err_vs_f64": float(err),
+                                             "leaf_level": ctx.fmm_last_level(),
+                                             "equivalent_pair_interactions_per_s":
+                                                 pairs_self / (ms * 1e-3)}
+    log("fmm: " + ", ".join(f"{k} {v['ms']:.2f} ms err {v['normwise_max_err_vs_f64']:.1e}"
+                            for k, v in out.items()))
+    return out
+
+
 def host_aos_rate(ctx, dvm, x, z, c, s):
     """Blob-on-blob sum through the host-pointer entry point (AoS particles in
     host memory, results back in host memory): the PCIe-inclusive rate."""
@@ -245,13 +271,14 @@ def served_phase(ctx, count, rate, steps_range, slots):
             "completed_ok": sum(1 for r in done if r.status == 0)}
 
 
-def rk3_plateau_phase(ctx, steps):
+def rk3_plateau_phase(ctx, steps, prec=None, fmm=None):
     """RK3 cylinder (512 panels, nu 5e-4, dt 0.0025, remesh M4' every 20
     steps) timed per step at a fixed-N snapshot of the plateau: a remeshed
     wake of ~1.5 M blobs is loaded into the simulation, one step runs untimed,
-    then `steps` full RK3 steps (3 Biot-Savart sums, 3 sheet solves and
+    then `steps` full RK3 steps (3 blob-on-blob sums, 3 sheet solves and
     sheet-on-blob sums, diffusion, release, crossing) are timed; one remesh of
-    the same state is timed separately."""
+    the same state is timed separately.  `prec` is the blob-on-blob sum form
+    (FP32C by default); `fmm` = (order, theta) for the FMM form."""
     import torch
 
     import dvmpp_amd as dvm
@@ -262,7 +289,10 @@ def rk3_plateau_phase(ctx, steps):
                             zmax=10.0, dx=0.01, dz=0.01, sigma=sig, circ_threshold=1e-10)
     snap = wake_snapshot(sigma=sig)
     n0 = snap["x"].size
-    p = dsim.SimParams(sp, dsim.RK3, dsim.REFLECT, 0, dvm.F32C, dsim.RNG_DEVICE, rm)
+    prec = dvm.F32C if prec is None else prec
+    if fmm is not None:
+        ctx.set_fmm(*fmm)
+    p = dsim.SimParams(sp, dsim.RK3, dsim.REFLECT, 0, prec, dsim.RNG_DEVICE, rm)
     s = dsim.Sim(ctx, px, pz, p)
     s.set_state(snap)
     s.run(1)
@@ -283,9 +313,12 @@ def rk3_plateau_phase(ctx, steps):
     torch.cuda.synchronize()
     tr = time.perf_counter() - t0
     nmean = 0.5 * (n1 + n2)
-    return {"workload": "RK3 cylinder, 512 panels, nu 5e-4, dt 0.0025, REFLECT, FP32C sums, "
+    form = ("FP32C sums" if prec == dvm.F32C else
+            f"FMM sums (order {fmm[0]}, theta {fmm[1]})" if prec == dvm.FMM else "FP64 sums")
+    return {"workload": f"RK3 cylinder, 512 panels, nu 5e-4, dt 0.0025, REFLECT, {form}, "
                         f"device random stream; remeshed-wake snapshot of {n0} blobs",
             "seconds_per_step": dt, "steps_timed": steps, "blobs": [n1, n2],
+            "steps_per_s": 1.0 / dt,
             "pair_interactions_per_s_in_step": 3.0 * nmean * (nmean - 1) / dt,
             "remesh_ms": tr * 1e3, "remesh_blobs_out": int(out[0].numel())}
 
@@ -319,6 +352,7 @@ def main():
                     help="comma list: sum (isolated sum, always run), cylinder, ensemble, "
                          "served, rk3")
     ap.add_argument("--rk3-steps", type=int, default=2)
+    ap.add_argument("--rk3-fmm-steps", type=int, default=10)
     ap.add_argument("--cylinder-steps", type=int, default=400)
     ap.add_argument("--cylinder-cpu-steps", type=int, default=40)
     ap.add_argument("--ensemble-count", type=int, default=128)
@@ -415,6 +449,7 @@ def main():
         },
         "probe_kernel_ms": {"f32c": tgt32_ms, "f64": tgt64_ms},
     }
+    line["fmm_self"] = fmm_rate(ctx, dvm, dev, u64, w64, pairs_self)
     line["host_aos_f32c"] = host_aos_rate(ctx, dvm, x, z, c, s)
     if args.copy_calibration_mib > 0:
         nbytes = args.copy_calibration_mib << 20
@@ -444,6 +479,8 @@ def main():
         line["served"] = served_phase(ctx, args.served_count, 20.0, (lo, hi), slots=32)
     if "rk3" in phases:
         line["rk3_cylinder_plateau"] = rk3_plateau_phase(ctx, args.rk3_steps)
+        line["rk3_cylinder_plateau_fmm"] = rk3_plateau_phase(ctx, args.rk3_fmm_steps, dvm.FMM,
+                                                             (12, 0.5))
     ctx.close()
     print(json.dumps(line), flush=True)
 

@@ -323,6 +323,41 @@ def rk3_plateau_phase(ctx, steps, prec=None, fmm=None):
             "remesh_ms": tr * 1e3, "remesh_blobs_out": int(out[0].numel())}
 
 
+def merge_phase(ctx, steps, thresholds=(1e-8, 1e-7)):
+    """merge_blobs on a grown wake: the C1 cylinder (256 panels, FP32C sums,
+    device stream) run `steps` steps, then one merge pass per threshold on a
+    copy of its blobs, timed, with the merged fraction and the work counts."""
+    import torch
+
+    import dvmpp_amd as dvm
+    from dvmpp_amd import sim as dsim
+    from dvmpp_amd.bodies import cylinder
+    px, pz = cylinder(256)
+    s = dsim.Sim(ctx, px, pz, dsim.sim_params(seed=3, prec=dvm.F32C, rng_mode=dsim.RNG_DEVICE))
+    s.run(steps)
+    st = s.get_state()
+    s.close()
+    n = st["x"].size
+    fields = ("x", "z", "circ", "sigma", "u", "w", "uvs", "wvs")
+    out = {"workload": f"circular cylinder r=0.5, 256 panels, Re=1000, {steps} Euler steps, "
+                       f"FP32C sums, device stream: {n} blobs", "blobs": n}
+    lib = dvm._lib.load()
+    import ctypes
+    for thr in thresholds:
+        d = [torch.from_numpy(st[k]).cuda() for k in fields]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        k = dsim.merge_blobs(ctx, thr, *d)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        lib.dvm_merge_stats(ctx.h, ctypes.byref(a), ctypes.byref(b))
+        out[f"threshold_{thr:g}"] = {"ms": ms, "merged": k, "merged_fraction": k / n,
+                                     "candidate_blobs": a.value, "recomputed_sequences": b.value}
+        log(f"merge threshold {thr:g}: {k} merges ({100 * k / n:.1f} %) in {ms:.1f} ms")
+    return out
+
+
 def load_traffic(kernel):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -348,7 +383,7 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (reference kernel "
                          "for scaling HBM byte counters)")
-    ap.add_argument("--phases", default="sum,cylinder,ensemble,served,rk3",
+    ap.add_argument("--phases", default="sum,cylinder,ensemble,served,rk3,merge",
                     help="comma list: sum (isolated sum, always run), cylinder, ensemble, "
                          "served, rk3")
     ap.add_argument("--rk3-steps", type=int, default=2)
@@ -477,6 +512,8 @@ def main():
     if "served" in phases:
         lo, hi = (int(v) for v in args.served_steps.split(","))
         line["served"] = served_phase(ctx, args.served_count, 20.0, (lo, hi), slots=32)
+    if "merge" in phases:
+        line["merge"] = merge_phase(ctx, args.cylinder_steps)
     if "rk3" in phases:
         line["rk3_cylinder_plateau"] = rk3_plateau_phase(ctx, args.rk3_steps)
         line["rk3_cylinder_plateau_fmm"] = rk3_plateau_phase(ctx, args.rk3_fmm_steps, dvm.FMM,

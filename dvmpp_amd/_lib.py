@@ -107,6 +107,7 @@ SIGNATURES.update({
     "dvm_sheet_set_gamma": (_st, [_vp, _vp]),
     "dvm_sheet_solve": (_st, [_vp, _int, _i64, _vp, _vp, _vp, _vp, _vp]),
     "dvm_sheet_on_blobs": (_st, [_vp, _i64, _vp, _vp, _vp, _vp]),
+    "dvm_sheet_on_blobs_far": (_st, [_vp, _i64, _vp, _vp, _vp, _vp]),
     "dvm_sheet_loads": (_st, [_vp, _dbl, _vp]),
     "dvm_sheet_probe_quirk": (_st, [_vp, _int, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
                                     _vp, _vp, _vp]),

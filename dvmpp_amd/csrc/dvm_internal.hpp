@@ -169,7 +169,18 @@ struct SheetDev {
     int64_t* off = nullptr;                        // release offsets (nl+1), off[nl] = Nrv
     double* cnew = nullptr;                        // circulation per nascent blob per panel
     int* flag = nullptr;                           // [0] NaN-count flag
+    // Far-field form of the sheet's velocity (fast sums only): the panels of
+    // sheet_pair are straight vortex segments, so outside kSheetFar * mr of
+    // (mx, mz) their velocity is a Laurent series with kSheetTerms moments,
+    // mA = sum_j integrals of -gamma(s) ((zeta(s) - c) / mr)^k ds over the
+    // panel, from the geometric integrals mI0 (weight 1 - s/ds) and mI1
+    // (weight s/ds) per panel (complex, interleaved re/im).
+    double mx = 0, mz = 0, mr = 1;
+    double *mI0 = nullptr, *mI1 = nullptr, *mA = nullptr;
 };
+
+constexpr int kSheetTerms = 40;
+constexpr double kSheetFar = 2.0;
 
 void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const double* pz,
                 const dvm_sheet_params* p);
@@ -178,7 +189,7 @@ void sheet_free(SheetDev& sh);
 void sheet_solve(dvm_ctx_t ctx, SheetDev& sh, int prec, int64_t n, const double* bx,
                  const double* bz, const double* bc, const double* bs);
 // vortexsheetbc
-void sheet_on_blobs(dvm_ctx_t ctx, const SheetDev& sh, int64_t n, const double* bx,
+void sheet_on_blobs(dvm_ctx_t ctx, const SheetDev& sh, int prec, int64_t n, const double* bx,
                     const double* bz, double* uvs, double* wvs);
 void sheet_loads(dvm_ctx_t ctx, SheetDev& sh, double ur);
 // probe_velocities in the reference's (quirk) form.

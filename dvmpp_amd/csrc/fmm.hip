@@ -44,6 +44,9 @@ namespace {
 
 constexpr int kFmmMaxLevel = 10;
 constexpr int kBinom = 64;  // binomial table side (2P <= 64)
+// Mean blobs per leaf over the square box that sets the depth: wakes fill a
+// fraction of the box, so occupied leaves hold several times more.
+constexpr double kLeafMean = 16.0;
 
 struct cplx {
     double re, im;
@@ -122,6 +125,7 @@ __global__ void fmm_key_kernel(int n, const double* __restrict__ x, const double
     idx[i] = i;
 }
 
+// Sorted particles as (x, z, circulation, 1 / sigma^2).
 __global__ void fmm_gather_kernel(int n, const int* __restrict__ perm, const double* __restrict__ x,
                                   const double* __restrict__ z, const double* __restrict__ c,
                                   const double* __restrict__ s, double4* __restrict__ sp)
@@ -129,16 +133,22 @@ __global__ void fmm_gather_kernel(int n, const int* __restrict__ perm, const dou
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int i = perm[k];
-    sp[k] = make_double4(x[i], z[i], c[i], s[i]);
+    sp[k] = make_double4(x[i], z[i], c[i], 1.0 / (s[i] * s[i]));
 }
 
+// lbeg[m] = first sorted position with key >= m, one thread per leaf (a
+// binary search, so long runs of empty leaves cost nothing extra).
 __global__ void fmm_lbeg_kernel(int n, const uint32_t* __restrict__ key, int nleaf, int* __restrict__ lbeg)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p > n) return;
-    const int lo = p == 0 ? -1 : (int)key[p - 1];
-    const int hi = p == n ? nleaf : (int)key[p];
-    for (int q = lo + 1; q <= hi; ++q) lbeg[q] = p;
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m > nleaf) return;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((int)key[mid] < m) lo = mid + 1;
+        else hi = mid;
+    }
+    lbeg[m] = lo;
 }
 
 __device__ __forceinline__ cplx box_centre(const FmmTree& t, int l, uint32_t m)
@@ -223,44 +233,55 @@ __global__ __launch_bounds__(64) void fmm_m2m_kernel(FmmTree t, int l)
     for (int k = 0; k < P; ++k) t.A[bi * P + k] = a[k];
 }
 
-// Level l locals: L2L from the parent (l > 2) plus M2L from the interaction
-// list.  One thread per box with particles.
+// Group width for P terms: lane l of a group holds coefficient l.
 template <int P>
-__global__ __launch_bounds__(64) void fmm_down_kernel(FmmTree t, int l)
+struct Lanes {
+    static constexpr int G = P <= 8 ? 8 : P <= 16 ? 16 : 32;
+};
+
+__device__ __forceinline__ cplx shfl_c(cplx v, int src, int width)
 {
+    return {__shfl(v.re, src, width), __shfl(v.im, src, width)};
+}
+
+// Level l locals: L2L from the parent (l > 2) plus M2L from the interaction
+// list, one group of G lanes per box with particles; lane l computes B_l.
+//   L2L  B_l  = 2^-(l+1) sum_{m>=l} C(m,l) Bp_m E^(m-l)
+//   M2L  B_l += (-1)^l t^l sum_k C(k+l,l) A_k t^(k+1),  t = 1/T
+// (A_k reaches lane l by shuffles; t^(l+1) by a prefix product over lanes).
+template <int P>
+__global__ __launch_bounds__(256) void fmm_down_kernel(FmmTree t, int l)
+{
+    constexpr int G = Lanes<P>::G;
     const int64_t nbox = int64_t(1) << (2 * l);
-    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= nbox) return;
+    const int64_t m = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const int lane = threadIdx.x & (G - 1);
+    if (m >= nbox) return;  // whole groups leave together
     const int64_t bi = level_off(l) + m;
     if (t.cnt[bi] == 0) return;
-    cplx b[P];
-#pragma unroll
-    for (int k = 0; k < P; ++k) b[k] = {0.0, 0.0};
+    const int p = t.p;
+    const bool act = lane < p;
+    cplx b = {0.0, 0.0};
     const int ix = (int)compact_bits((uint32_t)m), iz = (int)compact_bits((uint32_t)m >> 1);
     if (l > 2) {
         const int64_t pi = level_off(l - 1) + (m >> 2);
         const int ch = (int)(m & 3);
         const cplx e = {(ch & 1) ? 0.25 : -0.25, (ch & 2) ? 0.25 : -0.25};
-        cplx pb[P];
-#pragma unroll
-        for (int k = 0; k < P; ++k) pb[k] = t.B[pi * P + k];
-        cplx ep[P];
-        ep[0] = {1.0, 0.0};
-#pragma unroll
-        for (int k = 1; k < P; ++k) ep[k] = cmul(ep[k - 1], e);
-        double f = 0.5;
-#pragma unroll
-        for (int mm = 0; mm < P; ++mm) {
-            if (mm < t.p) {
-                cplx acc = {0.0, 0.0};
-#pragma unroll
-                for (int ll = mm; ll < P; ++ll)
-                    if (ll < t.p) acc = cadd(acc, cscale(cmul(pb[ll], ep[ll - mm]), t.binom[ll * kBinom + mm]));
-                b[mm] = cscale(acc, f);
+        const cplx mine = act ? t.B[pi * P + lane] : cplx{0.0, 0.0};
+        cplx ep = {1.0, 0.0};  // E^(mm - lane)
+        for (int mm = 0; mm < p; ++mm) {
+            const cplx bp = shfl_c(mine, mm, G);
+            if (act && mm >= lane) {
+                b = cadd(b, cscale(cmul(bp, ep), t.binom[mm * kBinom + lane]));
+                ep = cmul(ep, e);
             }
-            f *= 0.5;
         }
+        b = cscale(b, ldexp(1.0, -(lane + 1)));
     }
+    // binomials C(k + lane, lane) for this lane
+    double cb[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) cb[k] = t.binom[(k + lane) * kBinom + lane];
     const int side = 1 << l;
     const int pos = (ix & 1) | ((iz & 1) << 1);
     const int* il = t.il + pos * t.il_stride;
@@ -269,32 +290,27 @@ __global__ __launch_bounds__(64) void fmm_down_kernel(FmmTree t, int l)
         if (sx < 0 || sz < 0 || sx >= side || sz >= side) continue;
         const int64_t si = level_off(l) + morton((uint32_t)sx, (uint32_t)sz);
         if (t.cnt[si] == 0) continue;
-        // T = (c_tgt - c_src) / s; a'_k = A_k t^(k+1) with t = 1/T;
-        // b_l += (-1)^l t^l sum_k C(k+l, l) a'_k
-        const cplx tv = cinv({(double)(ix - sx), (double)(iz - sz)});
-        cplx ap[P];
+        const cplx T = {(double)(ix - sx), (double)(iz - sz)};
+        const cplx tv = cinv(T);
+        // t^(lane+1) by an inclusive prefix product over the group
         cplx pw = tv;
 #pragma unroll
+        for (int o = 1; o < G; o <<= 1) {
+            const cplx v = {__shfl_up(pw.re, o, G), __shfl_up(pw.im, o, G)};
+            if (lane >= o) pw = cmul(pw, v);
+        }
+        const cplx ak = act ? cmul(t.A[si * P + lane], pw) : cplx{0.0, 0.0};  // A_lane t^(lane+1)
+        cplx acc = {0.0, 0.0};
+#pragma unroll
         for (int k = 0; k < P; ++k) {
-            ap[k] = cmul(t.A[si * P + k], pw);
-            pw = cmul(pw, tv);
+            const cplx v = shfl_c(ak, k, G);
+            acc = cadd(acc, cscale(v, cb[k]));
         }
-        cplx tl = {1.0, 0.0};
-#pragma unroll
-        for (int ll = 0; ll < P; ++ll) {
-            if (ll < t.p) {
-                cplx acc = {0.0, 0.0};
-#pragma unroll
-                for (int k = 0; k < P; ++k)
-                    if (k < t.p) acc = cadd(acc, cscale(ap[k], t.binom[(k + ll) * kBinom + ll]));
-                acc = cmul(acc, tl);
-                b[ll] = (ll & 1) ? cadd(b[ll], cscale(acc, -1.0)) : cadd(b[ll], acc);
-            }
-            tl = cmul(tl, tv);
-        }
+        // (-1)^lane t^lane = (-1)^lane t^(lane+1) T
+        acc = cmul(acc, cmul(pw, T));
+        b = (lane & 1) ? cadd(b, cscale(acc, -1.0)) : cadd(b, acc);
     }
-#pragma unroll
-    for (int k = 0; k < P; ++k) t.B[bi * P + k] = b[k];
+    if (lane < P) t.B[bi * P + lane] = act ? b : cplx{0.0, 0.0};
 }
 
 // L2P + P2P: one thread per particle in sorted order.
@@ -336,8 +352,12 @@ __global__ void fmm_eval_kernel(FmmTree t, int n, const double4* __restrict__ sp
             const double dx = me.x - q4.x;
             const double dz = me.y - q4.y;
             const double r2 = dx * dx + dz * dz;
-            const double s2 = q4.w * q4.w;
-            const double dk = r2 < 10.0 * s2 ? (1.0 - exp(-r2 * (1.0 / s2))) / r2 : 1.0 / r2;
+            const double xs = r2 * q4.w;  // r^2 / sigma^2
+            // 1/r^2: hardware reciprocal and two Newton steps (about 1 ulp)
+            double inv = __builtin_amdgcn_rcp(r2);
+            inv = fma(inv, fma(-r2, inv, 1.0), inv);
+            inv = fma(inv, fma(-r2, inv, 1.0), inv);
+            const double dk = xs < 10.0 ? (1.0 - exp(-xs)) * inv : inv;
             nu -= dk * dz * q4.z;
             nw += dk * dx * q4.z;
         }
@@ -368,7 +388,7 @@ void fmm_levels(dvm_ctx_t ctx, const FmmTree& t, int n, const double4* sp, const
     {
         ProfScope ps(ctx, "fmm_down_kernel");
         for (int l = 2; l <= t.L; ++l) {
-            fmm_down_kernel<P><<<cdiv(int64_t(1) << (2 * l), 64), 64, 0, st>>>(t, l);
+            fmm_down_kernel<P><<<cdiv((int64_t(1) << (2 * l)) * Lanes<P>::G, 256), 256, 0, st>>>(t, l);
             DVM_LAUNCH_CHECK();
         }
     }
@@ -494,10 +514,10 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
     double S = std::fmax(x1 - x0, z1 - z0);
     if (!(S > 0.0)) return false;
     S *= 1.0 + 1e-9;
-    // depth: about 16 blobs per leaf over the box, leaves no narrower than
+    // depth: about kLeafMean blobs per leaf over the box, leaves no narrower than
     // sqrt(10) * max(sigma) / dmin (every regularised pair in the near field)
     int L = 0;
-    while (L < kFmmMaxLevel && (double)n / (double)(int64_t(1) << (2 * (L + 1))) >= 16.0) ++L;
+    while (L < kFmmMaxLevel && (double)n / (double)(int64_t(1) << (2 * (L + 1))) >= kLeafMean) ++L;
     const double smin = std::sqrt(10.0) * smax * (1.0 + 1e-9) / dmin;
     while (L > 0 && S / (double)(1 << L) < smin) --L;
     if (L < 2) return false;  // no well-separated boxes: direct sum
@@ -554,7 +574,7 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
                                                      0, 2 * L, st));
     fmm_gather_kernel<<<cdiv(n, 256), 256, 0, st>>>((int)n, perm, x, z, c, s, sp);
     DVM_LAUNCH_CHECK();
-    fmm_lbeg_kernel<<<cdiv(n + 1, 256), 256, 0, st>>>((int)n, skey, (int)nleaf, lbeg);
+    fmm_lbeg_kernel<<<cdiv(nleaf + 1, 256), 256, 0, st>>>((int)n, skey, (int)nleaf, lbeg);
     DVM_LAUNCH_CHECK();
 
     t.x0 = x0;

@@ -156,14 +156,19 @@ __global__ void merge_key_kernel(int n, const double* __restrict__ x, const doub
     idx[i] = i;
 }
 
-// cbeg[k] for every k in (key[p-1], key[p]] is p (lower bounds of each key).
+// cbeg[k] = first sorted position with key >= k, k = 0 .. ncell + 2, one
+// thread per key (a binary search: empty runs cost nothing extra).
 __global__ void merge_cbeg_kernel(int n, const int* __restrict__ key, int ncell, int* __restrict__ cbeg)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p > n) return;
-    const int lo = p == 0 ? -1 : key[p - 1];
-    const int hi = p == n ? ncell + 2 : key[p];
-    for (int q = lo + 1; q <= hi; ++q) cbeg[q] = p;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > ncell + 2) return;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (key[mid] < k) lo = mid + 1;
+        else hi = mid;
+    }
+    cbeg[k] = lo;
 }
 
 __device__ __forceinline__ int wave_min(int v)
@@ -703,7 +708,7 @@ int64_t merge_run(dvm_ctx_t ctx, double threshold, int64_t n, const BlobBuf& b, 
     require(sb <= L.sort_bytes, DVM_E_STATE, "merge_blobs: sort scratch too small");
     DVM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(sort_tmp, sb, key, key2, idx, sidx, (int)n, 0,
                                                      bits, st));
-    merge_cbeg_kernel<<<cdiv(n + 1, 256), 256, 0, st>>>((int)n, key2, g.ncell, cbeg);
+    merge_cbeg_kernel<<<cdiv(g.ncell + 3, 256), 256, 0, st>>>((int)n, key2, g.ncell, cbeg);
     DVM_LAUNCH_CHECK();
     g.sidx = sidx;
     g.cbeg = cbeg;

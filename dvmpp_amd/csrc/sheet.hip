@@ -155,6 +155,37 @@ void sheet_free(SheetDev& sh)
     sh.mem = nullptr;
 }
 
+// Nodes and weights of n-point Gauss-Legendre quadrature on [-1, 1]
+// (Newton iteration on the Legendre polynomial from the Chebyshev guess).
+static void gauss_legendre(int n, double* x, double* w)
+{
+    for (int i = 0; i < n; ++i) {
+        double t = std::cos(3.14159265358979323846 * (i + 0.75) / (n + 0.5));
+        double dp = 1.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = t;
+            for (int k = 2; k <= n; ++k) {
+                const double p2 = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (t * p1 - p0) / (t * t - 1.0);
+            const double dt = p1 / dp;
+            t -= dt;
+            if (std::fabs(dt) < 1e-16) break;
+        }
+        double p0 = 1.0, p1 = t;
+        for (int k = 2; k <= n; ++k) {
+            const double p2 = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        dp = n * (t * p1 - p0) / (t * t - 1.0);
+        x[i] = t;
+        w[i] = 2.0 / ((1.0 - t * t) * dp * dp);
+    }
+}
+
 void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const double* pz,
                 const dvm_sheet_params* p)
 {
@@ -192,12 +223,57 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
                            std::cos(-2.0 * th), std::sin(-2.0 * th),
                            std::pow(sh.hds[i], sh.cutoff_exp)};
     }
+    // far-field moments of the panels as sheet_pair places them: segments
+    // from the collocation point along theta, length ds (24-point
+    // Gauss-Legendre, exact for the polynomial integrands up to degree 47)
+    sh.mx = 0.0;
+    sh.mz = 0.0;
+    for (int i = 0; i < nl; ++i) {
+        sh.mx += px[i];
+        sh.mz += pz[i];
+    }
+    sh.mx /= nl;
+    sh.mz /= nl;
+    sh.mr = 0.0;
+    for (int i = 0; i < nl; ++i) {
+        const double ax = pc[i].xc - sh.mx, az = pc[i].zc - sh.mz;
+        const double bx = ax + pc[i].ds * pc[i].cs, bz = az + pc[i].ds * pc[i].sn;
+        sh.mr = std::max(sh.mr, std::max(std::hypot(ax, az), std::hypot(bx, bz)));
+    }
+    if (!(sh.mr > 0.0)) sh.mr = 1.0;
+    std::vector<double> mi0(2 * (size_t)nl * kSheetTerms), mi1(2 * (size_t)nl * kSheetTerms);
+    {
+        double gx[24], gw[24];
+        gauss_legendre(24, gx, gw);
+        for (int i = 0; i < nl; ++i) {
+            const double ds = pc[i].ds;
+            for (int q = 0; q < 24; ++q) {
+                const double t = 0.5 * (gx[q] + 1.0);  // s / ds
+                const double wq = 0.5 * gw[q] * ds;
+                const double zr = (pc[i].xc + t * ds * pc[i].cs - sh.mx) / sh.mr;
+                const double zi = (pc[i].zc + t * ds * pc[i].sn - sh.mz) / sh.mr;
+                double pr = 1.0, pi = 0.0;
+                for (int k = 0; k < kSheetTerms; ++k) {
+                    const size_t o = 2 * ((size_t)i * kSheetTerms + k);
+                    mi0[o] += wq * (1.0 - t) * pr;
+                    mi0[o + 1] += wq * (1.0 - t) * pi;
+                    mi1[o] += wq * t * pr;
+                    mi1[o + 1] += wq * t * pi;
+                    const double nr = pr * zr - pi * zi;
+                    pi = pr * zi + pi * zr;
+                    pr = nr;
+                }
+            }
+        }
+    }
     const size_t d = sizeof(double);
     const size_t nred = 3 * (size_t)nl > 1024 ? 3 * (size_t)nl : 1024;
+    const size_t nm = 2 * (size_t)nl * kSheetTerms;
     const size_t need = Carver::need({d * npts, d * npts, d * nl * 8, sizeof(PanelConst) * nl,
                                       d * (nl + 1) * nl, d * nl * (nl + 1), d * nl, d * (nl + 1),
                                       d * 2 * nl, d * 8, d * nred, sizeof(int) * nl,
-                                      sizeof(int64_t) * (nl + 1), d * nl, sizeof(int) * 4});
+                                      sizeof(int64_t) * (nl + 1), d * nl, sizeof(int) * 4, d * nm,
+                                      d * nm, d * 2 * kSheetTerms});
     DVM_HIP_CHECK(hipMalloc(&sh.mem, need));
     Carver cv(sh.mem);
     sh.x = cv.take<double>(npts);
@@ -217,6 +293,9 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
     sh.off = cv.take<int64_t>(nl + 1);
     sh.cnew = cv.take<double>(nl);
     sh.flag = cv.take<int>(4);
+    sh.mI0 = cv.take<double>(nm);
+    sh.mI1 = cv.take<double>(nm);
+    sh.mA = cv.take<double>(2 * kSheetTerms);
 
     std::vector<double> g((size_t)nl * 8);
     std::memcpy(&g[0], sh.hxc.data(), d * nl);
@@ -233,6 +312,8 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
     DVM_HIP_CHECK(hipMemcpyAsync(geo, g.data(), d * nl * 8, hipMemcpyHostToDevice, st));
     DVM_HIP_CHECK(hipMemcpyAsync(sh.panel, pc.data(), sizeof(PanelConst) * nl,
                                  hipMemcpyHostToDevice, st));
+    DVM_HIP_CHECK(hipMemcpyAsync(sh.mI0, mi0.data(), d * nm, hipMemcpyHostToDevice, st));
+    DVM_HIP_CHECK(hipMemcpyAsync(sh.mI1, mi1.data(), d * nm, hipMemcpyHostToDevice, st));
     DVM_HIP_CHECK(hipMemsetAsync(sh.gamma, 0, d * nl, st));
     DVM_HIP_CHECK(hipMemsetAsync(sh.scal, 0, d * 8, st));
 
@@ -372,13 +453,81 @@ __global__ __launch_bounds__(256) void sheet_on_blobs_kernel(
     wvs[i] = sw * kRpi2;
 }
 
-void sheet_on_blobs(dvm_ctx_t ctx, const SheetDev& sh, int64_t n, const double* bx,
+// Moments of the current gamma: mA_k = sum_j -(gamma_j I0_jk + gamma_{j+1} I1_jk)
+// (gamma(s) = gamma_j (1 - s/ds) + gamma_{j+1} s/ds on panel j, cyclic), one
+// thread per moment, panels in order.
+__global__ void sheet_moments_kernel(int nl, const double* __restrict__ gamma,
+                                     const double* __restrict__ i0, const double* __restrict__ i1,
+                                     double* __restrict__ ma)
+{
+    const int k = threadIdx.x;
+    if (k >= kSheetTerms) return;
+    double ar = 0.0, ai = 0.0;
+    for (int j = 0; j < nl; ++j) {
+        const double g0 = gamma[j], g1 = gamma[j + 1 == nl ? 0 : j + 1];
+        const size_t o = 2 * ((size_t)j * kSheetTerms + k);
+        ar -= g0 * i0[o] + g1 * i1[o];
+        ai -= g0 * i0[o + 1] + g1 * i1[o + 1];
+    }
+    ma[2 * k] = ar;
+    ma[2 * k + 1] = ai;
+}
+
+// Fast form: outside kSheetFar * mr of the body centre the Laurent series
+// phi = (1/mr) sum_k A_k / W^(k+1), W = (z - c) / mr, gives uvs = Im(phi) / 2pi,
+// wvs = Re(phi) / 2pi (truncation about kSheetFar^-kSheetTerms = 1e-12 of the
+// leading term); inside it, the panel sum as in sheet_on_blobs_kernel.
+__global__ __launch_bounds__(256) void sheet_on_blobs_far_kernel(
+    int64_t n, int nl, const double* __restrict__ bx, const double* __restrict__ bz,
+    const PanelConst* __restrict__ pc, const double* __restrict__ gamma,
+    const double* __restrict__ ma, double cx, double cz, double rr, double* __restrict__ uvs,
+    double* __restrict__ wvs)
+{
+    __shared__ double a[2 * kSheetTerms];
+    for (int k = threadIdx.x; k < 2 * kSheetTerms; k += blockDim.x) a[k] = ma[k];
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double xi = bx[i];
+    const double zi = bz[i];
+    const double wr = (xi - cx) / rr, wi = (zi - cz) / rr;
+    const double w2 = wr * wr + wi * wi;
+    if (w2 > kSheetFar * kSheetFar) {
+        // t = 1 / W; phi * mr = t (A_0 + t (A_1 + ...))
+        const double tr = wr / w2, ti = -wi / w2;
+        double pr = a[2 * (kSheetTerms - 1)], pim = a[2 * (kSheetTerms - 1) + 1];
+        for (int k = kSheetTerms - 2; k >= 0; --k) {
+            const double nr = pr * tr - pim * ti + a[2 * k];
+            pim = pr * ti + pim * tr + a[2 * k + 1];
+            pr = nr;
+        }
+        const double fr = (pr * tr - pim * ti) / rr, fi = (pr * ti + pim * tr) / rr;
+        uvs[i] = fi * kRpi2;
+        wvs[i] = fr * kRpi2;
+        return;
+    }
+    double su = 0.0, sw = 0.0;
+    for (int j = 0; j < nl; ++j)
+        sheet_pair(xi, zi, pc[j], gamma[j], gamma[j + 1 == nl ? 0 : j + 1], su, sw);
+    uvs[i] = su * kRpi2;
+    wvs[i] = sw * kRpi2;
+}
+
+void sheet_on_blobs(dvm_ctx_t ctx, const SheetDev& sh, int prec, int64_t n, const double* bx,
                     const double* bz, double* uvs, double* wvs)
 {
     if (n <= 0) return;
     ProfScope ps(ctx, "sheet_on_blobs");
-    sheet_on_blobs_kernel<<<cdiv(n, 256), 256, 0, ctx->stream>>>(n, sh.nl, bx, bz, sh.panel,
-                                                                 sh.gamma, uvs, wvs);
+    if (prec == DVM_F64) {
+        sheet_on_blobs_kernel<<<cdiv(n, 256), 256, 0, ctx->stream>>>(n, sh.nl, bx, bz, sh.panel,
+                                                                     sh.gamma, uvs, wvs);
+        DVM_LAUNCH_CHECK();
+        return;
+    }
+    sheet_moments_kernel<<<1, 64, 0, ctx->stream>>>(sh.nl, sh.gamma, sh.mI0, sh.mI1, sh.mA);
+    DVM_LAUNCH_CHECK();
+    sheet_on_blobs_far_kernel<<<cdiv(n, 256), 256, 0, ctx->stream>>>(
+        n, sh.nl, bx, bz, sh.panel, sh.gamma, sh.mA, sh.mx, sh.mz, sh.mr, uvs, wvs);
     DVM_LAUNCH_CHECK();
 }
 
@@ -811,7 +960,18 @@ dvm_status dvm_sheet_on_blobs(dvm_sheet_t s, int64_t n, const double* x, const d
     return guard(s->ctx, [&] {
         require(n >= 0, DVM_E_SIZE, "negative blob count");
         require(n == 0 || (x && z && uvs && wvs), DVM_E_INVALID, "null array");
-        sheet_on_blobs(s->ctx, s->sh, n, x, z, uvs, wvs);
+        sheet_on_blobs(s->ctx, s->sh, DVM_F64, n, x, z, uvs, wvs);
+    });
+}
+
+dvm_status dvm_sheet_on_blobs_far(dvm_sheet_t s, int64_t n, const double* x, const double* z,
+                                  double* uvs, double* wvs)
+{
+    if (!s) return DVM_E_INVALID;
+    return guard(s->ctx, [&] {
+        require(n >= 0, DVM_E_SIZE, "negative blob count");
+        require(n == 0 || (x && z && uvs && wvs), DVM_E_INVALID, "null array");
+        sheet_on_blobs(s->ctx, s->sh, DVM_F32C, n, x, z, uvs, wvs);
     });
 }
 

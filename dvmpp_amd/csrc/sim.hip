@@ -274,7 +274,7 @@ void rhs(dvm_sim_t s)
     const int64_t n = s->n;
     if (n > 0) bs_self_launch(s->ctx, s->p.prec, n, s->b.x, s->b.z, s->b.c, s->b.s, s->b.u, s->b.w);
     sheet_solve(s->ctx, s->sh, s->p.prec, n, s->b.x, s->b.z, s->b.c, s->b.s);
-    sheet_on_blobs(s->ctx, s->sh, n, s->b.x, s->b.z, s->b.uvs, s->b.wvs);
+    sheet_on_blobs(s->ctx, s->sh, s->p.prec, n, s->b.x, s->b.z, s->b.uvs, s->b.wvs);
 }
 
 void convect(dvm_sim_t s)

@@ -100,13 +100,16 @@ class Sheet:
                                                  ptr(sigma), _hp(b)))
         return b
 
-    def on_blobs(self, x, z):
+    def on_blobs(self, x, z, far=False):
+        """vortexsheetbc velocities at the blobs; far=True: the fast form
+        (Laurent series of the sheet beyond two body radii)."""
         import torch
         n = x.numel()
         uvs = torch.empty(n, dtype=torch.float64, device=x.device)
         wvs = torch.empty(n, dtype=torch.float64, device=x.device)
         ptr = (lambda t: ctypes.c_void_p(t.data_ptr()))
-        check(self.ctx.h, load().dvm_sheet_on_blobs(self.h, n, ptr(x), ptr(z), ptr(uvs), ptr(wvs)))
+        fn = load().dvm_sheet_on_blobs_far if far else load().dvm_sheet_on_blobs
+        check(self.ctx.h, fn(self.h, n, ptr(x), ptr(z), ptr(uvs), ptr(wvs)))
         return uvs, wvs
 
     def loads(self, ur):

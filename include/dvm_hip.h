@@ -190,6 +190,13 @@ dvm_status dvm_sheet_solve(dvm_sheet_t sheet, int prec, int64_t n, const double*
  * velocity at n device points, fused (no N x Nl coefficient matrices). */
 dvm_status dvm_sheet_on_blobs(dvm_sheet_t sheet, int64_t n, const double* x, const double* z,
                               double* uvs, double* wvs);
+/* The same velocities in the form the fast simulations use: blobs farther
+ * than 2 body radii from the body's centroid take a 40-term Laurent series of
+ * the sheet (its panels are straight vortex segments; truncation about 1e-12
+ * of the leading term), the others the panel sum above.  No counterpart in
+ * the reference. */
+dvm_status dvm_sheet_on_blobs_far(dvm_sheet_t sheet, int64_t n, const double* x, const double* z,
+                              double* uvs, double* wvs);
 /* VortexSheet::compute_loads (src/VortexSheet.cpp:487-514) and get_forces
  * (:552-555): out4 (HOST) = {fx, fz, cd, cl}. */
 dvm_status dvm_sheet_loads(dvm_sheet_t sheet, double ur, double* out4);

@@ -284,3 +284,27 @@ def test_f32c_device_rng_run(gpu_ctx):
     c = dsim.Sim(gpu_ctx, px, pz, g32)
     c.run(20)
     assert np.array_equal(c.get_state()["x"], sb["x"])
+
+
+@pytest.mark.parametrize("body", ["cyl256", "bodycoordinatesN40.txt", "simple_body.txt"])
+def test_sheet_far_field_form(gpu_ctx, body):
+    """The fast simulations' sheet-on-blob form: a 40-term Laurent series of
+    the sheet beyond two body radii (the panels of src/VortexSheet.cpp:225-272
+    are straight linear-strength vortex segments), the panel sum inside."""
+    px, pz = oracle.cylinder_points(256) if body == "cyl256" else load_body(body)
+    _, gp = both_params()
+    gsh = dsim.Sheet(gpu_ctx, px, pz, gp.sheet)
+    rng = np.random.default_rng(9)
+    gsh.gamma = rng.normal(size=px.size - 1)
+    cx, cz = px[:-1].mean(), pz[:-1].mean()
+    ext = max(np.ptp(px), np.ptp(pz))
+    r = ext * np.exp(rng.uniform(np.log(0.3), np.log(40.0), 20000))
+    th = rng.uniform(-np.pi, np.pi, r.size)
+    X, Z = to_dev(cx + r * np.cos(th), cz + r * np.sin(th))
+    ue, we = (t.cpu().numpy() for t in gsh.on_blobs(X, Z))
+    uf, wf = (t.cpu().numpy() for t in gsh.on_blobs(X, Z, far=True))
+    far = r > 1.5 * ext
+    assert far.sum() > 10000
+    assert np.array_equal(uf[~far & (r < 0.4 * ext)], ue[~far & (r < 0.4 * ext)])
+    scale = np.max(np.hypot(ue, we))
+    assert np.max(np.abs(uf - ue)) <= 1e-11 * scale and np.max(np.abs(wf - we)) <= 1e-11 * scale

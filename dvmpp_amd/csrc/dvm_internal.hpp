@@ -186,8 +186,9 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
                 const dvm_sheet_params* p);
 void sheet_free(SheetDev& sh);
 // solvevortexsheet: collocation sum + RHS + gamma = P b.
+// have_uw: the collocation sums are already in sh.uw.
 void sheet_solve(dvm_ctx_t ctx, SheetDev& sh, int prec, int64_t n, const double* bx,
-                 const double* bz, const double* bc, const double* bs);
+                 const double* bz, const double* bc, const double* bs, bool have_uw = false);
 // vortexsheetbc
 void sheet_on_blobs(dvm_ctx_t ctx, const SheetDev& sh, int prec, int64_t n, const double* bx,
                     const double* bz, double* uvs, double* wvs);
@@ -240,8 +241,11 @@ int64_t merge_run(dvm_ctx_t ctx, double threshold, int64_t n, const BlobBuf& b, 
 
 // DVM_FMM blob-on-blob sum (fmm.hip); false when it declined (the caller then
 // runs the direct FP64 sum).
+// With nt > 0 it also writes the sums at targets (tx, tz) into (tu, tw),
+// every source counted (the unguarded direct target sum).
 bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const double* c,
-              const double* s, double* u, double* w);
+              const double* s, double* u, double* w, int64_t nt = 0, const double* tx = nullptr,
+              const double* tz = nullptr, double* tu = nullptr, double* tw = nullptr);
 
 // Biot-Savart entry points shared with the sheet and simulation code.
 void bs_self_launch(dvm_ctx_t ctx, int prec, int64_t n, const double* x, const double* z,

@@ -95,6 +95,7 @@ struct FmmTree {
     int near_n;
     const int* lbeg;      // leaf particle bounds (4^L + 1)
     int* cnt;             // particles per box (all levels)
+    int* need;            // 1: a target lies in the box (its locals are needed)
     cplx* A;              // multipole coefficients per box
     cplx* B;              // local coefficients per box
 };
@@ -198,6 +199,8 @@ __global__ __launch_bounds__(64) void fmm_m2m_kernel(FmmTree t, int l)
 #pragma unroll
     for (int k = 0; k < P; ++k) a[k] = {0.0, 0.0};
     int total = 0;
+    t.need[bi] = t.need[level_off(l + 1) + 4 * m] | t.need[level_off(l + 1) + 4 * m + 1] |
+                 t.need[level_off(l + 1) + 4 * m + 2] | t.need[level_off(l + 1) + 4 * m + 3];
     for (int ch = 0; ch < 4; ++ch) {
         const int64_t ci = level_off(l + 1) + 4 * m + ch;
         const int cn = t.cnt[ci];
@@ -258,7 +261,7 @@ __global__ __launch_bounds__(256) void fmm_down_kernel(FmmTree t, int l)
     const int lane = threadIdx.x & (G - 1);
     if (m >= nbox) return;  // whole groups leave together
     const int64_t bi = level_off(l) + m;
-    if (t.cnt[bi] == 0) return;
+    if (t.cnt[bi] == 0 && t.need[bi] == 0) return;
     const int p = t.p;
     const bool act = lane < p;
     cplx b = {0.0, 0.0};
@@ -367,9 +370,79 @@ __global__ void fmm_eval_kernel(FmmTree t, int n, const double4* __restrict__ sp
     w[i] = (fw + nw) * kRpi2;
 }
 
+// Leaves holding a target need their local expansions.
+__global__ void fmm_mark_kernel(int nt, const double* __restrict__ tx, const double* __restrict__ tz,
+                                double x0, double z0, double sl, int L, int* __restrict__ need)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nt) return;
+    const int side = 1 << L;
+    const int ix = min(max((int)floor((tx[k] - x0) / sl), 0), side - 1);
+    const int iz = min(max((int)floor((tz[k] - z0) / sl), 0), side - 1);
+    need[level_off(L) + morton((uint32_t)ix, (uint32_t)iz)] = 1;
+}
+
+// Targets: L2P at the target's leaf plus every source of the near leaves
+// (no pair skipped, as the direct target sum with no guard).  One wavefront
+// per target: the lanes share the near-field sources and their partial sums
+// are added by shuffles.
+template <int P>
+__global__ __launch_bounds__(64) void fmm_target_kernel(FmmTree t, int nt, const double* __restrict__ tx,
+                                                        const double* __restrict__ tz,
+                                                        const double4* __restrict__ sp,
+                                                        double* __restrict__ u, double* __restrict__ w)
+{
+    const int k = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (k >= nt) return;
+    const int L = t.L;
+    const int side = 1 << L;
+    const double s = t.S / (double)side;
+    const double xk = tx[k], zk = tz[k];
+    const int ix = min(max((int)floor((xk - t.x0) / s), 0), side - 1);
+    const int iz = min(max((int)floor((zk - t.z0) / s), 0), side - 1);
+    const uint32_t m = morton((uint32_t)ix, (uint32_t)iz);
+    const int64_t bi = level_off(L) + m;
+    const cplx c = box_centre(t, L, m);
+    const cplx wv = {(xk - c.re) / s, (zk - c.im) / s};
+    cplx phi = {0.0, 0.0};
+#pragma unroll
+    for (int ll = P - 1; ll >= 0; --ll)
+        if (ll < t.p) phi = cadd(cmul(phi, wv), t.B[bi * P + ll]);
+    double nu = 0.0, nw = 0.0;
+    for (int q = 0; q < t.near_n; ++q) {
+        const int sx = ix + t.nearo[2 * q], sz = iz + t.nearo[2 * q + 1];
+        if (sx < 0 || sz < 0 || sx >= side || sz >= side) continue;
+        const uint32_t lm = morton((uint32_t)sx, (uint32_t)sz);
+        const int b = t.lbeg[lm], e = t.lbeg[lm + 1];
+        for (int j = b + lane; j < e; j += 64) {
+            const double4 q4 = sp[j];
+            const double dx = xk - q4.x;
+            const double dz = zk - q4.y;
+            const double r2 = dx * dx + dz * dz;
+            const double xs = r2 * q4.w;
+            double inv = __builtin_amdgcn_rcp(r2);
+            inv = fma(inv, fma(-r2, inv, 1.0), inv);
+            inv = fma(inv, fma(-r2, inv, 1.0), inv);
+            const double dk = xs < 10.0 ? (1.0 - exp(-xs)) * inv : inv;
+            nu -= dk * dz * q4.z;
+            nw += dk * dx * q4.z;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        nu += __shfl_xor(nu, o);
+        nw += __shfl_xor(nw, o);
+    }
+    if (lane == 0) {
+        u[k] = (phi.im / s + nu) * kRpi2;
+        w[k] = (phi.re / s + nw) * kRpi2;
+    }
+}
+
 template <int P>
 void fmm_levels(dvm_ctx_t ctx, const FmmTree& t, int n, const double4* sp, const uint32_t* skey,
-                const int* perm, double* u, double* w)
+                const int* perm, double* u, double* w, int nt, const double* tx, const double* tz,
+                double* tu, double* tw)
 {
     hipStream_t st = ctx->stream;
     const int64_t nleaf = int64_t(1) << (2 * t.L);
@@ -395,6 +468,11 @@ void fmm_levels(dvm_ctx_t ctx, const FmmTree& t, int n, const double4* sp, const
     {
         ProfScope ps(ctx, "fmm_eval_kernel");
         fmm_eval_kernel<P><<<cdiv(n, 128), 128, 0, st>>>(t, n, sp, skey, perm, u, w);
+        DVM_LAUNCH_CHECK();
+    }
+    if (nt > 0) {
+        ProfScope ps(ctx, "fmm_target_kernel");
+        fmm_target_kernel<P><<<nt, 64, 0, st>>>(t, nt, tx, tz, sp, tu, tw);
         DVM_LAUNCH_CHECK();
     }
 }
@@ -450,7 +528,8 @@ int fmm_terms(int order)
 }
 
 bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const double* c,
-              const double* s, double* u, double* w)
+              const double* s, double* u, double* w, int64_t nt, const double* tx,
+              const double* tz, double* tu, double* tw)
 {
     const int order = ctx->fmm_order;
     const double theta = ctx->fmm_theta;
@@ -458,6 +537,8 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
     const int P = fmm_terms(order);
     require(order >= 1 && P > 0, DVM_E_INVALID, "FMM expansion order must be 1..32");
     require(theta > 0.2 && theta <= 1.0, DVM_E_INVALID, "FMM theta must be in (0.2, 1]");
+    require(nt >= 0 && nt < (int64_t(1) << 31), DVM_E_SIZE, "target count out of range");
+    require(nt == 0 || (tx && tz && tu && tw), DVM_E_INVALID, "null target array");
     ctx->fmm_last_level = 0;
     if (n < 2) return false;
     hipStream_t st = ctx->stream;
@@ -498,8 +579,15 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
     double* red = static_cast<double*>(ws_get(ctx, sizeof(double) * 6 * kRedBlocks));
     fmm_bbox_kernel<<<kRedBlocks, 256, 0, st>>>((int)n, x, z, s, c, red);
     DVM_LAUNCH_CHECK();
-    double* hr = static_cast<double*>(pinned_get(ctx, sizeof(double) * 6 * kRedBlocks));
+    // (targets are few: their coordinates come to the host with the partials)
+    double* hr = static_cast<double*>(pinned_get(ctx, sizeof(double) * (6 * kRedBlocks + 2 * nt)));
     DVM_HIP_CHECK(hipMemcpyAsync(hr, red, sizeof(double) * 6 * kRedBlocks, hipMemcpyDeviceToHost, st));
+    if (nt > 0) {
+        DVM_HIP_CHECK(hipMemcpyAsync(hr + 6 * kRedBlocks, tx, sizeof(double) * nt,
+                                     hipMemcpyDeviceToHost, st));
+        DVM_HIP_CHECK(hipMemcpyAsync(hr + 6 * kRedBlocks + nt, tz, sizeof(double) * nt,
+                                     hipMemcpyDeviceToHost, st));
+    }
     DVM_HIP_CHECK(hipStreamSynchronize(st));
     double x0 = INFINITY, x1 = -INFINITY, z0 = INFINITY, z1 = -INFINITY, smax = 0.0, bad = 0.0;
     for (int k = 0; k < kRedBlocks; ++k) {
@@ -509,6 +597,14 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
         z1 = std::fmax(z1, hr[6 * k + 3]);
         smax = std::fmax(smax, hr[6 * k + 4]);
         bad += hr[6 * k + 5];
+    }
+    for (int64_t k = 0; k < nt; ++k) {  // the box holds the targets too
+        const double a = hr[6 * kRedBlocks + k], b = hr[6 * kRedBlocks + nt + k];
+        if (!std::isfinite(a) || !std::isfinite(b)) bad += 1.0;
+        x0 = std::fmin(x0, a);
+        x1 = std::fmax(x1, a);
+        z0 = std::fmin(z0, b);
+        z1 = std::fmax(z1, b);
     }
     if (bad > 0.0) return false;  // non-finite input: the direct sum reproduces it
     double S = std::fmax(x1 - x0, z1 - z0);
@@ -534,7 +630,7 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
     const size_t need = al(sizeof(double) * kBinom * kBinom) + al(sizeof(int) * 8 * il_stride) +
                         al(sizeof(int) * nearo.size()) + 2 * al(sizeof(uint32_t) * n) +
                         2 * al(sizeof(int) * n) + al(sizeof(double4) * n) + al(sizeof(int) * (nleaf + 1)) +
-                        al(sizeof(int) * nbox) + 2 * al(sizeof(cplx) * P * nbox) + al(sort_bytes);
+                        2 * al(sizeof(int) * nbox) + 2 * al(sizeof(cplx) * P * nbox) + al(sort_bytes);
     char* p = static_cast<char*>(ws_get(ctx, need));
     auto take = [&](size_t bytes) {
         char* q = p;
@@ -551,6 +647,7 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
     double4* sp = reinterpret_cast<double4*>(take(sizeof(double4) * n));
     int* lbeg = reinterpret_cast<int*>(take(sizeof(int) * (nleaf + 1)));
     int* cnt = reinterpret_cast<int*>(take(sizeof(int) * nbox));
+    int* needb = reinterpret_cast<int*>(take(sizeof(int) * nbox));
     cplx* A = reinterpret_cast<cplx*>(take(sizeof(cplx) * P * nbox));
     cplx* B = reinterpret_cast<cplx*>(take(sizeof(cplx) * P * nbox));
     void* sort_tmp = take(sort_bytes);
@@ -589,15 +686,21 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
     t.near_n = (int)(nearo.size() / 2);
     t.lbeg = lbeg;
     t.cnt = cnt;
+    t.need = needb;
+    DVM_HIP_CHECK(hipMemsetAsync(needb, 0, sizeof(int) * nbox, st));
+    if (nt > 0) {
+        fmm_mark_kernel<<<cdiv(nt, 256), 256, 0, st>>>((int)nt, tx, tz, x0, z0, sl, L, needb);
+        DVM_LAUNCH_CHECK();
+    }
     t.A = A;
     t.B = B;
     switch (P) {
-    case 8: fmm_levels<8>(ctx, t, (int)n, sp, skey, perm, u, w); break;
-    case 12: fmm_levels<12>(ctx, t, (int)n, sp, skey, perm, u, w); break;
-    case 16: fmm_levels<16>(ctx, t, (int)n, sp, skey, perm, u, w); break;
-    case 20: fmm_levels<20>(ctx, t, (int)n, sp, skey, perm, u, w); break;
-    case 24: fmm_levels<24>(ctx, t, (int)n, sp, skey, perm, u, w); break;
-    default: fmm_levels<32>(ctx, t, (int)n, sp, skey, perm, u, w); break;
+    case 8: fmm_levels<8>(ctx, t, (int)n, sp, skey, perm, u, w, (int)nt, tx, tz, tu, tw); break;
+    case 12: fmm_levels<12>(ctx, t, (int)n, sp, skey, perm, u, w, (int)nt, tx, tz, tu, tw); break;
+    case 16: fmm_levels<16>(ctx, t, (int)n, sp, skey, perm, u, w, (int)nt, tx, tz, tu, tw); break;
+    case 20: fmm_levels<20>(ctx, t, (int)n, sp, skey, perm, u, w, (int)nt, tx, tz, tu, tw); break;
+    case 24: fmm_levels<24>(ctx, t, (int)n, sp, skey, perm, u, w, (int)nt, tx, tz, tu, tw); break;
+    default: fmm_levels<32>(ctx, t, (int)n, sp, skey, perm, u, w, (int)nt, tx, tz, tu, tw); break;
     }
     // the host tables must outlive the copies that read them
     DVM_HIP_CHECK(hipStreamSynchronize(st));

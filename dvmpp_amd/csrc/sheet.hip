@@ -417,12 +417,13 @@ __global__ __launch_bounds__(256) void gamma_kernel(int nl, const double* __rest
 }
 
 void sheet_solve(dvm_ctx_t ctx, SheetDev& sh, int prec, int64_t n, const double* bx,
-                 const double* bz, const double* bc, const double* bs)
+                 const double* bz, const double* bc, const double* bs, bool have_uw)
 {
     const int nl = sh.nl;
     if (n > 0) {
-        bs_targets_launch(ctx, prec, n, bx, bz, bc, bs, nl, sh.xc, sh.zc, 0, 0.0, sh.uw,
-                          sh.uw + nl);
+        if (!have_uw)
+            bs_targets_launch(ctx, prec, n, bx, bz, bc, bs, nl, sh.xc, sh.zc, 0, 0.0, sh.uw,
+                              sh.uw + nl);
         device_sum(ctx, n, bc, sh.red, sh.scal);
     }
     const unsigned blocks = cdiv(nl, 4) < 64 ? cdiv(nl, 4) : 64;

@@ -272,8 +272,16 @@ double* pinned_tape(dvm_sim_t s, size_t count)
 void rhs(dvm_sim_t s)
 {
     const int64_t n = s->n;
-    if (n > 0) bs_self_launch(s->ctx, s->p.prec, n, s->b.x, s->b.z, s->b.c, s->b.s, s->b.u, s->b.w);
-    sheet_solve(s->ctx, s->sh, s->p.prec, n, s->b.x, s->b.z, s->b.c, s->b.s);
+    bool have_uw = false;
+    if (n > 0 && s->p.prec == DVM_FMM) {
+        // one tree for the blob velocities and the collocation sums
+        have_uw = fmm_self(s->ctx, n, s->b.x, s->b.z, s->b.c, s->b.s, s->b.u, s->b.w, s->sh.nl,
+                           s->sh.xc, s->sh.zc, s->sh.uw, s->sh.uw + s->sh.nl);
+        if (!have_uw) bs_self_launch(s->ctx, DVM_F64, n, s->b.x, s->b.z, s->b.c, s->b.s, s->b.u, s->b.w);
+    } else if (n > 0) {
+        bs_self_launch(s->ctx, s->p.prec, n, s->b.x, s->b.z, s->b.c, s->b.s, s->b.u, s->b.w);
+    }
+    sheet_solve(s->ctx, s->sh, s->p.prec, n, s->b.x, s->b.z, s->b.c, s->b.s, have_uw);
     sheet_on_blobs(s->ctx, s->sh, s->p.prec, n, s->b.x, s->b.z, s->b.uvs, s->b.wvs);
 }
 

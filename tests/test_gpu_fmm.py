@@ -106,5 +106,7 @@ def test_fmm_simulation_step(gpu_ctx):
         sim.substep(0)  # increment_step + convect
     assert gpu_ctx.fmm_last_level() >= 2  # b's sum was a tree
     sa, sb = a.get_state(), b.get_state()
+    # the collocation sums came from the same tree (FMM targets)
+    assert rel_err(b.gamma(), a.gamma()) <= 1e-9
     assert np.max(np.abs(sa["x"] - sb["x"])) <= 1e-9 and np.max(np.abs(sa["z"] - sb["z"])) <= 1e-9
     assert rel_err(sb["u"], sa["u"]) <= 1e-8

@@ -323,6 +323,44 @@ def rk3_plateau_phase(ctx, steps, prec=None, fmm=None):
             "remesh_ms": tr * 1e3, "remesh_blobs_out": int(out[0].numel())}
 
 
+def rk3_run_phase(ctx, steps, fmm=(12, 0.5)):
+    """The RK3 cylinder run itself: from rest, `steps` RK3 steps (512
+    panels, nu 5e-4, dt 0.0025, M4' remesh every 20 steps at dx = 0.01),
+    blob sums by the fast multipole form, device random stream; wall time of
+    the whole run, with the blob count and loads along the way."""
+    import torch
+
+    import dvmpp_amd as dvm
+    from dvmpp_amd import sim as dsim
+    from dvmpp_amd.workloads import rk3_cylinder
+    px, pz, sp, sig = rk3_cylinder()
+    rm = dsim.remesh_params(frequency=20, kernel=dsim.M4PRIME, xmin=-5.0, xmax=25.0, zmin=-10.0,
+                            zmax=10.0, dx=0.01, dz=0.01, sigma=sig, circ_threshold=1e-10)
+    ctx.set_fmm(*fmm)
+    s = dsim.Sim(ctx, px, pz, dsim.SimParams(sp, dsim.RK3, dsim.REFLECT, 0, dvm.FMM,
+                                             dsim.RNG_DEVICE, rm))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    trace = []
+    done = 0
+    while done < steps:
+        k = min(500, steps - done)
+        s.run(k)
+        done += k
+        trace.append([done, s.size(), round(time.perf_counter() - t0, 3)])
+        log(f"RK3 run: step {done}, N = {s.size()}, {trace[-1][2]:.1f} s")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cd, cl = s.loads()[2:]
+    s.close()
+    return {"workload": f"RK3 cylinder from rest, {steps} steps, 512 panels, nu 5e-4, dt 0.0025, "
+                        f"REFLECT, M4' remesh every 20 steps at dx 0.01, FMM sums (order {fmm[0]}, "
+                        f"theta {fmm[1]}), device random stream",
+            "steps": steps, "seconds": wall, "steps_per_s": steps / wall,
+            "final_blobs": trace[-1][1], "trace_step_blobs_seconds": trace,
+            "final_cd_cl": [cd, cl]}
+
+
 def merge_phase(ctx, steps, thresholds=(1e-8, 1e-7)):
     """merge_blobs on a grown wake: the C1 cylinder (256 panels, FP32C sums,
     device stream) run `steps` steps, then one merge pass per threshold on a
@@ -383,11 +421,12 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (reference kernel "
                          "for scaling HBM byte counters)")
-    ap.add_argument("--phases", default="sum,cylinder,ensemble,served,rk3,merge",
+    ap.add_argument("--phases", default="sum,cylinder,ensemble,served,rk3,rk3run,merge",
                     help="comma list: sum (isolated sum, always run), cylinder, ensemble, "
                          "served, rk3")
     ap.add_argument("--rk3-steps", type=int, default=2)
     ap.add_argument("--rk3-fmm-steps", type=int, default=10)
+    ap.add_argument("--rk3-run-steps", type=int, default=4000)
     ap.add_argument("--cylinder-steps", type=int, default=400)
     ap.add_argument("--cylinder-cpu-steps", type=int, default=40)
     ap.add_argument("--ensemble-count", type=int, default=128)
@@ -514,6 +553,8 @@ def main():
         line["served"] = served_phase(ctx, args.served_count, 20.0, (lo, hi), slots=32)
     if "merge" in phases:
         line["merge"] = merge_phase(ctx, args.cylinder_steps)
+    if "rk3run" in phases:
+        line["rk3_cylinder_run"] = rk3_run_phase(ctx, args.rk3_run_steps)
     if "rk3" in phases:
         line["rk3_cylinder_plateau"] = rk3_plateau_phase(ctx, args.rk3_steps)
         line["rk3_cylinder_plateau_fmm"] = rk3_plateau_phase(ctx, args.rk3_fmm_steps, dvm.FMM,

@@ -144,8 +144,9 @@ def log(msg):
 def plain_cylinder_phase(ctx, steps, cpu_steps):
     """Plain cylinder (256 panels, Euler, dt 0.005, REFLECT, seed 0): steps/s of
     the FP64 parity form (reference random stream), of the chosen form (FP32C
-    sums, device random stream), and of the CPU oracle over the first
-    `cpu_steps` steps (same window timed on the GPU)."""
+    sums, device random stream), of the fast multipole form (order 16,
+    device stream), and of the CPU oracle over the first `cpu_steps` steps
+    (same window timed on the GPU)."""
     import torch
 
     import dvmpp_amd as dvm
@@ -178,6 +179,11 @@ def plain_cylinder_phase(ctx, steps, cpu_steps):
                                    "window_steps_per_s": cpu_steps / w64}
     out["f32c_device_stream"] = {"steps_per_s": steps / a32, "final_blobs": n32, "cd": cd32,
                                  "window_steps_per_s": cpu_steps / w32}
+    ctx.set_fmm(16, 0.5)
+    wf, af, nf, cdf = run(dvm.FMM, dsim.RNG_DEVICE)
+    log(f"plain cylinder FMM: {steps} steps in {af:.2f} s, N = {nf}")
+    out["fmm_order16_device_stream"] = {"steps_per_s": steps / af, "final_blobs": nf, "cd": cdf,
+                                        "window_steps_per_s": cpu_steps / wf}
     import oracle
     sim = oracle.Sim(px, pz, oracle.make_params(rho=sp.rho, nu=sp.nu, dt=sp.dt, ux=sp.ux,
                                                 uz=sp.uz, cutoff_exp=sp.cutoff_exp,

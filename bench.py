@@ -402,6 +402,119 @@ def merge_phase(ctx, steps, thresholds=(1e-8, 1e-7)):
     return out
 
 
+def bandwidth_phase(ctx, reps=20):
+    """The HBM-bound blob operations at the RK3 plateau size: the remeshed
+    wake of ~1.5 M blobs (workloads.wake_snapshot) with its last 1 % of blobs
+    moved inside the 512-panel body (as blobs released last and diffused
+    through the surface would be) so that the crossing's compaction runs.
+    Single-kernel entry points are timed over `reps` back-to-back calls
+    between HIP events on the context's stream; the multi-kernel ones
+    (delete_inside: flags, scan, compaction; remesh: sort, gather, scan, emit,
+    with a host read of the output size) per call with a synchronisation.
+    GB/s = algorithmic bytes (FP64 SoA fields read + written, per blob) /
+    time, against the 8 TB/s HBM peak."""
+    import ctypes
+
+    import torch
+
+    import dvmpp_amd as dvm
+    from dvmpp_amd import sim as dsim
+    from dvmpp_amd.workloads import rk3_cylinder, wake_snapshot
+    lib = dvm._lib.load()
+    px, pz, sp, sig = rk3_cylinder()
+    snap = wake_snapshot(sigma=sig)
+    n = snap["x"].size
+    rng = np.random.default_rng(11)
+    k_in = n // 100
+    idx = np.arange(n - k_in, n)
+    rr = 0.45 * np.sqrt(rng.uniform(0.0, 1.0, k_in))
+    th = rng.uniform(-np.pi, np.pi, k_in)
+    x0, z0 = snap["x"].copy(), snap["z"].copy()
+    x0[idx], z0[idx] = rr * np.cos(th), rr * np.sin(th)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    X0, Z0 = dev(x0), dev(z0)
+    C0, S0 = dev(snap["circ"]), dev(snap["sigma"])
+    vel = [dev(rng.normal(0.0, 0.1, n)) for _ in range(4)]  # u w uvs wvs
+    q1, q2 = dev(np.zeros(n)), dev(np.zeros(n))
+    tape = dev(rng.normal(0.0, 1e-3, 2 * n))
+    ids = torch.arange(n, dtype=torch.int32, device="cuda")
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    sheet = dsim.Sheet(ctx, px, pz, sp)
+    flags = torch.empty(n, dtype=torch.int32, device="cuda")
+    X, Z = X0.clone(), Z0.clone()
+    u, w, uvs, wvs = vel
+
+    def events(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    chk = lambda st: dvm._lib.check(ctx.h, st)
+    out = {"workload": f"remeshed RK3-cylinder wake, {n} blobs ({k_in} inside the 512-panel "
+                       f"body), FP64 SoA resident in HBM", "blobs": n, "peak_GBps": 8000.0}
+
+    def rec(name, ms, bytes_per_blob, extra_bytes=0):
+        gbs = (bytes_per_blob * n + extra_bytes) / (ms * 1e-3) / 1e9
+        out[name] = {"ms": ms, "bytes_per_blob": bytes_per_blob, "GBps": gbs,
+                     "frac_of_hbm_peak": gbs / 8000.0}
+        log(f"bandwidth: {name} {ms * 1e3:.1f} us, {gbs:.0f} GB/s")
+
+    rec("advect_euler", events(lambda: chk(lib.dvm_advect_euler(
+        ctx.h, n, 1.0, 0.0, 0.0025, P(u), P(w), P(uvs), P(wvs), P(X), P(Z)))), 64)
+    rec("advect_rk3_stage", events(lambda: chk(lib.dvm_advect_rk3_stage(
+        ctx.h, n, -17.0 / 32.0, 8.0 / 9.0, 1.0, 0.0, 0.0025, P(u), P(w), P(uvs), P(wvs), P(q1),
+        P(q2), P(X), P(Z)))), 96)
+    rec("diffuse_tape", events(lambda: chk(lib.dvm_diffuse(ctx.h, n, P(tape), P(X), P(Z)))), 48)
+    rec("crossing_flags", events(lambda: chk(lib.dvm_sheet_inside(sheet.h, n, P(X0), P(Z0),
+                                                                  P(flags)))), 20)
+    inside = int(flags.sum().item())
+
+    # delete_inside: in place, so each call starts from a fresh copy (not timed)
+    fields0 = [X0, Z0, C0, S0] + vel
+    ms = []
+    for _ in range(5):
+        f = [t.clone() for t in fields0]
+        idc = ids.clone()
+        m = ctypes.c_int64()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        chk(lib.dvm_sheet_delete_inside(sheet.h, n, *[P(t) for t in f], P(idc), ctypes.byref(m)))
+        torch.cuda.synchronize()
+        ms.append((time.perf_counter() - t0) * 1e3)
+    # flags (x, z in, flag out) + stable compaction of 8 FP64 fields and the ID
+    rec("delete_inside", float(np.median(ms)), 20 + 136)
+    out["delete_inside"]["deleted"] = n - m.value
+    out["crossing_flags"]["inside"] = inside
+
+    rm = dsim.remesh_params(frequency=20, kernel=dsim.M4PRIME, xmin=-5.0, xmax=25.0, zmin=-10.0,
+                            zmax=10.0, dx=0.01, dz=0.01, sigma=sig, circ_threshold=1e-10)
+    # one dvm_remesh call into outputs sized beforehand (the call counts the
+    # output, reads the count to the host, then emits)
+    n_new = dsim.remesh(ctx, rm, X0, Z0, C0, S0)[0].numel()
+    o = [torch.empty(n_new, dtype=torch.float64, device="cuda") for _ in range(4)]
+    m = ctypes.c_int64()
+    ms = []
+    for _ in range(5):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        chk(lib.dvm_remesh(ctx.h, ctypes.byref(rm), n, P(X0), P(Z0), P(C0), P(S0), n_new,
+                           *[P(t) for t in o], ctypes.byref(m)))
+        torch.cuda.synchronize()
+        ms.append((time.perf_counter() - t0) * 1e3)
+    nodes = 3001 * 2001
+    rec("remesh_m4prime", float(np.median(ms)), 24, 16 * nodes + 32 * n_new)
+    out["remesh_m4prime"].update({"mesh_nodes": nodes, "blobs_out": n_new,
+                                  "bytes": "24 N + 16 nodes + 32 N_out"})
+    sheet.close()
+    return out
+
+
 def load_traffic(kernel):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -427,9 +540,9 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (reference kernel "
                          "for scaling HBM byte counters)")
-    ap.add_argument("--phases", default="sum,cylinder,ensemble,served,rk3,rk3run,merge",
+    ap.add_argument("--phases", default="sum,cylinder,ensemble,served,rk3,rk3run,merge,bandwidth",
                     help="comma list: sum (isolated sum, always run), cylinder, ensemble, "
-                         "served, rk3")
+                         "served, rk3, rk3run, merge, bandwidth")
     ap.add_argument("--rk3-steps", type=int, default=2)
     ap.add_argument("--rk3-fmm-steps", type=int, default=10)
     ap.add_argument("--rk3-run-steps", type=int, default=4000)
@@ -557,6 +670,8 @@ def main():
     if "served" in phases:
         lo, hi = (int(v) for v in args.served_steps.split(","))
         line["served"] = served_phase(ctx, args.served_count, 20.0, (lo, hi), slots=32)
+    if "bandwidth" in phases:
+        line["bandwidth_kernels"] = bandwidth_phase(ctx)
     if "merge" in phases:
         line["merge"] = merge_phase(ctx, args.cylinder_steps)
     if "rk3run" in phases:

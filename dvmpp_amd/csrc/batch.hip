@@ -36,6 +36,7 @@ struct SlotParam {
     int nl, max_npv, steps_target, pad0;
     uint32_t seed, pad1;
     double pad2;
+    BodyBox box;  // crossing-test cull box of the body
 };
 
 struct SlotState {
@@ -765,7 +766,7 @@ __global__ void bt_inside_kernel(int S, int64_t cap, int mp, const WorkList* wl,
         const double* px = bx + (int64_t)s * (mp + 1);
         const double* pz = bz + (int64_t)s * (mp + 1);
         const double x0 = b.x[o], z0 = b.z[o];
-        const bool in = inside_body(nl, px, pz, x0, z0);
+        const bool in = !outside_box(prm[s].box, x0, z0) && inside_body(nl, px, pz, x0, z0);
         flags[o] = in ? 1 : 0;
         if (!in || !reflect) return;
         // src/VortexSheet.cpp:345-405
@@ -1093,7 +1094,6 @@ void remesh_due(dvm_batch_t B)
         RemeshPlan plan;
         const int64_t m = remesh_count(B->ctx, rp, B->hst[s].n, in, plan);
         if (m > B->d.cap) {
-            DVM_HIP_CHECK(hipFreeAsync(plan.mem, st));
             SlotState bad = B->hst[s];
             bad.active = 0;
             bad.status = DVM_E_SIZE;
@@ -1235,6 +1235,7 @@ dvm_status dvm_batch_admit(dvm_batch_t B, int npts, const double* px, const doub
         sp.four_nu_dt = 4.0 * p->nu * p->dt;
         sp.nl = nl; sp.max_npv = p->max_npv; sp.steps_target = steps;
         sp.seed = (uint32_t)seed;
+        sp.box = sh.box;
         SlotState ss{};
         ss.active = 1;
         DVM_HIP_CHECK(hipMemcpyAsync(d.prm + s, &sp, sizeof(sp), hipMemcpyHostToDevice, st));

@@ -63,6 +63,16 @@ __device__ __forceinline__ bool inside_body(int nl, const double* __restrict__ b
     return cn == 1;
 }
 
+// Points where inside_body is false without visiting the edges: z < zmin or
+// z >= zmax (no edge straddles z under the half-open rule), x beyond the
+// largest x by more than the rounding of any crossing abscissa, or, for a
+// closed contour, below the smallest x by that margin (every straddling edge
+// counts and a closed contour has an even number of them, so cn != 1).
+__device__ __forceinline__ bool outside_box(const BodyBox& bb, double xc, double zc)
+{
+    return zc < bb.zlo || zc >= bb.zhi || xc > bb.xhi || xc < bb.xlo;
+}
+
 // One (blob, panel) term of src/VortexSheet.cpp:225-272: coefficients
 // px, qx, py, qy with the collocation midpoint as panel origin, against
 // gamma_j and gamma_{j+1}.

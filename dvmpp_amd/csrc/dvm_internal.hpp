@@ -146,11 +146,20 @@ struct PanelConst {
 };
 
 // Device-resident vortex sheet (one body).
+// Bounding box of a body contour for the crossing test's cull (outside_box):
+// zlo = min z, zhi = max z, xhi = max x + margin, xlo = min x - margin when
+// the contour is closed (last point == first), else -inf.
+struct BodyBox {
+    double zlo, zhi, xlo, xhi;
+};
+BodyBox body_box(int npts, const double* px, const double* pz);
+
 struct SheetDev {
     int nl = 0;
     double rho = 1, nu = 0, dt = 0, ux = 0, uz = 0, cutoff_exp = 0;
     int max_npv = 1;
     double aref = 0;
+    BodyBox box{};                                 // crossing-test cull box
     std::vector<double> hx, hz;                    // body points (host copy)
     std::vector<double> hxc, hzc, htheta, hds, henx, henz, hetx, hetz;
     char* mem = nullptr;                           // one device allocation
@@ -224,7 +233,7 @@ struct BlobBuf {
 // synchronizes); phase 2 writes the output into `out` (capacity >= count).
 struct RemeshPlan {
     int64_t n_in = 0, nx = 0, nz = 0, n_nodes_out = 0, n_kept = 0;
-    char* mem = nullptr;  // grid values, flags, scans (freed by remesh_finish)
+    char* mem = nullptr;  // grid values, flags, scans: the context's work arena
 };
 int64_t remesh_count(dvm_ctx_t ctx, const dvm_remesh_params& rp, int64_t n, const BlobBuf& in,
                      RemeshPlan& plan);

@@ -21,6 +21,7 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <climits>
 #include <cmath>
 
 namespace dvmi {
@@ -66,25 +67,106 @@ struct MeshGeo {
     int s, kernel;
 };
 
-__global__ void remesh_cell_kernel(int64_t n, MeshGeo g, const double* __restrict__ x,
-                                   const double* __restrict__ z, uint32_t* __restrict__ key,
-                                   uint32_t* __restrict__ idx, int* __restrict__ inside)
+// Per blob: its cell key (kKept for a blob whose stencil leaves the mesh)
+// and whether it is remeshed; per block: the smallest and largest cell
+// column and row of its remeshed blobs in part[4 * block + 0..3] (min cx,
+// max cx, min cz, max cz), folded by remesh_rect_kernel.
+__global__ __launch_bounds__(256) void remesh_cell_kernel(int64_t n, MeshGeo g,
+                                                          const double* __restrict__ x,
+                                                          const double* __restrict__ z,
+                                                          uint32_t* __restrict__ key,
+                                                          uint32_t* __restrict__ idx,
+                                                          int* __restrict__ inside,
+                                                          int* __restrict__ part)
 {
+    __shared__ int red[4][4];
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= n) return;
-    const double fx = (x[b] - g.xmin) / g.dx;
-    const double fz = (z[b] - g.zmin) / g.dz;
-    bool ok = fx >= -1e15 && fx <= 1e15 && fz >= -1e15 && fz <= 1e15;  // false for NaN
-    uint32_t k = kKept;
-    if (ok) {
-        const int64_t cx = (int64_t)floor(fx), cz = (int64_t)floor(fz);
-        ok = (cx - g.s + 1 >= 0) && (cx + g.s <= g.nx - 1) && (cz - g.s + 1 >= 0) &&
-             (cz + g.s <= g.nz - 1);
-        if (ok) k = (uint32_t)(cz * g.nx + cx);
+    int r0 = INT_MAX, r1 = -1, r2 = INT_MAX, r3 = -1;
+    if (b < n) {
+        const double fx = (x[b] - g.xmin) / g.dx;
+        const double fz = (z[b] - g.zmin) / g.dz;
+        bool ok = fx >= -1e15 && fx <= 1e15 && fz >= -1e15 && fz <= 1e15;  // false for NaN
+        uint32_t k = kKept;
+        if (ok) {
+            const int64_t cx = (int64_t)floor(fx), cz = (int64_t)floor(fz);
+            ok = (cx - g.s + 1 >= 0) && (cx + g.s <= g.nx - 1) && (cz - g.s + 1 >= 0) &&
+                 (cz + g.s <= g.nz - 1);
+            if (ok) {
+                k = (uint32_t)(cz * g.nx + cx);
+                r0 = r1 = (int)cx;
+                r2 = r3 = (int)cz;
+            }
+        }
+        key[b] = k;
+        idx[b] = (uint32_t)b;
+        inside[b] = ok ? 1 : 0;
     }
-    key[b] = k;
-    idx[b] = (uint32_t)b;
-    inside[b] = ok ? 1 : 0;
+    for (int o = 32; o > 0; o >>= 1) {
+        r0 = min(r0, __shfl_xor(r0, o));
+        r1 = max(r1, __shfl_xor(r1, o));
+        r2 = min(r2, __shfl_xor(r2, o));
+        r3 = max(r3, __shfl_xor(r3, o));
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[wave][0] = r0; red[wave][1] = r1; red[wave][2] = r2; red[wave][3] = r3;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int v = 1; v < (int)(blockDim.x >> 6); ++v) {
+            r0 = min(r0, red[v][0]); r1 = max(r1, red[v][1]);
+            r2 = min(r2, red[v][2]); r3 = max(r3, red[v][3]);
+        }
+        int* o = part + 4 * (int64_t)blockIdx.x;
+        o[0] = r0; o[1] = r1; o[2] = r2; o[3] = r3;
+    }
+}
+
+// rect[0..3] = the fold of the nb per-block rectangles (one block); with no
+// remeshed blob, {INT_MAX, -1, INT_MAX, -1}, which no node lies in.
+__global__ __launch_bounds__(256) void remesh_rect_kernel(int nb, const int* __restrict__ part,
+                                                          int* __restrict__ rect)
+{
+    __shared__ int red[4][4];
+    int r0 = INT_MAX, r1 = -1, r2 = INT_MAX, r3 = -1;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        r0 = min(r0, part[4 * b]); r1 = max(r1, part[4 * b + 1]);
+        r2 = min(r2, part[4 * b + 2]); r3 = max(r3, part[4 * b + 3]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        r0 = min(r0, __shfl_xor(r0, o));
+        r1 = max(r1, __shfl_xor(r1, o));
+        r2 = min(r2, __shfl_xor(r2, o));
+        r3 = max(r3, __shfl_xor(r3, o));
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[wave][0] = r0; red[wave][1] = r1; red[wave][2] = r2; red[wave][3] = r3;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int v = 1; v < (int)(blockDim.x >> 6); ++v) {
+            r0 = min(r0, red[v][0]); r1 = max(r1, red[v][1]);
+            r2 = min(r2, red[v][2]); r3 = max(r3, red[v][3]);
+        }
+        rect[0] = r0; rect[1] = r1; rect[2] = r2; rect[3] = r3;
+    }
+}
+
+// Sorted copies of the remeshed blobs' fractional cell coordinates and
+// circulation (position q of the (cell, index) order), so the node gather
+// reads them contiguously instead of through the permutation.
+__global__ void remesh_sorted_kernel(int64_t n, MeshGeo g, const uint32_t* __restrict__ sidx,
+                                     const double* __restrict__ x, const double* __restrict__ z,
+                                     const double* __restrict__ c, double* __restrict__ sfx,
+                                     double* __restrict__ sfz, double* __restrict__ sc)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t b = sidx[q];
+    sfx[q] = (x[b] - g.xmin) / g.dx;
+    sfz[q] = (z[b] - g.zmin) / g.dz;
+    sc[q] = c[b];
 }
 
 __global__ void remesh_range_kernel(int64_t n, const uint32_t* __restrict__ key,
@@ -98,10 +180,15 @@ __global__ void remesh_range_kernel(int64_t n, const uint32_t* __restrict__ key,
     if (p == n - 1 || key[p + 1] != k) end[k] = (int)p + 1;
 }
 
-__global__ void remesh_gather_kernel(MeshGeo g, const int* __restrict__ start,
-                                     const int* __restrict__ end, const uint32_t* __restrict__ sidx,
-                                     const double* __restrict__ x, const double* __restrict__ z,
-                                     const double* __restrict__ c, double threshold,
+// One thread per node: the sum over its stencil cells (z-major, ascending)
+// of each cell's blobs in index order.  Nodes outside the rectangle the
+// remeshed blobs' stencils reach (rect, widened by the stencil) have no
+// blobs in reach and take the value 0 without visiting their cells.
+__global__ void remesh_gather_kernel(MeshGeo g, const int* __restrict__ rect,
+                                     const int* __restrict__ start, const int* __restrict__ end,
+                                     const double* __restrict__ sfx,
+                                     const double* __restrict__ sfz,
+                                     const double* __restrict__ sc, double threshold,
                                      double* __restrict__ G, int* __restrict__ flag)
 {
     const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -110,19 +197,21 @@ __global__ void remesh_gather_kernel(MeshGeo g, const int* __restrict__ start,
     const int64_t i = node - k * g.nx;
     const int s = g.s;
     double acc = 0.0;
-    for (int64_t cz = k - s; cz <= k + s - 1; ++cz) {
-        if (cz < 0 || cz >= g.nz) continue;
-        for (int64_t cx = i - s; cx <= i + s - 1; ++cx) {
-            if (cx < 0 || cx >= g.nx) continue;
-            const int64_t cell = cz * g.nx + cx;
-            const int q1 = end[cell];
-            for (int q = start[cell]; q < q1; ++q) {
-                const uint32_t b = sidx[q];
-                const double fx = (x[b] - g.xmin) / g.dx;
-                const double fz = (z[b] - g.zmin) / g.dz;
-                const double wx = remesh_slot_w(g.kernel, s, fx, cx - s + 1, (int)(i - (cx - s + 1)));
-                const double wz = remesh_slot_w(g.kernel, s, fz, cz - s + 1, (int)(k - (cz - s + 1)));
-                acc += (c[b] * wx) * wz;
+    if (i >= (int64_t)rect[0] - s + 1 && i <= (int64_t)rect[1] + s &&
+        k >= (int64_t)rect[2] - s + 1 && k <= (int64_t)rect[3] + s) {
+        for (int64_t cz = k - s; cz <= k + s - 1; ++cz) {
+            if (cz < 0 || cz >= g.nz) continue;
+            for (int64_t cx = i - s; cx <= i + s - 1; ++cx) {
+                if (cx < 0 || cx >= g.nx) continue;
+                const int64_t cell = cz * g.nx + cx;
+                const int q1 = end[cell];
+                for (int q = start[cell]; q < q1; ++q) {
+                    const double wx = remesh_slot_w(g.kernel, s, sfx[q], cx - s + 1,
+                                                    (int)(i - (cx - s + 1)));
+                    const double wz = remesh_slot_w(g.kernel, s, sfz[q], cz - s + 1,
+                                                    (int)(k - (cz - s + 1)));
+                    acc += (sc[q] * wx) * wz;
+                }
             }
         }
     }
@@ -143,8 +232,8 @@ __global__ void remesh_emit_nodes_kernel(MeshGeo g, double sigma, const double* 
     out.z[o] = g.zmin + (double)k * g.dz;
     out.c[o] = G[node];
     out.s[o] = sigma;
-    out.u[o] = 0.0; out.w[o] = 0.0; out.uvs[o] = 0.0; out.wvs[o] = 0.0;
-    out.id[o] = (uint32_t)o;
+    if (out.u) { out.u[o] = 0.0; out.w[o] = 0.0; out.uvs[o] = 0.0; out.wvs[o] = 0.0; }
+    if (out.id) out.id[o] = (uint32_t)o;
 }
 
 __global__ void remesh_emit_kept_kernel(int64_t n_kept, int64_t first, int64_t base,
@@ -155,8 +244,18 @@ __global__ void remesh_emit_kept_kernel(int64_t n_kept, int64_t first, int64_t b
     const uint32_t b = sidx[first + q];
     const int64_t o = base + q;
     out.x[o] = in.x[b]; out.z[o] = in.z[b]; out.c[o] = in.c[b]; out.s[o] = in.s[b];
-    out.u[o] = in.u[b]; out.w[o] = in.w[b]; out.uvs[o] = in.uvs[b]; out.wvs[o] = in.wvs[b];
-    out.id[o] = (uint32_t)o;
+    if (out.u) {
+        out.u[o] = in.u[b]; out.w[o] = in.w[b]; out.uvs[o] = in.uvs[b]; out.wvs[o] = in.wvs[b];
+    }
+    if (out.id) out.id[o] = (uint32_t)o;
+}
+
+__global__ void remesh_tail_kernel(int64_t cells, const int* __restrict__ pos,
+                                   const int* __restrict__ flag, int* __restrict__ nin)
+{
+    if (threadIdx.x != 0) return;
+    nin[1] = pos[cells - 1];
+    nin[2] = flag[cells - 1];
 }
 
 static MeshGeo mesh_geo(const dvm_remesh_params& rp)
@@ -173,8 +272,8 @@ static MeshGeo mesh_geo(const dvm_remesh_params& rp)
 // Plan memory layout (offsets recomputed from the sizes on both phases).
 struct PlanPtrs {
     uint32_t *key, *key2, *idx, *sidx;
-    int *inside, *start, *end, *flag, *pos, *nin;
-    double* G;
+    int *inside, *start, *end, *flag, *pos, *nin, *rect, *rpart;
+    double *G, *sfx, *sfz, *sc;
     void* tmp;
     size_t tmp_bytes;
 };
@@ -193,7 +292,12 @@ static PlanPtrs plan_ptrs(char* mem, int64_t n, int64_t cells, size_t tmp_bytes)
     p.flag = cv.take<int>(cells);
     p.pos = cv.take<int>(cells);
     p.nin = cv.take<int>(4);
+    p.rect = cv.take<int>(4);
+    p.rpart = cv.take<int>(4 * cdiv(n > 0 ? n : 1, 256));
     p.G = cv.take<double>(cells);
+    p.sfx = cv.take<double>(n > 0 ? n : 1);
+    p.sfz = cv.take<double>(n > 0 ? n : 1);
+    p.sc = cv.take<double>(n > 0 ? n : 1);
     p.tmp = cv.take<char>(tmp_bytes);
     p.tmp_bytes = tmp_bytes;
     return p;
@@ -223,9 +327,13 @@ int64_t remesh_count(dvm_ctx_t ctx, const dvm_remesh_params& rp, int64_t n, cons
     const size_t tmp = plan_tmp_bytes(n, cells, st);
     const int64_t nn = n > 0 ? n : 1;
     const size_t bytes = Carver::need({4 * sizeof(uint32_t) * nn, sizeof(int) * nn,
-                                       4 * sizeof(int) * cells, sizeof(int) * 4,
-                                       sizeof(double) * cells, tmp}) + 16 * 256;
-    DVM_HIP_CHECK(hipMallocAsync((void**)&plan.mem, bytes, st));
+                                       4 * sizeof(int) * cells, sizeof(int) * 4, sizeof(int) * 4,
+                                       4 * sizeof(int) * (size_t)cdiv(nn, 256),
+                                       sizeof(double) * cells, 3 * sizeof(double) * nn, tmp}) +
+                         20 * 256;
+    // the context's work arena: nothing else takes it between this count and
+    // remesh_finish, and it persists, so repeated remeshes allocate nothing
+    plan.mem = static_cast<char*>(ws_get(ctx, bytes));
     PlanPtrs p = plan_ptrs(plan.mem, n, cells, tmp);
     plan.n_in = n;
     plan.nx = g.nx;
@@ -234,29 +342,39 @@ int64_t remesh_count(dvm_ctx_t ctx, const dvm_remesh_params& rp, int64_t n, cons
     DVM_HIP_CHECK(hipMemsetAsync(p.start, 0, sizeof(int) * cells, st));
     DVM_HIP_CHECK(hipMemsetAsync(p.end, 0, sizeof(int) * cells, st));
     DVM_HIP_CHECK(hipMemsetAsync(p.nin, 0, sizeof(int) * 4, st));
+    const int nb = (int)cdiv(n > 0 ? n : 1, 256);
     if (n > 0) {
-        remesh_cell_kernel<<<cdiv(n, 256), 256, 0, st>>>(n, g, in.x, in.z, p.key, p.idx, p.inside);
+        remesh_cell_kernel<<<nb, 256, 0, st>>>(n, g, in.x, in.z, p.key, p.idx, p.inside, p.rpart);
         DVM_LAUNCH_CHECK();
+    }
+    remesh_rect_kernel<<<1, 256, 0, st>>>(n > 0 ? nb : 0, p.rpart, p.rect);
+    DVM_LAUNCH_CHECK();
+    if (n > 0) {
         size_t t = p.tmp_bytes;
         DVM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(p.tmp, t, p.key, p.key2, p.idx, p.sidx,
                                                          (int)n, 0, 32, st));
         remesh_range_kernel<<<cdiv(n, 256), 256, 0, st>>>(n, p.key2, p.start, p.end);
         DVM_LAUNCH_CHECK();
+        remesh_sorted_kernel<<<cdiv(n, 256), 256, 0, st>>>(n, g, p.sidx, in.x, in.z, in.c, p.sfx,
+                                                           p.sfz, p.sc);
+        DVM_LAUNCH_CHECK();
         t = p.tmp_bytes;
         DVM_HIP_CHECK(hipcub::DeviceReduce::Sum(p.tmp, t, p.inside, p.nin, (int)n, st));
     }
-    remesh_gather_kernel<<<cdiv(cells, 256), 256, 0, st>>>(g, p.start, p.end, p.sidx, in.x, in.z,
-                                                           in.c, rp.circ_threshold, p.G, p.flag);
+    remesh_gather_kernel<<<cdiv(cells, 256), 256, 0, st>>>(g, p.rect, p.start, p.end, p.sfx,
+                                                           p.sfz, p.sc, rp.circ_threshold, p.G,
+                                                           p.flag);
     DVM_LAUNCH_CHECK();
     size_t t = p.tmp_bytes;
     DVM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(p.tmp, t, p.flag, p.pos, (int)cells, st));
-    int hv[3] = {0, 0, 0};
-    DVM_HIP_CHECK(hipMemcpyAsync(&hv[0], p.pos + cells - 1, sizeof(int), hipMemcpyDeviceToHost, st));
-    DVM_HIP_CHECK(hipMemcpyAsync(&hv[1], p.flag + cells - 1, sizeof(int), hipMemcpyDeviceToHost, st));
-    DVM_HIP_CHECK(hipMemcpyAsync(&hv[2], p.nin, sizeof(int), hipMemcpyDeviceToHost, st));
+    // one read-back: {remeshed blobs, last node's output offset, its flag}
+    remesh_tail_kernel<<<1, 64, 0, st>>>(cells, p.pos, p.flag, p.nin);
+    DVM_LAUNCH_CHECK();
+    int* hv = static_cast<int*>(pinned_get(ctx, sizeof(int) * 4));
+    DVM_HIP_CHECK(hipMemcpyAsync(hv, p.nin, sizeof(int) * 3, hipMemcpyDeviceToHost, st));
     DVM_HIP_CHECK(hipStreamSynchronize(st));
-    plan.n_nodes_out = (int64_t)hv[0] + hv[1];
-    plan.n_kept = n - hv[2];
+    plan.n_nodes_out = (int64_t)hv[1] + hv[2];
+    plan.n_kept = n - hv[0];
     return plan.n_nodes_out + plan.n_kept;
 }
 
@@ -276,7 +394,6 @@ void remesh_finish(dvm_ctx_t ctx, const dvm_remesh_params& rp, const BlobBuf& in
             plan.n_kept, n - plan.n_kept, plan.n_nodes_out, p.sidx, in, out);
         DVM_LAUNCH_CHECK();
     }
-    DVM_HIP_CHECK(hipFreeAsync(plan.mem, st));
     plan.mem = nullptr;
 }
 
@@ -297,33 +414,19 @@ dvm_status dvm_remesh(dvm_ctx_t ctx, const dvm_remesh_params* rp, int64_t n, con
         require(rp->dx > 0 && rp->dz > 0 && rp->xmax > rp->xmin && rp->zmax > rp->zmin,
                 DVM_E_INVALID, "invalid mesh");
         require(rp->kernel >= 0 && rp->kernel <= 2, DVM_E_INVALID, "unknown remesh kernel");
-        // the kept-blob copy reads velocity fields too: alias them to x (unused in output)
-        BlobBuf in;
+        // positions, circulation and core size only: no velocity fields or IDs
+        BlobBuf in{};
         in.x = const_cast<double*>(x); in.z = const_cast<double*>(z);
         in.c = const_cast<double*>(circ); in.s = const_cast<double*>(sigma);
-        in.u = in.w = in.uvs = in.wvs = const_cast<double*>(x);
         RemeshPlan plan;
-        // kept blobs need IDs only in the output; a scratch id buffer is used
         const int64_t m = remesh_count(ctx, *rp, n, in, plan);
         *n_out = m;
-        if (m > out_cap || (m > 0 && !(ox && oz && ocirc && osigma))) {
-            DVM_HIP_CHECK(hipFreeAsync(plan.mem, ctx->stream));
-            plan.mem = nullptr;
+        if (m > out_cap || (m > 0 && !(ox && oz && ocirc && osigma)))
             fail(DVM_E_SIZE, "remesh output larger than out_cap (n_out holds the size needed)");
-        }
-        if (m == 0) {
-            DVM_HIP_CHECK(hipFreeAsync(plan.mem, ctx->stream));
-            return;
-        }
-        double* junk = nullptr;
-        DVM_HIP_CHECK(hipMallocAsync((void**)&junk, (sizeof(double) * 4 + sizeof(uint32_t)) * m,
-                                     ctx->stream));
-        BlobBuf out;
+        if (m == 0) return;
+        BlobBuf out{};
         out.x = ox; out.z = oz; out.c = ocirc; out.s = osigma;
-        out.u = junk; out.w = junk + m; out.uvs = junk + 2 * m; out.wvs = junk + 3 * m;
-        out.id = reinterpret_cast<uint32_t*>(junk + 4 * m);
         remesh_finish(ctx, *rp, in, plan, out);
-        DVM_HIP_CHECK(hipFreeAsync(junk, ctx->stream));
     });
 }
 

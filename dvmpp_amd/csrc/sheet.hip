@@ -186,6 +186,28 @@ static void gauss_legendre(int n, double* x, double* w)
     }
 }
 
+BodyBox body_box(int npts, const double* px, const double* pz)
+{
+    for (int i = 0; i < npts; ++i)
+        if (!std::isfinite(px[i]) || !std::isfinite(pz[i]))
+            return BodyBox{-INFINITY, INFINITY, -INFINITY, INFINITY};  // no cull
+    double x0 = px[0], x1 = px[0], z0 = pz[0], z1 = pz[0];
+    for (int i = 1; i < npts; ++i) {
+        x0 = px[i] < x0 ? px[i] : x0; x1 = px[i] > x1 ? px[i] : x1;
+        z0 = pz[i] < z0 ? pz[i] : z0; z1 = pz[i] > z1 ? pz[i] : z1;
+    }
+    // a crossing abscissa x_i + vt (x_{i+1} - x_i) rounds to within a few
+    // units of |x|; the margin is many orders of magnitude wider
+    const double m = 1e-9 * (std::fabs(x0) + std::fabs(x1) + (x1 - x0)) + 1e-300;
+    const bool closed = px[npts - 1] == px[0] && pz[npts - 1] == pz[0];
+    BodyBox b;
+    b.zlo = z0;
+    b.zhi = z1;
+    b.xhi = x1 + m;
+    b.xlo = closed ? x0 - m : -INFINITY;
+    return b;
+}
+
 void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const double* pz,
                 const dvm_sheet_params* p)
 {
@@ -199,6 +221,7 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
     sh.cutoff_exp = p->cutoff_exp; sh.max_npv = p->max_npv;
     sh.hx.assign(px, px + npts);
     sh.hz.assign(pz, pz + npts);
+    sh.box = body_box(npts, px, pz);
     // read_input_coord / form_vortex_sheet (src/VortexSheet.cpp:56-102): O(Nl)
     // set-up on the host with the C library's libm.
     double mn = px[0], mx = px[0];
@@ -779,14 +802,15 @@ void sheet_release_emit(dvm_ctx_t ctx, const SheetDev& sh, int64_t nrv, const do
 // ----------------------------------------------------------------- crossing
 
 __global__ __launch_bounds__(256) void inside_kernel(int64_t n, int nl, const double* __restrict__ bxp,
-                                                     const double* __restrict__ bzp,
+                                                     const double* __restrict__ bzp, BodyBox bb,
                                                      const double* __restrict__ x,
                                                      const double* __restrict__ z,
                                                      int* __restrict__ flags)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    flags[i] = inside_body(nl, bxp, bzp, x[i], z[i]) ? 1 : 0;
+    const double xi = x[i], zi = z[i];
+    flags[i] = (!outside_box(bb, xi, zi) && inside_body(nl, bxp, bzp, xi, zi)) ? 1 : 0;
 }
 
 void sheet_inside_flags(dvm_ctx_t ctx, const SheetDev& sh, int64_t n, const double* x,
@@ -794,7 +818,8 @@ void sheet_inside_flags(dvm_ctx_t ctx, const SheetDev& sh, int64_t n, const doub
 {
     if (n <= 0) return;
     ProfScope ps(ctx, "crossing_flags");
-    inside_kernel<<<cdiv(n, 256), 256, 0, ctx->stream>>>(n, sh.nl, sh.x, sh.z, x, z, flags);
+    inside_kernel<<<cdiv(n, 256), 256, 0, ctx->stream>>>(n, sh.nl, sh.x, sh.z, sh.box, x, z,
+                                                         flags);
     DVM_LAUNCH_CHECK();
 }
 

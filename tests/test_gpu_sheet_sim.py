@@ -118,6 +118,37 @@ def test_probe_velocities_quirk_form(gpu_ctx, n):
                           .numpy())
 
 
+@pytest.mark.parametrize("closed", [True, False])
+def test_inside_bbox_edges_bit_exact(gpu_ctx, closed):
+    # The crossing kernel skips the edge loop for points outside the body's
+    # bounding box; the flags must still equal the full crossing-number test
+    # at the box edges and one unit in the last place either side of them,
+    # on a concave body, closed (last point == first) or not.
+    cx = np.array([0, 3, 3, 1, 1, 3, 3, 0, 0], dtype=float) * 0.3 - 0.2
+    cz = np.array([0, 0, 1, 1, 2, 2, 3, 3, 0], dtype=float) * 0.3 - 0.45
+    if not closed:
+        cx, cz = cx[:-1].copy(), cz[:-1].copy()
+    op, gp = both_params()
+    osh = oracle.Sheet(cx, cz, op)
+    gsh = dsim.Sheet(gpu_ctx, cx, cz, gp.sheet)
+
+    def around(v):
+        return [np.nextafter(v, -np.inf), v, np.nextafter(v, np.inf)]
+
+    xs = sorted(set(sum((around(v) for v in (cx.min(), cx.max(), 0.1)), []) + [-1.0, 0.05, 1.5]))
+    zs = sorted(set(sum((around(v) for v in (cz.min(), cz.max(), 0.0, -0.15, 0.15)), [])
+                    + [-1.0, 0.3, 1.0]))
+    gx, gz = np.meshgrid(np.array(xs), np.array(zs))
+    rng = np.random.default_rng(9)
+    x = np.concatenate([gx.ravel(), rng.uniform(-0.5, 1.0, 3000)])
+    z = np.concatenate([gz.ravel(), rng.uniform(-0.7, 0.7, 3000)])
+    X, Z = to_dev(x, z)
+    flags = gsh.inside(X, Z).cpu().numpy().astype(bool)
+    expect = np.array([osh.inside(a, b) for a, b in zip(x, z)])
+    assert np.array_equal(flags, expect)
+    assert expect.any() and not expect.all()
+
+
 def test_inside_and_reflect_bit_exact(gpu_ctx):
     px, pz = oracle.cylinder_points(64)
     op, gp = both_params()
@@ -253,6 +284,35 @@ def test_remesh_matches_oracle(gpu_ctx, kernel):
     rp.circ_threshold = 0.0
     gc0 = dsim.remesh(gpu_ctx, rp, *to_dev(x, z, c, s))[2].cpu().numpy()
     assert abs(gc0.sum() - c.sum()) <= 1e-12 * np.abs(c).sum()
+
+
+@pytest.mark.parametrize("kernel", [0, 2])
+def test_remesh_blobs_in_a_corner_of_the_mesh(gpu_ctx, kernel):
+    # Blobs occupying a small patch near one corner of a large mesh: the node
+    # gather visits only the rectangle their stencils reach; the result must
+    # equal the oracle's full-mesh remesh bit for bit, including blobs whose
+    # stencil leaves the mesh (kept) and a set with no remeshed blob at all.
+    rng = np.random.default_rng(40 + kernel)
+    n = 3000
+    x = rng.uniform(-2.9, -2.3, n)
+    z = rng.uniform(2.5, 2.95, n)
+    c = rng.normal(0.0, 1e-3, n)
+    s = np.full(n, 0.03)
+    o = oracle.make_params(remesh_frequency=1, remesh_kernel=kernel, remesh_xmin=-3.0,
+                           remesh_xmax=5.0, remesh_zmin=-3.0, remesh_zmax=3.0, remesh_dx=0.02,
+                           remesh_dz=0.02, remesh_sigma=0.03, remesh_circ_threshold=1e-9)
+    rp = dsim.remesh_params(frequency=1, kernel=kernel, xmin=-3.0, xmax=5.0, zmin=-3.0, zmax=3.0,
+                            dx=0.02, dz=0.02, sigma=0.03, circ_threshold=1e-9)
+    for sel in (slice(None), slice(0, 0)):
+        xs, zs, cs, ss = x[sel], z[sel], c[sel], s[sel]
+        if xs.size == 0:  # only blobs whose stencil leaves the mesh
+            xs, zs, cs, ss = (np.array([-2.99, 4.99]), np.array([0.0, 2.99]),
+                              np.array([1e-3, -1e-3]), np.array([0.03, 0.03]))
+        ox, oz, oc, osg = oracle.remesh(o, xs, zs, cs, ss)
+        g = [t.cpu().numpy() for t in dsim.remesh(gpu_ctx, rp, *to_dev(xs, zs, cs, ss))]
+        assert g[0].size == ox.size
+        for a, b in zip(g, (ox, oz, oc, osg)):
+            assert np.array_equal(a, b)
 
 
 def test_release_nan_is_vortex_exception(gpu_ctx):

@@ -444,8 +444,12 @@ void sheet_solve(dvm_ctx_t ctx, SheetDev& sh, int prec, int64_t n, const double*
 {
     const int nl = sh.nl;
     if (n > 0) {
+        // The collocation sums stay FP64 in every form: the loads take
+        // dp = -rho/dt gamma ds, so an FP32 rounding of the right-hand side
+        // reaches Cd, Cl amplified (about 4e-6 at 96 panels, dt 0.005), and
+        // Nl targets are a small share of a step's pairs.
         if (!have_uw)
-            bs_targets_launch(ctx, prec, n, bx, bz, bc, bs, nl, sh.xc, sh.zc, 0, 0.0, sh.uw,
+            bs_targets_launch(ctx, DVM_F64, n, bx, bz, bc, bs, nl, sh.xc, sh.zc, 0, 0.0, sh.uw,
                               sh.uw + nl);
         device_sum(ctx, n, bc, sh.red, sh.scal);
     }
@@ -968,6 +972,8 @@ dvm_status dvm_sheet_solve(dvm_sheet_t s, int prec, int64_t n, const double* x, 
 {
     if (!s) return DVM_E_INVALID;
     return guard(s->ctx, [&] {
+        require(prec == DVM_F64 || prec == DVM_F32C || prec == DVM_FMM, DVM_E_INVALID,
+                "unknown precision");
         require(n >= 0, DVM_E_SIZE, "negative blob count");
         require(n == 0 || (x && z && circ && sigma), DVM_E_INVALID, "null blob array");
         sheet_solve(s->ctx, s->sh, prec, n, x, z, circ, sigma);

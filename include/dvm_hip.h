@@ -182,7 +182,9 @@ dvm_status dvm_sheet_get_gamma(dvm_sheet_t sheet, double* gamma);
 dvm_status dvm_sheet_set_gamma(dvm_sheet_t sheet, const double* gamma);
 /* VortexSheet::solvevortexsheet (src/VortexSheet.cpp:167-215): collocation
  * sum of the n device blobs (unguarded), RHS b = (U+u).n, b_Nl = -sum circ,
- * gamma = (A^T A)^-1 A^T b.  b_host (Nl+1, may be NULL) receives b. */
+ * gamma = (A^T A)^-1 A^T b.  b_host (Nl+1, may be NULL) receives b.  The
+ * collocation sum is FP64 whatever prec names (a valid dvm_precision): the
+ * loads scale gamma by rho/dt, so its right-hand side keeps FP64. */
 dvm_status dvm_sheet_solve(dvm_sheet_t sheet, int prec, int64_t n, const double* x,
                            const double* z, const double* circ, const double* sigma,
                            double* b_host);

@@ -184,6 +184,39 @@ def plain_cylinder_phase(ctx, steps, cpu_steps):
     log(f"plain cylinder FMM: {steps} steps in {af:.2f} s, N = {nf}")
     out["fmm_order16_device_stream"] = {"steps_per_s": steps / af, "final_blobs": nf, "cd": cdf,
                                         "window_steps_per_s": cpu_steps / wf}
+
+    # The fast forms against FP64 from the same state (device stream, keyed by
+    # step and blob, so both draw the same increments while their blob counts
+    # agree): the developed wake of `steps` FP64 steps is copied into the fast
+    # form once, then both run k_steps free (no re-sync), compared each step.
+    def lockstep(prec, k_steps=10):
+        mk = lambda pr: dsim.Sim(ctx, px, pz, dsim.SimParams(sp, dsim.EULER, dsim.REFLECT, 0, pr,
+                                                              dsim.RNG_DEVICE,
+                                                              dsim.remesh_params()))
+        a, b = mk(dvm.F64), mk(prec)
+        a.run(steps)
+        b.run(steps)
+        b.set_state(a.get_state())
+        dcd = dcl = dv = 0.0
+        for _ in range(k_steps):
+            a.step()
+            b.step()
+            sa, sb = a.get_state(), b.get_state()
+            if sa["x"].size == sb["x"].size:
+                for f in ("u", "w"):
+                    dv = max(dv, float(np.abs(sb[f] - sa[f]).max() / np.abs(sa[f]).max()))
+            la, lb = a.loads(), b.loads()
+            dcd, dcl = max(dcd, abs(lb[2] - la[2])), max(dcl, abs(lb[3] - la[3]))
+        n = a.size()
+        a.close()
+        b.close()
+        return {"steps": k_steps, "blobs": n, "max_abs_dcd": dcd, "max_abs_dcl": dcl,
+                "max_normwise_velocity_err": dv}
+
+    out["fast_forms_vs_f64_from_same_state"] = {"f32c": lockstep(dvm.F32C),
+                                                 "fmm_order16": lockstep(dvm.FMM)}
+    log("fast forms vs FP64 from the same state: "
+        + json.dumps(out["fast_forms_vs_f64_from_same_state"]))
     import oracle
     sim = oracle.Sim(px, pz, oracle.make_params(rho=sp.rho, nu=sp.nu, dt=sp.dt, ux=sp.ux,
                                                 uz=sp.uz, cutoff_exp=sp.cutoff_exp,

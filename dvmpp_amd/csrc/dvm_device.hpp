@@ -213,9 +213,8 @@ __device__ __forceinline__ void far_tile_share4(const float4* __restrict__ sp, f
 // (truncation below 1.1e-8 relative); above, 1 - e^-x uses the hardware
 // exponential (v_exp_f32 on x log2 e; relative error below 1e-6).
 // Coincident distinct points (r2 = 0) give NaN, as the reference does.
-__device__ __forceinline__ float near_k(float r2, float2 nj)
+__device__ __forceinline__ float near_k_inv(float r2, float2 nj, float inv)
 {
-    const float inv = __builtin_amdgcn_rcpf(r2);
     const float y = -r2 * nj.y;
     float f = __builtin_fmaf(y, 1.0f / 40320.0f, 1.0f / 5040.0f);
     f = __builtin_fmaf(y, f, 1.0f / 720.0f);
@@ -229,12 +228,51 @@ __device__ __forceinline__ float near_k(float r2, float2 nj)
     return r2 < nj.x ? ((y > -0.5f && r2 > 0.0f) ? small : mid) : inv;
 }
 
+__device__ __forceinline__ float near_k(float r2, float2 nj)
+{
+    return near_k_inv(r2, nj, __builtin_amdgcn_rcpf(r2));
+}
+
 // Near-tile pairs k0 <= k < k1 of one source tile.  Offsets from the tile
 // centre are carried as two FP32 words (rounded value + rounding residual:
 // sj.xy + nj.zw for the source, txd + txl for the target), so dx is exact to
 // FP32 rounding of dx itself however far the pair sits from the centre.
 // nj = {10 sigma^2, 1/sigma^2, x residual, z residual}.  SELF skips source
 // j0 + k == i, GUARD skips r2 <= r2_skip.
+struct NearPair {
+    float dx, dz, r2, c;
+    float2 nj;
+};
+
+__device__ __forceinline__ NearPair near_prep(float4 sj, float4 nj4, float txd, float tzd,
+                                              float txl, float tzl)
+{
+    NearPair p;
+    p.dx = (txd - sj.x) + (txl - nj4.z);
+    p.dz = (tzd - sj.y) + (tzl - nj4.w);
+    p.r2 = __builtin_fmaf(p.dx, p.dx, p.dz * p.dz);
+    p.c = sj.z;
+    p.nj = make_float2(nj4.x, nj4.y);
+    return p;
+}
+
+template <bool SELF, bool GUARD>
+__device__ __forceinline__ void near_add(const NearPair& p, float K, int64_t j, int64_t i,
+                                         float r2_skip, float& au, float& aw)
+{
+    bool take = true;
+    if (SELF) take = j != i;
+    if (GUARD) take = take && (p.r2 > r2_skip);
+    const float kk = take ? K * p.c : 0.0f;
+    au = __builtin_fmaf(kk, p.dz, au);
+    aw = __builtin_fmaf(kk, p.dx, aw);
+}
+
+// Four sources at a time: their loads and separations come first, then one
+// branch for the group — the regularised factor (near_k) only when one of
+// the four pairs of some lane lies inside 10 sigma^2, else 1/r^2 (which
+// near_k also returns there) — so the group's scalar loads are in flight
+// together instead of one source's at a time behind each pair's branch.
 template <bool SELF, bool GUARD>
 __device__ __forceinline__ void near_range(const float4* __restrict__ sp,
                                            const float4* __restrict__ np, int k0, int k1,
@@ -242,21 +280,28 @@ __device__ __forceinline__ void near_range(const float4* __restrict__ sp,
                                            int64_t j0, int64_t i, float r2_skip, float& au,
                                            float& aw)
 {
-#pragma unroll 4
-    for (int k = k0; k < k1; ++k) {
-        const float4 sj = sp[k];
-        const float4 nj4 = np[k];
-        const float2 nj = make_float2(nj4.x, nj4.y);
-        const float dx = (txd - sj.x) + (txl - nj4.z);
-        const float dz = (tzd - sj.y) + (tzl - nj4.w);
-        const float r2 = __builtin_fmaf(dx, dx, dz * dz);
-        const float K = near_k(r2, nj);
-        bool take = true;
-        if (SELF) take = (j0 + k) != i;
-        if (GUARD) take = take && (r2 > r2_skip);
-        const float kk = take ? K * sj.z : 0.0f;
-        au = __builtin_fmaf(kk, dz, au);
-        aw = __builtin_fmaf(kk, dx, aw);
+    int k = k0;
+    for (; k + 4 <= k1; k += 4) {
+        const NearPair p0 = near_prep(sp[k], np[k], txd, tzd, txl, tzl);
+        const NearPair p1 = near_prep(sp[k + 1], np[k + 1], txd, tzd, txl, tzl);
+        const NearPair p2 = near_prep(sp[k + 2], np[k + 2], txd, tzd, txl, tzl);
+        const NearPair p3 = near_prep(sp[k + 3], np[k + 3], txd, tzd, txl, tzl);
+        float K0 = __builtin_amdgcn_rcpf(p0.r2), K1 = __builtin_amdgcn_rcpf(p1.r2);
+        float K2 = __builtin_amdgcn_rcpf(p2.r2), K3 = __builtin_amdgcn_rcpf(p3.r2);
+        if ((p0.r2 < p0.nj.x) | (p1.r2 < p1.nj.x) | (p2.r2 < p2.nj.x) | (p3.r2 < p3.nj.x)) {
+            K0 = near_k_inv(p0.r2, p0.nj, K0);
+            K1 = near_k_inv(p1.r2, p1.nj, K1);
+            K2 = near_k_inv(p2.r2, p2.nj, K2);
+            K3 = near_k_inv(p3.r2, p3.nj, K3);
+        }
+        near_add<SELF, GUARD>(p0, K0, j0 + k, i, r2_skip, au, aw);
+        near_add<SELF, GUARD>(p1, K1, j0 + k + 1, i, r2_skip, au, aw);
+        near_add<SELF, GUARD>(p2, K2, j0 + k + 2, i, r2_skip, au, aw);
+        near_add<SELF, GUARD>(p3, K3, j0 + k + 3, i, r2_skip, au, aw);
+    }
+    for (; k < k1; ++k) {
+        const NearPair p = near_prep(sp[k], np[k], txd, tzd, txl, tzl);
+        near_add<SELF, GUARD>(p, near_k(p.r2, p.nj), j0 + k, i, r2_skip, au, aw);
     }
 }
 

@@ -845,51 +845,24 @@ __global__ __launch_bounds__(1024) void bt_delete_kernel(int64_t cap, SlotState*
     if (t == 0) st[s].n = m;
 }
 
-// One lane per slot: loads (src/VortexSheet.cpp:487-514), then the step
-// counter; a slot that reaches its step count stops.
-__global__ void bt_loads_finish_kernel(int S, int mp, const SlotParam* __restrict__ prm,
-                                       SlotState* __restrict__ st, const double* __restrict__ gamma,
-                                       const double* __restrict__ ds, const double* __restrict__ enx,
-                                       const double* __restrict__ enz, double* __restrict__ work)
+// One workgroup per slot: the loads of src/VortexSheet.cpp:487-514
+// (loads_block; work holds 3 mp doubles per slot), then the step counter; a
+// slot that reaches its step count stops.  (Every lane reads `active` before
+// the first barrier of loads_block, so lane 0's update below is seen by none.)
+__global__ __launch_bounds__(kLoadsThreads) void bt_loads_finish_kernel(
+    int S, int mp, const SlotParam* __restrict__ prm, SlotState* __restrict__ st,
+    const double* __restrict__ gamma, const double* __restrict__ ds,
+    const double* __restrict__ enx, const double* __restrict__ enz, double* __restrict__ work)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.x;
     if (s >= S || !st[s].active) return;
-    const int nl = prm[s].nl;
-    const double* g = gamma + (int64_t)s * mp;
-    const double* d = ds + (int64_t)s * mp;
-    const double* ex = enx + (int64_t)s * mp;
-    const double* ez = enz + (int64_t)s * mp;
-    double* P = work + (int64_t)s * 3 * mp;
-    const double rho = prm[s].rho, ur = prm[s].ur;
-    const double k = -rho / prm[s].dt;
-    P[0] = 0.0;
-    for (int i = 0; i < nl - 1; i++) P[i + 1] = P[i] + k * (g[i] * d[i]);
-    for (int i = 0; i < nl - 1; i++) P[i] = 0.5 * (P[i] + P[i + 1]);
-    P[nl - 1] = 0.5 * (P[nl - 1] + P[0]);
-    double pmax = P[0];
-    for (int i = 1; i < nl; ++i) pmax = P[i] > pmax ? P[i] : pmax;
-    const double pref = 0.5 * rho * ur * ur;
-    double ax1 = 0.0, ax2 = 0.0, az1 = 0.0, az2 = 0.0;
-    int i;
-    for (i = 1; i < nl; i += 2) {
-        const double p0 = P[i - 1] + (pref - pmax) * 1.0, p1 = P[i] + (pref - pmax) * 1.0;
-        ax1 += p0 * ex[i - 1] * d[i - 1];
-        ax2 += p1 * ex[i] * d[i];
-        az1 += p0 * ez[i - 1] * d[i - 1];
-        az2 += p1 * ez[i] * d[i];
+    const int64_t o = (int64_t)s * mp;
+    loads_block(prm[s].nl, prm[s].rho, prm[s].dt, prm[s].ur, prm[s].aref, gamma + o, ds + o,
+                enx + o, enz + o, work + (int64_t)s * 3 * mp, st[s].loads);
+    if (threadIdx.x == 0) {
+        st[s].step += 1;
+        if (st[s].step >= prm[s].steps_target) st[s].active = 0;
     }
-    if ((i - 1) < nl) {
-        const double p0 = P[i - 1] + (pref - pmax) * 1.0;
-        ax1 += p0 * ex[i - 1] * d[i - 1];
-        az1 += p0 * ez[i - 1] * d[i - 1];
-    }
-    const double fx = -(ax1 + ax2), fz = -(az1 + az2);
-    st[s].loads[0] = fx;
-    st[s].loads[1] = fz;
-    st[s].loads[2] = fx / (0.5 * rho * prm[s].aref * ur * ur);
-    st[s].loads[3] = fz / (0.5 * rho * prm[s].aref * ur * ur);
-    st[s].step += 1;
-    if (st[s].step >= prm[s].steps_target) st[s].active = 0;
 }
 
 }  // namespace dvmi
@@ -1043,8 +1016,8 @@ void record_rest(dvm_batch_t B)
     bt_inside_kernel<<<gs, 256, 0, st>>>(S, cap, mp, d.wl, d.blob_pre, d.prm, d.bx, d.bz, d.enx, d.enz,
                                          reflect, d.b, d.flags);
     if (!reflect) bt_delete_kernel<<<S, 1024, 0, st>>>(cap, d.st, d.flags, d.b, d.alt);
-    bt_loads_finish_kernel<<<(S + 63) / 64, 64, 0, st>>>(S, mp, d.prm, d.st, d.gamma, d.ds, d.enx, d.enz,
-                                                         d.lwork);
+    bt_loads_finish_kernel<<<S, kLoadsThreads, 0, st>>>(S, mp, d.prm, d.st, d.gamma, d.ds, d.enx,
+                                                         d.enz, d.lwork);
     DVM_LAUNCH_CHECK();
 }
 

@@ -114,6 +114,80 @@ __device__ __forceinline__ void bs_pair_f64(double xi, double zi, double xj, dou
     wi += K * dx * cj;
 }
 
+// Armadillo-style two-accumulator sum (the oracle restates the same order).
+__device__ __forceinline__ double pair_sum(const double* a, int n)
+{
+    double acc1 = 0.0, acc2 = 0.0;
+    int j;
+    for (j = 1; j < n; j += 2) {
+        acc1 += a[j - 1];
+        acc2 += a[j];
+    }
+    if ((j - 1) < n) acc1 += a[j - 1];
+    return acc1 + acc2;
+}
+
+// src/VortexSheet.cpp:487-514 for one body, by one workgroup of kLoadsThreads
+// lanes: out = {fx, fz, cd, cl}.  Every value is formed by the reference's
+// expression in the reference's order — the pressure prefix, the maximum and
+// the two force sums sequentially by lane 0 — with the independent
+// elementwise steps spread over the lanes and the vectors staged in LDS
+// (work, 2 nl doubles, global memory, when nl > kLoadsLds).
+constexpr int kLoadsThreads = 256;
+constexpr int kLoadsLds = 1024;
+
+__device__ __forceinline__ void loads_block(int nl, double rho, double dt, double ur, double aref,
+                                            const double* __restrict__ g,
+                                            const double* __restrict__ d,
+                                            const double* __restrict__ ex,
+                                            const double* __restrict__ ez, double* work,
+                                            double* __restrict__ out)
+{
+    __shared__ double la[kLoadsLds], lb[kLoadsLds];
+    __shared__ double pmax_s;
+    double* A = nl <= kLoadsLds ? la : work;
+    double* B = nl <= kLoadsLds ? lb : work + nl;
+    const int t = threadIdx.x;
+    const double k = -rho / dt;
+    for (int i = t; i < nl; i += blockDim.x) A[i] = k * (g[i] * d[i]);
+    __syncthreads();
+    if (t == 0) {  // P[0] = 0, P[i+1] = P[i] + k (gamma_i ds_i)
+        double p = 0.0;
+        B[0] = 0.0;
+        for (int i = 0; i < nl - 1; ++i) {
+            p = p + A[i];
+            B[i + 1] = p;
+        }
+    }
+    __syncthreads();
+    // P[i] = (P[i] + P[i+1]) / 2 for i < nl - 1 (the old P[i+1]), then the last
+    // entry with the new P[0]
+    for (int i = t; i < nl - 1; i += blockDim.x) A[i] = 0.5 * (B[i] + B[i + 1]);
+    __syncthreads();
+    if (t == 0) {
+        A[nl - 1] = 0.5 * (B[nl - 1] + A[0]);
+        double pm = A[0];
+        for (int i = 1; i < nl; ++i) pm = A[i] > pm ? A[i] : pm;
+        pmax_s = pm;
+    }
+    __syncthreads();
+    const double shift = (0.5 * rho * ur * ur - pmax_s) * 1.0;
+    for (int i = t; i < nl; i += blockDim.x) {
+        const double pi = A[i] + shift;
+        B[i] = pi * ex[i] * d[i];
+        A[i] = pi * ez[i] * d[i];
+    }
+    __syncthreads();
+    if (t == 0) {
+        const double fx = -pair_sum(B, nl);
+        const double fz = -pair_sum(A, nl);
+        out[0] = fx;
+        out[1] = fz;
+        out[2] = fx / (0.5 * rho * aref * ur * ur);
+        out[3] = fz / (0.5 * rho * aref * ur * ur);
+    }
+}
+
 // Spreads the low 16 bits of v to the even bits (Morton interleave).
 __device__ inline uint32_t spread16(uint32_t v)
 {

@@ -625,56 +625,22 @@ void sheet_probe_quirk(dvm_ctx_t ctx, const SheetDev& sh, int prec, int64_t n, c
 
 // ----------------------------------------------------------------- loads
 
-// Armadillo-style two-accumulator sum (see the oracle's note: order unpinned).
-__device__ double pair_sum(const double* a, int n)
+// src/VortexSheet.cpp:487-514 (loads_block; one workgroup).  work holds
+// max(1024, 3 nl) doubles.
+__global__ __launch_bounds__(kLoadsThreads) void loads_kernel(
+    int nl, double rho, double dt, double ur, double aref, const double* __restrict__ gamma,
+    const double* __restrict__ ds, const double* __restrict__ enx, const double* __restrict__ enz,
+    double* __restrict__ work, double* __restrict__ out)
 {
-    double acc1 = 0.0, acc2 = 0.0;
-    int j;
-    for (j = 1; j < n; j += 2) {
-        acc1 += a[j - 1];
-        acc2 += a[j];
-    }
-    if ((j - 1) < n) acc1 += a[j - 1];
-    return acc1 + acc2;
-}
-
-// src/VortexSheet.cpp:487-514: sequential pressure integration (one lane).
-// work holds 3*nl doubles.
-__global__ void loads_kernel(int nl, double rho, double dt, double ur, double aref,
-                             const double* __restrict__ gamma, const double* __restrict__ ds,
-                             const double* __restrict__ enx, const double* __restrict__ enz,
-                             double* __restrict__ work, double* __restrict__ out)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double* P = work;
-    double* tx = work + nl;
-    double* tz = work + 2 * nl;
-    const double k = -rho / dt;
-    P[0] = 0.0;
-    for (int i = 0; i < nl - 1; i++) P[i + 1] = P[i] + k * (gamma[i] * ds[i]);
-    for (int i = 0; i < nl - 1; i++) P[i] = 0.5 * (P[i] + P[i + 1]);
-    P[nl - 1] = 0.5 * (P[nl - 1] + P[0]);
-    double pmax = P[0];
-    for (int i = 1; i < nl; ++i) pmax = P[i] > pmax ? P[i] : pmax;
-    const double pref = 0.5 * rho * ur * ur;
-    for (int i = 0; i < nl; ++i) {
-        const double pi = P[i] + (pref - pmax) * 1.0;
-        tx[i] = pi * enx[i] * ds[i];
-        tz[i] = pi * enz[i] * ds[i];
-    }
-    const double fx = -pair_sum(tx, nl);
-    const double fz = -pair_sum(tz, nl);
-    out[0] = fx;
-    out[1] = fz;
-    out[2] = fx / (0.5 * rho * aref * ur * ur);
-    out[3] = fz / (0.5 * rho * aref * ur * ur);
+    loads_block(nl, rho, dt, ur, aref, gamma, ds, enx, enz, work, out);
 }
 
 void sheet_loads(dvm_ctx_t ctx, SheetDev& sh, double ur)
 {
     // sh.red holds max(1024, 3 nl) doubles: the loads work vectors
-    loads_kernel<<<1, 64, 0, ctx->stream>>>(sh.nl, sh.rho, sh.dt, ur, sh.aref, sh.gamma, sh.ds,
-                                            sh.enx, sh.enz, sh.red, sh.scal + 1);
+    loads_kernel<<<1, kLoadsThreads, 0, ctx->stream>>>(sh.nl, sh.rho, sh.dt, ur, sh.aref,
+                                                       sh.gamma, sh.ds, sh.enx, sh.enz, sh.red,
+                                                       sh.scal + 1);
     DVM_LAUNCH_CHECK();
 }
 

@@ -37,7 +37,11 @@ struct SlotParam {
     uint32_t seed, pad1;
     double pad2;
     BodyBox box;  // crossing-test cull box of the body
+    int nslab, pad3;              // crossing-test slab lists (EdgeSlabs; 0: none)
+    double slab_z0, slab_inv_h;
 };
+
+constexpr int kSlabCap = 16;  // slab list entries per panel a slot holds
 
 struct SlotState {
     int64_t n;       // live blobs
@@ -97,6 +101,8 @@ struct BatchDev {
     int64_t* off = nullptr;     // S*(mp+1)
     double* cnew = nullptr;
     double* lwork = nullptr;    // S * 3*mp
+    int* slab_off = nullptr;    // S * (mp+1)
+    int* slab_edge = nullptr;   // S * kSlabCap*mp
 };
 
 // ----------------------------------------------------------------- work lists
@@ -754,46 +760,59 @@ __global__ void bt_commit_release_kernel(int S, SlotState* __restrict__ st)
     st[s].nrv = 0;
 }
 
-__global__ void bt_inside_kernel(int S, int64_t cap, int mp, const WorkList* wl,
-                                 const int64_t* __restrict__ blob_pre,
-                                 const SlotParam* __restrict__ prm, const double* __restrict__ bx,
-                                 const double* __restrict__ bz, const double* __restrict__ enx,
-                                 const double* __restrict__ enz, int reflect, BlobBuf b,
-                                 int* __restrict__ flags)
+// Crossing flags (inside_body_fast with the slot's box and slab lists) and,
+// for REFLECT, src/VortexSheet.cpp:345-405 for the inside blobs: each wave
+// takes its inside blobs one at a time, all lanes working on one
+// (reflect_wave, with that blob's slot body).  The loop bounds are
+// wave-uniform so every lane takes part in the shuffles.
+__global__ __launch_bounds__(256) void bt_inside_kernel(
+    int S, int64_t cap, int mp, const WorkList* wl, const int64_t* __restrict__ blob_pre,
+    const SlotParam* __restrict__ prm, const double* __restrict__ bx,
+    const double* __restrict__ bz, const double* __restrict__ enx,
+    const double* __restrict__ enz, const int* __restrict__ slab_off,
+    const int* __restrict__ slab_edge, int reflect, BlobBuf b, int* __restrict__ flags)
 {
-    for_each_blob(wl, blob_pre, S, cap, [&](int s, int64_t o) {
-        const int nl = prm[s].nl;
-        const double* px = bx + (int64_t)s * (mp + 1);
-        const double* pz = bz + (int64_t)s * (mp + 1);
-        const double x0 = b.x[o], z0 = b.z[o];
-        const bool in = !outside_box(prm[s].box, x0, z0) && inside_body(nl, px, pz, x0, z0);
-        flags[o] = in ? 1 : 0;
-        if (!in || !reflect) return;
-        // src/VortexSheet.cpp:345-405
-        int closest = 0;
-        double min_dist = 1e10;
-        for (int j = 0; j < nl; j++) {
-            const double xa = px[j], za = pz[j], xb2 = px[j + 1], zb2 = pz[j + 1];
-            const double vx = x0 - xa, vz = z0 - za;
-            const double dx = xb2 - xa, dz = zb2 - za;
-            const double len_sq = dx * dx + dz * dz;
-            double t = (vx * dx + vz * dz) / len_sq;
-            const double tm = (t < 1.0) ? t : 1.0;
-            t = (0.0 < tm) ? tm : 0.0;
-            const double cx = xa + t * dx, cz = za + t * dz;
-            const double dist = sqrt((x0 - cx) * (x0 - cx) + (z0 - cz) * (z0 - cz));
-            if (dist < min_dist) { min_dist = dist; closest = j; }
+    const int64_t total = wl->total_blobs;
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t gb = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); gb < total;
+         gb += stride) {
+        const int64_t g = gb + lane;
+        int s = 0;
+        int64_t o = 0;
+        bool in = false;
+        double x0 = 0.0, z0 = 0.0;
+        if (g < total) {
+            s = find_slot(blob_pre, S, g);
+            o = (int64_t)s * cap + (g - blob_pre[s]);
+            x0 = b.x[o];
+            z0 = b.z[o];
+            EdgeSlabs es;
+            es.off = slab_off + (int64_t)s * (mp + 1);
+            es.edge = slab_edge + (int64_t)s * kSlabCap * mp;
+            es.n = prm[s].nslab;
+            es.z0 = prm[s].slab_z0;
+            es.inv_h = prm[s].slab_inv_h;
+            in = inside_body_fast(prm[s].nl, bx + (int64_t)s * (mp + 1),
+                                  bz + (int64_t)s * (mp + 1), prm[s].box, es, x0, z0);
+            flags[o] = in ? 1 : 0;
         }
-        const double offset = 2.0 * min_dist + 1e-6;
-        const double ex = enx[(int64_t)s * mp + closest], ez = enz[(int64_t)s * mp + closest];
-        double nx = x0 + offset * ex, nz = z0 + offset * ez;
-        if (inside_body(nl, px, pz, nx, nz)) {
-            nx = x0 + 3.0 * offset * ex;
-            nz = z0 + 3.0 * offset * ez;
+        if (!reflect) continue;
+        uint64_t mask = __ballot(in);
+        while (mask) {
+            const int l = __ffsll((unsigned long long)mask) - 1;
+            mask &= mask - 1;
+            const int sl = __shfl(s, l);
+            const double px = __shfl(x0, l), pz = __shfl(z0, l);
+            const double2 p = reflect_wave(prm[sl].nl, bx + (int64_t)sl * (mp + 1),
+                                           bz + (int64_t)sl * (mp + 1), enx + (int64_t)sl * mp,
+                                           enz + (int64_t)sl * mp, px, pz);
+            if (lane == l) {
+                b.x[o] = p.x;
+                b.z[o] = p.y;
+            }
         }
-        b.x[o] = nx;
-        b.z[o] = nz;
-    });
+    }
 }
 
 // One workgroup per slot: stable compaction of the blobs outside the body
@@ -947,6 +966,8 @@ size_t batch_bytes(const dvm_batch_config& c, BatchDev* d, char* mem)
     o.off = cv.take<int64_t>(np1);
     o.cnew = cv.take<double>(np);
     o.lwork = cv.take<double>(S * 3 * mp);
+    o.slab_off = cv.take<int>(S * (mp + 1));
+    o.slab_edge = cv.take<int>(S * kSlabCap * mp);
     return cv.off + 256;
 }
 
@@ -1013,8 +1034,8 @@ void record_rest(dvm_batch_t B)
     bt_prefix_kernel<<<1, 1024, 0, st>>>(S, cap, d.st, d.wl, d.tile_pre, d.order, d.otile_pre,
                                          d.blob_pre, d.item_pre, d.seg_begin, d.seg_end);
     const int reflect = B->cfg.crossing != DVM_DELETE ? 1 : 0;
-    bt_inside_kernel<<<gs, 256, 0, st>>>(S, cap, mp, d.wl, d.blob_pre, d.prm, d.bx, d.bz, d.enx, d.enz,
-                                         reflect, d.b, d.flags);
+    bt_inside_kernel<<<gs, 256, 0, st>>>(S, cap, mp, d.wl, d.blob_pre, d.prm, d.bx, d.bz, d.enx,
+                                         d.enz, d.slab_off, d.slab_edge, reflect, d.b, d.flags);
     if (!reflect) bt_delete_kernel<<<S, 1024, 0, st>>>(cap, d.st, d.flags, d.b, d.alt);
     bt_loads_finish_kernel<<<S, kLoadsThreads, 0, st>>>(S, mp, d.prm, d.st, d.gamma, d.ds, d.enx,
                                                          d.enz, d.lwork);
@@ -1209,6 +1230,18 @@ dvm_status dvm_batch_admit(dvm_batch_t B, int npts, const double* px, const doub
         sp.nl = nl; sp.max_npv = p->max_npv; sp.steps_target = steps;
         sp.seed = (uint32_t)seed;
         sp.box = sh.box;
+        sp.nslab = 0;
+        if (sh.slabs.n > 0 && sh.hslab_edge.size() <= (size_t)kSlabCap * mp) {
+            DVM_HIP_CHECK(hipMemcpyAsync(d.slab_off + (size_t)s * (mp + 1), sh.hslab_off.data(),
+                                         sizeof(int) * sh.hslab_off.size(), hipMemcpyHostToDevice,
+                                         st));
+            DVM_HIP_CHECK(hipMemcpyAsync(d.slab_edge + (size_t)s * kSlabCap * mp,
+                                         sh.hslab_edge.data(), sizeof(int) * sh.hslab_edge.size(),
+                                         hipMemcpyHostToDevice, st));
+            sp.nslab = sh.slabs.n;
+            sp.slab_z0 = sh.slabs.z0;
+            sp.slab_inv_h = sh.slabs.inv_h;
+        }
         SlotState ss{};
         ss.active = 1;
         DVM_HIP_CHECK(hipMemcpyAsync(d.prm + s, &sp, sizeof(sp), hipMemcpyHostToDevice, st));

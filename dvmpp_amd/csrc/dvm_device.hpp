@@ -48,18 +48,26 @@ __device__ __forceinline__ double u01(uint32_t a, uint32_t b)
     return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
 }
 
+// src/VortexSheet.cpp:453-457: 1 when edge i (point i to i+1) straddles zc
+// under the half-open rule and crosses z = zc right of xc.
+__device__ __forceinline__ int edge_crossing(const double* __restrict__ bxp,
+                                             const double* __restrict__ bzp, int i, double xc,
+                                             double zc)
+{
+    const double zi = bzp[i], zi1 = bzp[i + 1];
+    if (((zi <= zc) && (zi1 > zc)) || ((zi > zc) && (zi1 <= zc))) {
+        const double vt = (zc - zi) / (zi1 - zi);
+        if (xc < bxp[i] + vt * (bxp[i + 1] - bxp[i])) return 1;
+    }
+    return 0;
+}
+
 // src/VortexSheet.cpp:447-464: crossing number over the Nl edges; inside iff 1.
 __device__ __forceinline__ bool inside_body(int nl, const double* __restrict__ bxp,
                                             const double* __restrict__ bzp, double xc, double zc)
 {
     int cn = 0;
-    for (int i = 0; i < nl; i++) {
-        const double zi = bzp[i], zi1 = bzp[i + 1];
-        if (((zi <= zc) && (zi1 > zc)) || ((zi > zc) && (zi1 <= zc))) {
-            const double vt = (zc - zi) / (zi1 - zi);
-            if (xc < bxp[i] + vt * (bxp[i + 1] - bxp[i])) ++cn;
-        }
-    }
+    for (int i = 0; i < nl; i++) cn += edge_crossing(bxp, bzp, i, xc, zc);
     return cn == 1;
 }
 
@@ -71,6 +79,75 @@ __device__ __forceinline__ bool inside_body(int nl, const double* __restrict__ b
 __device__ __forceinline__ bool outside_box(const BodyBox& bb, double xc, double zc)
 {
     return zc < bb.zlo || zc >= bb.zhi || xc > bb.xhi || xc < bb.xlo;
+}
+
+// inside_body with the bounding-box cull and the slab edge lists: only the
+// edges listed for zc's slab can straddle zc (EdgeSlabs), and the crossing
+// number is a count, so visiting them alone gives the same flag.
+__device__ __forceinline__ bool inside_body_fast(int nl, const double* __restrict__ bxp,
+                                                 const double* __restrict__ bzp,
+                                                 const BodyBox& bb, const EdgeSlabs& es, double xc,
+                                                 double zc)
+{
+    if (outside_box(bb, xc, zc)) return false;
+    if (es.n == 0) return inside_body(nl, bxp, bzp, xc, zc);
+    if (!(zc == zc)) return false;  // NaN: no edge straddles it
+    const double f = floor((zc - es.z0) * es.inv_h);
+    const int s = f < 0.0 ? 0 : (f > (double)(es.n - 1) ? es.n - 1 : (int)f);
+    int cn = 0;
+    for (int q = es.off[s]; q < es.off[s + 1]; ++q) cn += edge_crossing(bxp, bzp, es.edge[q], xc, zc);
+    return cn == 1;
+}
+
+// src/VortexSheet.cpp:345-405 for the point (px, pz), computed by all 64
+// lanes of a wavefront together (the point and the body are the same in
+// every lane): lane l searches edges l, l+64, ... keeping the first strict
+// minimum of the clamped-projection distance from 1e10 (the reference's
+// start), and the wave keeps the smallest distance and, among equal ones,
+// the smallest edge — the reference's first strict minimum over all edges
+// (no edge closer than 1e10: edge 0).  The moved point's inside test is the
+// same crossing count summed over the lanes.  Returns the new position.
+__device__ __forceinline__ double2 reflect_wave(int nl, const double* __restrict__ bxp,
+                                                const double* __restrict__ bzp,
+                                                const double* __restrict__ enx,
+                                                const double* __restrict__ enz, double px,
+                                                double pz)
+{
+    const int lane = threadIdx.x & 63;
+    double min_dist = 1e10;
+    int closest = nl;  // none closer than 1e10 in this lane
+    for (int j = lane; j < nl; j += 64) {
+        const double x0 = bxp[j], z0 = bzp[j], x1 = bxp[j + 1], z1 = bzp[j + 1];
+        const double vx = px - x0, vz = pz - z0;
+        const double dx = x1 - x0, dz = z1 - z0;
+        const double len_sq = dx * dx + dz * dz;
+        double t = (vx * dx + vz * dz) / len_sq;
+        const double tm = (t < 1.0) ? t : 1.0;      // std::min(1.0, t)
+        t = (0.0 < tm) ? tm : 0.0;                   // std::max(0.0, .)
+        const double cx = x0 + t * dx, cz = z0 + t * dz;
+        const double dist = sqrt((px - cx) * (px - cx) + (pz - cz) * (pz - cz));
+        if (dist < min_dist) { min_dist = dist; closest = j; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double od = __shfl_xor(min_dist, o);
+        const int oj = __shfl_xor(closest, o);
+        if (od < min_dist || (od == min_dist && oj < closest)) {
+            min_dist = od;
+            closest = oj;
+        }
+    }
+    if (closest == nl) closest = 0;
+    const double offset = 2.0 * min_dist + 1e-6;
+    double nx = px + offset * enx[closest];
+    double nz = pz + offset * enz[closest];
+    int cn = 0;
+    for (int j = lane; j < nl; j += 64) cn += edge_crossing(bxp, bzp, j, nx, nz);
+    for (int o = 32; o > 0; o >>= 1) cn += __shfl_xor(cn, o);
+    if (cn == 1) {
+        nx = px + 3.0 * offset * enx[closest];
+        nz = pz + 3.0 * offset * enz[closest];
+    }
+    return make_double2(nx, nz);
 }
 
 // One (blob, panel) term of src/VortexSheet.cpp:225-272: coefficients

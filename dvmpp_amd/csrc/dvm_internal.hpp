@@ -154,12 +154,30 @@ struct BodyBox {
 };
 BodyBox body_box(int npts, const double* px, const double* pz);
 
+// Edges per horizontal slab of a body's z range, for the crossing test: the
+// range [z0, z0 + n/inv_h) is cut into n slabs (n = Nl), slab s lists (in
+// edge order) every edge whose z extent meets slabs s-1..s+1 of its ends, so
+// every edge that can straddle a z in slab s is listed there.  n = 0: none
+// (degenerate or non-finite body, or more entries than a caller can hold).
+struct EdgeSlabs {
+    const int* off = nullptr;   // n + 1 offsets into edge
+    const int* edge = nullptr;
+    int n = 0;
+    int pad = 0;
+    double z0 = 0.0, inv_h = 0.0;
+};
+// Host build: off (n + 1) and edge lists; returns n (0 for none).
+int edge_slabs_build(int npts, const double* px, const double* pz, std::vector<int>& off,
+                     std::vector<int>& edge, double& z0, double& inv_h);
+
 struct SheetDev {
     int nl = 0;
     double rho = 1, nu = 0, dt = 0, ux = 0, uz = 0, cutoff_exp = 0;
     int max_npv = 1;
     double aref = 0;
     BodyBox box{};                                 // crossing-test cull box
+    EdgeSlabs slabs;                               // crossing-test edge lists (device)
+    std::vector<int> hslab_off, hslab_edge;        // the same on the host
     std::vector<double> hx, hz;                    // body points (host copy)
     std::vector<double> hxc, hzc, htheta, hds, henx, henz, hetx, hetz;
     char* mem = nullptr;                           // one device allocation

@@ -208,6 +208,51 @@ BodyBox body_box(int npts, const double* px, const double* pz)
     return b;
 }
 
+int edge_slabs_build(int npts, const double* px, const double* pz, std::vector<int>& off,
+                     std::vector<int>& edge, double& z0, double& inv_h)
+{
+    off.clear();
+    edge.clear();
+    const int nl = npts - 1;
+    double lo = INFINITY, hi = -INFINITY;
+    for (int i = 0; i < npts; ++i) {
+        if (!std::isfinite(px[i]) || !std::isfinite(pz[i])) return 0;
+        lo = pz[i] < lo ? pz[i] : lo;
+        hi = pz[i] > hi ? pz[i] : hi;
+    }
+    if (!(hi > lo) || nl < 1) return 0;
+    const int n = nl;
+    z0 = lo;
+    inv_h = (double)n / (hi - lo);
+    auto slab = [&](double z) {
+        const double f = std::floor((z - z0) * inv_h);
+        return f < 0.0 ? 0 : (f > (double)(n - 1) ? n - 1 : (int)f);
+    };
+    std::vector<int> cnt(n + 1, 0);
+    auto span = [&](int i, int& a, int& b) {
+        const double za = pz[i], zb = pz[i + 1];
+        a = slab(za < zb ? za : zb) - 1;
+        b = slab(za < zb ? zb : za) + 1;
+        a = a < 0 ? 0 : a;
+        b = b > n - 1 ? n - 1 : b;
+    };
+    for (int i = 0; i < nl; ++i) {
+        int a, b;
+        span(i, a, b);
+        for (int q = a; q <= b; ++q) ++cnt[q + 1];
+    }
+    off.assign(n + 1, 0);
+    for (int q = 0; q < n; ++q) off[q + 1] = off[q] + cnt[q + 1];
+    edge.resize(off[n]);
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (int i = 0; i < nl; ++i) {
+        int a, b;
+        span(i, a, b);
+        for (int q = a; q <= b; ++q) edge[fill[q]++] = i;
+    }
+    return n;
+}
+
 void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const double* pz,
                 const dvm_sheet_params* p)
 {
@@ -222,6 +267,8 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
     sh.hx.assign(px, px + npts);
     sh.hz.assign(pz, pz + npts);
     sh.box = body_box(npts, px, pz);
+    double sz0 = 0.0, sinv = 0.0;
+    const int nslab = edge_slabs_build(npts, px, pz, sh.hslab_off, sh.hslab_edge, sz0, sinv);
     // read_input_coord / form_vortex_sheet (src/VortexSheet.cpp:56-102): O(Nl)
     // set-up on the host with the C library's libm.
     double mn = px[0], mx = px[0];
@@ -296,7 +343,9 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
                                       d * (nl + 1) * nl, d * nl * (nl + 1), d * nl, d * (nl + 1),
                                       d * 2 * nl, d * 8, d * nred, sizeof(int) * nl,
                                       sizeof(int64_t) * (nl + 1), d * nl, sizeof(int) * 4, d * nm,
-                                      d * nm, d * 2 * kSheetTerms});
+                                      d * nm, d * 2 * kSheetTerms,
+                                      sizeof(int) * (sh.hslab_off.size() + 1),
+                                      sizeof(int) * (sh.hslab_edge.size() + 1)});
     DVM_HIP_CHECK(hipMalloc(&sh.mem, need));
     Carver cv(sh.mem);
     sh.x = cv.take<double>(npts);
@@ -319,6 +368,21 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
     sh.mI0 = cv.take<double>(nm);
     sh.mI1 = cv.take<double>(nm);
     sh.mA = cv.take<double>(2 * kSheetTerms);
+    int* slab_off = cv.take<int>(sh.hslab_off.size() + 1);
+    int* slab_edge = cv.take<int>(sh.hslab_edge.size() + 1);
+    if (nslab > 0) {
+        DVM_HIP_CHECK(hipMemcpy(slab_off, sh.hslab_off.data(), sizeof(int) * sh.hslab_off.size(),
+                                hipMemcpyHostToDevice));
+        DVM_HIP_CHECK(hipMemcpy(slab_edge, sh.hslab_edge.data(),
+                                sizeof(int) * sh.hslab_edge.size(), hipMemcpyHostToDevice));
+        sh.slabs.off = slab_off;
+        sh.slabs.edge = slab_edge;
+        sh.slabs.n = nslab;
+        sh.slabs.z0 = sz0;
+        sh.slabs.inv_h = sinv;
+    } else {
+        sh.slabs = EdgeSlabs{};
+    }
 
     std::vector<double> g((size_t)nl * 8);
     std::memcpy(&g[0], sh.hxc.data(), d * nl);
@@ -773,14 +837,13 @@ void sheet_release_emit(dvm_ctx_t ctx, const SheetDev& sh, int64_t nrv, const do
 
 __global__ __launch_bounds__(256) void inside_kernel(int64_t n, int nl, const double* __restrict__ bxp,
                                                      const double* __restrict__ bzp, BodyBox bb,
-                                                     const double* __restrict__ x,
+                                                     EdgeSlabs es, const double* __restrict__ x,
                                                      const double* __restrict__ z,
                                                      int* __restrict__ flags)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double xi = x[i], zi = z[i];
-    flags[i] = (!outside_box(bb, xi, zi) && inside_body(nl, bxp, bzp, xi, zi)) ? 1 : 0;
+    flags[i] = inside_body_fast(nl, bxp, bzp, bb, es, x[i], z[i]) ? 1 : 0;
 }
 
 void sheet_inside_flags(dvm_ctx_t ctx, const SheetDev& sh, int64_t n, const double* x,
@@ -788,43 +851,38 @@ void sheet_inside_flags(dvm_ctx_t ctx, const SheetDev& sh, int64_t n, const doub
 {
     if (n <= 0) return;
     ProfScope ps(ctx, "crossing_flags");
-    inside_kernel<<<cdiv(n, 256), 256, 0, ctx->stream>>>(n, sh.nl, sh.x, sh.z, sh.box, x, z,
-                                                         flags);
+    inside_kernel<<<cdiv(n, 256), 256, 0, ctx->stream>>>(n, sh.nl, sh.x, sh.z, sh.box, sh.slabs,
+                                                         x, z, flags);
     DVM_LAUNCH_CHECK();
 }
 
-// src/VortexSheet.cpp:345-405 for the flagged (inside) blobs.
-__global__ void reflect_kernel(int64_t n, int nl, const double* __restrict__ bxp,
-                               const double* __restrict__ bzp, const double* __restrict__ enx,
-                               const double* __restrict__ enz, const int* __restrict__ flags,
-                               double* __restrict__ x, double* __restrict__ z)
+// src/VortexSheet.cpp:345-405 for the flagged (inside) blobs: each
+// wavefront takes the flagged blobs among its 64 one at a time, all lanes
+// working on one (reflect_wave).  No lane leaves early: every lane takes
+// part in the shuffles.
+__global__ __launch_bounds__(256) void reflect_kernel(int64_t n, int nl,
+                                                      const double* __restrict__ bxp,
+                                                      const double* __restrict__ bzp,
+                                                      const double* __restrict__ enx,
+                                                      const double* __restrict__ enz,
+                                                      const int* __restrict__ flags,
+                                                      double* __restrict__ x,
+                                                      double* __restrict__ z)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !flags[i]) return;
-    const double px = x[i], pz = z[i];
-    int closest = 0;
-    double min_dist = 1e10;
-    for (int j = 0; j < nl; j++) {
-        const double x0 = bxp[j], z0 = bzp[j], x1 = bxp[j + 1], z1 = bzp[j + 1];
-        const double vx = px - x0, vz = pz - z0;
-        const double dx = x1 - x0, dz = z1 - z0;
-        const double len_sq = dx * dx + dz * dz;
-        double t = (vx * dx + vz * dz) / len_sq;
-        const double tm = (t < 1.0) ? t : 1.0;      // std::min(1.0, t)
-        t = (0.0 < tm) ? tm : 0.0;                   // std::max(0.0, .)
-        const double cx = x0 + t * dx, cz = z0 + t * dz;
-        const double dist = sqrt((px - cx) * (px - cx) + (pz - cz) * (pz - cz));
-        if (dist < min_dist) { min_dist = dist; closest = j; }
+    const bool mine = i < n && flags[i] != 0;
+    const double xi = mine ? x[i] : 0.0, zi = mine ? z[i] : 0.0;
+    uint64_t mask = __ballot(mine);
+    while (mask) {
+        const int l = __ffsll((unsigned long long)mask) - 1;
+        mask &= mask - 1;
+        const double px = __shfl(xi, l), pz = __shfl(zi, l);
+        const double2 p = reflect_wave(nl, bxp, bzp, enx, enz, px, pz);
+        if ((int)(threadIdx.x & 63) == l) {
+            x[i] = p.x;
+            z[i] = p.y;
+        }
     }
-    const double offset = 2.0 * min_dist + 1e-6;
-    double nx = px + offset * enx[closest];
-    double nz = pz + offset * enz[closest];
-    if (inside_body(nl, bxp, bzp, nx, nz)) {
-        nx = px + 3.0 * offset * enx[closest];
-        nz = pz + 3.0 * offset * enz[closest];
-    }
-    x[i] = nx;
-    z[i] = nz;
 }
 
 void sheet_reflect(dvm_ctx_t ctx, const SheetDev& sh, int64_t n, const int* flags, double* x,

@@ -101,6 +101,7 @@ struct BatchDev {
     int64_t* off = nullptr;     // S*(mp+1)
     double* cnew = nullptr;
     double* lwork = nullptr;    // S * 3*mp
+    double* rhs = nullptr;      // S * (mp+1) right-hand sides of the gamma solve
     int* slab_off = nullptr;    // S * (mp+1)
     int* slab_edge = nullptr;   // S * kSlabCap*mp
 };
@@ -525,18 +526,17 @@ __global__ __launch_bounds__(256) void bt_colloc_kernel(int64_t cap, int mp,
     if (threadIdx.x == 0) out[2 * mp] = red[0];
 }
 
-// One workgroup per slot: b = (U + u).n, b_Nl = -sum circ; gamma = P b.
-__global__ __launch_bounds__(256) void bt_gamma_kernel(int mp, const SlotParam* __restrict__ prm,
-                                                       const SlotState* __restrict__ st,
-                                                       const double* __restrict__ enx,
-                                                       const double* __restrict__ enz,
-                                                       const double* __restrict__ P,
-                                                       const double* __restrict__ col,
-                                                       double* __restrict__ gamma)
+// Per slot: b = (U + u).n, b_Nl = -sum circ (the chunk partials added in
+// chunk order), into rhs (mp + 1 per slot).
+__global__ __launch_bounds__(256) void bt_rhs_kernel(int mp, const SlotParam* __restrict__ prm,
+                                                     const SlotState* __restrict__ st,
+                                                     const double* __restrict__ enx,
+                                                     const double* __restrict__ enz,
+                                                     const double* __restrict__ col,
+                                                     double* __restrict__ rhs)
 {
     const int s = blockIdx.x;
     if (!st[s].active) return;
-    extern __shared__ double sb[];
     const int nl = prm[s].nl;
     const bool have = st[s].n > 0;
     const double* cs = col + (int64_t)s * kColChunks * (2 * mp + 1);
@@ -557,13 +557,32 @@ __global__ __launch_bounds__(256) void bt_gamma_kernel(int mp, const SlotParam* 
             v = have ? (prm[s].ux + u * kRpi2) * ex + (prm[s].uz + w * kRpi2) * ez
                      : prm[s].ux * ex + prm[s].uz * ez;
         }
-        sb[i] = v;
+        rhs[(int64_t)s * (mp + 1) + i] = v;
     }
+}
+
+// gamma = P b per slot, kGammaBlocks workgroups per slot, one wavefront per
+// row: lane l sums columns l, l+64, ... in order, then the lanes' sums are
+// added by shuffles — the row arithmetic of gamma_kernel, so a slot's gamma
+// equals the single simulation's bit for bit.
+constexpr int kGammaBlocks = 16;
+
+__global__ __launch_bounds__(256) void bt_gamma_kernel(int mp, const SlotParam* __restrict__ prm,
+                                                       const SlotState* __restrict__ st,
+                                                       const double* __restrict__ P,
+                                                       const double* __restrict__ rhs,
+                                                       double* __restrict__ gamma)
+{
+    const int s = blockIdx.y;
+    if (!st[s].active) return;
+    extern __shared__ double sb[];
+    const int nl = prm[s].nl;
+    const int ncol = nl + 1;
+    for (int i = threadIdx.x; i < ncol; i += 256) sb[i] = rhs[(int64_t)s * (mp + 1) + i];
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int ncol = nl + 1;
     const double* Ps = P + (int64_t)s * mp * (mp + 1);
-    for (int i = wave; i < nl; i += 4) {
+    for (int i = blockIdx.x * 4 + wave; i < nl; i += kGammaBlocks * 4) {
         double acc = 0.0;
         for (int c = lane; c < ncol; c += 64) acc += Ps[(int64_t)i * ncol + c] * sb[c];
         for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
@@ -966,6 +985,7 @@ size_t batch_bytes(const dvm_batch_config& c, BatchDev* d, char* mem)
     o.off = cv.take<int64_t>(np1);
     o.cnew = cv.take<double>(np);
     o.lwork = cv.take<double>(S * 3 * mp);
+    o.rhs = cv.take<double>(S * (mp + 1));
     o.slab_off = cv.take<int>(S * (mp + 1));
     o.slab_edge = cv.take<int>(S * kSlabCap * mp);
     return cv.off + 256;
@@ -1006,8 +1026,9 @@ void record_convect(dvm_batch_t B)
             bt_bs64_kernel<<<persistent, kBT, 0, st>>>(S, cap, d.wl, d.order, d.otile_pre, d.st, d.b);
         }
         bt_colloc_kernel<<<dim3(kColChunks, S), 256, 0, st>>>(cap, mp, d.prm, d.st, d.b, d.panel, d.col);
-        bt_gamma_kernel<<<S, 256, sizeof(double) * (mp + 1), st>>>(mp, d.prm, d.st, d.enx, d.enz, d.P,
-                                                                    d.col, d.gamma);
+        bt_rhs_kernel<<<S, 256, 0, st>>>(mp, d.prm, d.st, d.enx, d.enz, d.col, d.rhs);
+        bt_gamma_kernel<<<dim3(kGammaBlocks, S), 256, sizeof(double) * (mp + 1), st>>>(
+            mp, d.prm, d.st, d.P, d.rhs, d.gamma);
         bt_sheet_kernel<<<gs, 256, 0, st>>>(S, cap, mp, d.wl, d.blob_pre, d.prm, d.b, d.panel, d.gamma);
         bt_advect_kernel<<<gs, 256, 0, st>>>(S, cap, d.wl, d.blob_pre, d.prm, rk ? 1 : 0, a[k], bb[k],
                                              d.b, d.q1, d.q2);

@@ -149,6 +149,36 @@ def test_inside_bbox_edges_bit_exact(gpu_ctx, closed):
     assert expect.any() and not expect.all()
 
 
+def test_inside_spiky_star_bit_exact(gpu_ctx):
+    # A 96-edge star whose spikes span most of the height: every slab lists
+    # many edges (long lists, many straddles per z); flags and reflected
+    # positions equal the oracle's full crossing-number test.
+    k = np.arange(97)
+    r = np.where(k % 2 == 0, 1.0, 0.25)
+    th = 2 * np.pi * k / 96 + 0.3
+    px, pz = r * np.cos(th), r * np.sin(th)
+    px[-1], pz[-1] = px[0], pz[0]
+    op, gp = both_params()
+    osh = oracle.Sheet(px, pz, op)
+    gsh = dsim.Sheet(gpu_ctx, px, pz, gp.sheet)
+    rng = np.random.default_rng(17)
+    x = rng.uniform(-1.1, 1.1, 6000)
+    z = rng.uniform(-1.1, 1.1, 6000)
+    X, Z = to_dev(x, z)
+    flags = gsh.inside(X, Z).cpu().numpy().astype(bool)
+    expect = np.array([osh.inside(a, b) for a, b in zip(x, z)])
+    assert np.array_equal(flags, expect) and expect.any()
+    osim = oracle.Sim(px, pz, op)
+    st = {f: np.zeros(x.size) for f in oracle.STATE_FIELDS}
+    st["x"], st["z"] = x.copy(), z.copy()
+    st["id"] = np.arange(x.size, dtype=np.uint32)
+    osim.set_state(st)
+    osim.crossing()
+    ost = osim.get_state()
+    gsh.reflect(X, Z)
+    assert np.array_equal(X.cpu().numpy(), ost["x"]) and np.array_equal(Z.cpu().numpy(), ost["z"])
+
+
 def test_inside_and_reflect_bit_exact(gpu_ctx):
     px, pz = oracle.cylinder_points(64)
     op, gp = both_params()

@@ -546,45 +546,73 @@ __global__ __launch_bounds__(256) void sheet_on_blobs_kernel(
 }
 
 // Moments of the current gamma: mA_k = sum_j -(gamma_j I0_jk + gamma_{j+1} I1_jk)
-// (gamma(s) = gamma_j (1 - s/ds) + gamma_{j+1} s/ds on panel j, cyclic), one
-// thread per moment, panels in order.
-__global__ void sheet_moments_kernel(int nl, const double* __restrict__ gamma,
-                                     const double* __restrict__ i0, const double* __restrict__ i1,
-                                     double* __restrict__ ma)
+// (gamma(s) = gamma_j (1 - s/ds) + gamma_{j+1} s/ds on panel j, cyclic): one
+// workgroup per moment, lane t summing panels t, t+256, ... and the lanes'
+// sums added by a fixed tree.  Block 0 also clears the near-blob counter of
+// sheet_on_blobs_far_kernel.
+__global__ __launch_bounds__(256) void sheet_moments_kernel(int nl, const double* __restrict__ gamma,
+                                                            const double* __restrict__ i0,
+                                                            const double* __restrict__ i1,
+                                                            double* __restrict__ ma,
+                                                            int* __restrict__ near_count)
 {
-    const int k = threadIdx.x;
-    if (k >= kSheetTerms) return;
+    __shared__ double rr[256], ri[256];
+    const int k = blockIdx.x, t = threadIdx.x;
+    if (k == 0 && t == 0) *near_count = 0;
     double ar = 0.0, ai = 0.0;
-    for (int j = 0; j < nl; ++j) {
+    for (int j = t; j < nl; j += 256) {
         const double g0 = gamma[j], g1 = gamma[j + 1 == nl ? 0 : j + 1];
         const size_t o = 2 * ((size_t)j * kSheetTerms + k);
         ar -= g0 * i0[o] + g1 * i1[o];
         ai -= g0 * i0[o + 1] + g1 * i1[o + 1];
     }
-    ma[2 * k] = ar;
-    ma[2 * k + 1] = ai;
+    rr[t] = ar;
+    ri[t] = ai;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) {
+            rr[t] += rr[t + h];
+            ri[t] += ri[t + h];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        ma[2 * k] = rr[0];
+        ma[2 * k + 1] = ri[0];
+    }
 }
 
 // Fast form: outside kSheetFar * mr of the body centre the Laurent series
 // phi = (1/mr) sum_k A_k / W^(k+1), W = (z - c) / mr, gives uvs = Im(phi) / 2pi,
 // wvs = Re(phi) / 2pi (truncation about kSheetFar^-kSheetTerms = 1e-12 of the
-// leading term); inside it, the panel sum as in sheet_on_blobs_kernel.
+// leading term).  Blobs inside it are listed (near_list, near_count) for
+// sheet_on_blobs_near_kernel.
 __global__ __launch_bounds__(256) void sheet_on_blobs_far_kernel(
-    int64_t n, int nl, const double* __restrict__ bx, const double* __restrict__ bz,
-    const PanelConst* __restrict__ pc, const double* __restrict__ gamma,
+    int64_t n, const double* __restrict__ bx, const double* __restrict__ bz,
     const double* __restrict__ ma, double cx, double cz, double rr, double* __restrict__ uvs,
-    double* __restrict__ wvs)
+    double* __restrict__ wvs, int* __restrict__ near_list, int* __restrict__ near_count)
 {
     __shared__ double a[2 * kSheetTerms];
     for (int k = threadIdx.x; k < 2 * kSheetTerms; k += blockDim.x) a[k] = ma[k];
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double xi = bx[i];
-    const double zi = bz[i];
+    const double xi = i < n ? bx[i] : 0.0;
+    const double zi = i < n ? bz[i] : 0.0;
     const double wr = (xi - cx) / rr, wi = (zi - cz) / rr;
     const double w2 = wr * wr + wi * wi;
-    if (w2 > kSheetFar * kSheetFar) {
+    // near blobs: one counter update per wavefront
+    const bool is_near = i < n && !(w2 > kSheetFar * kSheetFar);
+    const uint64_t m = __ballot(is_near);
+    if (m) {
+        const int lane = threadIdx.x & 63;
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(near_count, __popcll(m));
+        base = __shfl(base, leader);
+        if (is_near) near_list[base + __popcll(m & ((uint64_t(1) << lane) - 1))] = (int)i;
+    }
+    if (i >= n || is_near) return;
+    {
         // t = 1 / W; phi * mr = t (A_0 + t (A_1 + ...))
         const double tr = wr / w2, ti = -wi / w2;
         double pr = a[2 * (kSheetTerms - 1)], pim = a[2 * (kSheetTerms - 1) + 1];
@@ -596,13 +624,38 @@ __global__ __launch_bounds__(256) void sheet_on_blobs_far_kernel(
         const double fr = (pr * tr - pim * ti) / rr, fi = (pr * ti + pim * tr) / rr;
         uvs[i] = fi * kRpi2;
         wvs[i] = fr * kRpi2;
-        return;
     }
-    double su = 0.0, sw = 0.0;
-    for (int j = 0; j < nl; ++j)
-        sheet_pair(xi, zi, pc[j], gamma[j], gamma[j + 1 == nl ? 0 : j + 1], su, sw);
-    uvs[i] = su * kRpi2;
-    wvs[i] = sw * kRpi2;
+}
+
+// The panel sum (sheet_pair) for the listed near blobs, one wavefront per
+// blob: lane l sums panels l, l+64, ..., and the lanes' sums are added by
+// shuffles.  A fixed set of wavefronts walks the list (its length is read on
+// the device).
+__global__ __launch_bounds__(256) void sheet_on_blobs_near_kernel(
+    int nl, const double* __restrict__ bx, const double* __restrict__ bz,
+    const PanelConst* __restrict__ pc, const double* __restrict__ gamma,
+    const int* __restrict__ near_list, const int* __restrict__ near_count,
+    double* __restrict__ uvs, double* __restrict__ wvs)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
+    const int cnt = *near_count;
+    for (int q = wave; q < cnt; q += nwaves) {
+        const int i = near_list[q];
+        const double xi = bx[i], zi = bz[i];
+        double su = 0.0, sw = 0.0;
+        for (int j = lane; j < nl; j += 64)
+            sheet_pair(xi, zi, pc[j], gamma[j], gamma[j + 1 == nl ? 0 : j + 1], su, sw);
+        for (int o = 32; o > 0; o >>= 1) {
+            su += __shfl_xor(su, o);
+            sw += __shfl_xor(sw, o);
+        }
+        if (lane == 0) {
+            uvs[i] = su * kRpi2;
+            wvs[i] = sw * kRpi2;
+        }
+    }
 }
 
 void sheet_on_blobs(dvm_ctx_t ctx, const SheetDev& sh, int prec, int64_t n, const double* bx,
@@ -616,10 +669,17 @@ void sheet_on_blobs(dvm_ctx_t ctx, const SheetDev& sh, int prec, int64_t n, cons
         DVM_LAUNCH_CHECK();
         return;
     }
-    sheet_moments_kernel<<<1, 64, 0, ctx->stream>>>(sh.nl, sh.gamma, sh.mI0, sh.mI1, sh.mA);
+    require(n < (int64_t(1) << 31), DVM_E_SIZE, "too many blobs");
+    int* near_list = static_cast<int*>(ws_get(ctx, sizeof(int) * (size_t)n));
+    int* near_count = sh.flag + 1;
+    sheet_moments_kernel<<<kSheetTerms, 256, 0, ctx->stream>>>(sh.nl, sh.gamma, sh.mI0, sh.mI1,
+                                                                sh.mA, near_count);
     DVM_LAUNCH_CHECK();
     sheet_on_blobs_far_kernel<<<cdiv(n, 256), 256, 0, ctx->stream>>>(
-        n, sh.nl, bx, bz, sh.panel, sh.gamma, sh.mA, sh.mx, sh.mz, sh.mr, uvs, wvs);
+        n, bx, bz, sh.mA, sh.mx, sh.mz, sh.mr, uvs, wvs, near_list, near_count);
+    DVM_LAUNCH_CHECK();
+    sheet_on_blobs_near_kernel<<<ctx->num_cus * 8, 256, 0, ctx->stream>>>(
+        sh.nl, bx, bz, sh.panel, sh.gamma, near_list, near_count, uvs, wvs);
     DVM_LAUNCH_CHECK();
 }
 

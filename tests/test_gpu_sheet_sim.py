@@ -413,7 +413,9 @@ def test_f32c_device_rng_run(gpu_ctx):
 def test_sheet_far_field_form(gpu_ctx, body):
     """The fast simulations' sheet-on-blob form: a 40-term Laurent series of
     the sheet beyond two body radii (the panels of src/VortexSheet.cpp:225-272
-    are straight linear-strength vortex segments), the panel sum inside."""
+    are straight linear-strength vortex segments), the panel sum inside (one
+    wavefront per blob, the panels split over the lanes: the same terms added
+    in another order)."""
     px, pz = oracle.cylinder_points(256) if body == "cyl256" else load_body(body)
     _, gp = both_params()
     gsh = dsim.Sheet(gpu_ctx, px, pz, gp.sheet)
@@ -428,6 +430,8 @@ def test_sheet_far_field_form(gpu_ctx, body):
     uf, wf = (t.cpu().numpy() for t in gsh.on_blobs(X, Z, far=True))
     far = r > 1.5 * ext
     assert far.sum() > 10000
-    assert np.array_equal(uf[~far & (r < 0.4 * ext)], ue[~far & (r < 0.4 * ext)])
+    inner = ~far & (r < 0.4 * ext)
+    assert inner.sum() > 100
+    assert rel_err(uf[inner], ue[inner]) <= 1e-13 and rel_err(wf[inner], we[inner]) <= 1e-13
     scale = np.max(np.hypot(ue, we))
     assert np.max(np.abs(uf - ue)) <= 1e-11 * scale and np.max(np.abs(wf - we)) <= 1e-11 * scale

@@ -98,7 +98,14 @@ struct FmmTree {
     int* need;            // 1: a target lies in the box (its locals are needed)
     cplx* A;              // multipole coefficients per box
     cplx* B;              // local coefficients per box
+    int* heavy;           // [0] count, then the heavy targets (fmm_eval_kernel)
 };
+
+// Targets whose near field holds more sources than this are summed by a
+// wavefront each (fmm_heavy_kernel) instead of one lane: dense clusters (the
+// nascent blobs along a body) would otherwise leave a few lanes running long
+// after the rest of the launch.
+constexpr int kFmmHeavy = 2048;
 
 __host__ __device__ __forceinline__ int64_t level_off(int l) { return ((int64_t(1) << (2 * l)) - 1) / 3; }
 
@@ -323,10 +330,32 @@ __global__ void fmm_eval_kernel(FmmTree t, int n, const double4* __restrict__ sp
                                 double* __restrict__ u, double* __restrict__ w)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const double4 me = sp[k];
-    const uint32_t m = skey[k];
+    const bool valid = k < n;
+    const uint32_t m = valid ? skey[k] : 0u;
     const int L = t.L;
+    const int ix = (int)compact_bits(m), iz = (int)compact_bits(m >> 1);
+    const int side = 1 << L;
+    // near-field size of this target's leaf: heavy ones go to fmm_heavy_kernel
+    int work = 0;
+    if (valid)
+        for (int q = 0; q < t.near_n; ++q) {
+            const int sx = ix + t.nearo[2 * q], sz = iz + t.nearo[2 * q + 1];
+            if (sx < 0 || sz < 0 || sx >= side || sz >= side) continue;
+            const uint32_t lm = morton((uint32_t)sx, (uint32_t)sz);
+            work += t.lbeg[lm + 1] - t.lbeg[lm];
+        }
+    const bool heavy = work > kFmmHeavy;
+    const uint64_t hm = __ballot(heavy);
+    if (hm) {
+        const int lane = threadIdx.x & 63;
+        const int leader = __ffsll((unsigned long long)hm) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(t.heavy, __popcll(hm));
+        base = __shfl(base, leader);
+        if (heavy) t.heavy[1 + base + __popcll(hm & ((uint64_t(1) << lane) - 1))] = k;
+    }
+    if (!valid || heavy) return;
+    const double4 me = sp[k];
     const double s = t.S / (double)(1 << L);
     double fu = 0.0, fw = 0.0;
     if (L >= 2) {
@@ -341,8 +370,6 @@ __global__ void fmm_eval_kernel(FmmTree t, int n, const double4* __restrict__ sp
         fw = phi.re / s;
     }
     // near field with the reference's regularised kernel
-    const int ix = (int)compact_bits(m), iz = (int)compact_bits(m >> 1);
-    const int side = 1 << L;
     double nu = 0.0, nw = 0.0;
     for (int q = 0; q < t.near_n; ++q) {
         const int sx = ix + t.nearo[2 * q], sz = iz + t.nearo[2 * q + 1];
@@ -368,6 +395,74 @@ __global__ void fmm_eval_kernel(FmmTree t, int n, const double4* __restrict__ sp
     const int i = perm[k];
     u[i] = (fu + nu) * kRpi2;
     w[i] = (fw + nw) * kRpi2;
+}
+
+// The heavy targets of fmm_eval_kernel, one wavefront each: the lanes split
+// every near leaf's sources (lane l takes l, l+64, ...), their sums are added
+// by shuffles, and lane 0 adds the local expansion (L2P).  A fixed set of
+// wavefronts walks the list.
+template <int P>
+__global__ __launch_bounds__(256) void fmm_heavy_kernel(FmmTree t, const double4* __restrict__ sp,
+                                                        const uint32_t* __restrict__ skey,
+                                                        const int* __restrict__ perm,
+                                                        double* __restrict__ u,
+                                                        double* __restrict__ w)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
+    const int cnt = t.heavy[0];
+    const int L = t.L;
+    const int side = 1 << L;
+    const double s = t.S / (double)side;
+    for (int q = wave; q < cnt; q += nwaves) {
+        const int k = t.heavy[1 + q];
+        const double4 me = sp[k];
+        const uint32_t m = skey[k];
+        const int ix = (int)compact_bits(m), iz = (int)compact_bits(m >> 1);
+        double nu = 0.0, nw = 0.0;
+        for (int qq = 0; qq < t.near_n; ++qq) {
+            const int sx = ix + t.nearo[2 * qq], sz = iz + t.nearo[2 * qq + 1];
+            if (sx < 0 || sz < 0 || sx >= side || sz >= side) continue;
+            const uint32_t lm = morton((uint32_t)sx, (uint32_t)sz);
+            const int b = t.lbeg[lm], e = t.lbeg[lm + 1];
+            for (int j = b + lane; j < e; j += 64) {
+                if (j == k) continue;
+                const double4 q4 = sp[j];
+                const double dx = me.x - q4.x;
+                const double dz = me.y - q4.y;
+                const double r2 = dx * dx + dz * dz;
+                const double xs = r2 * q4.w;
+                double inv = __builtin_amdgcn_rcp(r2);
+                inv = fma(inv, fma(-r2, inv, 1.0), inv);
+                inv = fma(inv, fma(-r2, inv, 1.0), inv);
+                const double dk = xs < 10.0 ? (1.0 - exp(-xs)) * inv : inv;
+                nu -= dk * dz * q4.z;
+                nw += dk * dx * q4.z;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            nu += __shfl_xor(nu, o);
+            nw += __shfl_xor(nw, o);
+        }
+        if (lane == 0) {
+            double fu = 0.0, fw = 0.0;
+            if (L >= 2) {
+                const int64_t bi = level_off(L) + m;
+                const cplx c = box_centre(t, L, m);
+                const cplx wv = {(me.x - c.re) / s, (me.y - c.im) / s};
+                cplx phi = {0.0, 0.0};
+#pragma unroll
+                for (int ll = P - 1; ll >= 0; --ll)
+                    if (ll < t.p) phi = cadd(cmul(phi, wv), t.B[bi * P + ll]);
+                fu = phi.im / s;
+                fw = phi.re / s;
+            }
+            const int i = perm[k];
+            u[i] = (fu + nu) * kRpi2;
+            w[i] = (fw + nw) * kRpi2;
+        }
+    }
 }
 
 // Leaves holding a target need their local expansions.
@@ -467,7 +562,10 @@ void fmm_levels(dvm_ctx_t ctx, const FmmTree& t, int n, const double4* sp, const
     }
     {
         ProfScope ps(ctx, "fmm_eval_kernel");
+        DVM_HIP_CHECK(hipMemsetAsync(t.heavy, 0, sizeof(int), st));
         fmm_eval_kernel<P><<<cdiv(n, 128), 128, 0, st>>>(t, n, sp, skey, perm, u, w);
+        DVM_LAUNCH_CHECK();
+        fmm_heavy_kernel<P><<<ctx->num_cus * 8, 256, 0, st>>>(t, sp, skey, perm, u, w);
         DVM_LAUNCH_CHECK();
     }
     if (nt > 0) {
@@ -630,7 +728,8 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
     const size_t need = al(sizeof(double) * kBinom * kBinom) + al(sizeof(int) * 8 * il_stride) +
                         al(sizeof(int) * nearo.size()) + 2 * al(sizeof(uint32_t) * n) +
                         2 * al(sizeof(int) * n) + al(sizeof(double4) * n) + al(sizeof(int) * (nleaf + 1)) +
-                        2 * al(sizeof(int) * nbox) + 2 * al(sizeof(cplx) * P * nbox) + al(sort_bytes);
+                        2 * al(sizeof(int) * nbox) + 2 * al(sizeof(cplx) * P * nbox) + al(sort_bytes) +
+                        al(sizeof(int) * (n + 1));
     char* p = static_cast<char*>(ws_get(ctx, need));
     auto take = [&](size_t bytes) {
         char* q = p;
@@ -651,6 +750,7 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
     cplx* A = reinterpret_cast<cplx*>(take(sizeof(cplx) * P * nbox));
     cplx* B = reinterpret_cast<cplx*>(take(sizeof(cplx) * P * nbox));
     void* sort_tmp = take(sort_bytes);
+    int* heavy = reinterpret_cast<int*>(take(sizeof(int) * (n + 1)));
 
     // tables: interaction lists padded to il_stride entries per child position
     std::vector<int> ilh(8 * il_stride, 0);
@@ -694,6 +794,7 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
     }
     t.A = A;
     t.B = B;
+    t.heavy = heavy;
     switch (P) {
     case 8: fmm_levels<8>(ctx, t, (int)n, sp, skey, perm, u, w, (int)nt, tx, tz, tu, tw); break;
     case 12: fmm_levels<12>(ctx, t, (int)n, sp, skey, perm, u, w, (int)nt, tx, tz, tu, tw); break;

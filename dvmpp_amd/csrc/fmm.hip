@@ -165,22 +165,18 @@ __device__ __forceinline__ cplx box_centre(const FmmTree& t, int l, uint32_t m)
     return {t.x0 + (compact_bits(m) + 0.5) * s, t.z0 + (compact_bits(m >> 1) + 0.5) * s};
 }
 
-// P2M: one thread per leaf.
+// P2M, one thread per leaf; a leaf of more than kP2mHeavy particles is
+// listed in t.heavy instead and done by fmm_p2m_heavy_kernel (a dense leaf
+// would otherwise hold one lane long after the rest of the launch).
+constexpr int kP2mHeavy = 256;
+
 template <int P>
-__global__ __launch_bounds__(64) void fmm_p2m_kernel(FmmTree t, const double4* __restrict__ sp)
+__device__ __forceinline__ void p2m_accumulate(const FmmTree& t, const double4* __restrict__ sp,
+                                               int64_t m, int j0, int j1, int step, cplx* a)
 {
-    const int64_t nleaf = int64_t(1) << (2 * t.L);
-    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= nleaf) return;
-    const int64_t bi = level_off(t.L) + m;
-    const int b = t.lbeg[m], e = t.lbeg[m + 1];
-    t.cnt[bi] = e - b;
-    cplx a[P];
-#pragma unroll
-    for (int k = 0; k < P; ++k) a[k] = {0.0, 0.0};
     const double s = t.S / (double)(1 << t.L), rs = 1.0 / s;
     const cplx c = box_centre(t, t.L, (uint32_t)m);
-    for (int j = b; j < e; ++j) {
+    for (int j = j0; j < j1; j += step) {
         const double4 q = sp[j];
         const cplx zj = {(q.x - c.re) * rs, (q.y - c.im) * rs};
         cplx pw = {q.z, 0.0};
@@ -190,8 +186,67 @@ __global__ __launch_bounds__(64) void fmm_p2m_kernel(FmmTree t, const double4* _
             pw = cmul(pw, zj);
         }
     }
+}
+
+template <int P>
+__global__ __launch_bounds__(64) void fmm_p2m_kernel(FmmTree t, const double4* __restrict__ sp)
+{
+    const int64_t nleaf = int64_t(1) << (2 * t.L);
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = m < nleaf;
+    const int b = valid ? t.lbeg[m] : 0, e = valid ? t.lbeg[m + 1] : 0;
+    const bool heavy = e - b > kP2mHeavy;
+    const uint64_t hm = __ballot(heavy);
+    if (hm) {
+        const int lane = threadIdx.x & 63;
+        const int leader = __ffsll((unsigned long long)hm) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(t.heavy, __popcll(hm));
+        base = __shfl(base, leader);
+        if (heavy) t.heavy[1 + base + __popcll(hm & ((uint64_t(1) << lane) - 1))] = (int)m;
+    }
+    if (!valid || heavy) return;
+    const int64_t bi = level_off(t.L) + m;
+    t.cnt[bi] = e - b;
+    cplx a[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) a[k] = {0.0, 0.0};
+    p2m_accumulate<P>(t, sp, m, b, e, 1, a);
 #pragma unroll
     for (int k = 0; k < P; ++k) t.A[bi * P + k] = a[k];
+}
+
+// The listed dense leaves, one wavefront each: lane l takes particles l,
+// l+64, ..., and the lanes' sums are added by shuffles.
+template <int P>
+__global__ __launch_bounds__(256) void fmm_p2m_heavy_kernel(FmmTree t, const double4* __restrict__ sp)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
+    const int cnt = t.heavy[0];
+    for (int q = wave; q < cnt; q += nwaves) {
+        const int64_t m = t.heavy[1 + q];
+        const int64_t bi = level_off(t.L) + m;
+        const int b = t.lbeg[m], e = t.lbeg[m + 1];
+        cplx a[P];
+#pragma unroll
+        for (int k = 0; k < P; ++k) a[k] = {0.0, 0.0};
+        p2m_accumulate<P>(t, sp, m, b + lane, e, 64, a);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            if (k >= t.p) continue;
+            for (int o = 32; o > 0; o >>= 1) {
+                a[k].re += __shfl_xor(a[k].re, o);
+                a[k].im += __shfl_xor(a[k].im, o);
+            }
+        }
+        if (lane == 0) {
+            t.cnt[bi] = e - b;
+#pragma unroll
+            for (int k = 0; k < P; ++k) t.A[bi * P + k] = a[k];
+        }
+    }
 }
 
 // M2M into level l from its children at l + 1: one thread per box.
@@ -543,7 +598,10 @@ void fmm_levels(dvm_ctx_t ctx, const FmmTree& t, int n, const double4* sp, const
     const int64_t nleaf = int64_t(1) << (2 * t.L);
     {
         ProfScope ps(ctx, "fmm_p2m_kernel");
+        DVM_HIP_CHECK(hipMemsetAsync(t.heavy, 0, sizeof(int), st));
         fmm_p2m_kernel<P><<<cdiv(nleaf, 64), 64, 0, st>>>(t, sp);
+        DVM_LAUNCH_CHECK();
+        fmm_p2m_heavy_kernel<P><<<ctx->num_cus * 4, 256, 0, st>>>(t, sp);
         DVM_LAUNCH_CHECK();
     }
     {

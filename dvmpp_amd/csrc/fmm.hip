@@ -310,7 +310,10 @@ __device__ __forceinline__ cplx shfl_c(cplx v, int src, int width)
 }
 
 // Level l locals: L2L from the parent (l > 2) plus M2L from the interaction
-// list, one group of G lanes per box with particles; lane l computes B_l.
+// list, one wavefront per box with particles, made of 64/G groups of G lanes;
+// lane l of a group computes B_l.  Group g takes interaction-list entries g,
+// g + 64/G, ... (group 0 also the L2L), and the groups' sums are added by
+// shuffles, so the list is walked 64/G entries at a time.
 //   L2L  B_l  = 2^-(l+1) sum_{m>=l} C(m,l) Bp_m E^(m-l)
 //   M2L  B_l += (-1)^l t^l sum_k C(k+l,l) A_k t^(k+1),  t = 1/T
 // (A_k reaches lane l by shuffles; t^(l+1) by a prefix product over lanes).
@@ -318,17 +321,19 @@ template <int P>
 __global__ __launch_bounds__(256) void fmm_down_kernel(FmmTree t, int l)
 {
     constexpr int G = Lanes<P>::G;
+    constexpr int NG = 64 / G;
     const int64_t nbox = int64_t(1) << (2 * l);
-    const int64_t m = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const int64_t m = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & (G - 1);
-    if (m >= nbox) return;  // whole groups leave together
+    const int grp = (int)((threadIdx.x & 63) / G);
+    if (m >= nbox) return;  // whole wavefronts leave together
     const int64_t bi = level_off(l) + m;
     if (t.cnt[bi] == 0 && t.need[bi] == 0) return;
     const int p = t.p;
     const bool act = lane < p;
     cplx b = {0.0, 0.0};
     const int ix = (int)compact_bits((uint32_t)m), iz = (int)compact_bits((uint32_t)m >> 1);
-    if (l > 2) {
+    if (l > 2 && grp == 0) {
         const int64_t pi = level_off(l - 1) + (m >> 2);
         const int ch = (int)(m & 3);
         const cplx e = {(ch & 1) ? 0.25 : -0.25, (ch & 2) ? 0.25 : -0.25};
@@ -350,7 +355,7 @@ __global__ __launch_bounds__(256) void fmm_down_kernel(FmmTree t, int l)
     const int side = 1 << l;
     const int pos = (ix & 1) | ((iz & 1) << 1);
     const int* il = t.il + pos * t.il_stride;
-    for (int q = 0; q < t.il_n[pos]; ++q) {
+    for (int q = grp; q < t.il_n[pos]; q += NG) {
         const int sx = ix + il[2 * q], sz = iz + il[2 * q + 1];
         if (sx < 0 || sz < 0 || sx >= side || sz >= side) continue;
         const int64_t si = level_off(l) + morton((uint32_t)sx, (uint32_t)sz);
@@ -375,7 +380,11 @@ __global__ __launch_bounds__(256) void fmm_down_kernel(FmmTree t, int l)
         acc = cmul(acc, cmul(pw, T));
         b = (lane & 1) ? cadd(b, cscale(acc, -1.0)) : cadd(b, acc);
     }
-    if (lane < P) t.B[bi * P + lane] = act ? b : cplx{0.0, 0.0};
+    for (int o = G; o < 64; o <<= 1) {
+        b.re += __shfl_xor(b.re, o);
+        b.im += __shfl_xor(b.im, o);
+    }
+    if (grp == 0 && lane < P) t.B[bi * P + lane] = act ? b : cplx{0.0, 0.0};
 }
 
 // L2P + P2P: one thread per particle in sorted order.
@@ -614,7 +623,7 @@ void fmm_levels(dvm_ctx_t ctx, const FmmTree& t, int n, const double4* sp, const
     {
         ProfScope ps(ctx, "fmm_down_kernel");
         for (int l = 2; l <= t.L; ++l) {
-            fmm_down_kernel<P><<<cdiv((int64_t(1) << (2 * l)) * Lanes<P>::G, 256), 256, 0, st>>>(t, l);
+            fmm_down_kernel<P><<<cdiv((int64_t(1) << (2 * l)) * 64, 256), 256, 0, st>>>(t, l);
             DVM_LAUNCH_CHECK();
         }
     }

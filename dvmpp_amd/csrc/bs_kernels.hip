@@ -594,9 +594,11 @@ static void launch_f64(dvm_ctx_t ctx, bool self, int64_t ns, const double* sx, c
                        const double* tz, int guard, double r2_skip, double* u, double* w)
 {
     const int64_t bx = cdiv(nt, kBlock);
-    // Keep chunks of >= 2048 sources; a single chunk keeps the reference's
-    // exact summation order.
-    const int nch = pick_chunks(ctx, bx, (ns + 2047) / 2048);
+    // Keep chunks of >= 2048 sources (>= 512 when the targets fill at most
+    // 16 blocks — collocation points, probes — so that enough blocks run); a
+    // single chunk keeps the reference's exact summation order.
+    const int64_t min_chunk = bx <= 16 ? 512 : 2048;
+    const int nch = pick_chunks(ctx, bx, (ns + min_chunk - 1) / min_chunk);
     const int64_t chunk = (ns + nch - 1) / nch;
     double* part = nullptr;
     if (nch > 1) part = static_cast<double*>(ws_get(ctx, sizeof(double) * 2 * nt * nch));

@@ -104,6 +104,11 @@ struct BatchDev {
     double* rhs = nullptr;      // S * (mp+1) right-hand sides of the gamma solve
     int* slab_off = nullptr;    // S * (mp+1)
     int* slab_edge = nullptr;   // S * kSlabCap*mp
+    // FP32C sheet sum by panel groups (sheet_groups): mg groups per slot
+    int mg = 0;
+    double4* gcen = nullptr;    // S * mg
+    double *gI0 = nullptr, *gI1 = nullptr;  // S * mp * 2*kSheetTerms
+    double* gA = nullptr;       // S * mg * 2*kSheetTerms moments of the current gamma
 };
 
 // ----------------------------------------------------------------- work lists
@@ -609,6 +614,84 @@ __global__ __launch_bounds__(256) void bt_sheet_kernel(int S, int64_t cap, int m
     });
 }
 
+// FP32C form of the sheet's velocity by panel groups (sheet_groups): moments
+// of the current gamma per (group, term), A_k = sum over the group's panels of
+// -(gamma_j I0_jk + gamma_{j+1} I1_jk), summed in panel order.
+__global__ __launch_bounds__(64) void bt_group_moments_kernel(int mp, int mg,
+                                                              const SlotParam* __restrict__ prm,
+                                                              const SlotState* __restrict__ st,
+                                                              const double* __restrict__ gamma,
+                                                              const double* __restrict__ i0,
+                                                              const double* __restrict__ i1,
+                                                              double* __restrict__ ga)
+{
+    const int g = blockIdx.x, s = blockIdx.y, k = threadIdx.x;
+    if (!st[s].active || k >= kSheetTerms) return;
+    const int nl = prm[s].nl;
+    const int j0 = g * kSheetGroup;
+    if (j0 >= nl) return;
+    const int j1 = min(j0 + kSheetGroup, nl);
+    const double* gm = gamma + (int64_t)s * mp;
+    double ar = 0.0, ai = 0.0;
+    for (int j = j0; j < j1; ++j) {
+        const double g0 = gm[j], g1 = gm[j + 1 == nl ? 0 : j + 1];
+        const int64_t o = 2 * (((int64_t)s * mp + j) * kSheetTerms + k);
+        ar -= g0 * i0[o] + g1 * i1[o];
+        ai -= g0 * i0[o + 1] + g1 * i1[o + 1];
+    }
+    const int64_t w = 2 * (((int64_t)s * mg + g) * kSheetTerms + k);
+    ga[w] = ar;
+    ga[w + 1] = ai;
+}
+
+// The FP32C sheet velocity of every blob, in each slot's Morton order (the
+// sorted positions p64 and the permutation perm of the pair sum's sort, so a
+// wavefront's blobs sit close together): per panel group, the Laurent series
+// (phi = (1/r) sum_k A_k / W^(k+1), W = (z - c) / r; u += Im phi, w += Re phi)
+// when the blob lies beyond kSheetFar * r of the group centre, else
+// sheet_pair over the group's panels.  Each blob's choice depends on its own
+// position only.
+__global__ __launch_bounds__(256) void bt_sheet_groups_kernel(
+    int S, int64_t cap, int mp, int mg, const WorkList* wl, const int64_t* __restrict__ blob_pre,
+    const SlotParam* __restrict__ prm, const uint32_t* __restrict__ perm,
+    const double2* __restrict__ p64, const PanelConst* __restrict__ panel,
+    const double* __restrict__ gamma, const double4* __restrict__ gcen,
+    const double* __restrict__ ga, BlobBuf b)
+{
+    for_each_blob(wl, blob_pre, S, cap, [&](int s, int64_t o) {
+        const double2 p = p64[o];  // o = s cap + sorted rank
+        const int nl = prm[s].nl;
+        const int ng = (nl + kSheetGroup - 1) / kSheetGroup;
+        const PanelConst* pc = panel + (int64_t)s * mp;
+        const double* gm = gamma + (int64_t)s * mp;
+        double su = 0.0, sw = 0.0;
+        for (int g = 0; g < ng; ++g) {
+            const double4 c = gcen[(int64_t)s * mg + g];
+            const double wr = (p.x - c.x) / c.z, wi = (p.y - c.y) / c.z;
+            const double w2 = wr * wr + wi * wi;
+            if (w2 > kSheetFar * kSheetFar) {
+                const double* a = ga + 2 * ((int64_t)s * mg + g) * kSheetTerms;
+                const double tr = wr / w2, ti = -wi / w2;  // 1 / W
+                double pr = a[2 * (kSheetTerms - 1)], pim = a[2 * (kSheetTerms - 1) + 1];
+                for (int k = kSheetTerms - 2; k >= 0; --k) {
+                    const double nr = pr * tr - pim * ti + a[2 * k];
+                    pim = pr * ti + pim * tr + a[2 * k + 1];
+                    pr = nr;
+                }
+                su += (pr * ti + pim * tr) / c.z;
+                sw += (pr * tr - pim * ti) / c.z;
+            } else {
+                const int j1 = min((g + 1) * kSheetGroup, nl);
+                for (int j = g * kSheetGroup; j < j1; ++j)
+                    sheet_pair(p.x, p.y, pc[j], gm[j], gm[j + 1 == nl ? 0 : j + 1], su, sw);
+            }
+        }
+        const int64_t dst = (int64_t)s * cap + perm[o];
+        b.uvs[dst] = su * kRpi2;
+        b.wvs[dst] = sw * kRpi2;
+    });
+}
+
 // ----------------------------------------------------------------- blob updates
 
 __global__ void bt_zero_q_kernel(int S, int64_t cap, const WorkList* wl,
@@ -988,6 +1071,12 @@ size_t batch_bytes(const dvm_batch_config& c, BatchDev* d, char* mem)
     o.rhs = cv.take<double>(S * (mp + 1));
     o.slab_off = cv.take<int>(S * (mp + 1));
     o.slab_edge = cv.take<int>(S * kSlabCap * mp);
+    const size_t mg = (mp + kSheetGroup - 1) / kSheetGroup;
+    o.mg = (int)mg;
+    o.gcen = cv.take<double4>(f32c ? S * mg : 0);
+    o.gI0 = cv.take<double>(f32c ? np * 2 * kSheetTerms : 0);
+    o.gI1 = cv.take<double>(f32c ? np * 2 * kSheetTerms : 0);
+    o.gA = cv.take<double>(f32c ? S * mg * 2 * kSheetTerms : 0);
     return cv.off + 256;
 }
 
@@ -1029,7 +1118,16 @@ void record_convect(dvm_batch_t B)
         bt_rhs_kernel<<<S, 256, 0, st>>>(mp, d.prm, d.st, d.enx, d.enz, d.col, d.rhs);
         bt_gamma_kernel<<<dim3(kGammaBlocks, S), 256, sizeof(double) * (mp + 1), st>>>(
             mp, d.prm, d.st, d.P, d.rhs, d.gamma);
-        bt_sheet_kernel<<<gs, 256, 0, st>>>(S, cap, mp, d.wl, d.blob_pre, d.prm, d.b, d.panel, d.gamma);
+        if (f32c) {
+            bt_group_moments_kernel<<<dim3(d.mg, S), 64, 0, st>>>(mp, d.mg, d.prm, d.st, d.gamma,
+                                                                  d.gI0, d.gI1, d.gA);
+            bt_sheet_groups_kernel<<<gs, 256, 0, st>>>(S, cap, mp, d.mg, d.wl, d.blob_pre, d.prm,
+                                                       d.perm, d.p64, d.panel, d.gamma, d.gcen,
+                                                       d.gA, d.b);
+        } else {
+            bt_sheet_kernel<<<gs, 256, 0, st>>>(S, cap, mp, d.wl, d.blob_pre, d.prm, d.b, d.panel,
+                                                d.gamma);
+        }
         bt_advect_kernel<<<gs, 256, 0, st>>>(S, cap, d.wl, d.blob_pre, d.prm, rk ? 1 : 0, a[k], bb[k],
                                              d.b, d.q1, d.q2);
     }
@@ -1243,6 +1341,15 @@ dvm_status dvm_batch_admit(dvm_batch_t B, int npts, const double* px, const doub
         d2d(d.panel + (size_t)s * mp, sh.panel, sizeof(PanelConst) * nl);
         d2d(d.P + (size_t)s * mp * (mp + 1), sh.P, db * nl * (nl + 1));
         DVM_HIP_CHECK(hipMemsetAsync(d.gamma + (size_t)s * mp, 0, db * nl, st));
+        std::vector<double> gcen, gi0, gi1;  // live until the synchronisation below
+        if (B->cfg.prec == DVM_F32C) {
+            sheet_groups(sh, gcen, gi0, gi1);
+            DVM_HIP_CHECK(hipMemcpyAsync(d.gcen + (size_t)s * d.mg, gcen.data(), db * gcen.size(),
+                                         hipMemcpyHostToDevice, st));
+            const size_t mo = (size_t)s * mp * 2 * kSheetTerms;
+            DVM_HIP_CHECK(hipMemcpyAsync(d.gI0 + mo, gi0.data(), db * gi0.size(), hipMemcpyHostToDevice, st));
+            DVM_HIP_CHECK(hipMemcpyAsync(d.gI1 + mo, gi1.data(), db * gi1.size(), hipMemcpyHostToDevice, st));
+        }
         SlotParam sp{};
         sp.rho = p->rho; sp.nu = p->nu; sp.dt = p->dt; sp.ux = p->ux; sp.uz = p->uz;
         sp.ur = std::sqrt(p->ux * p->ux + p->uz * p->uz);

@@ -209,6 +209,17 @@ struct SheetDev {
 constexpr int kSheetTerms = 40;
 constexpr double kSheetFar = 2.0;
 
+// Panel groups of the batched fast sheet sum: kSheetGroup consecutive panels,
+// with centre c (mean of the segments' midpoints) and radius r (largest
+// distance from c to a segment end).  Beyond kSheetFar * r of c a group's
+// velocity is the Laurent series above with the group's own moments.
+// sheet_groups fills cen (cx, cz, r, 0 per group; r = inf for a degenerate
+// group, which is always summed panel by panel) and the per-panel geometric
+// integrals i0, i1 about each panel's group centre (the layout of mI0, mI1).
+constexpr int kSheetGroup = 16;
+void sheet_groups(const SheetDev& sh, std::vector<double>& cen, std::vector<double>& i0,
+                  std::vector<double>& i1);
+
 void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const double* pz,
                 const dvm_sheet_params* p);
 void sheet_free(SheetDev& sh);

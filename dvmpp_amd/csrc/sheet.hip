@@ -186,6 +186,79 @@ static void gauss_legendre(int n, double* x, double* w)
     }
 }
 
+// Geometric moments of one panel as sheet_pair places it (the segment from
+// (xc, zc) along theta, length ds) about centre (cx, cz), radius r: for
+// k < kSheetTerms, i0[k] = int (1 - s/ds) W^k ds and i1[k] = int (s/ds) W^k ds
+// with W = (zeta(s) - c) / r (complex, interleaved re/im), by 24-point
+// Gauss-Legendre quadrature, exact for these polynomials up to degree 47.
+static void panel_moments(const PanelConst& q, double cx, double cz, double r, double* i0,
+                          double* i1)
+{
+    struct Rule {
+        double x[24], w[24];
+        Rule() { gauss_legendre(24, x, w); }
+    };
+    static const Rule gl;
+    const double ds = q.ds;
+    for (int k = 0; k < 2 * kSheetTerms; ++k) i0[k] = i1[k] = 0.0;
+    for (int n = 0; n < 24; ++n) {
+        const double t = 0.5 * (gl.x[n] + 1.0);  // s / ds
+        const double wq = 0.5 * gl.w[n] * ds;
+        const double zr = (q.xc + t * ds * q.cs - cx) / r;
+        const double zi = (q.zc + t * ds * q.sn - cz) / r;
+        double pr = 1.0, pi = 0.0;
+        for (int k = 0; k < kSheetTerms; ++k) {
+            i0[2 * k] += wq * (1.0 - t) * pr;
+            i0[2 * k + 1] += wq * (1.0 - t) * pi;
+            i1[2 * k] += wq * t * pr;
+            i1[2 * k + 1] += wq * t * pi;
+            const double nr = pr * zr - pi * zi;
+            pi = pr * zi + pi * zr;
+            pr = nr;
+        }
+    }
+}
+
+void sheet_groups(const SheetDev& sh, std::vector<double>& cen, std::vector<double>& i0,
+                  std::vector<double>& i1)
+{
+    const int nl = sh.nl;
+    const int ng = (nl + kSheetGroup - 1) / kSheetGroup;
+    cen.assign(4 * (size_t)ng, 0.0);
+    i0.assign(2 * (size_t)nl * kSheetTerms, 0.0);
+    i1.assign(2 * (size_t)nl * kSheetTerms, 0.0);
+    std::vector<PanelConst> pc(nl);
+    for (int j = 0; j < nl; ++j) {
+        const double th = sh.htheta[j];
+        pc[j] = PanelConst{sh.hxc[j], sh.hzc[j], sh.hds[j], std::cos(th), std::sin(th), 0.0, 0.0, 0.0};
+    }
+    for (int g = 0; g < ng; ++g) {
+        const int j0 = g * kSheetGroup, j1 = std::min(j0 + kSheetGroup, nl);
+        double cx = 0.0, cz = 0.0;
+        for (int j = j0; j < j1; ++j) {
+            cx += pc[j].xc + 0.5 * pc[j].ds * pc[j].cs;
+            cz += pc[j].zc + 0.5 * pc[j].ds * pc[j].sn;
+        }
+        cx /= (j1 - j0);
+        cz /= (j1 - j0);
+        double r = 0.0;
+        for (int j = j0; j < j1; ++j) {
+            const double ax = pc[j].xc - cx, az = pc[j].zc - cz;
+            const double bx = ax + pc[j].ds * pc[j].cs, bz = az + pc[j].ds * pc[j].sn;
+            r = std::max(r, std::max(std::hypot(ax, az), std::hypot(bx, bz)));
+        }
+        // a degenerate or non-finite group never takes the series
+        const bool ok = r > 0.0 && std::isfinite(r) && std::isfinite(cx) && std::isfinite(cz);
+        cen[4 * (size_t)g] = ok ? cx : 0.0;
+        cen[4 * (size_t)g + 1] = ok ? cz : 0.0;
+        cen[4 * (size_t)g + 2] = ok ? r : INFINITY;
+        if (!ok) continue;
+        for (int j = j0; j < j1; ++j)
+            panel_moments(pc[j], cx, cz, r, &i0[2 * (size_t)j * kSheetTerms],
+                          &i1[2 * (size_t)j * kSheetTerms]);
+    }
+}
+
 BodyBox body_box(int npts, const double* px, const double* pz)
 {
     for (int i = 0; i < npts; ++i)
@@ -312,30 +385,9 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
     }
     if (!(sh.mr > 0.0)) sh.mr = 1.0;
     std::vector<double> mi0(2 * (size_t)nl * kSheetTerms), mi1(2 * (size_t)nl * kSheetTerms);
-    {
-        double gx[24], gw[24];
-        gauss_legendre(24, gx, gw);
-        for (int i = 0; i < nl; ++i) {
-            const double ds = pc[i].ds;
-            for (int q = 0; q < 24; ++q) {
-                const double t = 0.5 * (gx[q] + 1.0);  // s / ds
-                const double wq = 0.5 * gw[q] * ds;
-                const double zr = (pc[i].xc + t * ds * pc[i].cs - sh.mx) / sh.mr;
-                const double zi = (pc[i].zc + t * ds * pc[i].sn - sh.mz) / sh.mr;
-                double pr = 1.0, pi = 0.0;
-                for (int k = 0; k < kSheetTerms; ++k) {
-                    const size_t o = 2 * ((size_t)i * kSheetTerms + k);
-                    mi0[o] += wq * (1.0 - t) * pr;
-                    mi0[o + 1] += wq * (1.0 - t) * pi;
-                    mi1[o] += wq * t * pr;
-                    mi1[o + 1] += wq * t * pi;
-                    const double nr = pr * zr - pi * zi;
-                    pi = pr * zi + pi * zr;
-                    pr = nr;
-                }
-            }
-        }
-    }
+    for (int i = 0; i < nl; ++i)
+        panel_moments(pc[i], sh.mx, sh.mz, sh.mr, &mi0[2 * (size_t)i * kSheetTerms],
+                      &mi1[2 * (size_t)i * kSheetTerms]);
     const size_t d = sizeof(double);
     const size_t nred = 3 * (size_t)nl > 1024 ? 3 * (size_t)nl : 1024;
     const size_t nm = 2 * (size_t)nl * kSheetTerms;

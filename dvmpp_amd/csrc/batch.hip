@@ -1341,14 +1341,14 @@ dvm_status dvm_batch_admit(dvm_batch_t B, int npts, const double* px, const doub
         d2d(d.panel + (size_t)s * mp, sh.panel, sizeof(PanelConst) * nl);
         d2d(d.P + (size_t)s * mp * (mp + 1), sh.P, db * nl * (nl + 1));
         DVM_HIP_CHECK(hipMemsetAsync(d.gamma + (size_t)s * mp, 0, db * nl, st));
-        std::vector<double> gcen, gi0, gi1;  // live until the synchronisation below
+        std::vector<double> gcen;  // live until the synchronisation below
         if (B->cfg.prec == DVM_F32C) {
-            sheet_groups(sh, gcen, gi0, gi1);
+            sheet_groups(sh, gcen);
             DVM_HIP_CHECK(hipMemcpyAsync(d.gcen + (size_t)s * d.mg, gcen.data(), db * gcen.size(),
                                          hipMemcpyHostToDevice, st));
             const size_t mo = (size_t)s * mp * 2 * kSheetTerms;
-            DVM_HIP_CHECK(hipMemcpyAsync(d.gI0 + mo, gi0.data(), db * gi0.size(), hipMemcpyHostToDevice, st));
-            DVM_HIP_CHECK(hipMemcpyAsync(d.gI1 + mo, gi1.data(), db * gi1.size(), hipMemcpyHostToDevice, st));
+            panel_moments(st, nl, d.panel + (size_t)s * mp, 0.0, 0.0, 1.0, d.gcen + (size_t)s * d.mg,
+                          d.gI0 + mo, d.gI1 + mo);
         }
         SlotParam sp{};
         sp.rho = p->rho; sp.nu = p->nu; sp.dt = p->dt; sp.ux = p->ux; sp.uz = p->uz;

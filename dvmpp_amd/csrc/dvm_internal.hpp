@@ -214,11 +214,14 @@ constexpr double kSheetFar = 2.0;
 // distance from c to a segment end).  Beyond kSheetFar * r of c a group's
 // velocity is the Laurent series above with the group's own moments.
 // sheet_groups fills cen (cx, cz, r, 0 per group; r = inf for a degenerate
-// group, which is always summed panel by panel) and the per-panel geometric
-// integrals i0, i1 about each panel's group centre (the layout of mI0, mI1).
+// group, which is always summed panel by panel).
 constexpr int kSheetGroup = 16;
-void sheet_groups(const SheetDev& sh, std::vector<double>& cen, std::vector<double>& i0,
-                  std::vector<double>& i1);
+void sheet_groups(const SheetDev& sh, std::vector<double>& cen);
+// Geometric integrals (the layout of mI0, mI1) of the nl panels pc on stream
+// st: about (cx, cz) with radius r, or, when gcen is given, about each
+// panel's group centre gcen[i / kSheetGroup].
+void panel_moments(hipStream_t st, int nl, const PanelConst* pc, double cx, double cz, double r,
+                   const double4* gcen, double* i0, double* i1);
 
 void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const double* pz,
                 const dvm_sheet_params* p);

@@ -186,65 +186,109 @@ static void gauss_legendre(int n, double* x, double* w)
     }
 }
 
-// Geometric moments of one panel as sheet_pair places it (the segment from
-// (xc, zc) along theta, length ds) about centre (cx, cz), radius r: for
-// k < kSheetTerms, i0[k] = int (1 - s/ds) W^k ds and i1[k] = int (s/ds) W^k ds
-// with W = (zeta(s) - c) / r (complex, interleaved re/im), by 24-point
-// Gauss-Legendre quadrature, exact for these polynomials up to degree 47.
-static void panel_moments(const PanelConst& q, double cx, double cz, double r, double* i0,
-                          double* i1)
+struct GlRule24 {
+    double x[24], w[24];
+};
+
+static const GlRule24& gl_rule24()
 {
-    struct Rule {
-        double x[24], w[24];
-        Rule() { gauss_legendre(24, x, w); }
-    };
-    static const Rule gl;
-    const double ds = q.ds;
-    for (int k = 0; k < 2 * kSheetTerms; ++k) i0[k] = i1[k] = 0.0;
-    for (int n = 0; n < 24; ++n) {
-        const double t = 0.5 * (gl.x[n] + 1.0);  // s / ds
-        const double wq = 0.5 * gl.w[n] * ds;
-        const double zr = (q.xc + t * ds * q.cs - cx) / r;
-        const double zi = (q.zc + t * ds * q.sn - cz) / r;
-        double pr = 1.0, pi = 0.0;
+    static const GlRule24 r = [] {
+        GlRule24 g;
+        gauss_legendre(24, g.x, g.w);
+        return g;
+    }();
+    return r;
+}
+
+// Geometric moments of the panels as sheet_pair places them (panel i is the
+// segment from (xc, zc) along theta, length ds) about a centre c and radius r:
+// for k < kSheetTerms, i0[i][k] = int (1 - s/ds) W^k ds and
+// i1[i][k] = int (s/ds) W^k ds with W = (zeta(s) - c) / r (complex,
+// interleaved re/im), by 24-point Gauss-Legendre quadrature, exact for these
+// polynomials up to degree 47.  c, r are (cx, cz, r), or with gcen the centre
+// of panel i's group (r = inf: a degenerate group, moments zero).  One
+// workgroup per panel: lanes q < 24 form the powers at their quadrature point,
+// lanes k < kSheetTerms add the weighted powers over the points in order.
+__global__ __launch_bounds__(64) void panel_moments_kernel(int nl, const PanelConst* __restrict__ pc,
+                                                           double cx, double cz, double r,
+                                                           const double4* __restrict__ gcen,
+                                                           GlRule24 gl, double* __restrict__ i0,
+                                                           double* __restrict__ i1)
+{
+    __shared__ double pw[24][2 * kSheetTerms];
+    __shared__ double wt[24][2];
+    const int i = blockIdx.x, t = threadIdx.x;
+    if (i >= nl) return;
+    if (gcen) {
+        const double4 c = gcen[i / kSheetGroup];
+        cx = c.x;
+        cz = c.y;
+        r = c.z;
+    }
+    const bool ok = r < INFINITY;
+    const PanelConst q = pc[i];
+    if (t < 24) {
+        const double tt = 0.5 * (gl.x[t] + 1.0);  // s / ds
+        const double wq = 0.5 * gl.w[t] * q.ds;
+        wt[t][0] = wq * (1.0 - tt);
+        wt[t][1] = wq * tt;
+        const double zr = (q.xc + tt * q.ds * q.cs - cx) / r;
+        const double zi = (q.zc + tt * q.ds * q.sn - cz) / r;
+        double pr = 1.0, pim = 0.0;
         for (int k = 0; k < kSheetTerms; ++k) {
-            i0[2 * k] += wq * (1.0 - t) * pr;
-            i0[2 * k + 1] += wq * (1.0 - t) * pi;
-            i1[2 * k] += wq * t * pr;
-            i1[2 * k + 1] += wq * t * pi;
-            const double nr = pr * zr - pi * zi;
-            pi = pr * zi + pi * zr;
+            pw[t][2 * k] = pr;
+            pw[t][2 * k + 1] = pim;
+            const double nr = pr * zr - pim * zi;
+            pim = pr * zi + pim * zr;
             pr = nr;
         }
     }
+    __syncthreads();
+    if (t < kSheetTerms) {
+        double a0r = 0.0, a0i = 0.0, a1r = 0.0, a1i = 0.0;
+        if (ok) {
+            for (int n = 0; n < 24; ++n) {
+                a0r += wt[n][0] * pw[n][2 * t];
+                a0i += wt[n][0] * pw[n][2 * t + 1];
+                a1r += wt[n][1] * pw[n][2 * t];
+                a1i += wt[n][1] * pw[n][2 * t + 1];
+            }
+        }
+        const size_t o = 2 * ((size_t)i * kSheetTerms + t);
+        i0[o] = a0r;
+        i0[o + 1] = a0i;
+        i1[o] = a1r;
+        i1[o + 1] = a1i;
+    }
 }
 
-void sheet_groups(const SheetDev& sh, std::vector<double>& cen, std::vector<double>& i0,
-                  std::vector<double>& i1)
+void panel_moments(hipStream_t st, int nl, const PanelConst* pc, double cx, double cz, double r,
+                   const double4* gcen, double* i0, double* i1)
+{
+    if (nl <= 0) return;
+    panel_moments_kernel<<<nl, 64, 0, st>>>(nl, pc, cx, cz, r, gcen, gl_rule24(), i0, i1);
+    DVM_LAUNCH_CHECK();
+}
+
+void sheet_groups(const SheetDev& sh, std::vector<double>& cen)
 {
     const int nl = sh.nl;
     const int ng = (nl + kSheetGroup - 1) / kSheetGroup;
     cen.assign(4 * (size_t)ng, 0.0);
-    i0.assign(2 * (size_t)nl * kSheetTerms, 0.0);
-    i1.assign(2 * (size_t)nl * kSheetTerms, 0.0);
-    std::vector<PanelConst> pc(nl);
-    for (int j = 0; j < nl; ++j) {
-        const double th = sh.htheta[j];
-        pc[j] = PanelConst{sh.hxc[j], sh.hzc[j], sh.hds[j], std::cos(th), std::sin(th), 0.0, 0.0, 0.0};
-    }
     for (int g = 0; g < ng; ++g) {
         const int j0 = g * kSheetGroup, j1 = std::min(j0 + kSheetGroup, nl);
         double cx = 0.0, cz = 0.0;
         for (int j = j0; j < j1; ++j) {
-            cx += pc[j].xc + 0.5 * pc[j].ds * pc[j].cs;
-            cz += pc[j].zc + 0.5 * pc[j].ds * pc[j].sn;
+            cx += sh.hxc[j] + 0.5 * sh.hds[j] * std::cos(sh.htheta[j]);
+            cz += sh.hzc[j] + 0.5 * sh.hds[j] * std::sin(sh.htheta[j]);
         }
         cx /= (j1 - j0);
         cz /= (j1 - j0);
         double r = 0.0;
         for (int j = j0; j < j1; ++j) {
-            const double ax = pc[j].xc - cx, az = pc[j].zc - cz;
-            const double bx = ax + pc[j].ds * pc[j].cs, bz = az + pc[j].ds * pc[j].sn;
+            const double ax = sh.hxc[j] - cx, az = sh.hzc[j] - cz;
+            const double bx = ax + sh.hds[j] * std::cos(sh.htheta[j]);
+            const double bz = az + sh.hds[j] * std::sin(sh.htheta[j]);
             r = std::max(r, std::max(std::hypot(ax, az), std::hypot(bx, bz)));
         }
         // a degenerate or non-finite group never takes the series
@@ -252,10 +296,6 @@ void sheet_groups(const SheetDev& sh, std::vector<double>& cen, std::vector<doub
         cen[4 * (size_t)g] = ok ? cx : 0.0;
         cen[4 * (size_t)g + 1] = ok ? cz : 0.0;
         cen[4 * (size_t)g + 2] = ok ? r : INFINITY;
-        if (!ok) continue;
-        for (int j = j0; j < j1; ++j)
-            panel_moments(pc[j], cx, cz, r, &i0[2 * (size_t)j * kSheetTerms],
-                          &i1[2 * (size_t)j * kSheetTerms]);
     }
 }
 
@@ -384,10 +424,6 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
         sh.mr = std::max(sh.mr, std::max(std::hypot(ax, az), std::hypot(bx, bz)));
     }
     if (!(sh.mr > 0.0)) sh.mr = 1.0;
-    std::vector<double> mi0(2 * (size_t)nl * kSheetTerms), mi1(2 * (size_t)nl * kSheetTerms);
-    for (int i = 0; i < nl; ++i)
-        panel_moments(pc[i], sh.mx, sh.mz, sh.mr, &mi0[2 * (size_t)i * kSheetTerms],
-                      &mi1[2 * (size_t)i * kSheetTerms]);
     const size_t d = sizeof(double);
     const size_t nred = 3 * (size_t)nl > 1024 ? 3 * (size_t)nl : 1024;
     const size_t nm = 2 * (size_t)nl * kSheetTerms;
@@ -451,8 +487,7 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
     DVM_HIP_CHECK(hipMemcpyAsync(geo, g.data(), d * nl * 8, hipMemcpyHostToDevice, st));
     DVM_HIP_CHECK(hipMemcpyAsync(sh.panel, pc.data(), sizeof(PanelConst) * nl,
                                  hipMemcpyHostToDevice, st));
-    DVM_HIP_CHECK(hipMemcpyAsync(sh.mI0, mi0.data(), d * nm, hipMemcpyHostToDevice, st));
-    DVM_HIP_CHECK(hipMemcpyAsync(sh.mI1, mi1.data(), d * nm, hipMemcpyHostToDevice, st));
+    panel_moments(st, nl, sh.panel, sh.mx, sh.mz, sh.mr, nullptr, sh.mI0, sh.mI1);
     DVM_HIP_CHECK(hipMemsetAsync(sh.gamma, 0, d * nl, st));
     DVM_HIP_CHECK(hipMemsetAsync(sh.scal, 0, d * 8, st));
 

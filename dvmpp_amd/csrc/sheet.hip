@@ -86,55 +86,227 @@ __global__ void gj_build_kernel(int n, const double* __restrict__ A, double* __r
     }
 }
 
-constexpr int kGjThreads = 256;
-constexpr int kGjMaxRows = 4096;  // at most 4096 panels per body
+constexpr int kGjMaxRows = 4096;   // at most 4096 panels per body
+constexpr int kGjThreads = 512;
+constexpr int kGjStrip = 17408;    // doubles of the LDS strip: n rows x (KB + 1)
+constexpr int kGjRows = 8;         // rows per workgroup of the trailing update
 
-// Step k: every workgroup finds the same pivot row p, workgroup 0 records
-// it, and each workgroup eliminates column k from its rows (r = blockIdx.x +
-// m gridDim.x, r != p) over columns k+1 .. 2n.
-__global__ __launch_bounds__(kGjThreads) void gj_step_kernel(int n, int k, double* __restrict__ G,
-                                                            int* __restrict__ pivrow)
+// Blocked form of the elimination, KB pivot columns per pass.  Within a pass
+// the steps only touch the strip of columns k0 .. k0+KB-1 and the trailing
+// columns; column k is not written after step k, so the multiplier of row r
+// at step k is f(r, k) = G[r][k] / G[p_k][k] from the final strip.  The value
+// of pivot row p_m when it is used, U_m = T0[p_m] - sum_{c<m} f(p_m, c) U_c,
+// and then every row r, T[r] -= f(r, m) U_m for m = 0 .. KB-1 (m with
+// p_m != r), give the trailing columns the same operations, in the same order,
+// as KB single steps.
+
+// Pivot-candidate order: larger |value| first, then the lower row; n (no
+// row) last.
+__device__ __forceinline__ void gj_better(double ov, int oi, int n, double& bv, int& bi)
 {
+    if (oi < n && (bi == n || ov > bv || (ov == bv && oi < bi))) {
+        bv = ov;
+        bi = oi;
+    }
+}
+
+// The pass's pivot steps on the strip (in LDS, row stride KB + 1), one
+// workgroup, a thread per row (rows t, t + kGjThreads, ...; which of them are
+// used as pivots kept in a register mask): each step takes the unused row
+// with the largest |G[r][k]| (the first on ties) and eliminates column k from
+// the other rows over the strip's later columns, the row segment in
+// registers, and offers the row's next-column value as the next pivot
+// candidate; one barrier per step (the per-wave candidates double-buffered).
+// Then the strip back to G and the pivot rows' trailing columns into U.
+template <int KB>
+__global__ __launch_bounds__(kGjThreads) void gj_panel_kernel(int n, int k0, double* __restrict__ G,
+                                                              int* __restrict__ pivrow,
+                                                              double* __restrict__ U)
+{
+    constexpr int LD = KB + 1;
+    constexpr int NW = kGjThreads / 64;
+    __shared__ double S[kGjStrip];
     __shared__ unsigned char used[kGjMaxRows];
-    __shared__ double bv[kGjThreads];
-    __shared__ int bi[kGjThreads];
-    const int t = threadIdx.x;
+    __shared__ double wv[2][NW];
+    __shared__ int wi[2][NW];
+    __shared__ int piv[KB];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int W = 2 * n + 1;
+    const int kb = min(KB, n - k0);
+    for (int e = t; e < n * KB; e += kGjThreads) {
+        const int r = e / KB, c = e - r * KB;
+        S[r * LD + c] = c < kb ? G[(int64_t)r * W + k0 + c] : 0.0;
+    }
     for (int r = t; r < n; r += kGjThreads) used[r] = 0;
     __syncthreads();
-    for (int m = t; m < k; m += kGjThreads) used[pivrow[m]] = 1;
+    for (int m = t; m < k0; m += kGjThreads) used[pivrow[m]] = 1;
     __syncthreads();
+    uint32_t mine = 0;  // bit i: row t + i kGjThreads already a pivot (n <= 32 kGjThreads)
     double best = -1.0;
     int bidx = n;
-    for (int r = t; r < n; r += kGjThreads) {
-        if (used[r]) continue;
-        const double v = fabs(G[(int64_t)r * W + k]);
-        if (bidx == n || v > best) { best = v; bidx = r; }  // any unused row beats none
+    for (int r = t, i = 0; r < n; r += kGjThreads, ++i) {
+        if (used[r]) {
+            mine |= 1u << i;
+            continue;
+        }
+        const double v = fabs(S[r * LD]);
+        if (bidx == n || v > best) { best = v; bidx = r; }
     }
-    bv[t] = best;
-    bi[t] = bidx;
-    __syncthreads();
-    for (int s = kGjThreads / 2; s > 0; s >>= 1) {
-        if (t < s) {
-            const double ov = bv[t + s];
-            const int oi = bi[t + s];
-            if (oi < n && (bi[t] == n || ov > bv[t] || (ov == bv[t] && oi < bi[t]))) {
-                bv[t] = ov;
-                bi[t] = oi;
-            }
+    // All KB steps run (unrolled, so the column tests below fold away); in a
+    // last, narrower pass the steps c >= kb touch only the strip's padding
+    // columns and record nothing.
+#pragma unroll
+    for (int c = 0; c < KB; ++c) {
+        const int buf = c & 1;
+        for (int o = 32; o > 0; o >>= 1) gj_better(__shfl_xor(best, o), __shfl_xor(bidx, o), n, best, bidx);
+        if (lane == 0) {
+            wv[buf][wave] = best;
+            wi[buf][wave] = bidx;
         }
         __syncthreads();
+        double bv = wv[buf][0];
+        int p = wi[buf][0];
+        for (int w = 1; w < NW; ++w) gj_better(wv[buf][w], wi[buf][w], n, bv, p);
+        if (p >= n) p = 0;  // no unused row left: a padding step, any row reads in bounds
+        if (t == 0) {
+            piv[c] = p;
+            if (c < kb) pivrow[k0 + c] = p;
+        }
+        if (p % kGjThreads == t) mine |= 1u << (p / kGjThreads);
+        const double pk = S[p * LD + c];
+        double prow[KB];
+#pragma unroll
+        for (int j = 0; j < KB; ++j) prow[j] = S[p * LD + j];
+        best = -1.0;
+        bidx = n;
+        for (int r = t, i = 0; r < n; r += kGjThreads, ++i) {
+            if (r == p) continue;
+            const double f = S[r * LD + c] / pk;
+            double row[KB];
+#pragma unroll
+            for (int j = 0; j < KB; ++j) row[j] = S[r * LD + j];
+            double next = 0.0;
+#pragma unroll
+            for (int j = 0; j < KB; ++j) {
+                double v = row[j];
+                v -= f * prow[j];
+                next = j == c + 1 ? v : next;
+                row[j] = j > c ? v : row[j];  // columns up to k keep their values
+            }
+#pragma unroll
+            for (int j = 0; j < KB; ++j) S[r * LD + j] = row[j];
+            if (!(mine >> i & 1u)) {
+                const double v = fabs(next);
+                if (bidx == n || v > best) { best = v; bidx = r; }
+            }
+        }
     }
-    const int p = bi[0];
-    if (blockIdx.x == 0 && t == 0) pivrow[k] = p;
-    const double* __restrict__ gp = G + (int64_t)p * W;
-    const double pk = gp[k];
-    for (int r = blockIdx.x; r < n; r += gridDim.x) {
-        if (r == p) continue;
-        double* __restrict__ gr = G + (int64_t)r * W;
-        const double f = gr[k] / pk;
-        for (int j = k + 1 + t; j < W; j += kGjThreads) gr[j] -= f * gp[j];
+    __syncthreads();
+    for (int e = t; e < n * kb; e += kGjThreads) {
+        const int r = e / kb, c = e - r * kb;
+        G[(int64_t)r * W + k0 + c] = S[r * LD + c];
     }
+    // the pivot rows' trailing columns before the update (T0 of U_m)
+    const int wt = W - (k0 + kb);
+    for (int e = t; e < kb * wt; e += kGjThreads) {
+        const int m = e / wt, j = k0 + kb + (e - m * wt);
+        U[(int64_t)m * W + j] = G[(int64_t)piv[m] * W + j];
+    }
+}
+
+// U_m = T0[p_m] - sum_{c<m} f(p_m, c) U_c of 64 trailing columns per
+// workgroup (a thread per column, the values so far in LDS), in place in U.
+template <int KB>
+__global__ __launch_bounds__(64) void gj_pivot_rows_kernel(int n, int k0, const double* __restrict__ G,
+                                                           const int* __restrict__ pivrow,
+                                                           double* __restrict__ U)
+{
+    __shared__ double fm[KB][KB];
+    __shared__ double us[KB][64];
+    const int lane = threadIdx.x;
+    const int W = 2 * n + 1;
+    const int kb = min(KB, n - k0);
+    const int j = k0 + kb + blockIdx.x * 64 + lane;
+    for (int e = lane; e < KB * KB; e += 64) {
+        const int m = e / KB, c = e - m * KB;
+        fm[m][c] = (c < m && m < kb) ? G[(int64_t)pivrow[k0 + m] * W + k0 + c] /
+                                           G[(int64_t)pivrow[k0 + c] * W + k0 + c]
+                                     : 0.0;
+    }
+    __syncthreads();
+    if (j >= W) return;
+    for (int m = 0; m < kb; ++m) {
+        double v = U[(int64_t)m * W + j];
+#pragma unroll 4
+        for (int c = 0; c < m; ++c) v -= fm[m][c] * us[c][lane];
+        us[m][lane] = v;
+        U[(int64_t)m * W + j] = v;
+    }
+}
+
+// The pass's trailing update, rows r0 .. r0+kGjRows-1 and 64 trailing
+// columns per workgroup: T[r][j] -= f(r, m) U_m[j] for m in order (m with
+// p_m != r).
+template <int KB>
+__global__ __launch_bounds__(64) void gj_update_kernel(int n, int k0, double* __restrict__ G,
+                                                       const int* __restrict__ pivrow,
+                                                       const double* __restrict__ U)
+{
+    __shared__ double f[kGjRows][KB];
+    __shared__ int pv[KB];
+    const int lane = threadIdx.x;
+    const int W = 2 * n + 1;
+    const int kb = min(KB, n - k0);
+    const int r0 = blockIdx.y * kGjRows;
+    const int j = k0 + kb + blockIdx.x * 64 + lane;
+    for (int m = lane; m < KB; m += 64) pv[m] = m < kb ? pivrow[k0 + m] : -1;
+    for (int e = lane; e < kGjRows * KB; e += 64) {
+        const int rr = e / KB, m = e - rr * KB;
+        const int r = r0 + rr;
+        double v = 0.0;
+        if (r < n && m < kb) {
+            const int p = pivrow[k0 + m];
+            v = p == r ? 0.0 : G[(int64_t)r * W + k0 + m] / G[(int64_t)p * W + k0 + m];
+        }
+        f[rr][m] = v;
+    }
+    __syncthreads();
+    if (j >= W) return;
+    double u[KB];
+#pragma unroll
+    for (int m = 0; m < KB; ++m) u[m] = m < kb ? U[(int64_t)m * W + j] : 0.0;
+    const int r1 = min(r0 + kGjRows, n);
+    for (int r = r0; r < r1; ++r) {
+        double acc = G[(int64_t)r * W + j];
+#pragma unroll
+        for (int m = 0; m < KB; ++m)
+            if (pv[m] >= 0 && pv[m] != r) acc -= f[r - r0][m] * u[m];
+        G[(int64_t)r * W + j] = acc;
+    }
+}
+
+template <int KB>
+static void gj_passes(hipStream_t st, int n, double* G, int* pivrow, double* U)
+{
+    const int W = 2 * n + 1;
+    for (int k0 = 0; k0 < n; k0 += KB) {
+        const int kb = std::min(KB, n - k0);
+        gj_panel_kernel<KB><<<1, kGjThreads, 0, st>>>(n, k0, G, pivrow, U);
+        DVM_LAUNCH_CHECK();
+        const int wt = W - (k0 + kb);
+        gj_pivot_rows_kernel<KB><<<cdiv(wt, 64), 64, 0, st>>>(n, k0, G, pivrow, U);
+        DVM_LAUNCH_CHECK();
+        gj_update_kernel<KB><<<dim3(cdiv(wt, 64), cdiv(n, kGjRows)), 64, 0, st>>>(n, k0, G, pivrow, U);
+        DVM_LAUNCH_CHECK();
+    }
+}
+
+// Widest pass whose strip fits the LDS strip.
+static int gj_pass_width(int n)
+{
+    for (int kb = 32; kb > 2; kb >>= 1)
+        if ((int64_t)n * (kb + 1) <= kGjStrip) return kb;
+    return 2;
 }
 
 // P[i][c] = G[p][n + c] / G[p][i] with p the pivot row of column i.
@@ -493,9 +665,11 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
 
     // compute_influence_matrix, then the least-squares operator (A^T A)^-1 A^T.
     const int W = 2 * nl + 1;
+    const int kb = gj_pass_width(nl);
     double* G = nullptr;
-    DVM_HIP_CHECK(hipMallocAsync((void**)&G, d * nl * W + sizeof(int) * nl, st));
-    int* pivrow = reinterpret_cast<int*>(G + (size_t)nl * W);
+    DVM_HIP_CHECK(hipMallocAsync((void**)&G, d * (size_t)(nl + kb) * W + sizeof(int) * nl, st));
+    double* U = G + (size_t)nl * W;
+    int* pivrow = reinterpret_cast<int*>(U + (size_t)kb * W);
     {
         ProfScope ps(ctx, "panel_influence");
         influence_kernel<<<dim3(cdiv(nl, 128), nl + 1), 128, 0, st>>>(nl, sh.x, sh.z, sh.xc, sh.zc,
@@ -506,10 +680,12 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
         ProfScope ps(ctx, "sheet_operator");
         gj_build_kernel<<<dim3(cdiv(W, 128), nl), 128, 0, st>>>(nl, sh.A, G);
         DVM_LAUNCH_CHECK();
-        const unsigned blocks = (unsigned)(nl < ctx->num_cus ? nl : ctx->num_cus);
-        for (int k = 0; k < nl; ++k) {
-            gj_step_kernel<<<blocks, kGjThreads, 0, st>>>(nl, k, G, pivrow);
-            DVM_LAUNCH_CHECK();
+        switch (kb) {
+            case 32: gj_passes<32>(st, nl, G, pivrow, U); break;
+            case 16: gj_passes<16>(st, nl, G, pivrow, U); break;
+            case 8: gj_passes<8>(st, nl, G, pivrow, U); break;
+            case 4: gj_passes<4>(st, nl, G, pivrow, U); break;
+            default: gj_passes<2>(st, nl, G, pivrow, U); break;
         }
         gj_extract_kernel<<<dim3(cdiv(nl + 1, 128), nl), 128, 0, st>>>(nl, G, pivrow, sh.P);
         DVM_LAUNCH_CHECK();

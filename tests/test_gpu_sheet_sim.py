@@ -74,6 +74,45 @@ def test_solve_gamma(gpu_ctx, n):
     assert rel_err(gsh.gamma, g_o) <= TOL
 
 
+@pytest.mark.parametrize("nl", [600, 1100])
+def test_solve_gamma_many_panels(gpu_ctx, nl):
+    # above 512 panels the operator's elimination runs in 16-column (600) and
+    # 8-column (1100) passes
+    px, pz = oracle.cylinder_points(nl)
+    op, gp = both_params()
+    osh = oracle.Sheet(px, pz, op)
+    gsh = dsim.Sheet(gpu_ctx, px, pz, gp.sheet)
+    x, z, c, s = wake_blobs(2000, seed=nl)
+    b_o, g_o = osh.solve(x, z, c, s)
+    X, Z, C, S = to_dev(x, z, c, s)
+    b_g = gsh.solve(X, Z, C, S)
+    assert rel_err(b_g, b_o) <= 1e-13
+    assert rel_err(gsh.gamma, g_o) <= TOL
+
+
+@pytest.mark.parametrize("nl", [2100, 3500])
+def test_operator_normal_equations(gpu_ctx, nl):
+    # 4-column (2100) and 2-column (3500) passes, checked by what the
+    # least-squares solution satisfies: A^T (A gamma - b) = 0, to rounding
+    # (about 1e-15 of the terms' size for these well-conditioned bodies)
+    px, pz = oracle.cylinder_points(nl)
+    _, gp = both_params(ux=1.0, uz=0.3)
+    gsh = dsim.Sheet(gpu_ctx, px, pz, gp.sheet)
+    A = gsh.influence()
+    x, z, c, s = wake_blobs(500, seed=3)
+    X, Z, C, S = to_dev(x, z, c, s)
+    b = gsh.solve(X, Z, C, S)
+    g = gsh.gamma
+    r = A.T @ (A @ g - b)
+    scale = np.abs(A.T) @ (np.abs(A) @ np.abs(g) + np.abs(b))
+    assert np.max(np.abs(r) / scale) <= 1e-11
+    # a planted 1e-6 error in one gamma entry is far outside that bound
+    g2 = g.copy()
+    g2[nl // 3] *= 1.0 + 1e-6
+    r2 = A.T @ (A @ g2 - b)
+    assert np.max(np.abs(r2) / scale) > 1e-9
+
+
 def test_sheet_on_blobs_and_loads(gpu_ctx):
     px, pz = load_body("bodycoordinatesN40.txt")
     op, gp = both_params(rho=1.3, dt=0.004)

@@ -75,6 +75,39 @@ def test_slot_matches_single_simulation(gpu_ctx, prec, scheme, crossing):
     s.close()
 
 
+@pytest.mark.parametrize("case", [CASES[2], (bodies.ellipse(300, aspect=3.0, aoa_deg=20),
+                                             dict(nu=5e-4, dt=0.004), 21, 10)])
+def test_f32c_panel_groups_match_single_simulation(gpu_ctx, case):
+    # The FP32C slot takes the sheet's velocity by 16-panel groups (40
+    # panels: two full groups and one of 8; 300: 18 and one of 12); the
+    # single simulation by the body-wide series and panel sums.
+    (px, pz), kw, seed, steps = case
+    out, infos, _ = run_batch_wide(gpu_ctx, [case])
+    sp = dsim.sim_params(seed=seed, prec=dvm.F32C, rng_mode=dsim.RNG_DEVICE, **kw)
+    s = dsim.Sim(gpu_ctx, px, pz, sp)
+    s.run(steps)
+    ref = s.get_state()
+    assert ref["x"].size == out[0]["x"].size
+    scale = np.max(np.abs(ref["x"]))
+    assert np.max(np.abs(ref["x"] - out[0]["x"])) <= 1e-6 * scale
+    assert np.max(np.abs(ref["z"] - out[0]["z"])) <= 1e-6 * scale
+    assert abs(s.loads()[2] - infos[0]["loads"][2]) <= 1e-6 * max(1.0, abs(s.loads()[2]))
+    s.close()
+
+
+def run_batch_wide(ctx, cases):
+    cfg = batch_config(slots=2, slot_capacity=1 << 15, max_panels=512, prec=dvm.F32C)
+    b = Batch(ctx, cfg)
+    slots = [b.admit(px, pz, dsim.sheet_params(**kw), seed, steps)
+             for (px, pz), kw, seed, steps in cases]
+    b.step(max(c[3] for c in cases))
+    out = [b.state(s) for s in slots]
+    infos = [b.info(s) for s in slots]
+    done = b.poll()
+    b.close()
+    return out, infos, done
+
+
 def test_batch_remesh_matches_single_simulation(gpu_ctx):
     rp = dsim.remesh_params(frequency=4, kernel=dsim.M4PRIME, xmin=-2.0, xmax=4.0, zmin=-2.0,
                             zmax=2.0, dx=0.05, dz=0.05, sigma=0.06, circ_threshold=1e-9)

@@ -615,8 +615,7 @@ __global__ __launch_bounds__(256) void bt_sheet_kernel(int S, int64_t cap, int m
 }
 
 // FP32C form of the sheet's velocity by panel groups (sheet_groups): moments
-// of the current gamma per (group, term), A_k = sum over the group's panels of
-// -(gamma_j I0_jk + gamma_{j+1} I1_jk), summed in panel order.
+// of the current gamma per (group, term), then per blob sheet_by_groups.
 __global__ __launch_bounds__(64) void bt_group_moments_kernel(int mp, int mg,
                                                               const SlotParam* __restrict__ prm,
                                                               const SlotState* __restrict__ st,
@@ -625,32 +624,16 @@ __global__ __launch_bounds__(64) void bt_group_moments_kernel(int mp, int mg,
                                                               const double* __restrict__ i1,
                                                               double* __restrict__ ga)
 {
-    const int g = blockIdx.x, s = blockIdx.y, k = threadIdx.x;
-    if (!st[s].active || k >= kSheetTerms) return;
-    const int nl = prm[s].nl;
-    const int j0 = g * kSheetGroup;
-    if (j0 >= nl) return;
-    const int j1 = min(j0 + kSheetGroup, nl);
-    const double* gm = gamma + (int64_t)s * mp;
-    double ar = 0.0, ai = 0.0;
-    for (int j = j0; j < j1; ++j) {
-        const double g0 = gm[j], g1 = gm[j + 1 == nl ? 0 : j + 1];
-        const int64_t o = 2 * (((int64_t)s * mp + j) * kSheetTerms + k);
-        ar -= g0 * i0[o] + g1 * i1[o];
-        ai -= g0 * i0[o + 1] + g1 * i1[o + 1];
-    }
-    const int64_t w = 2 * (((int64_t)s * mg + g) * kSheetTerms + k);
-    ga[w] = ar;
-    ga[w + 1] = ai;
+    const int s = blockIdx.y;
+    if (!st[s].active) return;
+    const int64_t io = (int64_t)s * mp * 2 * kSheetTerms;
+    group_moments(blockIdx.x, threadIdx.x, prm[s].nl, gamma + (int64_t)s * mp, i0 + io, i1 + io,
+                  ga + (int64_t)s * mg * 2 * kSheetTerms);
 }
 
-// The FP32C sheet velocity of every blob, in each slot's Morton order (the
-// sorted positions p64 and the permutation perm of the pair sum's sort, so a
-// wavefront's blobs sit close together): per panel group, the Laurent series
-// (phi = (1/r) sum_k A_k / W^(k+1), W = (z - c) / r; u += Im phi, w += Re phi)
-// when the blob lies beyond kSheetFar * r of the group centre, else
-// sheet_pair over the group's panels.  Each blob's choice depends on its own
-// position only.
+// Each slot's blobs in its Morton order (the sorted positions p64 and the
+// permutation perm of the pair sum's sort), so a wavefront's blobs sit close
+// together and mostly take the same branch per group.
 __global__ __launch_bounds__(256) void bt_sheet_groups_kernel(
     int S, int64_t cap, int mp, int mg, const WorkList* wl, const int64_t* __restrict__ blob_pre,
     const SlotParam* __restrict__ prm, const uint32_t* __restrict__ perm,
@@ -660,32 +643,9 @@ __global__ __launch_bounds__(256) void bt_sheet_groups_kernel(
 {
     for_each_blob(wl, blob_pre, S, cap, [&](int s, int64_t o) {
         const double2 p = p64[o];  // o = s cap + sorted rank
-        const int nl = prm[s].nl;
-        const int ng = (nl + kSheetGroup - 1) / kSheetGroup;
-        const PanelConst* pc = panel + (int64_t)s * mp;
-        const double* gm = gamma + (int64_t)s * mp;
         double su = 0.0, sw = 0.0;
-        for (int g = 0; g < ng; ++g) {
-            const double4 c = gcen[(int64_t)s * mg + g];
-            const double wr = (p.x - c.x) / c.z, wi = (p.y - c.y) / c.z;
-            const double w2 = wr * wr + wi * wi;
-            if (w2 > kSheetFar * kSheetFar) {
-                const double* a = ga + 2 * ((int64_t)s * mg + g) * kSheetTerms;
-                const double tr = wr / w2, ti = -wi / w2;  // 1 / W
-                double pr = a[2 * (kSheetTerms - 1)], pim = a[2 * (kSheetTerms - 1) + 1];
-                for (int k = kSheetTerms - 2; k >= 0; --k) {
-                    const double nr = pr * tr - pim * ti + a[2 * k];
-                    pim = pr * ti + pim * tr + a[2 * k + 1];
-                    pr = nr;
-                }
-                su += (pr * ti + pim * tr) / c.z;
-                sw += (pr * tr - pim * ti) / c.z;
-            } else {
-                const int j1 = min((g + 1) * kSheetGroup, nl);
-                for (int j = g * kSheetGroup; j < j1; ++j)
-                    sheet_pair(p.x, p.y, pc[j], gm[j], gm[j + 1 == nl ? 0 : j + 1], su, sw);
-            }
-        }
+        sheet_by_groups(p.x, p.y, prm[s].nl, panel + (int64_t)s * mp, gamma + (int64_t)s * mp,
+                        gcen + (int64_t)s * mg, ga + (int64_t)s * mg * 2 * kSheetTerms, su, sw);
         const int64_t dst = (int64_t)s * cap + perm[o];
         b.uvs[dst] = su * kRpi2;
         b.wvs[dst] = sw * kRpi2;
@@ -1341,14 +1301,11 @@ dvm_status dvm_batch_admit(dvm_batch_t B, int npts, const double* px, const doub
         d2d(d.panel + (size_t)s * mp, sh.panel, sizeof(PanelConst) * nl);
         d2d(d.P + (size_t)s * mp * (mp + 1), sh.P, db * nl * (nl + 1));
         DVM_HIP_CHECK(hipMemsetAsync(d.gamma + (size_t)s * mp, 0, db * nl, st));
-        std::vector<double> gcen;  // live until the synchronisation below
         if (B->cfg.prec == DVM_F32C) {
-            sheet_groups(sh, gcen);
-            DVM_HIP_CHECK(hipMemcpyAsync(d.gcen + (size_t)s * d.mg, gcen.data(), db * gcen.size(),
-                                         hipMemcpyHostToDevice, st));
             const size_t mo = (size_t)s * mp * 2 * kSheetTerms;
-            panel_moments(st, nl, d.panel + (size_t)s * mp, 0.0, 0.0, 1.0, d.gcen + (size_t)s * d.mg,
-                          d.gI0 + mo, d.gI1 + mo);
+            d2d(d.gcen + (size_t)s * d.mg, sh.gcen, sizeof(double4) * sh.ng);
+            d2d(d.gI0 + mo, sh.gI0, db * 2 * kSheetTerms * nl);
+            d2d(d.gI1 + mo, sh.gI1, db * 2 * kSheetTerms * nl);
         }
         SlotParam sp{};
         sp.rho = p->rho; sp.nu = p->nu; sp.dt = p->dt; sp.ux = p->ux; sp.uz = p->uz;

@@ -172,6 +172,60 @@ __device__ __forceinline__ void sheet_pair(double xi, double zi, const PanelCons
     sw += py * gj + qy * gj1;
 }
 
+// The sheet's velocity at (x, z) by panel groups (sheet_groups): for each
+// group, beyond kSheetFar * r of its centre the Laurent series
+// phi = (1/r) sum_k A_k / W^(k+1), W = (z - c) / r (u += Im phi, w += Re phi),
+// else sheet_pair over the group's panels.  pc, gm (gamma), gc (centres) and
+// ga (moments, 2 kSheetTerms per group) are one body's; the choice depends on
+// (x, z) alone.  Adds to su, sw (without the 1/2pi).
+__device__ __forceinline__ void sheet_by_groups(double x, double z, int nl, const PanelConst* pc,
+                                                const double* gm, const double4* gc,
+                                                const double* ga, double& su, double& sw)
+{
+    const int ng = (nl + kSheetGroup - 1) / kSheetGroup;
+    for (int g = 0; g < ng; ++g) {
+        const double4 c = gc[g];
+        const double wr = (x - c.x) / c.z, wi = (z - c.y) / c.z;
+        const double w2 = wr * wr + wi * wi;
+        if (w2 > kSheetFar * kSheetFar) {
+            const double* a = ga + 2 * g * kSheetTerms;
+            const double tr = wr / w2, ti = -wi / w2;  // 1 / W
+            double pr = a[2 * (kSheetTerms - 1)], pim = a[2 * (kSheetTerms - 1) + 1];
+            for (int k = kSheetTerms - 2; k >= 0; --k) {
+                const double nr = pr * tr - pim * ti + a[2 * k];
+                pim = pr * ti + pim * tr + a[2 * k + 1];
+                pr = nr;
+            }
+            su += (pr * ti + pim * tr) / c.z;
+            sw += (pr * tr - pim * ti) / c.z;
+        } else {
+            const int j1 = min((g + 1) * kSheetGroup, nl);
+            for (int j = g * kSheetGroup; j < j1; ++j)
+                sheet_pair(x, z, pc[j], gm[j], gm[j + 1 == nl ? 0 : j + 1], su, sw);
+        }
+    }
+}
+
+// Group moments A_k = sum over group g's panels, in order, of
+// -(gamma_j I0_jk + gamma_{j+1} I1_jk) (k = lane < kSheetTerms); i0, i1 in
+// the layout of sheet_groups' integrals, one body's.
+__device__ __forceinline__ void group_moments(int g, int k, int nl, const double* gm,
+                                              const double* i0, const double* i1, double* ga)
+{
+    const int j0 = g * kSheetGroup;
+    if (j0 >= nl || k >= kSheetTerms) return;
+    const int j1 = min(j0 + kSheetGroup, nl);
+    double ar = 0.0, ai = 0.0;
+    for (int j = j0; j < j1; ++j) {
+        const double g0 = gm[j], g1 = gm[j + 1 == nl ? 0 : j + 1];
+        const int64_t o = 2 * ((int64_t)j * kSheetTerms + k);
+        ar -= g0 * i0[o] + g1 * i1[o];
+        ai -= g0 * i0[o + 1] + g1 * i1[o + 1];
+    }
+    ga[2 * (g * kSheetTerms + k)] = ar;
+    ga[2 * (g * kSheetTerms + k) + 1] = ai;
+}
+
 // One pair of the reference kernel (src/VelocityKernel.cpp:44-60), FP64.
 __device__ __forceinline__ void bs_pair_f64(double xi, double zi, double xj, double zj, double cj,
                                             double sj, double& ui, double& wi)

@@ -204,6 +204,11 @@ struct SheetDev {
     // (weight s/ds) per panel (complex, interleaved re/im).
     double mx = 0, mz = 0, mr = 1;
     double *mI0 = nullptr, *mI1 = nullptr, *mA = nullptr;
+    // Panel groups (sheet_groups) for the batched fast form, which copies
+    // them: group centres and per-panel integrals about them.
+    int ng = 0;
+    double4* gcen = nullptr;
+    double *gI0 = nullptr, *gI1 = nullptr;
 };
 
 constexpr int kSheetTerms = 40;

@@ -599,13 +599,17 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
     const size_t d = sizeof(double);
     const size_t nred = 3 * (size_t)nl > 1024 ? 3 * (size_t)nl : 1024;
     const size_t nm = 2 * (size_t)nl * kSheetTerms;
+    std::vector<double> gcen;  // live until the synchronisation at the end
+    sheet_groups(sh, gcen);
+    const int ng = (int)(gcen.size() / 4);
     const size_t need = Carver::need({d * npts, d * npts, d * nl * 8, sizeof(PanelConst) * nl,
                                       d * (nl + 1) * nl, d * nl * (nl + 1), d * nl, d * (nl + 1),
                                       d * 2 * nl, d * 8, d * nred, sizeof(int) * nl,
                                       sizeof(int64_t) * (nl + 1), d * nl, sizeof(int) * 4, d * nm,
                                       d * nm, d * 2 * kSheetTerms,
                                       sizeof(int) * (sh.hslab_off.size() + 1),
-                                      sizeof(int) * (sh.hslab_edge.size() + 1)});
+                                      sizeof(int) * (sh.hslab_edge.size() + 1),
+                                      sizeof(double4) * ng, d * nm, d * nm});
     DVM_HIP_CHECK(hipMalloc(&sh.mem, need));
     Carver cv(sh.mem);
     sh.x = cv.take<double>(npts);
@@ -630,6 +634,10 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
     sh.mA = cv.take<double>(2 * kSheetTerms);
     int* slab_off = cv.take<int>(sh.hslab_off.size() + 1);
     int* slab_edge = cv.take<int>(sh.hslab_edge.size() + 1);
+    sh.ng = ng;
+    sh.gcen = cv.take<double4>(ng);
+    sh.gI0 = cv.take<double>(nm);
+    sh.gI1 = cv.take<double>(nm);
     if (nslab > 0) {
         DVM_HIP_CHECK(hipMemcpy(slab_off, sh.hslab_off.data(), sizeof(int) * sh.hslab_off.size(),
                                 hipMemcpyHostToDevice));
@@ -660,6 +668,8 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
     DVM_HIP_CHECK(hipMemcpyAsync(sh.panel, pc.data(), sizeof(PanelConst) * nl,
                                  hipMemcpyHostToDevice, st));
     panel_moments(st, nl, sh.panel, sh.mx, sh.mz, sh.mr, nullptr, sh.mI0, sh.mI1);
+    DVM_HIP_CHECK(hipMemcpyAsync(sh.gcen, gcen.data(), d * gcen.size(), hipMemcpyHostToDevice, st));
+    panel_moments(st, nl, sh.panel, 0.0, 0.0, 1.0, sh.gcen, sh.gI0, sh.gI1);
     DVM_HIP_CHECK(hipMemsetAsync(sh.gamma, 0, d * nl, st));
     DVM_HIP_CHECK(hipMemsetAsync(sh.scal, 0, d * 8, st));
 

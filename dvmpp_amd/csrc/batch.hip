@@ -373,7 +373,9 @@ __global__ __launch_bounds__(256) void bt_bs32_kernel(int S, int64_t cap, int ma
     for (;;) {
         int item = 0;
         if (lane == 0) item = atomicAdd(&wl->work_counter, 1);
-        item = __shfl(item, 0);
+        // wave-uniform (an SGPR), so the item's slot, tiles and source
+        // records are scalar loads, as in the single-simulation kernel
+        item = __builtin_amdgcn_readfirstlane(item);
         if (item >= total) break;
         const int p = find_slot(item_pre, S, item);
         const int s = order[p];

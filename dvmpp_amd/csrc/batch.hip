@@ -634,24 +634,31 @@ __global__ __launch_bounds__(64) void bt_group_moments_kernel(int mp, int mg,
 }
 
 // Each slot's blobs in its Morton order (the sorted positions p64 and the
-// permutation perm of the pair sum's sort), so a wavefront's blobs sit close
-// together and mostly take the same branch per group.
-__global__ __launch_bounds__(256) void bt_sheet_groups_kernel(
-    int S, int64_t cap, int mp, int mg, const WorkList* wl, const int64_t* __restrict__ blob_pre,
-    const SlotParam* __restrict__ prm, const uint32_t* __restrict__ perm,
-    const double2* __restrict__ p64, const PanelConst* __restrict__ panel,
-    const double* __restrict__ gamma, const double4* __restrict__ gcen,
-    const double* __restrict__ ga, BlobBuf b)
+// permutation perm of the pair sum's sort), a workgroup per 256-blob tile of
+// one slot: a wavefront's blobs sit close together and mostly take the same
+// branch per group, and the slot's panels, group centres and moments are
+// workgroup-uniform (scalar loads).
+__global__ __launch_bounds__(kBT) void bt_sheet_groups_kernel(
+    int S, int64_t cap, int mp, int mg, const WorkList* wl, const int* __restrict__ tile_pre,
+    const SlotState* __restrict__ st, const SlotParam* __restrict__ prm,
+    const uint32_t* __restrict__ perm, const double2* __restrict__ p64,
+    const PanelConst* __restrict__ panel, const double* __restrict__ gamma,
+    const double4* __restrict__ gcen, const double* __restrict__ ga, BlobBuf b)
 {
-    for_each_blob(wl, blob_pre, S, cap, [&](int s, int64_t o) {
-        const double2 p = p64[o];  // o = s cap + sorted rank
+    const int total = wl->total_tiles;
+    for (int g = blockIdx.x; g < total; g += gridDim.x) {
+        const int s = find_slot(tile_pre, S, g);
+        const int64_t k = (int64_t)(g - tile_pre[s]) * kBT + threadIdx.x;  // sorted rank
+        if (k >= st[s].n) continue;
+        const int64_t o = (int64_t)s * cap + k;
+        const double2 p = p64[o];
         double su = 0.0, sw = 0.0;
         sheet_by_groups(p.x, p.y, prm[s].nl, panel + (int64_t)s * mp, gamma + (int64_t)s * mp,
                         gcen + (int64_t)s * mg, ga + (int64_t)s * mg * 2 * kSheetTerms, su, sw);
         const int64_t dst = (int64_t)s * cap + perm[o];
         b.uvs[dst] = su * kRpi2;
         b.wvs[dst] = sw * kRpi2;
-    });
+    }
 }
 
 // ----------------------------------------------------------------- blob updates
@@ -1083,9 +1090,9 @@ void record_convect(dvm_batch_t B)
         if (f32c) {
             bt_group_moments_kernel<<<dim3(d.mg, S), 64, 0, st>>>(mp, d.mg, d.prm, d.st, d.gamma,
                                                                   d.gI0, d.gI1, d.gA);
-            bt_sheet_groups_kernel<<<gs, 256, 0, st>>>(S, cap, mp, d.mg, d.wl, d.blob_pre, d.prm,
-                                                       d.perm, d.p64, d.panel, d.gamma, d.gcen,
-                                                       d.gA, d.b);
+            bt_sheet_groups_kernel<<<gs, kBT, 0, st>>>(S, cap, mp, d.mg, d.wl, d.tile_pre, d.st,
+                                                       d.prm, d.perm, d.p64, d.panel, d.gamma,
+                                                       d.gcen, d.gA, d.b);
         } else {
             bt_sheet_kernel<<<gs, 256, 0, st>>>(S, cap, mp, d.wl, d.blob_pre, d.prm, d.b, d.panel,
                                                 d.gamma);

@@ -184,6 +184,33 @@ __device__ __forceinline__ void wave_sync_memory()
     __builtin_amdgcn_wave_barrier();
 }
 
+// The absorbed flags as the searches and the commit see them: none (the
+// speculative pass), the global array, or a bitmap in the commit wavefront's
+// LDS (one bit per blob; written back to the global array at the end).  sync()
+// makes a wavefront's own flag writes visible to its next reads.
+struct TakenNone {
+    __device__ bool get(int) const { return false; }
+};
+struct TakenGlobal {
+    int* r;
+    __device__ bool get(int j) const { return r[j] != 0; }
+    __device__ void set(int j) const { r[j] = 1; }
+    __device__ void sync() const { wave_sync_memory(); }
+};
+struct TakenLds {
+    uint32_t* bits;
+    __device__ bool get(int j) const { return (bits[j >> 5] >> (j & 31)) & 1u; }
+    __device__ void set(int j) const { atomicOr(&bits[j >> 5], 1u << (j & 31)); }
+    // LDS operations of one wavefront complete in order: only the compiler
+    // must not move them
+    __device__ void sync() const
+    {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+};
+constexpr int kTakenWords = 32768;  // LDS bitmap: up to 1,048,576 blobs
+
 // Cell range [i0, i1] on one axis covering (v - r, v + r); a cell boundary
 // within 1e-6 of a cell of the range edge adds that cell (rounding of the
 // binning).  Returns false when the range is not finite.
@@ -206,16 +233,16 @@ __device__ __forceinline__ bool cell_range(double v, double r, double v0, double
 // Lowest j >= j0 with !removed[j] that passes against (cg, cx, cz), by a
 // forward scan of all later blobs; INT_MAX if none.  Lanes hold consecutive
 // indices, so the lowest set ballot bit is the lowest passing index.
-template <bool REMOVALS>
+template <class T>
 __device__ int next_linear(int n, const double* __restrict__ x, const double* __restrict__ z,
-                           const double* __restrict__ c, const int* removed, double cg, double cx,
+                           const double* __restrict__ c, const T& taken, double cg, double cx,
                            double cz, int j0, double thr, double rej)
 {
     const int lane = threadIdx.x & (kW - 1);
     for (; j0 < n; j0 += kW) {
         const int j = j0 + lane;
         bool hit = false;
-        if (j < n && (!REMOVALS || !removed[j])) hit = merge_pass(cg, cx, cz, c[j], x[j], z[j], thr, rej);
+        if (j < n && !taken.get(j)) hit = merge_pass(cg, cx, cz, c[j], x[j], z[j], thr, rej);
         const unsigned long long b = __ballot(hit);
         if (b) return j0 + __ffsll((long long)b) - 1;
     }
@@ -323,23 +350,23 @@ __device__ bool visit_near(double cg, double cx, double cz, int j0, const MergeG
 }
 
 // Lowest j >= j0 with !removed[j] that passes against (cg, cx, cz), by a
-// cell-list query (falls back to next_linear).  REMOVALS = false ignores the
+// cell-list query (falls back to next_linear).  TakenNone ignores the
 // absorbed flags (the speculative pass, where none are set).
-template <bool REMOVALS>
+template <class T>
 __device__ int next_partner(int n, const double* __restrict__ x, const double* __restrict__ z,
-                            const double* __restrict__ c, const int* removed, double cg, double cx,
+                            const double* __restrict__ c, const T& taken, double cg, double cx,
                             double cz, int j0, double thr, double rej, const MergeGrid& g, WaveLds& L)
 {
     int best = INT_MAX;
     auto visit = [&](int j, bool ordered) {
-        const bool hit = j >= 0 && j < best && (!REMOVALS || !removed[j]) &&
+        const bool hit = j >= 0 && j < best && !taken.get(j) &&
                          merge_pass(cg, cx, cz, c[j], x[j], z[j], thr, rej);
         const bool any = __ballot(hit) != 0;
         if (any) best = min(best, wave_min(hit ? j : INT_MAX));
         return ordered && any;  // in index order the first hit is the answer
     };
     if (!visit_near(cg, cx, cz, j0, g, L, best, visit))
-        return next_linear<REMOVALS>(n, x, z, c, removed, cg, cx, cz, j0, thr, rej);
+        return next_linear(n, x, z, c, taken, cg, cx, cz, j0, thr, rej);
     return best;
 }
 
@@ -435,7 +462,7 @@ __global__ __launch_bounds__(256) void merge_spec_kernel(
             if (lane == 0) jl[(size_t)slot * kSpecMax + k] = jm;
             ++k;
             absorb(cx, cz, cg, cs, x[jm], z[jm], c[jm], s[jm]);
-            jm = next_partner<false>(n, x, z, c, nullptr, cg, cx, cz, jm + 1, thr, rej, g, L);
+            jm = next_partner(n, x, z, c, TakenNone{}, cg, cx, cz, jm + 1, thr, rej, g, L);
         }
         if (lane == 0) {
             jn[slot] = k;
@@ -452,16 +479,16 @@ __global__ __launch_bounds__(256) void merge_spec_kernel(
 // committed, the others recomputed against the absorptions so far, starting
 // from the first input-state partner still available.  Lane 0 writes the
 // shared results.  out[0] = merges, out[1] = recomputed sequences.
-__global__ __launch_bounds__(kW) void merge_commit_kernel(
-    int n, double* __restrict__ x, double* __restrict__ z, double* __restrict__ c,
+template <class T>
+__device__ void merge_commit(
+    const T& taken, int n, double* __restrict__ x, double* __restrict__ z, double* __restrict__ c,
     double* __restrict__ s, double* __restrict__ u, double* __restrict__ w,
     double* __restrict__ uvs, double* __restrict__ wvs, const int* __restrict__ cand,
     const int* __restrict__ list, const int* __restrict__ nlist, const int* __restrict__ jn,
     const int* __restrict__ jl, const double* __restrict__ fs, const int* __restrict__ cn,
-    const int* __restrict__ cl, int* removed, double thr, double rej, MergeGrid g,
+    const int* __restrict__ cl, double thr, double rej, const MergeGrid& g, WaveLds& L,
     int* __restrict__ out)
 {
-    __shared__ WaveLds L;
     const int lane = threadIdx.x;
     const int nl = *nlist;
     int total = 0, redo = 0;
@@ -471,12 +498,12 @@ __global__ __launch_bounds__(kW) void merge_commit_kernel(
         const int np = cn[slot];
         const int jv = lane < min(nj, kSpecMax) ? jl[(size_t)slot * kSpecMax + lane] : -1;
         const int pv = lane < min(np, kCandMax) ? cl[(size_t)slot * kCandMax + lane] : -1;
-        if (removed[ii]) continue;  // absorbed by an earlier blob
+        if (taken.get(ii)) continue;  // absorbed by an earlier blob
         bool valid = nj <= kSpecMax;
-        if (valid && __ballot(jv >= 0 && removed[jv])) valid = false;
+        if (valid && __ballot(jv >= 0 && taken.get(jv))) valid = false;
         double cx, cz, cg, cs;
         if (valid) {
-            if (jv >= 0) removed[jv] = 1;
+            if (jv >= 0) taken.set(jv);
             cx = fs[4 * (size_t)slot];
             cz = fs[4 * (size_t)slot + 1];
             cg = fs[4 * (size_t)slot + 2];
@@ -490,23 +517,23 @@ __global__ __launch_bounds__(kW) void merge_commit_kernel(
             int jm;
             if (np >= 0) {
                 // the first still-available partner is the first absorption
-                const unsigned long long b = __ballot(pv >= 0 && !removed[pv]);
+                const unsigned long long b = __ballot(pv >= 0 && !taken.get(pv));
                 if (b) jm = __shfl(pv, __ffsll((long long)b) - 1);
                 else if (np <= kCandMax) continue;  // every partner is taken: no merge
-                else jm = next_partner<true>(n, x, z, c, removed, cg, cx, cz,
+                else jm = next_partner(n, x, z, c, taken, cg, cx, cz,
                                              __shfl(pv, kCandMax - 1) + 1, thr, rej, g, L);
             } else {
                 // every j in (ii, cand) fails against the input state
-                jm = next_partner<true>(n, x, z, c, removed, cg, cx, cz, cand[ii], thr, rej, g, L);
+                jm = next_partner(n, x, z, c, taken, cg, cx, cz, cand[ii], thr, rej, g, L);
             }
             ++redo;
             int k = 0;
             while (jm != INT_MAX) {
                 absorb(cx, cz, cg, cs, x[jm], z[jm], c[jm], s[jm]);
-                if (lane == 0) removed[jm] = 1;
+                if (lane == 0) taken.set(jm);
                 ++k;
-                wave_sync_memory();
-                jm = next_partner<true>(n, x, z, c, removed, cg, cx, cz, jm + 1, thr, rej, g, L);
+                taken.sync();
+                jm = next_partner(n, x, z, c, taken, cg, cx, cz, jm + 1, thr, rej, g, L);
             }
             total += k;
             if (k == 0) continue;
@@ -521,12 +548,50 @@ __global__ __launch_bounds__(kW) void merge_commit_kernel(
             uvs[ii] = 0.0;
             wvs[ii] = 0.0;
         }
-        wave_sync_memory();
+        taken.sync();
     }
     if (lane == 0) {
         out[0] = total;
         out[1] = redo;
     }
+}
+
+__global__ __launch_bounds__(kW) void merge_commit_kernel(
+    int n, double* __restrict__ x, double* __restrict__ z, double* __restrict__ c,
+    double* __restrict__ s, double* __restrict__ u, double* __restrict__ w,
+    double* __restrict__ uvs, double* __restrict__ wvs, const int* __restrict__ cand,
+    const int* __restrict__ list, const int* __restrict__ nlist, const int* __restrict__ jn,
+    const int* __restrict__ jl, const double* __restrict__ fs, const int* __restrict__ cn,
+    const int* __restrict__ cl, int* removed, double thr, double rej, MergeGrid g,
+    int* __restrict__ out)
+{
+    __shared__ WaveLds L;
+    merge_commit(TakenGlobal{removed}, n, x, z, c, s, u, w, uvs, wvs, cand, list, nlist, jn, jl,
+                 fs, cn, cl, thr, rej, g, L, out);
+}
+
+// The same with the absorbed flags in LDS (n <= 32 kTakenWords): no global
+// round trip per candidate; the flags go to `removed` at the end.
+__global__ __launch_bounds__(kW) void merge_commit_lds_kernel(
+    int n, double* __restrict__ x, double* __restrict__ z, double* __restrict__ c,
+    double* __restrict__ s, double* __restrict__ u, double* __restrict__ w,
+    double* __restrict__ uvs, double* __restrict__ wvs, const int* __restrict__ cand,
+    const int* __restrict__ list, const int* __restrict__ nlist, const int* __restrict__ jn,
+    const int* __restrict__ jl, const double* __restrict__ fs, const int* __restrict__ cn,
+    const int* __restrict__ cl, int* removed, double thr, double rej, MergeGrid g,
+    int* __restrict__ out)
+{
+    __shared__ WaveLds L;
+    __shared__ uint32_t bits[kTakenWords];
+    const int lane = threadIdx.x;
+    const int nw = (n + 31) >> 5;
+    for (int k = lane; k < nw; k += kW) bits[k] = 0u;
+    __syncthreads();
+    const TakenLds taken{bits};
+    merge_commit(taken, n, x, z, c, s, u, w, uvs, wvs, cand, list, nlist, jn, jl, fs, cn, cl, thr,
+                 rej, g, L, out);
+    __syncthreads();
+    for (int i = lane; i < n; i += kW) removed[i] = taken.get(i) ? 1 : 0;
 }
 
 __global__ void merge_keep_kernel(int n, const int* __restrict__ removed, int* __restrict__ keep)
@@ -720,9 +785,14 @@ int64_t merge_run(dvm_ctx_t ctx, double threshold, int64_t n, const BlobBuf& b, 
     }
     {
         ProfScope ps(ctx, "merge_commit_kernel");
-        merge_commit_kernel<<<1, kW, 0, st>>>((int)n, b.x, b.z, b.c, b.s, b.u, b.w, b.uvs, b.wvs, cand,
-                                              list, counts, jn, jl, fs, cn, cl, removed, threshold,
-                                              rej, g, counts + 2);
+        if (n <= 32 * (int64_t)kTakenWords)
+            merge_commit_lds_kernel<<<1, kW, 0, st>>>((int)n, b.x, b.z, b.c, b.s, b.u, b.w, b.uvs,
+                                                      b.wvs, cand, list, counts, jn, jl, fs, cn, cl,
+                                                      removed, threshold, rej, g, counts + 2);
+        else
+            merge_commit_kernel<<<1, kW, 0, st>>>((int)n, b.x, b.z, b.c, b.s, b.u, b.w, b.uvs, b.wvs,
+                                                  cand, list, counts, jn, jl, fs, cn, cl, removed,
+                                                  threshold, rej, g, counts + 2);
         DVM_LAUNCH_CHECK();
     }
     int* hc = static_cast<int*>(pinned_get(ctx, sizeof(int) * 4));

@@ -89,3 +89,40 @@ def test_merge_large_set_invariants(gpu_ctx):
     assert kg2 == ko
     for f in FIELDS:
         assert np.array_equal(got2[f], ref[f]), f
+
+
+def _pairs(npairs, seed):
+    """Separated pairs of close blobs on a square lattice (spacing 1e-2, pair
+    distance below 2e-4, circulations of one sign): at threshold 1e-6 each
+    pair merges and nothing else does, so the result of a prefix of the set
+    does not depend on the rest."""
+    rng = np.random.default_rng(seed)
+    side = int(np.ceil(np.sqrt(npairs)))
+    k = np.arange(npairs)
+    cx, cz = (k % side) * 1e-2, (k // side) * 1e-2
+    ang = rng.uniform(0, 2 * np.pi, npairs)
+    d = rng.uniform(2e-5, 2e-4, npairs)
+    n = 2 * npairs
+    st = {"id": np.arange(n, dtype=np.uint32), "x": np.empty(n), "z": np.empty(n),
+          "circ": rng.uniform(5e-4, 1.5e-3, n), "sigma": rng.uniform(0.01, 0.05, n)}
+    st["x"][0::2], st["z"][0::2] = cx, cz
+    st["x"][1::2], st["z"][1::2] = cx + d * np.cos(ang), cz + d * np.sin(ang)
+    for f in ("u", "w", "uvs", "wvs"):
+        st[f] = rng.normal(0, 1, n)
+    return st
+
+
+def test_merge_flags_in_global_memory_above_lds_capacity(gpu_ctx):
+    # Above 1,048,576 blobs the ordered commit keeps the absorbed flags in
+    # global memory instead of its LDS bitmap; the first 100,000 pairs must
+    # come out the same as when merged on their own (LDS path).
+    big = _pairs(550_000, 7)
+    kb, out_big = _device_merge(gpu_ctx, 1e-6, big, with_ids=False)
+    assert kb == 550_000
+    small = {f: v[:200_000].copy() for f, v in big.items()}
+    ks, out_small = _device_merge(gpu_ctx, 1e-6, small, with_ids=False)
+    assert ks == 100_000
+    for f in FIELDS:
+        assert np.array_equal(out_big[f][:100_000], out_small[f]), f
+    # each survivor is its pair, merged: circulation adds
+    assert np.array_equal(out_small["circ"], small["circ"][0::2] + small["circ"][1::2])

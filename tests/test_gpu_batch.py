@@ -108,23 +108,26 @@ def run_batch_wide(ctx, cases):
     return out, infos, done
 
 
-def test_batch_remesh_matches_single_simulation(gpu_ctx):
+@pytest.mark.parametrize("prec,tol", [(dvm.F64, 1e-10), (dvm.F32C, 1e-6)])
+def test_batch_remesh_matches_single_simulation(gpu_ctx, prec, tol):
+    # FP32C: the remeshed slot's next stage re-sorts its blobs for the pair
+    # sum and the panel-group sheet velocity
     rp = dsim.remesh_params(frequency=4, kernel=dsim.M4PRIME, xmin=-2.0, xmax=4.0, zmin=-2.0,
                             zmax=2.0, dx=0.05, dz=0.05, sigma=0.06, circ_threshold=1e-9)
     (px, pz), kw, seed, _ = CASES[0]
-    cfg = batch_config(slots=2, slot_capacity=1 << 15, max_panels=128, prec=dvm.F64, remesh=rp)
+    cfg = batch_config(slots=2, slot_capacity=1 << 15, max_panels=128, prec=prec, remesh=rp)
     b = Batch(gpu_ctx, cfg)
     slot = b.admit(px, pz, dsim.sheet_params(**kw), seed, 10)
     b.step(10)
     out = b.state(slot)
     b.close()
-    s = dsim.Sim(gpu_ctx, px, pz, dsim.sim_params(seed=seed, prec=dvm.F64, rng_mode=dsim.RNG_DEVICE,
+    s = dsim.Sim(gpu_ctx, px, pz, dsim.sim_params(seed=seed, prec=prec, rng_mode=dsim.RNG_DEVICE,
                                                    remesh=rp, **kw))
     s.run(10)
     ref = s.get_state()
     assert ref["x"].size == out["x"].size
-    assert np.max(np.abs(ref["x"] - out["x"])) <= 1e-10 * np.max(np.abs(ref["x"]))
-    assert np.max(np.abs(ref["circ"] - out["circ"])) <= 1e-10 * np.max(np.abs(ref["circ"]))
+    assert np.max(np.abs(ref["x"] - out["x"])) <= tol * np.max(np.abs(ref["x"]))
+    assert np.max(np.abs(ref["circ"] - out["circ"])) <= tol * np.max(np.abs(ref["circ"]))
     s.close()
 
 

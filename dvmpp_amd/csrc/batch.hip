@@ -834,28 +834,28 @@ __global__ void bt_commit_release_kernel(int S, SlotState* __restrict__ st)
 // Crossing flags (inside_body_fast with the slot's box and slab lists) and,
 // for REFLECT, src/VortexSheet.cpp:345-405 for the inside blobs: each wave
 // takes its inside blobs one at a time, all lanes working on one
-// (reflect_wave, with that blob's slot body).  The loop bounds are
-// wave-uniform so every lane takes part in the shuffles.
-__global__ __launch_bounds__(256) void bt_inside_kernel(
-    int S, int64_t cap, int mp, const WorkList* wl, const int64_t* __restrict__ blob_pre,
-    const SlotParam* __restrict__ prm, const double* __restrict__ bx,
-    const double* __restrict__ bz, const double* __restrict__ enx,
+// (reflect_wave).  A workgroup per 256-blob tile of one slot, so the slot's
+// body, box and slab lists are workgroup-uniform (scalar loads) and the loop
+// bounds wave-uniform (every lane takes part in the shuffles).
+__global__ __launch_bounds__(kBT) void bt_inside_kernel(
+    int S, int64_t cap, int mp, const WorkList* wl, const int* __restrict__ tile_pre,
+    const SlotState* __restrict__ st, const SlotParam* __restrict__ prm,
+    const double* __restrict__ bx, const double* __restrict__ bz, const double* __restrict__ enx,
     const double* __restrict__ enz, const int* __restrict__ slab_off,
     const int* __restrict__ slab_edge, int reflect, BlobBuf b, int* __restrict__ flags)
 {
-    const int64_t total = wl->total_blobs;
+    const int total = wl->total_tiles;
     const int lane = threadIdx.x & 63;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t gb = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); gb < total;
-         gb += stride) {
-        const int64_t g = gb + lane;
-        int s = 0;
-        int64_t o = 0;
+    for (int g = blockIdx.x; g < total; g += gridDim.x) {
+        const int s = find_slot(tile_pre, S, g);
+        const int64_t k = (int64_t)(g - tile_pre[s]) * kBT + threadIdx.x;
+        const int64_t o = (int64_t)s * cap + k;
+        const int nl = prm[s].nl;
+        const double* sbx = bx + (int64_t)s * (mp + 1);
+        const double* sbz = bz + (int64_t)s * (mp + 1);
         bool in = false;
         double x0 = 0.0, z0 = 0.0;
-        if (g < total) {
-            s = find_slot(blob_pre, S, g);
-            o = (int64_t)s * cap + (g - blob_pre[s]);
+        if (k < st[s].n) {
             x0 = b.x[o];
             z0 = b.z[o];
             EdgeSlabs es;
@@ -864,8 +864,7 @@ __global__ __launch_bounds__(256) void bt_inside_kernel(
             es.n = prm[s].nslab;
             es.z0 = prm[s].slab_z0;
             es.inv_h = prm[s].slab_inv_h;
-            in = inside_body_fast(prm[s].nl, bx + (int64_t)s * (mp + 1),
-                                  bz + (int64_t)s * (mp + 1), prm[s].box, es, x0, z0);
+            in = inside_body_fast(nl, sbx, sbz, prm[s].box, es, x0, z0);
             flags[o] = in ? 1 : 0;
         }
         if (!reflect) continue;
@@ -873,11 +872,9 @@ __global__ __launch_bounds__(256) void bt_inside_kernel(
         while (mask) {
             const int l = __ffsll((unsigned long long)mask) - 1;
             mask &= mask - 1;
-            const int sl = __shfl(s, l);
             const double px = __shfl(x0, l), pz = __shfl(z0, l);
-            const double2 p = reflect_wave(prm[sl].nl, bx + (int64_t)sl * (mp + 1),
-                                           bz + (int64_t)sl * (mp + 1), enx + (int64_t)sl * mp,
-                                           enz + (int64_t)sl * mp, px, pz);
+            const double2 p = reflect_wave(nl, sbx, sbz, enx + (int64_t)s * mp, enz + (int64_t)s * mp,
+                                           px, pz);
             if (lane == l) {
                 b.x[o] = p.x;
                 b.z[o] = p.y;
@@ -1122,8 +1119,9 @@ void record_rest(dvm_batch_t B)
     bt_prefix_kernel<<<1, 1024, 0, st>>>(S, cap, d.st, d.wl, d.tile_pre, d.order, d.otile_pre,
                                          d.blob_pre, d.item_pre, d.seg_begin, d.seg_end);
     const int reflect = B->cfg.crossing != DVM_DELETE ? 1 : 0;
-    bt_inside_kernel<<<gs, 256, 0, st>>>(S, cap, mp, d.wl, d.blob_pre, d.prm, d.bx, d.bz, d.enx,
-                                         d.enz, d.slab_off, d.slab_edge, reflect, d.b, d.flags);
+    bt_inside_kernel<<<gs, kBT, 0, st>>>(S, cap, mp, d.wl, d.tile_pre, d.st, d.prm, d.bx, d.bz,
+                                         d.enx, d.enz, d.slab_off, d.slab_edge, reflect, d.b,
+                                         d.flags);
     if (!reflect) bt_delete_kernel<<<S, 1024, 0, st>>>(cap, d.st, d.flags, d.b, d.alt);
     bt_loads_finish_kernel<<<S, kLoadsThreads, 0, st>>>(S, mp, d.prm, d.st, d.gamma, d.ds, d.enx,
                                                          d.enz, d.lwork);

@@ -11,12 +11,14 @@
  * so the restatement is pinned by the reference's own known-answer tests,
  * closed forms, invariants and long-double brute force (tests/test_oracle_*.py).
  *
- * Parity unpinned (nothing outside the reading of the reference fixes them):
- * the summation order of arma::sum (totalcirc, the force sums), the BLAS
- * order of A^T A and A^T b and the factorisation arma::solve uses, and the
+ * Every function has such a check.  What only the reference binary could fix
+ * is the last bits of a few results, which the tests compare to a tolerance
+ * instead: the summation order of arma::sum (totalcirc, the force sums), the
+ * BLAS order of A^T A and A^T b and the factorisation arma::solve uses.  The
  * w component of the sheet term of orc_sheet_probe_quirk (probe_velocities'
- * py, qy coefficients, src/VortexSheet.cpp:621-626, match no panel-velocity
- * closed form; its u component is pinned by quadrature).
+ * altered py, qy coefficients, src/VortexSheet.cpp:611-637) is pinned by
+ * rebuilding it from the panel's geometry and the quadrature-pinned
+ * vortexsheetbc pair (tests/test_oracle_sheet.py).
  */
 #ifndef DVM_ORACLE_H
 #define DVM_ORACLE_H

@@ -291,3 +291,84 @@ def test_probe_quirk_sheet_term(params):
     sh.gamma = gam2
     _, _, uvs2, wvs2 = sh.probe_quirk([], [], [], [], prx, prz)
     assert np.array_equal(uvs, uvs2) and np.array_equal(wvs, wvs2)
+
+
+def _quirk_w_from_geometry(prx, prz, xa, za, th, ds, ga, g0, variant=None):
+    """The w part of probe_velocities' sheet term (src/VortexSheet.cpp:611-637),
+    built from the panel's geometry and the vortexsheetbc coefficients.
+
+    vortexsheetbc's (py, qy) pair of the panel (:248-250) is minus 2 pi times
+    the quadrature w velocity of a unit linear strength at either end, which
+    test_sheet_bc_matches_quadrature pins.  probe_velocities changes three
+    things against it: c9 takes +dz sin(-2 theta) where vortexsheetbc takes
+    -dz sin(-2 theta) (:619-620 against :240), qy's c7 factor is
+    (c1 cos - c5 sin) where vortexsheetbc has (c1 sin - c5 cos) (:624-625
+    against :248-249), and py's c6 term has the opposite sign (:626 against
+    :250); the gamma_0 product is subtracted (:634).  With s and n the
+    probe's tangential and normal coordinates about the panel origin A and B
+    the panel's far end, c1 = -s, c5 = n, c6 = ln(|P-B|^2 / |P-A|^2) and c7
+    is the signed angle from P-B to P-A (the angle AB subtends at P), so
+
+        qy' = qy - dz sin(2 theta) c6 / ds + (n - s)(cos theta - sin theta) c7 / ds
+        py' = py - c6 cos theta - (qy' - qy)
+        w   = py' ga - qy' g0.
+
+    `variant` plants one plausible transcription mistake in this formula so a
+    test can show that the tolerance separates it.
+    """
+    c, s_ = math.cos(th), math.sin(th)
+    xb, zb = xa + ds * c, za + ds * s_
+    pa = np.array([prx - xa, prz - za])
+    pb = np.array([prx - xb, prz - zb])
+    c6 = math.log(float(pb @ pb) / float(pa @ pa))
+    c7 = math.atan2(pb[0] * pa[1] - pb[1] * pa[0], float(pa @ pb))
+    t = pa[0] * c + pa[1] * s_          # tangential coordinate of P
+    nn = pa[0] * s_ - pa[1] * c         # normal coordinate (c5's sign)
+    _, w10 = panel_velocity(prx, prz, xa, za, th, ds, 1.0, 0.0)
+    _, w01 = panel_velocity(prx, prz, xa, za, th, ds, 0.0, 1.0)
+    py = -w10 / RP2
+    qy = -w01 / RP2
+    dq = -pa[1] * math.sin(2 * th) * c6 / ds + (nn - t) * (c - s_) * c7 / ds
+    dp = -c6 * c
+    if variant == "c9_as_bc":
+        dq = (nn - t) * (c - s_) * c7 / ds
+    elif variant == "qy_c7_as_bc":
+        dq = -pa[1] * math.sin(2 * th) * c6 / ds
+    elif variant == "py_c6_as_bc":
+        dp = 0.0
+    elif variant == "gamma0_added":
+        return (py + dp - dq) * ga + (qy + dq) * g0
+    return (py + dp - dq) * ga - (qy + dq) * g0
+
+
+def test_probe_quirk_w_term_geometry(params):
+    # Pins the w component of probe_velocities' sheet term, which no closed
+    # form of a panel velocity gives: it is vortexsheetbc's pair plus the
+    # terms the source's sign changes add, each from the geometry.
+    rng = np.random.default_rng(17)
+    for body in ("bodycoordinatesN40.txt", "simple_body.txt"):
+        x, z = load_body(body)
+        sh = oracle.Sheet(x, z, params)
+        g = sh.geometry()
+        nl = sh.nl
+        gam = rng.normal(size=nl)
+        sh.gamma = gam
+        ang = rng.uniform(0, 2 * math.pi, 12)
+        rad = rng.uniform(0.7, 4.0, 12)
+        prx = g["xc"].mean() + rad * np.cos(ang)
+        prz = g["zc"].mean() + rad * np.sin(ang)
+        _, w, _, wvs = sh.probe_quirk([], [], [], [], prx, prz)
+        L = nl - 1
+        args = (g["xc"][L], g["zc"][L], g["theta"][L], g["ds"][L], gam[L], gam[0])
+        scale = max(1.0, np.abs(wvs).max())
+        for k in range(prx.size):
+            ref = _quirk_w_from_geometry(prx[k], prz[k], *args)
+            assert abs(wvs[k] - ref) < 1e-10 * scale, (body, k, wvs[k], ref)
+            # no blobs: w = sheet term + Uz (Uz = 0 here)
+            assert w[k] == wvs[k] + params.uz
+        # each plausible slip in the three changed lines moves w by far more
+        # than the tolerance at these probes
+        for variant in ("c9_as_bc", "qy_c7_as_bc", "py_c6_as_bc", "gamma0_added"):
+            worst = max(abs(wvs[k] - _quirk_w_from_geometry(prx[k], prz[k], *args, variant=variant))
+                        for k in range(prx.size))
+            assert worst > 1e-4 * scale, variant

@@ -40,11 +40,59 @@ constexpr int kBlock = 256;  // targets per FP64 block (one per lane, 4 waves)
 
 // ----------------------------------------------------------------- FP64 form
 
+// Per-source record of the FP64 form: position, circulation and the
+// near-field threshold thr = 10 * sigma^2 exactly as the reference forms it
+// per pair (src/VelocityKernel.cpp:48, sigma^2 = sigma * sigma), so a pair is
+// near or far exactly when it is in the reference.  Inside the near branch
+// 1/sigma^2 is taken as 10 / thr, within an ulp of the reference's 1/sigma^2.
+// Records of the FP64 form are padded by this many entries so the prefetch of
+// a group past the end of a source range stays inside the allocation.
+constexpr int kRec64Pad = 8;
+
+// Writes n records and kRec64Pad zero records after them: a group's unused
+// slots then hold finite values, whose terms are multiplied by zero.
+__global__ void src_rec64_kernel(int64_t n, const double* __restrict__ x,
+                                 const double* __restrict__ z, const double* __restrict__ c,
+                                 const double* __restrict__ s, double4* __restrict__ rec)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n + kRec64Pad) return;
+    if (j >= n) {
+        rec[j] = make_double4(0.0, 0.0, 0.0, 0.0);
+        return;
+    }
+    const double sj = s[j];
+    rec[j] = make_double4(x[j], z[j], c[j], 10.0 * (sj * sj));
+}
+
+// 1/a from the hardware reciprocal and two Newton steps on the FMA: below one
+// unit in the last place (the correctly rounded quotient the reference's
+// divide gives can differ in that last bit).  a = 0 gives NaN, as 0/0 does.
+__device__ __forceinline__ double recip64(double a)
+{
+    double y = __builtin_amdgcn_rcp(a);
+    double e = __builtin_fma(-a, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-a, y, 1.0);
+    return __builtin_fma(y, e, y);
+}
+
+struct Rec64x4 {
+    double4 r[4];
+};
+
+// One target per lane; sources in index order, four at a time, through
+// wave-uniform (scalar) loads with the next group's records in flight while
+// the current group is summed.  The pair is that of
+// src/VelocityKernel.cpp:44-60 with K = num * (1/r^2) in place of num / r^2
+// and the two accumulations as fused multiply-adds: each pair term is within
+// a few units in the last place of the reference's; the far pairs of a group
+// run without branches and only a group with a near pair in some lane enters
+// the near-field branch.
 template <bool SELF, bool GUARD>
 __global__ __launch_bounds__(kBlock) void bs_f64_kernel(
     int64_t nt, const double* __restrict__ tx, const double* __restrict__ tz, int64_t ns,
-    const double* __restrict__ sx, const double* __restrict__ sz, const double* __restrict__ sc,
-    const double* __restrict__ ss, int64_t chunk, double r2_skip, double* __restrict__ u,
+    const double4* __restrict__ rec, int64_t chunk, double r2_skip, double* __restrict__ u,
     double* __restrict__ w, double* __restrict__ part)
 {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -56,30 +104,55 @@ __global__ __launch_bounds__(kBlock) void bs_f64_kernel(
     double ui = 0.0;
     double wi = 0.0;
 
-    // One pair, in the order of src/VelocityKernel.cpp:44-60.
-    auto pair = [&](int64_t j, bool skip_self) {
-        const double xj = sx[j];
-        const double zj = sz[j];
-        const double cj = sc[j];
-        const double sj = ss[j];
-        const double dx = xi - xj;
-        const double dz = zi - zj;
-        const double r2 = dx * dx + dz * dz;
-        bool take = true;
-        if (SELF) take = !(skip_self && j == i);
-        if (GUARD) take = take && (r2 > r2_skip);
-        const double s2 = sj * sj;
-        const double thr = 10.0 * s2;
-        double num = 1.0;
-        if (take && r2 < thr) {
-            const double rs = 1.0 / s2;
-            num = 1.0 - exp(-r2 * rs);
+    // Sum of the pairs with sources g.r[0..cnt-1] (source index jb + q).
+    auto group = [&](const Rec64x4& g, int cnt, int64_t jb, bool self_check) {
+        double dx[4], dz[4], r2[4], K[4];
+        bool near[4], take[4];
+        bool any_near = false;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            dx[q] = xi - g.r[q].x;
+            dz[q] = zi - g.r[q].y;
+            r2[q] = __builtin_fma(dx[q], dx[q], dz[q] * dz[q]);
+            take[q] = q < cnt;
+            if (SELF) take[q] = take[q] && !(self_check && jb + q == i);
+            if (GUARD) take[q] = take[q] && (r2[q] > r2_skip);
+            K[q] = recip64(r2[q]);
+            near[q] = take[q] && r2[q] < g.r[q].w;
+            any_near = any_near || near[q];
         }
-        const double K = num / r2;
-        if (take) {
-            ui -= K * dz * cj;
-            wi += K * dx * cj;
+        if (any_near) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (near[q]) K[q] *= 1.0 - exp(-r2[q] * (10.0 / g.r[q].w));
         }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double kc = take[q] ? K[q] * g.r[q].z : 0.0;
+            ui = __builtin_fma(-kc, dz[q], ui);
+            wi = __builtin_fma(kc, dx[q], wi);
+        }
+    };
+
+    // Sources jb .. je-1; the records of the next group are loaded before the
+    // current group is summed (they may lie past je: the array is padded).
+    auto range = [&](int64_t jb, int64_t je, bool self_check) {
+        if (jb >= je) return;
+        const Rec64x4* q = reinterpret_cast<const Rec64x4*>(rec + jb);
+        Rec64x4 a = q[0];
+        int64_t j = jb;
+        for (; j + 4 <= je; j += 4) {
+            __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+            __builtin_amdgcn_sched_barrier(0);
+            const Rec64x4 b = *reinterpret_cast<const Rec64x4*>(
+                rec + __builtin_amdgcn_readfirstlane((int)(j + 4)));
+            __builtin_amdgcn_sched_barrier(0);
+            group(a, 4, j, self_check);
+            __builtin_amdgcn_sched_barrier(0);
+            a = b;
+        }
+        if (j < je) group(a, (int)(je - j), j, self_check);
     };
 
     if (SELF) {
@@ -87,17 +160,13 @@ __global__ __launch_bounds__(kBlock) void bs_f64_kernel(
         const int64_t b0 = (int64_t)blockIdx.x * kBlock;
         const int64_t b1 = b0 + kBlock < nt ? b0 + kBlock : nt;
         const int64_t a_end = j1 < b0 ? j1 : b0;
-        int64_t j = j0;
-#pragma unroll 4
-        for (; j < a_end; ++j) pair(j, false);
+        range(j0, a_end, false);
+        const int64_t m_beg = j0 > b0 ? j0 : b0;
         const int64_t m_end = j1 < b1 ? j1 : b1;
-        for (j = (j0 > b0 ? j0 : b0); j < m_end; ++j) pair(j, true);
-        j = j0 > b1 ? j0 : b1;
-#pragma unroll 4
-        for (; j < j1; ++j) pair(j, false);
+        range(m_beg, m_end, true);
+        range(j0 > b1 ? j0 : b1, j1, false);
     } else {
-#pragma unroll 4
-        for (int64_t j = j0; j < j1; ++j) pair(j, false);
+        range(j0, j1, false);
     }
 
     if (i < nt) {
@@ -596,24 +665,29 @@ static void launch_f64(dvm_ctx_t ctx, bool self, int64_t ns, const double* sx, c
     const int64_t bx = cdiv(nt, kBlock);
     // Keep chunks of >= 2048 sources (>= 512 when the targets fill at most
     // 16 blocks — collocation points, probes — so that enough blocks run); a
-    // single chunk keeps the reference's exact summation order.
+    // single chunk keeps the reference's summation order.
     const int64_t min_chunk = bx <= 16 ? 512 : 2048;
     const int nch = pick_chunks(ctx, bx, (ns + min_chunk - 1) / min_chunk);
     const int64_t chunk = (ns + nch - 1) / nch;
-    double* part = nullptr;
-    if (nch > 1) part = static_cast<double*>(ws_get(ctx, sizeof(double) * 2 * nt * nch));
+    const size_t part_bytes = nch > 1 ? sizeof(double) * 2 * nt * nch : 0;
+    Carver cv(ws_get(ctx, Carver::need({sizeof(double4) * (size_t)(ns + kRec64Pad), part_bytes})));
+    double4* rec = cv.take<double4>((size_t)(ns + kRec64Pad));
+    double* part = nch > 1 ? cv.take<double>(2 * (size_t)nt * nch) : nullptr;
     dim3 grid((unsigned)bx, (unsigned)nch);
     {
         ProfScope ps(ctx, self ? "bs_self_f64" : "bs_targets_f64");
+        src_rec64_kernel<<<cdiv(ns + kRec64Pad, 256), 256, 0, ctx->stream>>>(ns, sx, sz, sc, ss,
+                                                                              rec);
+        DVM_LAUNCH_CHECK();
         if (self) {
             bs_f64_kernel<true, false><<<grid, kBlock, 0, ctx->stream>>>(
-                nt, tx, tz, ns, sx, sz, sc, ss, chunk, r2_skip, u, w, part);
+                nt, tx, tz, ns, rec, chunk, r2_skip, u, w, part);
         } else if (guard) {
             bs_f64_kernel<false, true><<<grid, kBlock, 0, ctx->stream>>>(
-                nt, tx, tz, ns, sx, sz, sc, ss, chunk, r2_skip, u, w, part);
+                nt, tx, tz, ns, rec, chunk, r2_skip, u, w, part);
         } else {
             bs_f64_kernel<false, false><<<grid, kBlock, 0, ctx->stream>>>(
-                nt, tx, tz, ns, sx, sz, sc, ss, chunk, r2_skip, u, w, part);
+                nt, tx, tz, ns, rec, chunk, r2_skip, u, w, part);
         }
         DVM_LAUNCH_CHECK();
     }

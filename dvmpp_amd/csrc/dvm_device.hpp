@@ -375,38 +375,77 @@ __device__ __forceinline__ void far_tile_share2(const float4* __restrict__ sp, f
     }
 }
 
+// s_waitcnt immediate that waits for every outstanding scalar (and LDS) load
+// and for nothing else: lgkmcnt = 0, vmcnt = 63 and expcnt = 7 (no wait).
+constexpr unsigned kWaitLgkm0 = 0xc07f;
+
+// Eight consecutive source records: one 128-byte scalar load.
+struct Src8 {
+    float4 v[8];
+};
+
+__device__ __forceinline__ void far_quad(const float4 sa, const float4 sb, const float4 sc,
+                                         const float4 sd, float txd, float tzd, float& au,
+                                         float& aw)
+{
+    const float dxa = txd - sa.x, dza = tzd - sa.y;
+    const float dxb = txd - sb.x, dzb = tzd - sb.y;
+    const float dxc = txd - sc.x, dzc = tzd - sc.y;
+    const float dxd = txd - sd.x, dzd = tzd - sd.y;
+    const float ra = __builtin_fmaf(dxa, dxa, dza * dza);
+    const float rb = __builtin_fmaf(dxb, dxb, dzb * dzb);
+    const float rc = __builtin_fmaf(dxc, dxc, dzc * dzc);
+    const float rd = __builtin_fmaf(dxd, dxd, dzd * dzd);
+    const float pab = ra * rb;
+    const float pcd = rc * rd;
+    const float inv = __builtin_amdgcn_rcpf(pab * pcd);
+    const float iab = inv * pcd;
+    const float icd = inv * pab;
+    const float ka = (rb * iab) * sa.z;
+    const float kb = (ra * iab) * sb.z;
+    const float kc = (rd * icd) * sc.z;
+    const float kd = (rc * icd) * sd.z;
+    au = __builtin_fmaf(ka, dza, au);
+    aw = __builtin_fmaf(ka, dxa, aw);
+    au = __builtin_fmaf(kb, dzb, au);
+    aw = __builtin_fmaf(kb, dxb, aw);
+    au = __builtin_fmaf(kc, dzc, au);
+    aw = __builtin_fmaf(kc, dxc, aw);
+    au = __builtin_fmaf(kd, dzd, au);
+    aw = __builtin_fmaf(kd, dxd, aw);
+}
+
+// One reciprocal for four sources.  The records go through scalar loads in
+// batches of eight with the next batch in flight while the current one is
+// summed: an explicit wait for all scalar loads (they complete out of order,
+// so no partial wait exists) comes before each batch's loads are issued, and
+// scheduling barriers keep the compiler from moving that wait below the
+// loads or the arithmetic above them.  The batch index goes through
+// readfirstlane so the address stays a base plus immediate offsets.
 template <int T>
 __device__ __forceinline__ void far_tile_share4(const float4* __restrict__ sp, float txd,
                                                 float tzd, float& au, float& aw)
 {
-#pragma unroll 4
-    for (int k = 0; k < T; k += 4) {
-        const float4 sa = sp[k], sb = sp[k + 1], sc = sp[k + 2], sd = sp[k + 3];
-        const float dxa = txd - sa.x, dza = tzd - sa.y;
-        const float dxb = txd - sb.x, dzb = tzd - sb.y;
-        const float dxc = txd - sc.x, dzc = tzd - sc.y;
-        const float dxd = txd - sd.x, dzd = tzd - sd.y;
-        const float ra = __builtin_fmaf(dxa, dxa, dza * dza);
-        const float rb = __builtin_fmaf(dxb, dxb, dzb * dzb);
-        const float rc = __builtin_fmaf(dxc, dxc, dzc * dzc);
-        const float rd = __builtin_fmaf(dxd, dxd, dzd * dzd);
-        const float pab = ra * rb;
-        const float pcd = rc * rd;
-        const float inv = __builtin_amdgcn_rcpf(pab * pcd);
-        const float iab = inv * pcd;
-        const float icd = inv * pab;
-        const float ka = (rb * iab) * sa.z;
-        const float kb = (ra * iab) * sb.z;
-        const float kc = (rd * icd) * sc.z;
-        const float kd = (rc * icd) * sd.z;
-        au = __builtin_fmaf(ka, dza, au);
-        aw = __builtin_fmaf(ka, dxa, aw);
-        au = __builtin_fmaf(kb, dzb, au);
-        aw = __builtin_fmaf(kb, dxb, aw);
-        au = __builtin_fmaf(kc, dzc, au);
-        aw = __builtin_fmaf(kc, dxc, aw);
-        au = __builtin_fmaf(kd, dzd, au);
-        aw = __builtin_fmaf(kd, dxd, aw);
+    const Src8* __restrict__ q = reinterpret_cast<const Src8*>(sp);
+    Src8 a = q[0];
+#pragma unroll 1
+    for (int b = 0; b < T / 8; b += 2) {
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+        __builtin_amdgcn_sched_barrier(0);
+        const Src8 c = q[__builtin_amdgcn_readfirstlane(b + 1)];
+        __builtin_amdgcn_sched_barrier(0);
+        far_quad(a.v[0], a.v[1], a.v[2], a.v[3], txd, tzd, au, aw);
+        __builtin_amdgcn_sched_barrier(0);
+        far_quad(a.v[4], a.v[5], a.v[6], a.v[7], txd, tzd, au, aw);
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+        __builtin_amdgcn_sched_barrier(0);
+        a = q[__builtin_amdgcn_readfirstlane(b + 2 < T / 8 ? b + 2 : b + 1)];
+        __builtin_amdgcn_sched_barrier(0);
+        far_quad(c.v[0], c.v[1], c.v[2], c.v[3], txd, tzd, au, aw);
+        __builtin_amdgcn_sched_barrier(0);
+        far_quad(c.v[4], c.v[5], c.v[6], c.v[7], txd, tzd, au, aw);
     }
 }
 

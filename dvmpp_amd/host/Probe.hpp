@@ -1,12 +1,13 @@
 // Velocity probe points (src/Probe.hpp:16-31): positions and the velocities
-// VortexSheet::probe_velocities writes.  The XML constructor and the output
-// file of the reference are not part of this layer.
+// VortexSheet::probe_velocities writes, and the reference's output file.
 #ifndef DVMPP_AMD_PROBE_HPP
 #define DVMPP_AMD_PROBE_HPP
 
+#include <string>
 #include <vector>
 
 #include "BaseTypes.hpp"
+#include "XmlHandler.hpp"
 
 class Probe {
 public:
@@ -27,7 +28,15 @@ public:
             m_z(i) = z[i];
         }
     }
+    /// src/Probe.cpp:12-34: positions from <probe> x and z; creates
+    /// <output_dir>/<timestamp>_probe.dat with its header.
+    Probe(const XmlHandler& xml, const std::string& timestamp);
     [[nodiscard]] unsigned size() const { return (unsigned)m_x.size(); }
+    /// Appends one row per probe (src/Probe.cpp:40-51).
+    void write_step(double time);
+
+private:
+    std::string m_outfile;
 };
 
 #endif

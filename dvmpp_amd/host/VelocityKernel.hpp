@@ -44,8 +44,6 @@ public:
                                   std::vector<VelocityResult>& velocities) = 0;
     [[nodiscard]] static std::unique_ptr<VelocityKernel> create(const std::string& type);
     [[nodiscard]] virtual std::string name() const = 0;
-    /// dvm_precision this kernel passes to the C-ABI.
-    [[nodiscard]] virtual int precision() const = 0;
 };
 
 class DirectKernel : public VelocityKernel {
@@ -57,7 +55,9 @@ public:
                           const std::vector<double>& target_z,
                           std::vector<VelocityResult>& velocities) override;
     [[nodiscard]] std::string name() const override { return "direct"; }
-    [[nodiscard]] int precision() const override;
+    /// dvm_precision this kernel passes to the C-ABI (not part of the
+    /// reference's abstract interface, so its subclasses need not define it).
+    [[nodiscard]] virtual int precision() const;
 };
 
 class DirectF32CKernel : public DirectKernel {

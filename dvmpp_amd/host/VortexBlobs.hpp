@@ -1,6 +1,6 @@
 // Vortex blobs (src/VortexBlobs.hpp:21-109): public SoA fields in host memory;
-// the compute methods run on the GPU through the C-ABI.  Output-file members
-// of the reference are not part of this layer.
+// the compute methods run on the GPU through the C-ABI; the output files are
+// the reference's.
 #ifndef DVMPP_AMD_VORTEXBLOBS_HPP
 #define DVMPP_AMD_VORTEXBLOBS_HPP
 
@@ -11,6 +11,7 @@
 #include "BaseTypes.hpp"
 #include "Exceptions.hpp"
 #include "Random.hpp"
+#include "XmlHandler.hpp"
 
 namespace dvm {
 class VelocityKernel;
@@ -29,6 +30,9 @@ public:
     Vector m_wvs;
 
     VortexBlobs();
+    /// src/VortexBlobs.cpp:16-42: creates <output_dir>/<timestamp>_vortex.dat
+    /// and _vortex_num.dat with their headers (no blobs yet).
+    VortexBlobs(const XmlHandler& xml, const std::string& timestamp);
     explicit VortexBlobs(unsigned N);
     ~VortexBlobs();
     VortexBlobs(VortexBlobs&& other) noexcept;
@@ -49,10 +53,18 @@ public:
     unsigned merge_blobs(double threshold);
     /// Throws SizeMismatchException if the fields disagree (src/VortexBlobs.cpp:272-285).
     [[nodiscard]] unsigned size() const;
+    void print_location() const;
+    void print_velocity() const;
+    void print_circulation() const;
+    /// Appends the blob rows and the count row (src/VortexBlobs.cpp:310-331).
+    void write_step(double time, unsigned step);
     void setKernelType(const std::string& type);
     [[nodiscard]] std::string kernelType() const;
+    void copyOutputConfig(const VortexBlobs& other);
 
 private:
+    std::string m_blobsfile;
+    std::string m_numfile;
     std::unique_ptr<dvm::VelocityKernel> m_kernel;
 };
 

@@ -146,6 +146,24 @@ void DirectKernel::computeInducedAt(const std::vector<VortexParticle>& sources,
 
 VortexBlobs::VortexBlobs() : m_kernel(dvm::VelocityKernel::create("direct")) {}
 
+VortexBlobs::VortexBlobs(const XmlHandler& xml, const std::string& stamp) : VortexBlobs()
+{
+    const std::filesystem::path outpath(xml.getStringAttribute("io", "output_dir", true));
+    const double dt = xml.getValueAttribute("time", "dt");
+    const double steps = xml.getValueAttribute("time", "steps");
+    m_blobsfile = (outpath / (stamp + "_vortex.dat")).string();
+    std::ofstream bf(m_blobsfile);
+    if (!bf) throw dvm::FileIOException::cannotOpen(m_blobsfile);
+    bf << size() << " # Number of nodes\n"
+       << dt << " # Time step\n"
+       << steps << " # Steps\n"
+       << "Time [s]\tx-position [m]\tz-position [m]\tcirculation\n";
+    m_numfile = (outpath / (stamp + "_vortex_num.dat")).string();
+    std::ofstream nf(m_numfile);
+    if (!nf) throw dvm::FileIOException::cannotOpen(m_numfile);
+    nf << "Time [s]\tNumber of vortices\n";
+}
+
 VortexBlobs::VortexBlobs(unsigned N) : VortexBlobs()
 {
     m_ID.assign(N, 0u);
@@ -158,8 +176,8 @@ VortexBlobs& VortexBlobs::operator=(VortexBlobs&& other) noexcept = default;
 
 VortexBlobs::VortexBlobs(const VortexBlobs& o)
     : m_ID(o.m_ID), m_x(o.m_x), m_z(o.m_z), m_circ(o.m_circ), m_sigma(o.m_sigma), m_u(o.m_u),
-      m_w(o.m_w), m_uvs(o.m_uvs), m_wvs(o.m_wvs),
-      m_kernel(dvm::VelocityKernel::create(o.m_kernel ? o.m_kernel->name() : "direct"))
+      m_w(o.m_w), m_uvs(o.m_uvs), m_wvs(o.m_wvs), m_blobsfile(o.m_blobsfile),
+      m_numfile(o.m_numfile), m_kernel(dvm::VelocityKernel::create(o.m_kernel ? o.m_kernel->name() : "direct"))
 {
 }
 
@@ -168,6 +186,7 @@ VortexBlobs& VortexBlobs::operator=(const VortexBlobs& o)
     if (this != &o) {
         m_ID = o.m_ID; m_x = o.m_x; m_z = o.m_z; m_circ = o.m_circ; m_sigma = o.m_sigma;
         m_u = o.m_u; m_w = o.m_w; m_uvs = o.m_uvs; m_wvs = o.m_wvs;
+        m_blobsfile = o.m_blobsfile; m_numfile = o.m_numfile;
         m_kernel = dvm::VelocityKernel::create(o.m_kernel ? o.m_kernel->name() : "direct");
     }
     return *this;
@@ -176,6 +195,70 @@ VortexBlobs& VortexBlobs::operator=(const VortexBlobs& o)
 void VortexBlobs::setKernelType(const std::string& type) { m_kernel = dvm::VelocityKernel::create(type); }
 
 std::string VortexBlobs::kernelType() const { return m_kernel->name(); }
+
+void VortexBlobs::copyOutputConfig(const VortexBlobs& other)
+{
+    m_blobsfile = other.m_blobsfile;
+    m_numfile = other.m_numfile;
+}
+
+void VortexBlobs::print_location() const
+{
+    for (unsigned i = 0; i < size(); i++)
+        std::cout << " The location of Vortex Blob " << m_ID[i] << " is (x,z) = (" << m_x(i) << ","
+                  << m_z(i) << ")" << std::endl;
+}
+
+void VortexBlobs::print_velocity() const
+{
+    for (unsigned i = 0; i < size(); i++)
+        std::cout << "(u,w) = (" << m_u(i) << "," << m_w(i) << ")" << std::endl;
+}
+
+void VortexBlobs::print_circulation() const
+{
+    for (unsigned i = 0; i < size(); i++) std::cout << " circ = " << m_circ(i) << std::endl;
+}
+
+void VortexBlobs::write_step(double time, unsigned step)
+{
+    if (m_blobsfile.empty() || m_numfile.empty())
+        throw dvm::FileIOException("VortexBlobs::write_step: must set filenames before writing");
+    std::ofstream bf(m_blobsfile, std::ios_base::app);
+    if (!bf) throw dvm::FileIOException::cannotWrite(m_blobsfile);
+    for (unsigned i = 0; i < size(); ++i)
+        bf << time << "\t" << m_x(i) << "\t" << m_z(i) << "\t" << m_circ(i) << "\n";
+    std::ofstream nf(m_numfile, std::ios_base::app);
+    if (!nf) throw dvm::FileIOException::cannotWrite(m_numfile);
+    nf << step << "\t" << size() << "\n";
+}
+
+// ---------------------------------------------------------------- Probe
+
+Probe::Probe(const XmlHandler& xml, const std::string& stamp)
+{
+    const std::vector<double> x = xml.getList("probe", "x");
+    const std::vector<double> z = xml.getList("probe", "z");
+    m_x.zeros(x.size());
+    m_z.zeros(z.size());
+    for (std::size_t i = 0; i < x.size(); ++i) m_x(i) = x[i];
+    for (std::size_t i = 0; i < z.size(); ++i) m_z(i) = z[i];
+    for (Vector* v : {&m_u, &m_w, &m_uvs, &m_wvs}) v->zeros(x.size());
+    const std::filesystem::path outpath(xml.getStringAttribute("io", "output_dir", true));
+    m_outfile = (outpath / (stamp + "_probe.dat")).string();
+    std::ofstream of(m_outfile);
+    if (!of) throw dvm::FileIOException::cannotOpen(m_outfile);
+    of << xml.getValueAttribute("time", "dt") << " # Time step\n"
+       << xml.getValueAttribute("time", "steps") << " # Steps\n"
+       << "Time [s]\tu [m/s]\tw [m/s]\n";
+}
+
+void Probe::write_step(double time)
+{
+    std::ofstream of(m_outfile, std::ios_base::app);
+    if (!of) throw dvm::FileIOException::cannotWrite(m_outfile);
+    for (unsigned i = 0; i < size(); ++i) of << time << "\t" << m_u(i) << "\t" << m_w(i) << "\n";
+}
 
 void VortexBlobs::append_vortices(VortexBlobs& nb)
 {
@@ -317,6 +400,43 @@ VortexSheet::VortexSheet(const std::string& body_file, const dvm_sheet_params& p
     init(x, z);
 }
 
+VortexSheet::VortexSheet() = default;
+
+VortexSheet::VortexSheet(const XmlHandler& xml, const std::string& stamp, int precision)
+    : m_prec(precision)
+{
+    m_params.rho = xml.getValueAttribute("constants", "density");
+    m_params.nu = xml.getValueAttribute("constants", "nu");
+    m_params.dt = xml.getValueAttribute("time", "dt");
+    m_params.ux = xml.getValueAttribute("flow", "ux");
+    m_params.uz = xml.getValueAttribute("flow", "uz");
+    m_params.max_npv = (int)static_cast<unsigned>(xml.getValueAttribute("constants", "max_NumPanelVort"));
+    m_params.cutoff_exp = xml.getValueAttribute("constants", "cutoff_exp");
+
+    const std::filesystem::path domain = std::filesystem::path(xml.getStringAttribute("io", "input_dir")) /
+                                         xml.getStringAttribute("io", "domain_file");
+    std::vector<double> x, z;
+    read_body(domain.string(), x, z);
+    std::cout << "Successfully loaded coordinate file with " << x.size() << " points." << std::endl;
+    std::cout << "A = " << *std::max_element(x.begin(), x.end()) - *std::min_element(x.begin(), x.end())
+              << " (assuming cylinder)\n";
+    init(x, z);
+    std::cout << "Created vortex sheet of size " << m_nl << std::endl;
+
+    const std::filesystem::path outpath(xml.getStringAttribute("io", "output_dir", true));
+    const std::string gamma_path = (outpath / (stamp + "_gamma.dat")).string();
+    m_gammafile.open(gamma_path);
+    if (!m_gammafile) throw dvm::FileIOException::cannotOpen(gamma_path);
+    m_gammafile << m_nl << " # Number of collocation points\n"
+                << m_params.dt << " # Time Step\n"
+                << xml.getValueAttribute("time", "steps") << " # Steps\n"
+                << "Time [s] Gamma - Vortex sheet strength\n";
+    const std::string force_path = (outpath / (stamp + "_loads.dat")).string();
+    m_forcefile.open(force_path);
+    if (!m_forcefile) throw dvm::FileIOException::cannotOpen(force_path);
+    m_forcefile << "Time [s]\tF_x [-]\tF_z [s]\n";
+}
+
 void VortexSheet::init(const std::vector<double>& x, const std::vector<double>& z)
 {
     if (x.size() != z.size()) throw dvm::SizeMismatchException("VortexSheet body");
@@ -326,6 +446,50 @@ void VortexSheet::init(const std::vector<double>& x, const std::vector<double>& 
     int nl = 0;
     dvm::check(dvm_sheet_size(m_sheet, &nl), "VortexSheet::size", ctx);
     m_nl = (unsigned)nl;
+    for (Vector* v : {&m_xc, &m_zc, &m_ds, &m_enx, &m_enz, &m_etx, &m_etz}) v->zeros(m_nl);
+    double aref = 0.0;
+    dvm::check(dvm_sheet_geometry(m_sheet, m_xc.memptr(), m_zc.memptr(), nullptr, m_ds.memptr(),
+                                  m_enx.memptr(), m_enz.memptr(), m_etx.memptr(), m_etz.memptr(),
+                                  &aref),
+               "VortexSheet geometry", ctx);
+}
+
+void VortexSheet::need_body(const char* where) const
+{
+    if (!m_sheet)
+        throw dvm::DVMException(std::string("VortexSheet::") + where +
+                                ": the sheet has no body (default-constructed)");
+}
+
+void VortexSheet::write_step(double time)
+{
+    if (!m_gammafile.is_open() || !m_forcefile.is_open())
+        throw dvm::FileIOException("VortexSheet::write_step: no output files (not built from an input file)");
+    const Vector g = m_nl ? gamma() : Vector();
+    for (unsigned i = 0; i < m_nl; ++i)
+        m_gammafile << time << "\t " << m_xc(i) << "\t" << m_zc(i) << "\t" << g(i) << "\t" << m_ds(i)
+                    << "\n";
+    m_forcefile << time << "\t" << m_fx << "\t" << m_fz << "\n";
+}
+
+void VortexSheet::print_collocation() const
+{
+    for (unsigned i = 0; i < m_nl; i++)
+        std::cout << "(xc,zc) = (" << m_xc(i) << "," << m_zc(i) << ")" << std::endl;
+}
+
+void VortexSheet::print_unit_vectors() const
+{
+    for (unsigned i = 0; i < m_nl; i++)
+        std::cout << "For panel No " << i << " the normal vectors are en = (" << m_enx(i) << ","
+                  << m_enz(i) << ") and et = (" << m_etx(i) << "," << m_etz(i) << ")" << std::endl;
+}
+
+void VortexSheet::print_gamma() const
+{
+    // one value per line (the reference prints the Armadillo column)
+    const Vector g = m_nl ? gamma() : Vector();
+    for (unsigned i = 0; i < m_nl; i++) std::cout << g(i) << std::endl;
 }
 
 VortexSheet::~VortexSheet()
@@ -340,6 +504,7 @@ std::tuple<double, double> VortexSheet::get_coefficients() const { return {m_cd,
 
 Vector VortexSheet::gamma() const
 {
+    need_body("gamma");
     Vector g(m_nl);
     dvm::check(dvm_sheet_get_gamma(m_sheet, g.memptr()), "gamma", dvm::device_context());
     return g;
@@ -347,12 +512,14 @@ Vector VortexSheet::gamma() const
 
 void VortexSheet::set_gamma(const Vector& g)
 {
+    need_body("set_gamma");
     if (g.size() != m_nl) throw dvm::SizeMismatchException("VortexSheet gamma");
     dvm::check(dvm_sheet_set_gamma(m_sheet, g.memptr()), "set_gamma", dvm::device_context());
 }
 
 void VortexSheet::solvevortexsheet(VortexBlobs& blobs)
 {
+    need_body("solvevortexsheet");
     const unsigned N = blobs.size();
     const size_t b = sizeof(double) * N;
     dvm::DeviceBuffer buf(4 * b + 8);
@@ -369,6 +536,7 @@ void VortexSheet::solvevortexsheet(VortexBlobs& blobs)
 
 void VortexSheet::vortexsheetbc(VortexBlobs& blobs)
 {
+    need_body("vortexsheetbc");
     const unsigned N = blobs.size();
     if (N == 0) return;
     const size_t b = sizeof(double) * N;
@@ -384,6 +552,7 @@ void VortexSheet::vortexsheetbc(VortexBlobs& blobs)
 
 void VortexSheet::probe_velocities(const VortexBlobs& blobs, Probe& probe)
 {
+    need_body("probe_velocities");
     const unsigned N = blobs.size();
     const unsigned P = probe.size();
     if (probe.m_z.size() != P) throw dvm::SizeMismatchException("Probe");
@@ -414,6 +583,7 @@ void VortexSheet::probe_velocities(const VortexBlobs& blobs, Probe& probe)
 
 VortexBlobs VortexSheet::release_nascent_vortices_rw(Random& rand)
 {
+    need_body("release_nascent_vortices_rw");
     int64_t nrv = 0;
     dvm::check(dvm_sheet_release_counts(m_sheet, &nrv), "VortexSheet::release_nascent_vortices_rw",
                dvm::device_context());
@@ -441,6 +611,7 @@ VortexBlobs VortexSheet::release_nascent_vortices_rw(Random& rand)
 
 void VortexSheet::reflect(VortexBlobs& v)
 {
+    need_body("reflect");
     const unsigned N = v.size();
     if (N == 0) return;
     const size_t b = sizeof(double) * N;
@@ -455,6 +626,7 @@ void VortexSheet::reflect(VortexBlobs& v)
 
 void VortexSheet::delete_inside(VortexBlobs& v)
 {
+    need_body("delete_inside");
     const unsigned N = v.size();
     if (N == 0) return;
     const size_t b = sizeof(double) * N;
@@ -480,6 +652,7 @@ void VortexSheet::delete_inside(VortexBlobs& v)
 
 void VortexSheet::compute_loads(double Urel)
 {
+    need_body("compute_loads");
     double out[4];
     dvm::check(dvm_sheet_loads(m_sheet, Urel, out), "compute_loads", dvm::device_context());
     m_fx = out[0];

@@ -5,11 +5,16 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <filesystem>
+#include <fstream>
 #include <functional>
+#include <sstream>
 #include <random>
 #include <set>
 #include <string>
 #include <vector>
+
+#include <unistd.h>
 
 #include "BackgroundMesh.hpp"
 #include "DVMBase.hpp"
@@ -19,6 +24,7 @@
 #include "VelocityKernel.hpp"
 #include "VortexBlobs.hpp"
 #include "VortexSheet.hpp"
+#include "XmlHandler.hpp"
 
 namespace {
 struct Case {
@@ -544,6 +550,145 @@ TEST(DVMBaseRunsCylinder)
     auto [fx, fz, cd, cl] = dvm.loads();
     EXPECT(std::isfinite(cd) && std::isfinite(cl) && std::isfinite(fx) && std::isfinite(fz));
     EXPECT_NEAR(dvm.time(), 0.05, 1e-15);
+}
+
+// tests/unit/test_VortexSheet.cpp:12-27
+TEST(VortexSheetDefaultConstructorWorks)
+{
+    VortexSheet sheet;
+    EXPECT(sheet.size() == 0u);
+}
+
+TEST(VortexSheetGetForcesReturnsZeroForDefaultSheet)
+{
+    VortexSheet sheet;
+    auto [fx, fz] = sheet.get_forces();
+    EXPECT(fx == 0.0);
+    EXPECT(fz == 0.0);
+    // a default sheet has no body: the methods that need one say so
+    VortexBlobs none;
+    EXPECT_THROW(sheet.solvevortexsheet(none), dvm::DVMException);
+}
+
+// tests/unit/test_Probe.cpp:10-20
+TEST(ProbeDefaultConstructorCreatesEmptyProbe)
+{
+    Probe probe;
+    EXPECT(probe.size() == 0u);
+}
+
+TEST(ProbeSizeReturnsCorrectCount)
+{
+    Probe probe;
+    EXPECT(probe.size() == 0u);
+}
+
+// A kernel written against the reference's abstract interface
+// (src/VelocityKernel.hpp:29-62: two compute methods and name()) compiles
+// and is usable through the base class.
+namespace {
+class ZeroKernel : public dvm::VelocityKernel {
+public:
+    void computeSelfInduced(const std::vector<dvm::VortexParticle>& p,
+                            std::vector<dvm::VelocityResult>& v) override
+    {
+        v.assign(p.size(), dvm::VelocityResult{});
+    }
+    void computeInducedAt(const std::vector<dvm::VortexParticle>&, const std::vector<double>& tx,
+                          const std::vector<double>&, std::vector<dvm::VelocityResult>& v) override
+    {
+        v.assign(tx.size(), dvm::VelocityResult{});
+    }
+    [[nodiscard]] std::string name() const override { return "zero"; }
+};
+}  // namespace
+
+TEST(ReferenceStyleKernelSubclass)
+{
+    std::unique_ptr<dvm::VelocityKernel> k = std::make_unique<ZeroKernel>();
+    std::vector<dvm::VortexParticle> p{{0.0, 0.0, 1.0, 0.1}, {1.0, 0.0, -1.0, 0.1}};
+    std::vector<dvm::VelocityResult> v;
+    k->computeSelfInduced(p, v);
+    EXPECT(v.size() == 2u && v[0].u == 0.0 && k->name() == "zero");
+}
+
+// The input-file constructors (src/VortexSheet.cpp:12-54,
+// src/VortexBlobs.cpp:16-42, src/Probe.cpp:12-34): the body from
+// <io><input_dir>/<domain_file>, the output files with the reference headers.
+namespace {
+std::string read_file(const std::string& f)
+{
+    std::ifstream in(f);
+    std::stringstream ss;
+    ss << in.rdbuf();
+    return ss.str();
+}
+}  // namespace
+
+TEST(InputFileConstructors)
+{
+    namespace fs = std::filesystem;
+    const fs::path dir = fs::temp_directory_path() / ("dvmpp_dropin_" + std::to_string(::getpid()));
+    fs::create_directories(dir);
+    const std::string xml_text =
+        "<?xml version=\"1.0\"?>\n<DVMpp>\n"
+        "<algorithms><surface_crossing string=\"REFLECT\" /></algorithms>\n"
+        "<constants><density val=\"1.0\" /><nu val=\"0.001\" /><max_NumPanelVort val=\"3\" />"
+        "<cutoff_exp val=\"0.675\" /><seed val=\"42\" /></constants>\n"
+        "<io><input_dir string=\"" + g_golden + "\" /><output_dir string=\"" + dir.string() +
+        "\" /><domain_file string=\"simple_body.txt\" /></io>\n"
+        "<flow><ux val=\"3.0\" /><uz val=\"0.0\" /></flow>\n"
+        "<probe><x string=\"2.0 3.0\" /><z string=\"0.0 0.5\" /></probe>\n"
+        "<time><scheme string=\"euler\" /><dt val=\"0.005\" /><steps val=\"10\" /></time>\n"
+        "</DVMpp>\n";
+    const std::string xml_file = (dir / "case.xml").string();
+    std::ofstream(xml_file) << xml_text;
+    XmlHandler xml(xml_file);
+    unsigned nblobs = 0;
+    {  // the sheet's files are flushed when it goes (as the reference's are)
+    VortexSheet sheet(xml, "stamp");
+    EXPECT(sheet.size() == 20u);
+    VortexBlobs blobs(xml, "stamp");
+    EXPECT(blobs.size() == 0u);
+    Probe probe(xml, "stamp");
+    EXPECT(probe.size() == 2u && probe.m_x(1) == 3.0 && probe.m_z(1) == 0.5);
+
+    // one step of the reference's loop through the classes, then the writers
+    sheet.solvevortexsheet(blobs);
+    Random rng;
+    rng.seed(42);
+    VortexBlobs nascent = sheet.release_nascent_vortices_rw(rng);
+    blobs.append_vortices(nascent);
+    sheet.compute_loads(3.0);
+    sheet.write_step(0.005);
+    blobs.write_step(0.005, 1);
+    sheet.probe_velocities(blobs, probe);
+    probe.write_step(0.005);
+    nblobs = blobs.size();
+    // a copy keeps the output files (src/VortexBlobs.cpp:64-77)
+    VortexBlobs copy(blobs);
+    copy.write_step(0.01, 2);
+    }
+
+    const std::string gam = read_file((dir / "stamp_gamma.dat").string());
+    EXPECT(gam.rfind("20 # Number of collocation points\n0.005 # Time Step\n10 # Steps\n"
+                     "Time [s] Gamma - Vortex sheet strength\n0.005\t ", 0) == 0);
+    EXPECT(std::count(gam.begin(), gam.end(), '\n') == 4 + 20);
+    const std::string loads = read_file((dir / "stamp_loads.dat").string());
+    EXPECT(loads.rfind("Time [s]\tF_x [-]\tF_z [s]\n0.005\t", 0) == 0);
+    const std::string vort = read_file((dir / "stamp_vortex.dat").string());
+    EXPECT(vort.rfind("0 # Number of nodes\n0.005 # Time step\n10 # Steps\n"
+                      "Time [s]\tx-position [m]\tz-position [m]\tcirculation\n", 0) == 0);
+    EXPECT(std::count(vort.begin(), vort.end(), '\n') == 4 + 2 * (long)nblobs);
+    const std::string num = read_file((dir / "stamp_vortex_num.dat").string());
+    EXPECT(num == "Time [s]\tNumber of vortices\n1\t" + std::to_string(nblobs) + "\n2\t" +
+                      std::to_string(nblobs) + "\n");
+    const std::string pr = read_file((dir / "stamp_probe.dat").string());
+    EXPECT(pr.rfind("0.005 # Time step\n10 # Steps\nTime [s]\tu [m/s]\tw [m/s]\n0.005\t", 0) == 0);
+    EXPECT(std::count(pr.begin(), pr.end(), '\n') == 3 + 2);
+    VortexBlobs plain;
+    EXPECT_THROW(plain.write_step(0.0, 0), dvm::FileIOException);
+    fs::remove_all(dir);
 }
 
 int main(int argc, char** argv)

@@ -29,6 +29,7 @@ class Params(ctypes.Structure):
         ("remesh_dx", ctypes.c_double), ("remesh_dz", ctypes.c_double),
         ("remesh_sigma", ctypes.c_double), ("remesh_circ_threshold", ctypes.c_double),
         ("merge_threshold", ctypes.c_double), ("merge_frequency", ctypes.c_int),
+        ("rng_mode", ctypes.c_int),
     ]
 
 
@@ -38,7 +39,7 @@ def make_params(**kw):
                     scheme=0, crossing=2, seed=0, remesh_frequency=0, remesh_kernel=0,
                     remesh_xmin=-5.0, remesh_xmax=25.0, remesh_zmin=-10.0, remesh_zmax=10.0,
                     remesh_dx=0.01, remesh_dz=0.01, remesh_sigma=0.05, remesh_circ_threshold=1e-10,
-                    merge_threshold=0.0, merge_frequency=0)
+                    merge_threshold=0.0, merge_frequency=0, rng_mode=0)
     defaults.update(kw)
     for k, v in defaults.items():
         setattr(p, k, v)
@@ -84,6 +85,9 @@ def _load():
         "orc_sim_get_loads": (None, [vp, _dp, _dp, _dp, _dp]),
         "orc_sim_step_index": (ctypes.c_int, [vp]),
         "orc_sim_rng_draws": (ctypes.c_uint64, [vp]),
+        "orc_sim_set_clock": (None, [vp, ctypes.c_int, ctypes.c_uint64]),
+        "orc_philox4x32": (None, [vp, vp, vp]),
+        "orc_u01": (ctypes.c_double, [ctypes.c_uint32, ctypes.c_uint32]),
         "orc_sim_convect": (None, [vp]),
         "orc_sim_remesh": (None, [vp]),
         "orc_sim_diffuse": (None, [vp]),
@@ -121,6 +125,20 @@ def _p(a):
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def philox4x32(ctr, key):
+    """Philox4x32-10 words for a 4-word counter and a 2-word key."""
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    o = np.empty(4, dtype=np.uint32)
+    lib().orc_philox4x32(c.ctypes.data, k.ctypes.data, o.ctypes.data)
+    return o
+
+
+def u01(a, b):
+    """The device stream's 53-bit uniform in (0, 1) from two words."""
+    return lib().orc_u01(int(a), int(b))
 
 
 def bs_self(x, z, circ, sigma):
@@ -296,6 +314,10 @@ class Sim:
 
     def rng_draws(self):
         return lib().orc_sim_rng_draws(self.h)
+
+    def set_clock(self, step, draws):
+        """Step index and random-stream position of another run."""
+        lib().orc_sim_set_clock(self.h, step, draws)
 
     def convect(self):
         lib().orc_sim_convect(self.h)

@@ -55,6 +55,50 @@ struct Rng {
     void seed(int s) { if (s >= 0) re.seed(s); }
 };
 
+// The library's device random stream (not the reference's): Philox4x32-10
+// (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
+// SC'11) with the published multipliers and Weyl increments, 10 rounds.
+void philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out)
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n1 = (uint32_t)p1;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// 53 random bits from two words, centred in their interval: (m + 0.5) 2^-53.
+double u01(uint32_t a, uint32_t b)
+{
+    const uint64_t m = ((uint64_t)(a >> 5) << 26) | (uint64_t)(b >> 6);
+    return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// Draws of the device stream: index i of step `step`, stream tag 0 (diffusion:
+// two uniforms per blob) or 1 (release: one per nascent blob).
+struct DeviceDraw {
+    uint32_t seed;
+    uint32_t step;
+    void two(uint64_t i, uint32_t tag, double& r1, double& r2) const
+    {
+        const uint32_t ctr[4] = {(uint32_t)i, (uint32_t)(i >> 32), step, tag};
+        const uint32_t key[2] = {seed, tag == 0 ? 0x5eed0001u : 0x5eed0002u};
+        uint32_t o[4];
+        philox(ctr, key, o);
+        r1 = u01(o[0], o[1]);
+        r2 = u01(o[2], o[3]);
+    }
+};
+
 // ---------------------------------------------------------------- Blobs
 // src/VortexBlobs.hpp:24-32 (public SoA fields)
 struct Blobs {
@@ -438,6 +482,13 @@ struct Sheet {
     // Returns false on the VortexException path (NaN counts).
     bool release(Rng& rng, Blobs& out) const
     {
+        return release_with([&rng](unsigned) { return rng.rand(); }, out);
+    }
+
+    // The same with the k-th nascent blob's uniform from draw(k).
+    template <class Draw>
+    bool release_with(Draw draw, Blobs& out) const
+    {
         const int n = Nl;
         Vec pc(n), cnt(n), cnew(n);
         for (int i = 0; i < n; ++i) pc[i] = gamma[i] * ds[i];
@@ -455,7 +506,7 @@ struct Sheet {
             while (j < static_cast<unsigned>(cnt[i])) {
                 const double xm = x[i] + ds[i] * etx[i] / (cnt[i] + 1);
                 const double zm = z[i] + ds[i] * etz[i] / (cnt[i] + 1);
-                const double R = rng.rand();
+                const double R = draw(counter);
                 const double rrw = std::fabs(std::sqrt(4.0 * nu * dt * std::log(1.0 / R)));
                 out.x[counter] = xm + rrw * enx[i];
                 out.z[counter] = zm + rrw * enz[i];
@@ -777,9 +828,15 @@ struct Sim {
     // src/VortexBlobs.cpp:251-265 (diffusion_random_walk)
     void diffuse()
     {
+        const DeviceDraw dd{(uint32_t)p.seed, (uint32_t)step};
         for (size_t i = 0; i < v.size(); i++) {
-            const double R1 = rng.rand();
-            const double R2 = rng.rand();
+            double R1, R2;
+            if (p.rng_mode == 1) {
+                dd.two(i, 0, R1, R2);
+            } else {
+                R1 = rng.rand();
+                R2 = rng.rand();
+            }
             const double rrw = std::sqrt(4.0 * p.nu * p.dt * std::log(1.0 / R1));
             const double thetarw = 2.0 * pi * R2;
             v.x[i] += rrw * std::cos(thetarw);
@@ -790,7 +847,17 @@ struct Sim {
     bool release()
     {
         Blobs nb;
-        if (!sheet.release(rng, nb)) return false;
+        if (p.rng_mode == 1) {
+            const DeviceDraw dd{(uint32_t)p.seed, (uint32_t)step};
+            auto draw = [&dd](unsigned k) {
+                double r1, r2;
+                dd.two(k, 1, r1, r2);
+                return r1;
+            };
+            if (!sheet.release_with(draw, nb)) return false;
+        } else if (!sheet.release(rng, nb)) {
+            return false;
+        }
         v.append(nb);
         return true;
     }
@@ -1010,6 +1077,18 @@ void orc_sim_get_loads(void* sim, double* fx, double* fz, double* cd, double* cl
 }
 int orc_sim_step_index(void* sim) { return static_cast<Sim*>(sim)->step; }
 uint64_t orc_sim_rng_draws(void* sim) { return static_cast<Sim*>(sim)->rng.draws; }
+void orc_philox4x32(const uint32_t* ctr, const uint32_t* key, uint32_t* out) { philox(ctr, key, out); }
+double orc_u01(uint32_t a, uint32_t b) { return u01(a, b); }
+void orc_sim_set_clock(void* sim, int step, uint64_t draws)
+{
+    // libstdc++ generate_canonical<double, 53> over mt19937 takes 2 words per
+    // draw (the stream that src/Random.cpp:3-18 produces)
+    Sim* s = static_cast<Sim*>(sim);
+    s->step = step;
+    s->rng.seed(s->p.seed);
+    s->rng.re.discard(2 * draws);
+    s->rng.draws = draws;
+}
 void orc_sim_convect(void* sim) { Sim* s = static_cast<Sim*>(sim); s->step++; s->convect(); }
 void orc_sim_remesh(void* sim) { static_cast<Sim*>(sim)->do_remesh(); }
 void orc_sim_diffuse(void* sim) { static_cast<Sim*>(sim)->diffuse(); }

@@ -48,6 +48,10 @@ typedef struct orc_params {
     /* algorithms/merge_threshold, merge_frequency (src/DVMBase.cpp:49-60); 0 = off */
     double merge_threshold;
     int merge_frequency;
+    /* random stream: 0 = the reference's (mt19937, src/Random.cpp:3-18);
+     * 1 = the library's device stream (Philox4x32-10 keyed by seed, step and
+     * blob index), so batched and device-stream runs can be checked */
+    int rng_mode;
 } orc_params;
 
 /* ---- Random (src/Random.cpp:3-18) ---- */
@@ -56,6 +60,11 @@ void     orc_rng_destroy(void* rng);
 double   orc_rng_rand(void* rng);
 void     orc_rng_fill(void* rng, int64_t n, double* out);
 uint32_t orc_rng_raw(void* rng);
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): out = the 4 words for
+ * counter ctr[4] and key key[2]; orc_u01 = the 53-bit uniform in (0, 1) the
+ * device stream builds from two words. */
+void     orc_philox4x32(const uint32_t* ctr, const uint32_t* key, uint32_t* out);
+double   orc_u01(uint32_t a, uint32_t b);
 /* OpenMP threads the parallel loops use (OMP_NUM_THREADS). */
 int      orc_num_threads(void);
 
@@ -109,6 +118,10 @@ void    orc_sim_set_gamma(void* sim, const double* gamma);
 void    orc_sim_get_loads(void* sim, double* fx, double* fz, double* cd, double* cl);
 int     orc_sim_step_index(void* sim);
 uint64_t orc_sim_rng_draws(void* sim);
+/* Puts the simulation at step `step` with the random stream `draws` doubles
+ * past the seed (the stream re-seeded, then 2 words per draw discarded), so a
+ * state taken from another run at that step continues as that run would. */
+void    orc_sim_set_clock(void* sim, int step, uint64_t draws);
 /* Individual sub-steps on the sim's state, in compute_step order. */
 void    orc_sim_convect(void* sim);
 void    orc_sim_remesh(void* sim);

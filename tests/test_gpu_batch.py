@@ -1,12 +1,15 @@
 """Batched ensemble (dvm_batch_*): each simulation's result is the same alone
 and in a mix (bit-for-bit: per-item sum orders are fixed), the graph replay
-equals eager launches, a batch slot follows the single-simulation engine with
-the same device random stream (FP64: to 1e-10; FP32C: to the FP32C
-tolerance), and a failing simulation stops alone."""
+equals eager launches, a batch slot follows the oracle's step in its
+device-stream mode (FP64: to 1e-10) and the single-simulation engine with the
+same device random stream (FP32C: to the FP32C tolerance), and a failing
+simulation stops alone."""
 import numpy as np
 import pytest
 
 import dvmpp_amd as dvm
+import oracle
+from conftest import rel_err
 from dvmpp_amd import bodies
 from dvmpp_amd import sim as dsim
 from dvmpp_amd.batch import Batch, batch_config
@@ -127,7 +130,9 @@ def test_batch_remesh_matches_single_simulation(gpu_ctx, prec, tol):
     ref = s.get_state()
     assert ref["x"].size == out["x"].size
     assert np.max(np.abs(ref["x"] - out["x"])) <= tol * np.max(np.abs(ref["x"]))
+    assert np.max(np.abs(ref["z"] - out["z"])) <= tol * np.max(np.abs(ref["z"]))
     assert np.max(np.abs(ref["circ"] - out["circ"])) <= tol * np.max(np.abs(ref["circ"]))
+    assert np.array_equal(ref["sigma"], out["sigma"])
     s.close()
 
 
@@ -156,4 +161,45 @@ def test_slot_reuse_and_capacity_error(gpu_ctx):
     assert s2 in (s0, s1)
     b.step(2)
     assert dict(b.poll())[s2] == 0
+    b.close()
+
+
+@pytest.mark.parametrize("scheme,crossing", [(dsim.EULER, dsim.REFLECT), (dsim.RK3, dsim.DELETE)])
+def test_fp64_slots_match_oracle(gpu_ctx, scheme, crossing):
+    """FP64 slots of one batch against the oracle's time step
+    (src/DVMBase.cpp:86-133) with the same device random stream (Philox
+    keyed by seed, step and blob index; the oracle's Philox is pinned by the
+    generator's known-answer vectors): 8 steps free-running from rest, blob
+    counts and core sizes equal, positions, circulations and Cd, Cl to 1e-10."""
+    cases = [(bodies.cylinder(128), dict(nu=1e-3, dt=0.005), 31),
+             (bodies.ellipse(96, aspect=2.0, aoa_deg=10), dict(nu=5e-4, dt=0.004, ux=0.8), 32),
+             (bodies.rounded_square(80), dict(nu=2e-3, dt=0.006), 33)]
+    steps = 8
+    cfg = batch_config(slots=4, slot_capacity=1 << 14, max_panels=128, scheme=scheme,
+                       crossing=crossing, prec=dvm.F64)
+    b = Batch(gpu_ctx, cfg)
+    slots = [b.admit(px, pz, dsim.sheet_params(**kw), seed, steps) for (px, pz), kw, seed in cases]
+    b.step(steps)
+    for slot, ((px, pz), kw, seed) in zip(slots, cases):
+        got, info = b.state(slot), b.info(slot)
+        assert info["status"] == 0 and info["step"] == steps
+        sp = dsim.sheet_params(**kw)
+        op = oracle.make_params(rho=sp.rho, nu=sp.nu, dt=sp.dt, ux=sp.ux, uz=sp.uz,
+                                cutoff_exp=sp.cutoff_exp, max_npv=sp.max_npv, scheme=scheme,
+                                crossing=crossing, seed=seed, rng_mode=1)
+        osim = oracle.Sim(px, pz, op)
+        for _ in range(steps):
+            assert osim.step() == 0
+        ref = osim.get_state()
+        assert ref["x"].size == got["x"].size, (slot, ref["x"].size, got["x"].size)
+        assert np.array_equal(ref["sigma"], got["sigma"])
+        assert rel_err(got["circ"], ref["circ"]) <= 1e-10
+        scale = max(1.0, float(np.max(np.abs(ref["x"]))))
+        assert np.max(np.abs(got["x"] - ref["x"])) <= 1e-10 * scale
+        assert np.max(np.abs(got["z"] - ref["z"])) <= 1e-10 * scale
+        for f in ("u", "w"):
+            assert rel_err(got[f], ref[f]) <= 1e-9, (slot, f)
+        lo = osim.loads()
+        for k in (2, 3):
+            assert abs(info["loads"][k] - lo[k]) <= 1e-10 * max(1.0, abs(lo[k])), (slot, k)
     b.close()

@@ -346,16 +346,20 @@ def test_chosen_form_loads_lockstep(gpu_ctx, form):
     a.run(40)  # a developed near wake, ~8K blobs
     b.run(40)
     b.set_state(a.get_state())
+    compared = 0
     try:
         for k in range(8):
             a.step()
             b.step()
             sa, sb = a.get_state(), b.get_state()
             if sa["x"].size == sb["x"].size:  # release counts agree (they may differ by one)
+                compared += 1
                 for f in ("u", "w"):
                     assert rel_err(sb[f], sa[f]) <= 1e-5, (form, k, f)
             la, lb = a.loads(), b.loads()
             assert abs(lb[2] - la[2]) <= 1e-6 and abs(lb[3] - la[3]) <= 1e-6, (form, k, la, lb)
+        # the velocity check must not pass by skipping: most steps compared
+        assert compared >= 4, (form, compared)
     finally:
         a.close()
         b.close()

@@ -199,3 +199,87 @@ def test_rk3_weights_integrate_constant_velocity():
         out = sim.get_state()
         assert abs(out["x"][0] - 500.0 - 0.007) < 1e-8
         assert abs(out["z"][0] - 300.0 + 0.003) < 1e-8
+
+
+def test_sim_set_clock_continues_the_run():
+    """A state taken at step k with its step index and random-stream position
+    continues bit for bit as the run it came from (the lock-step tests
+    re-synchronise the oracle to a GPU run this way)."""
+    px, pz = oracle.cylinder_points(32)
+    p = oracle.make_params(seed=3)
+    a = oracle.Sim(px, pz, p)
+    for _ in range(5):
+        a.step()
+    st, draws, k = a.get_state(), a.rng_draws(), a.step_index()
+    b = oracle.Sim(px, pz, p)
+    b.set_state(st)
+    b.set_clock(k, draws)
+    for _ in range(2):
+        a.step()
+        b.step()
+    sa, sb = a.get_state(), b.get_state()
+    assert all(np.array_equal(sa[f], sb[f]) for f in sa)
+    assert a.rng_draws() == b.rng_draws() and a.step_index() == b.step_index() == k + 2
+    assert a.loads() == b.loads()
+    # a wrong stream position changes the diffusion
+    c = oracle.Sim(px, pz, p)
+    c.set_state(st)
+    c.set_clock(k, draws + 1)
+    c.step()
+    a2 = oracle.Sim(px, pz, p)
+    a2.set_state(st)
+    a2.set_clock(k, draws)
+    a2.step()
+    assert not np.array_equal(c.get_state()["x"], a2.get_state()["x"])
+
+
+# Philox4x32-10 known-answer vectors of the generator's authors (Random123,
+# kat_vectors: philox4x32_10 for the zero, all-ones and pi-digit inputs).
+PHILOX_KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,out", PHILOX_KAT)
+def test_philox_known_answers(ctr, key, out):
+    assert tuple(int(v) for v in oracle.philox4x32(ctr, key)) == out
+
+
+def test_u01_bounds_and_bits():
+    # (m + 0.5) 2^-53 over the 53 bits taken: a > 5 high bits, b > 6 high bits
+    assert oracle.u01(0, 0) == 0.5 * 2.0 ** -53
+    assert oracle.u01(0xFFFFFFFF, 0xFFFFFFFF) == 1.0 - 0.5 * 2.0 ** -53
+    assert oracle.u01(0x20, 0) == (2.0 ** 26 + 0.5) * 2.0 ** -53
+    assert oracle.u01(0, 0x40) == 1.5 * 2.0 ** -53
+    assert oracle.u01(0x1F, 0x3F) == oracle.u01(0, 0)  # dropped low bits
+
+
+def test_device_stream_diffusion_statistics():
+    """The oracle's device-stream random walk: each component N(0, 2 nu dt),
+    components and steps uncorrelated (the device mode's reference moments,
+    src/VortexBlobs.cpp:251-265 on Philox draws)."""
+    n = 200_000
+    px, pz = oracle.cylinder_points(16)
+    p = oracle.make_params(rng_mode=1, seed=9, nu=1e-3, dt=0.005)
+    sim = oracle.Sim(px, pz, p)
+    zero = np.zeros(n)
+    st = {"x": np.full(n, 50.0), "z": zero.copy(), "circ": zero.copy(), "sigma": np.full(n, 0.01),
+          "u": zero.copy(), "w": zero.copy(), "uvs": zero.copy(), "wvs": zero.copy(),
+          "id": np.arange(n, dtype=np.uint32)}
+    steps = []
+    for k in (1, 2):
+        sim.set_state(st)
+        sim.set_clock(k, 0)
+        sim.diffuse()
+        s = sim.get_state()
+        steps.append((s["x"] - 50.0, s["z"]))
+    var = 2 * 1e-3 * 0.005
+    for dx, dz in steps:
+        for d in (dx, dz):
+            assert abs(d.mean()) < 5 * np.sqrt(var / n)
+            assert abs(d.var() / var - 1) < 5 * np.sqrt(2.0 / n)
+        assert abs(np.corrcoef(dx, dz)[0, 1]) < 5 / np.sqrt(n)
+    assert abs(np.corrcoef(steps[0][0], steps[1][0])[0, 1]) < 5 / np.sqrt(n)

@@ -53,6 +53,8 @@ SIGNATURES = {
     "dvm_memcpy_d2h": (_st, [_vp, _vp, _vp, _sz]),
     "dvm_memcpy_d2d": (_st, [_vp, _vp, _vp, _sz]),
     "dvm_bs_self": (_st, [_vp, _int, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "dvm_bs_self_noalloc": (_st, [_vp, _int, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "dvm_bs_self_noalloc_scratch_bytes": (_sz, []),
     "dvm_bs_targets": (_st, [_vp, _int, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _int, _dbl,
                              _vp, _vp]),
     "dvm_bs_self_host": (_st, [_vp, _int, _i64, _vp, _vp]),

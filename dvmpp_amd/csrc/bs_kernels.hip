@@ -32,6 +32,8 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 namespace dvmi {
@@ -667,7 +669,12 @@ static void launch_f64(dvm_ctx_t ctx, bool self, int64_t ns, const double* sx, c
     // 16 blocks — collocation points, probes — so that enough blocks run); a
     // single chunk keeps the reference's summation order.
     const int64_t min_chunk = bx <= 16 ? 512 : 2048;
-    const int nch = pick_chunks(ctx, bx, (ns + min_chunk - 1) / min_chunk);
+    int64_t max_chunks = (ns + min_chunk - 1) / min_chunk;
+    if (ctx->ws_fixed) {
+        const int64_t m = (int64_t)(ctx->ws_fixed_partials / (2 * sizeof(double) * (size_t)nt));
+        max_chunks = m < max_chunks ? (m < 1 ? 1 : m) : max_chunks;
+    }
+    const int nch = pick_chunks(ctx, bx, max_chunks);
     const int64_t chunk = (ns + nch - 1) / nch;
     const size_t part_bytes = nch > 1 ? sizeof(double) * 2 * nt * nch : 0;
     Carver cv(ws_get(ctx, Carver::need({sizeof(double4) * (size_t)(ns + kRec64Pad), part_bytes})));
@@ -821,7 +828,8 @@ static void f32c_run(dvm_ctx_t ctx, int kind, int v, int occ, int64_t ns, const 
     const int nbb = 64;
     const int64_t want = (int64_t)occ * ctx->num_cus * 256 / (matrix ? 256 : K.block);
     int64_t nch64 = (want + gx - 1) / gx;
-    const int64_t nch_mem = (int64_t)(kPartialBytes / (2 * sizeof(double) * (size_t)nt));
+    const size_t part_cap = ctx->ws_fixed ? ctx->ws_fixed_partials : kPartialBytes;
+    const int64_t nch_mem = (int64_t)(part_cap / (2 * sizeof(double) * (size_t)nt));
     nch64 = nch64 > nch_mem ? nch_mem : nch64;
     nch64 = nch64 > ntiles ? ntiles : (nch64 < 1 ? 1 : nch64);
     const int nch = (int)nch64;
@@ -964,7 +972,7 @@ static void launch_f32c(dvm_ctx_t ctx, bool self, int64_t ns, const double* sx, 
     if (ctx->f32c_forced[0] != 0) {
         v = f32c_find(ctx->f32c_forced[0], ctx->f32c_forced[1], ctx->f32c_forced[2]);
         occ = ctx->f32c_forced[3];
-    } else if (ctx->tuning == DVM_TUNE_MEASURE) {
+    } else if (ctx->tuning == DVM_TUNE_MEASURE && !ctx->ws_fixed) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         DVM_HIP_CHECK(hipStreamIsCapturing(ctx->stream, &cs));
         if (cs == hipStreamCaptureStatusNone) {
@@ -1040,11 +1048,84 @@ void bs_targets_launch(dvm_ctx_t ctx, int prec, int64_t ns, const double* sx, co
         launch_f32c(ctx, false, ns, sx, sz, sc, ss, nt, tx, tz, guard, r2_skip, u, w);
 }
 
+// ------------------------------------------- allocation-free blob-on-blob sum
+//
+// dvm_bs_self_noalloc runs dvm_bs_self's kernels with the scratch memory
+// (sort keys, tiles, source copies, chunk partials, FP64 records) in this
+// file-scope device array instead of the context's growable arena, so it
+// calls no hipMalloc or hipFree.  Each GPU the library runs on holds its own
+// copy until the process ends.  Every byte a call uses is written before it is
+// read, so nothing needs clearing between calls; calls that overlap on one GPU
+// share the array and race.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr size_t kScratchPartialBytes = size_t(16) << 20;  // of which chunk partials
+constexpr int kScratchMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of `scratch` on `device`, taken once per device with that
+// device current (the host-side name is a stand-in, never a device address).
+static void* scratch_address(int device)
+{
+    static std::mutex mu;
+    static void* ptr[kScratchMaxDevices] = {};
+    require(device >= 0 && device < kScratchMaxDevices, DVM_E_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[device]) {
+        int cur = -1;
+        DVM_HIP_CHECK(hipGetDevice(&cur));
+        if (cur != device) DVM_HIP_CHECK(hipSetDevice(device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) DVM_HIP_CHECK(hipSetDevice(cur));
+        if (e != hipSuccess || !p)
+            fail(DVM_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        ptr[device] = p;
+    }
+    return ptr[device];
+}
+
+// Points the context's arena at the scratch array for one call (table tuning,
+// no kernel timing) and restores it afterwards.
+struct FixedScratch {
+    dvm_ctx_t ctx;
+    bool prof;
+    FixedScratch(dvm_ctx_t c, void* p) : ctx(c), prof(c->prof)
+    {
+        ctx->ws_fixed = p;
+        ctx->ws_fixed_bytes = kScratchBytes;
+        ctx->ws_fixed_partials = kScratchPartialBytes;
+        ctx->prof = false;
+    }
+    ~FixedScratch()
+    {
+        ctx->ws_fixed = nullptr;
+        ctx->ws_fixed_bytes = 0;
+        ctx->ws_fixed_partials = 0;
+        ctx->prof = prof;
+    }
+};
+
 }  // namespace dvmi
 
 using namespace dvmi;
 
 extern "C" {
+
+dvm_status dvm_bs_self_noalloc(dvm_ctx_t ctx, int prec, int64_t n, const double* x,
+                               const double* z, const double* circ, const double* sigma,
+                               double* u, double* w)
+{
+    if (!ctx) return DVM_E_INVALID;
+    return guard(ctx, [&] {
+        require(prec == DVM_F64 || prec == DVM_F32C, DVM_E_INVALID,
+                "dvm_bs_self_noalloc: precision must be DVM_F64 or DVM_F32C");
+        void* p = scratch_address(ctx->device);
+        FixedScratch fs(ctx, p);
+        bs_self_launch(ctx, prec, n, x, z, circ, sigma, u, w);
+    });
+}
+
+size_t dvm_bs_self_noalloc_scratch_bytes(void) { return kScratchBytes; }
 
 dvm_status dvm_bs_self(dvm_ctx_t ctx, int prec, int64_t n, const double* x, const double* z,
                        const double* circ, const double* sigma, double* u, double* w)

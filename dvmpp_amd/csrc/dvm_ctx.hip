@@ -7,6 +7,14 @@ namespace dvmi {
 
 void* ws_get(dvm_ctx_t ctx, size_t bytes)
 {
+    if (ctx->ws_fixed) {
+        if (bytes > ctx->ws_fixed_bytes)
+            fail(DVM_E_SIZE, "this call needs " + std::to_string(bytes) +
+                                 " bytes of scratch, more than the " +
+                                 std::to_string(ctx->ws_fixed_bytes) +
+                                 "-byte array of the allocation-free entry point");
+        return ctx->ws_fixed;
+    }
     if (bytes <= ctx->ws_bytes && ctx->ws) return ctx->ws;
     // Earlier work on this stream may still read the old arena.
     DVM_HIP_CHECK(hipStreamSynchronize(ctx->stream));

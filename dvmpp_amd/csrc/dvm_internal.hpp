@@ -23,6 +23,11 @@ struct dvm_ctx_s {
     // call on the context's stream (calls on one stream run in order).
     void* ws = nullptr;
     size_t ws_bytes = 0;
+    // While set (dvm_bs_self_noalloc), ws_get hands out this fixed region
+    // instead and never allocates; ws_fixed_partials caps the chunk partials.
+    void* ws_fixed = nullptr;
+    size_t ws_fixed_bytes = 0;
+    size_t ws_fixed_partials = 0;
     // Pinned host staging for small device->host reads.
     void* pinned = nullptr;
     size_t pinned_bytes = 0;

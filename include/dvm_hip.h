@@ -100,6 +100,23 @@ dvm_status dvm_memcpy_d2d(dvm_ctx_t ctx, void* dst, const void* src, size_t byte
 dvm_status dvm_bs_self(dvm_ctx_t ctx, int prec, int64_t n, const double* x, const double* z,
                        const double* circ, const double* sigma, double* u, double* w);
 
+/* Allocation-free form of dvm_bs_self; it has no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the sum's
+ * scratch (sort keys, tiles, source copies, chunk partials) lives in a device
+ * array of the library, dvm_bs_self_noalloc_scratch_bytes() = 64 MiB, one per
+ * GPU, held until the process ends; the chunk partials take at most 16 MiB of
+ * it.  prec is DVM_F64 or DVM_F32C.  A call that needs more scratch returns
+ * DVM_E_SIZE before launching anything (about 1.5M blobs in FP64, 0.5M in
+ * FP32C).  The caller passes the inputs and outputs, as for dvm_bs_self; the
+ * results agree with it to the form's tolerance (bit for bit unless the
+ * partial cap changes the chunking).  Calls that overlap on one GPU share the
+ * array and race. */
+dvm_status dvm_bs_self_noalloc(dvm_ctx_t ctx, int prec, int64_t n, const double* x,
+                               const double* z, const double* circ, const double* sigma,
+                               double* u, double* w);
+size_t dvm_bs_self_noalloc_scratch_bytes(void);
+
 /* Sources -> arbitrary targets.  Pairs with r2 <= r2_skip are skipped when
  * guard != 0: guard = 1, r2_skip = 1e-15 gives computeInducedAt
  * (src/VelocityKernel.cpp:69-108, decl src/VelocityKernel.hpp:48-52);

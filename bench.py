@@ -71,10 +71,46 @@ def time_form(ctx, dvm, dev, prec, steps, warmup, outs):
     return elapsed, (self_ms / max(self_n, 1)), (tgt_ms / max(tgt_n, 1)), variants
 
 
+def host_cpu():
+    """CPU model, sockets x cores per socket x threads per core (lscpu)."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    sockets = int(info.get("Socket(s)", "0") or 0)
+    cores = int(info.get("Core(s) per socket", "0") or 0)
+    return {"model": info.get("Model name", "unknown"), "sockets": sockets,
+            "cores_per_socket": cores, "physical_cores": sockets * cores,
+            "threads_per_core": int(info.get("Thread(s) per core", "0") or 0),
+            "logical_cpus": os.cpu_count()}
+
+
 def cpu_baseline(x, z, c, s, budget_s):
     """The oracle (reference loop order, FP64, OpenMP) on a bounded sample of
-    targets; each target sums over all N sources with the self pair skipped."""
+    targets; each target sums over all N sources with the self pair skipped.
+    Timed with a build of the reference's flags for this host (-O3
+    -march=native, src/CMakeLists.txt:43), made here before timing; the
+    checker's portable build (contraction off) is the fallback."""
+    import subprocess
+    import tempfile
+
     import oracle
+    path = os.path.join(tempfile.gettempdir(), f"dvmpp_oracle_native_{os.getpid()}.so")
+    build = "-O3 -march=native -fopenmp (the reference's flags, built on this host)"
+    try:
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native",
+                        f"NATIVE_OUT={path}"], check=True, timeout=300,
+                       capture_output=True)
+        using = oracle.native_lib(path)
+    except (OSError, subprocess.SubprocessError, RuntimeError) as e:
+        log(f"cpu baseline: native build failed ({e}); timing the checker's build")
+        using = oracle.lib()
+        build = "-O3 -march=x86-64-v3 -fopenmp -ffp-contract=off (the checker's build)"
     n = x.size
     order = np.random.default_rng(1).permutation(n)
     chunk = 4096
@@ -82,13 +118,18 @@ def cpu_baseline(x, z, c, s, budget_s):
     t0 = time.perf_counter()
     while done < n and time.perf_counter() - t0 < budget_s:
         idx = order[done:done + chunk]
-        oracle.bs_targets(x, z, c, s, x[idx], z[idx])
+        oracle.bs_targets(x, z, c, s, x[idx], z[idx], using=using)
         done += idx.size
     dt = time.perf_counter() - t0
+    try:
+        os.remove(path)
+    except OSError:
+        pass
     return {"value": done * n / dt, "unit": "pair-interactions/s",
-            "cores": int(oracle.lib().orc_num_threads()), "kind": "port",
+            "cores": int(using.orc_num_threads()), "kind": "port",
             "sample": f"{done} blob targets x {n} sources (self pair skipped), FP64 reference "
-                      f"loop order, OpenMP, {dt:.1f} s"}
+                      f"loop order, OpenMP, {dt:.1f} s",
+            "build": build, "host_cpu": host_cpu()}
 
 
 def fmm_rate(ctx, dvm, dev, u64, w64, pairs_self, reps=10):
@@ -232,94 +273,136 @@ def plain_cylinder_phase(ctx, steps, cpu_steps):
     return out
 
 
-def ensemble_phase(ctx, count, steps, plain_count):
-    """Uneven ensemble (ellipses and rounded squares, 128-512 panels, nu
-    log-uniform in [2e-4, 2e-3], dt 0.005) advanced `steps` steps from rest:
-    one batch (FP32C, graph-replayed steps) against the plain FP64 form run one
-    simulation at a time (measured on the first `plain_count` bodies)."""
+def c4_remesh():
+    """The C3 remesh settings the ensemble and served workloads run with (M4'
+    every 20 steps on a 0.01 grid over [-5, 25] x [-10, 10], core size 0.0321),
+    which hold each wake to the 1k-400k blobs of the ensemble configuration."""
+    from dvmpp_amd import sim as dsim
+    return dsim.remesh_params(frequency=20, kernel=dsim.M4PRIME, xmin=-5.0, xmax=25.0,
+                              zmin=-10.0, zmax=10.0, dx=0.01, dz=0.01, sigma=0.0321,
+                              circ_threshold=1e-10)
+
+
+def ensemble_phase(ctx, count, window, plain_count, ref_count, seed=3):
+    """The 128-body ensemble (steps ~ U{200..2000}) in one batch (FP32C,
+    graph-replayed steps, remesh), every body run to its own step count or to
+    `window` steps if that is smaller (window 0: the full counts).  Beside it
+    the plain FP64 form one simulation at a time, on a random subset of the
+    bodies over the same steps, with the device random stream and with the
+    reference stream; the ratio is batched aggregate steps/s over plain
+    aggregate steps/s of that subset."""
     import torch
 
     import dvmpp_amd as dvm
     from dvmpp_amd import sim as dsim
     from dvmpp_amd.batch import Batch, batch_config
     from dvmpp_amd.workloads import ensemble
-    jobs = ensemble(count, steps_range=(steps, steps))
-    cap = 1 << 18
+    rp = c4_remesh()
+    jobs = ensemble(count)
+    run = [min(st, window) if window else st for *_, st in jobs]
+    cap = 1 << 17 if 0 < window <= 300 else 1 << 20
     b = Batch(ctx, batch_config(slots=count, slot_capacity=cap, max_panels=512, prec=dvm.F32C,
-                                use_graph=True))
+                                use_graph=True, remesh=rp))
     t0 = time.perf_counter()
-    for px, pz, sp, seed, st in jobs:
-        b.admit(px, pz, sp, seed, st)
+    slots = [b.admit(px, pz, sp, sd, st) for (px, pz, sp, sd, _), st in zip(jobs, run)]
     torch.cuda.synchronize()
     ta = time.perf_counter() - t0
     ctx.prof_reset()
     ctx.prof_enable(True)
+    status = {}
+    pairs = 0.0
+    chunk = 50
     t0 = time.perf_counter()
-    b.step(steps)
+    while len(status) < count:
+        b.step(chunk)
+        # sizes after the chunk: a coarse sum of N^2 over the chunk's steps
+        for s in slots:
+            if s not in status:
+                pairs += float(b.info(s)["n"]) ** 2 * chunk
+        status.update(dict(b.poll(count)))
     torch.cuda.synchronize()
     tb = time.perf_counter() - t0
     ctx.prof_enable(False)
-    done = b.poll()
-    blobs = [b.info(s)["n"] for s, _ in done]
-    ok = sum(1 for _, st in done if st == 0)
+    blobs = [b.info(s)["n"] for s in slots]
+    ok = sum(1 for st in status.values() if st == 0)
     b.close()
-    log(f"ensemble batch: {count} x {steps} steps in {tb:.2f} s (admission {ta:.2f} s)")
-    # both forms timed over the stepping only; set-up (panel influence matrix
-    # and least-squares operator per body) is reported beside it
-    t0 = time.perf_counter()
-    sims = []
-    for px, pz, sp, seed, st in jobs[:plain_count]:
-        p = dsim.SimParams(sp, dsim.EULER, dsim.REFLECT, seed, dvm.F64, dsim.RNG_REFERENCE,
-                           dsim.remesh_params())
-        sims.append((dsim.Sim(ctx, px, pz, p), st))
-    torch.cuda.synchronize()
-    tpa = (time.perf_counter() - t0) * count / plain_count
-    t0 = time.perf_counter()
-    for s, st in sims:
-        s.run(st)
-    torch.cuda.synchronize()
-    tp = (time.perf_counter() - t0) * count / plain_count
-    for s, _ in sims:
-        s.close()
-    log(f"ensemble plain FP64: {plain_count} x {steps} steps in {tp * plain_count / count:.2f} s")
+    total = sum(run)
+    log(f"ensemble batch: {count} bodies, {total} steps in {tb:.2f} s (admission {ta:.2f} s)")
+    pick = np.random.default_rng(seed).choice(count, max(plain_count, ref_count), replace=False)
+    plain = {}
+    for name, mode, m in (("device_stream", dsim.RNG_DEVICE, plain_count),
+                          ("reference_stream", dsim.RNG_REFERENCE, ref_count)):
+        t_set = t_run = 0.0
+        steps = 0
+        for k in pick[:m]:
+            px, pz, sp, sd, _ = jobs[k]
+            t1 = time.perf_counter()
+            sim = dsim.Sim(ctx, px, pz, dsim.SimParams(sp, dsim.EULER, dsim.REFLECT, sd, dvm.F64,
+                                                        mode, rp))
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            sim.run(run[k])
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
+            sim.close()
+            t_set += t2 - t1
+            t_run += t3 - t2
+            steps += run[k]
+        plain[name] = {"bodies": int(m), "steps": steps, "seconds": t_run, "setup_seconds": t_set,
+                       "aggregate_steps_per_s": steps / t_run,
+                       "speedup_of_batched": (total / tb) / (steps / t_run),
+                       "speedup_of_batched_with_setup": (total / (tb + ta)) /
+                                                        (steps / (t_run + t_set))}
+        log(f"ensemble plain FP64 ({name}): {m} bodies, {steps} steps in {t_run:.2f} s")
+    win = f"the first {window} steps of each" if window else "each body's full step count"
     return {"workload": f"{count} bodies (ellipses AR~U[1,4], AoA~U[0,30] deg; rounded squares), "
-                        f"128-512 panels, nu log-U[2e-4,2e-3], dt 0.005, {steps} steps from rest",
-            "batched_f32c_graph": {"aggregate_steps_per_s": count * steps / tb, "seconds": tb,
+                        f"128-512 panels, nu log-U[2e-4,2e-3], dt 0.005, steps ~ U{{200..2000}} "
+                        f"({win}: {total} simulation steps), Euler, REFLECT, M4' remesh every 20 "
+                        f"steps at 0.01 (sigma 0.0321)",
+            "batched_f32c_graph": {"aggregate_steps_per_s": total / tb, "seconds": tb,
                                    "admission_seconds": ta, "completed_ok": ok,
-                                   "final_blobs_min_max": [min(blobs), max(blobs)]},
-            "plain_f64_per_simulation": {"aggregate_steps_per_s": count * steps / tp,
-                                         "seconds_extrapolated": tp,
-                                         "setup_seconds_extrapolated": tpa,
-                                         "measured_simulations": plain_count},
-            "speedup": tp / tb,
-            "speedup_with_setup": (tp + tpa) / (tb + ta)}
+                                   "slot_capacity": cap,
+                                   "final_blobs_min_median_max": [int(min(blobs)),
+                                                                  int(np.median(blobs)),
+                                                                  int(max(blobs))],
+                                   "pair_interactions_per_s_estimate": pairs / tb},
+            "plain_f64_per_simulation": plain,
+            "plain_subset": f"{len(pick)} bodies drawn at random (seed {seed}): "
+                            f"{sorted(int(k) for k in pick)}"}
 
 
 def served_phase(ctx, count, rate, steps_range, slots):
-    """Poisson arrivals at `rate`/s of ensemble-distribution requests, served by
-    continuous batching (FP32C, graph-replayed steps) with admission by
-    projected HBM footprint against a 200 GB budget."""
-    import numpy as np
-
+    """Poisson arrivals at `rate`/s of ensemble-distribution requests (128-512
+    panels, step counts ~ U{steps_range}, remesh), served by continuous
+    batching (FP32C, graph-replayed steps) with capacity classes of 128K, 512K
+    and 1M blobs per slot allocated within a 200 GB HBM budget."""
     import dvmpp_amd as dvm
     from dvmpp_amd.batch import batch_config
     from dvmpp_amd.serve import Request, Server
     from dvmpp_amd.workloads import request_trace
     trace = request_trace(count, rate=rate, steps_range=steps_range)
     reqs = [Request(i, t, px, pz, sp, seed, st) for i, (t, (px, pz, sp, seed, st)) in enumerate(trace)]
-    srv = Server(ctx, batch_config(slots=slots, slot_capacity=1 << 18, max_panels=512,
-                                   prec=dvm.F32C, use_graph=True), budget_bytes=200 << 30)
+    srv = Server(ctx, batch_config(slots=slots, slot_capacity=1 << 20, max_panels=512,
+                                   prec=dvm.F32C, use_graph=True, remesh=c4_remesh()),
+                 budget_bytes=200 << 30, classes=[1 << 17, 1 << 19, 1 << 20])
     done, wall = srv.run(reqs)
+    peak = srv.peak_bytes
     srv.close()
-    lat = np.array([r.latency for r in done])
-    steps = sum(r.steps for r in done if r.status == 0)
+    ok = [r for r in done if r.status == 0]
+    lat = np.array([r.latency for r in ok]) if ok else np.array([np.nan])
+    steps = sum(r.steps for r in ok)
     log(f"served: {len(done)} requests in {wall:.2f} s")
-    return {"workload": f"{count} requests, Poisson {rate}/s, ensemble bodies, steps ~ "
-                        f"U{{{steps_range[0]}..{steps_range[1]}}}, {slots} slots, 200 GB budget",
+    return {"workload": f"{count} requests, Poisson {rate}/s, ensemble bodies (128-512 panels), "
+                        f"steps ~ U{{{steps_range[0]}..{steps_range[1]}}}, M4' remesh every 20 "
+                        f"steps at 0.01, {slots} slots per batch region, capacity classes "
+                        f"128K/512K/1M blobs, 200 GB budget",
             "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)),
             "aggregate_steps_per_s": steps / wall, "wall_s": wall,
-            "completed_ok": sum(1 for r in done if r.status == 0)}
+            "completed_ok": len(ok), "rejected": sum(1 for r in done if r.status == 2),
+            "peak_allocated_bytes": peak,
+            "requests_by_class": {str(c): sum(1 for r in done if r.capacity == c)
+                                  for c in (1 << 17, 1 << 19, 1 << 20)}}
 
 
 def rk3_plateau_phase(ctx, steps, prec=None, fmm=None):
@@ -594,10 +677,12 @@ def main():
     ap.add_argument("--cylinder-steps", type=int, default=400)
     ap.add_argument("--cylinder-cpu-steps", type=int, default=40)
     ap.add_argument("--ensemble-count", type=int, default=128)
-    ap.add_argument("--ensemble-steps", type=int, default=40)
-    ap.add_argument("--ensemble-plain", type=int, default=32)
-    ap.add_argument("--served-count", type=int, default=24)
-    ap.add_argument("--served-steps", default="40,160")
+    ap.add_argument("--ensemble-window", type=int, default=300,
+                    help="steps per body in the ensemble phase (0: each body's full count)")
+    ap.add_argument("--ensemble-plain", type=int, default=8)
+    ap.add_argument("--ensemble-plain-ref", type=int, default=8)
+    ap.add_argument("--served-count", type=int, default=8)
+    ap.add_argument("--served-steps", default="500,3000")
     args = ap.parse_args()
     phases = set(args.phases.split(","))
 
@@ -710,8 +795,8 @@ def main():
         line["sim_steps_plain_cylinder"] = plain_cylinder_phase(ctx, args.cylinder_steps,
                                                                 args.cylinder_cpu_steps)
     if "ensemble" in phases:
-        line["ensemble"] = ensemble_phase(ctx, args.ensemble_count, args.ensemble_steps,
-                                          args.ensemble_plain)
+        line["ensemble"] = ensemble_phase(ctx, args.ensemble_count, args.ensemble_window,
+                                          args.ensemble_plain, args.ensemble_plain_ref)
     if "served" in phases:
         lo, hi = (int(v) for v in args.served_steps.split(","))
         line["served"] = served_phase(ctx, args.served_count, 20.0, (lo, hi), slots=32)

@@ -1,23 +1,27 @@
 """Continuous-batching request loop over the batched ensemble.
 
 Requests (a body, its constants, a seed and a step count) arrive over time.
-The loop admits a waiting request as soon as a slot is free and the projected
-HBM footprint of everything admitted stays within the budget, advances every
-admitted simulation together (dvm_batch_step), and retires simulations as they
-finish — so requests that arrive while others run share the device with them.
-All blob and sheet state stays device-resident; results are read back once, on
-completion.
+Device memory is reserved in capacity classes: a class is a batch region
+(dvm_batch_create) whose slots each hold up to that class's blob count.  A
+request goes to the smallest class that holds its projected blob count; it is
+admitted into a free slot of a live batch of that class, or into a new batch
+of that class when the new region fits in the HBM budget beside the regions
+already allocated.  Every live batch advances together each poll
+(dvm_batch_step), finished simulations are retired, and a batch whose slots
+are all free is released, so the allocated footprint follows the admitted
+work.  A request whose projection exceeds the largest class, or whose class
+region alone exceeds the budget, is rejected on arrival (status
+DVM_E_SIZE) instead of waiting forever.  All blob and sheet state stays
+device-resident; results are read back once, on completion.
 """
 import time
 from dataclasses import dataclass, field
 
+from ._lib import BatchConfig
 from .batch import Batch, footprint
 from .workloads import projected_blobs
 
-# Per blob of slot capacity in an FP32C batch: two SoA sets (136), RK stores
-# (16), sorted FP32 tile copies and FP64 positions (48), sort keys/values and
-# permutation (16) plus sort scratch (8), chunk partials (64), flags (4).
-BYTES_PER_BLOB = 292
+STATUS_REJECTED = 2  # DVM_E_SIZE: the request cannot fit any class or the budget
 
 
 @dataclass
@@ -33,6 +37,7 @@ class Request:
     finished: float = -1.0
     status: int = -1
     slot: int = -1
+    capacity: int = 0   # blob capacity of the slot it ran in
     loads: tuple = ()
     state: dict = field(default_factory=dict)
 
@@ -41,60 +46,155 @@ class Request:
         return self.finished - self.arrival
 
 
+def _class_config(cfg, capacity):
+    return BatchConfig(cfg.slots, capacity, cfg.max_panels, cfg.scheme, cfg.crossing, cfg.prec,
+                       cfg.use_graph, cfg.remesh)
+
+
 class Server:
-    def __init__(self, ctx, cfg, budget_bytes, steps_per_poll=1, keep_state=False):
+    """`cfg` is the template of every batch (slots per batch, largest slot
+    capacity, body size limit, scheme, crossing, arithmetic, graph replay,
+    remesh); `classes` lists the slot capacities (default: the template's
+    alone).  `budget_bytes` bounds the device bytes of all live batches."""
+
+    def __init__(self, ctx, cfg, budget_bytes, classes=None, steps_per_poll=1, keep_state=False):
         self.ctx = ctx
         self.cfg = cfg
-        self.batch = Batch(ctx, cfg)
-        self.fixed_bytes = footprint(cfg)
+        self.classes = sorted(classes) if classes else [int(cfg.slot_capacity)]
+        self.class_bytes = {c: footprint(_class_config(cfg, c)) for c in self.classes}
         self.budget = budget_bytes
         self.steps_per_poll = steps_per_poll
         self.keep_state = keep_state
-        self.running = {}       # slot -> Request
-        self.projected = {}     # slot -> bytes
-        self.steps_done = 0     # sum over simulations of completed steps
+        self.batches = []        # [capacity, Batch, {slot: Request}]
+        self.steps_done = 0      # sum over simulations of completed steps
+        self.peak_bytes = 0
+
+    def allocated_bytes(self):
+        return sum(self.class_bytes[c] for c, _, _ in self.batches)
+
+    def projected(self, req):
+        """Blob count the request is expected to reach: without remesh every
+        panel releases about 2.3 blobs per step; with remesh the wake is held
+        on the grid, so at most the nodes of the strip it can cover (length
+        U t plus the body, width 4 body lengths), plus one step of nascent
+        blobs."""
+        nl = req.px.size - 1
+        n = projected_blobs(nl, req.steps)
+        rm = self.cfg.remesh
+        if rm.frequency > 0:
+            sp = req.params
+            length = abs(sp.ux) * sp.dt * req.steps + 2.0
+            strip = int(length * 4.0 / (rm.dx * rm.dz))
+            n = min(n, strip + 4 * nl * rm.frequency)
+        return n
 
     def projected_bytes(self, req):
-        nl = req.px.size - 1
-        n = min(projected_blobs(nl, req.steps), self.cfg.slot_capacity)
-        return n * BYTES_PER_BLOB
+        """Device bytes of the slot class the request would occupy (a batch's
+        region divided over its slots)."""
+        c = self._class_of(req)
+        if c is None:
+            return None
+        return self.class_bytes[c] // self.cfg.slots
 
-    def _can_admit(self, req):
-        if len(self.running) >= self.cfg.slots:
-            return False
-        return sum(self.projected.values()) + self.projected_bytes(req) <= self.budget
+    def _class_of(self, req):
+        n = self.projected(req)
+        for c in self.classes:
+            if n <= c:
+                return c
+        return None
+
+    def _place(self, req):
+        """A (batch entry) with a free slot for the request's class, creating
+        a batch if the budget allows; None if it must wait."""
+        c = self._class_of(req)
+        for entry in self.batches:
+            if entry[0] == c and len(entry[2]) < self.cfg.slots:
+                return entry
+        if self.allocated_bytes() + self.class_bytes[c] <= self.budget:
+            entry = [c, Batch(self.ctx, _class_config(self.cfg, c)), {}]
+            self.batches.append(entry)
+            self.peak_bytes = max(self.peak_bytes, self.allocated_bytes())
+            return entry
+        return None
+
+    def _rejectable(self, req):
+        c = self._class_of(req)
+        return c is None or self.class_bytes[c] > self.budget
 
     def run(self, requests, clock=time.perf_counter):
         pending = sorted(requests, key=lambda r: r.arrival)
         done = []
         t0 = clock()
-        while pending or self.running:
+        while pending or any(e[2] for e in self.batches):
             now = clock() - t0
-            while pending and pending[0].arrival <= now and self._can_admit(pending[0]):
-                req = pending.pop(0)
-                req.slot = self.batch.admit(req.px, req.pz, req.params, req.seed, req.steps)
+            waiting = []
+            for req in pending:
+                if req.arrival > now:
+                    waiting.append(req)
+                    continue
+                if self._rejectable(req):
+                    req.status = STATUS_REJECTED
+                    req.admitted = req.finished = clock() - t0
+                    done.append(req)
+                    continue
+                entry = self._place(req)
+                if entry is None:
+                    waiting.append(req)
+                    continue
+                req.slot = entry[1].admit(req.px, req.pz, req.params, req.seed, req.steps)
+                req.capacity = entry[0]
                 req.admitted = clock() - t0
-                self.running[req.slot] = req
-                self.projected[req.slot] = self.projected_bytes(req)
-            if not self.running:
-                # idle until the next arrival
-                wait = pending[0].arrival - (clock() - t0)
-                if wait > 0:
-                    time.sleep(min(wait, 0.05))
+                entry[2][req.slot] = req
+            pending = waiting
+            if not any(e[2] for e in self.batches):
+                if not pending:
+                    break
+                now = clock() - t0
+                if pending[0].arrival <= now:
+                    # an arrived request waits while nothing runs: idle
+                    # regions of other classes hold the budget; free them
+                    # (with none left it would have been placed: its class
+                    # region fits the budget, or it was rejected)
+                    for _, b, _ in self.batches:
+                        b.close()
+                    if not self.batches:
+                        for req in [r for r in pending if r.arrival <= now]:
+                            req.status = STATUS_REJECTED
+                            req.admitted = req.finished = now
+                            done.append(req)
+                            pending.remove(req)
+                    self.batches = []
+                    continue
+                time.sleep(min(pending[0].arrival - now, 0.05))
                 continue
-            self.batch.step(self.steps_per_poll)
-            for slot, status in self.batch.poll():
-                req = self.running.pop(slot)
-                self.projected.pop(slot)
-                info = self.batch.info(slot)
-                req.finished = clock() - t0
-                req.status = status
-                req.loads = info["loads"]
-                self.steps_done += info["step"]
-                if self.keep_state:
-                    req.state = self.batch.state(slot)
-                done.append(req)
+            for entry in self.batches:
+                if entry[2]:
+                    entry[1].step(self.steps_per_poll)
+            for entry in self.batches:
+                if not entry[2]:
+                    continue
+                for slot, status in entry[1].poll():
+                    req = entry[2].pop(slot)
+                    info = entry[1].info(slot)
+                    req.finished = clock() - t0
+                    req.status = status
+                    req.loads = info["loads"]
+                    self.steps_done += info["step"]
+                    if self.keep_state:
+                        req.state = entry[1].state(slot)
+                    done.append(req)
+            # release the regions that hold nothing, keeping one empty batch
+            # per class a waiting request will need
+            needed = {self._class_of(r) for r in pending}
+            for entry in [e for e in self.batches if not e[2]]:
+                if entry[0] in needed:
+                    needed.discard(entry[0])
+                    continue
+                entry[1].close()
+                self.batches.remove(entry)
         return done, clock() - t0
 
     def close(self):
-        self.batch.close()
+        for _, b, _ in self.batches:
+            b.close()
+        self.batches = []

@@ -46,10 +46,11 @@ def make_params(**kw):
     return p
 
 
-def _load():
-    if not os.path.exists(_LIB_PATH):
-        raise RuntimeError(f"oracle library missing: {_LIB_PATH} (run `make -C oracle`)")
-    lib = ctypes.CDLL(_LIB_PATH)
+def _load(path=None):
+    path = path or _LIB_PATH
+    if not os.path.exists(path):
+        raise RuntimeError(f"oracle library missing: {path} (run `make -C oracle`)")
+    lib = ctypes.CDLL(path)
     vp = ctypes.c_void_p
     sig = {
         "orc_rng_create": (vp, [ctypes.c_int]),
@@ -109,6 +110,12 @@ def _load():
 _lib = None
 
 
+def native_lib(path):
+    """The oracle built with the reference's flags for the host running it
+    (`make -C oracle native`): the CPU baseline's timing build only."""
+    return _load(path)
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -159,12 +166,14 @@ def bs_self_ld(x, z, circ, sigma):
     return u, w
 
 
-def bs_targets(sx, sz, scirc, ssigma, tx, tz):
+def bs_targets(sx, sz, scirc, ssigma, tx, tz, using=None):
+    """computeInducedAt (src/VelocityKernel.cpp:69-108); `using` picks another
+    build of the library (the CPU baseline's)."""
     sx, sz, scirc, ssigma, tx, tz = map(_f64, (sx, sz, scirc, ssigma, tx, tz))
     nt = tx.size
     u = np.empty(nt)
     w = np.empty(nt)
-    lib().orc_bs_targets(sx.size, _p(sx), _p(sz), _p(scirc), _p(ssigma), nt, _p(tx), _p(tz),
+    (using or lib()).orc_bs_targets(sx.size, _p(sx), _p(sz), _p(scirc), _p(ssigma), nt, _p(tx), _p(tz),
                          _p(u), _p(w))
     return u, w
 

@@ -72,3 +72,29 @@ def test_product_sources_do_not_reference_oracle(path):
         if os.path.isfile(f) and f.endswith((".py", ".hip", ".hpp", ".cpp", ".h")):
             assert "import oracle" not in open(f).read(), f
             assert "orc_" not in open(f).read(), f
+
+
+def test_server_rejects_what_cannot_fit_without_gpu():
+    """The serving loop rejects on arrival a request whose class region alone
+    exceeds the budget (no device work; a fake clock), so run() returns."""
+    import numpy as np
+
+    from dvmpp_amd import bodies, sim
+    from dvmpp_amd.batch import batch_config
+    from dvmpp_amd.serve import STATUS_REJECTED, Request, Server
+    now = [0.0]
+
+    def clock():
+        now[0] += 0.01
+        return now[0]
+
+    srv = Server(None, batch_config(slots=4, slot_capacity=1 << 14, max_panels=64),
+                 budget_bytes=1024)
+    px, pz = bodies.cylinder(32)
+    reqs = [Request(i, 0.02 * i, px, pz, sim.sheet_params(), i, 10) for i in range(3)]
+    # one more whose projection exceeds every class
+    reqs.append(Request(3, 0.0, px, pz, sim.sheet_params(), 3, 10**6))
+    done, wall = srv.run(reqs, clock=clock)
+    assert len(done) == 4 and all(r.status == STATUS_REJECTED for r in done)
+    assert not srv.batches and wall < 1.0
+    assert np.isfinite(wall)

@@ -42,12 +42,31 @@ def test_served_equals_alone(gpu_ctx):
 
 
 def test_budget_limits_admission(gpu_ctx):
-    reqs = make_requests(4)
-    srv = Server(gpu_ctx, batch_config(slots=4, slot_capacity=1 << 14, max_panels=64,
-                                       prec=dvm.F32C), budget_bytes=0)
-    one = max(srv.projected_bytes(r) for r in reqs)
+    # a budget of one single-slot region: one request on the device at a time
+    cfg = batch_config(slots=1, slot_capacity=1 << 14, max_panels=64, prec=dvm.F32C)
+    srv = Server(gpu_ctx, cfg, budget_bytes=0)
+    one = srv.class_bytes[1 << 14]
     srv.close()
-    done, _ = serve(gpu_ctx, make_requests(4), slots=4, budget=one)  # room for one at a time
-    spans = sorted((r.admitted, r.finished) for r in done.values())
+    srv = Server(gpu_ctx, cfg, budget_bytes=one, keep_state=True)
+    done, _ = srv.run(make_requests(4))
+    assert srv.peak_bytes == one
+    srv.close()
+    spans = sorted((r.admitted, r.finished) for r in done)
+    assert len(spans) == 4
     assert all(spans[k + 1][0] >= spans[k][1] for k in range(len(spans) - 1))
-    assert all(r.status == 0 for r in done.values())
+    assert all(r.status == 0 for r in done)
+
+
+def test_capacity_classes(gpu_ctx):
+    # small requests take the small class; results equal the single-class run
+    reqs = make_requests(6)
+    cfg = batch_config(slots=4, slot_capacity=1 << 14, max_panels=64, prec=dvm.F32C)
+    srv = Server(gpu_ctx, cfg, budget_bytes=200 << 30, classes=[1 << 12, 1 << 14], keep_state=True)
+    done, _ = srv.run(reqs)
+    srv.close()
+    single, _ = serve(gpu_ctx, make_requests(6), slots=4)
+    assert all(r.status == 0 for r in done)
+    for r in done:
+        assert r.capacity == min(c for c in (1 << 12, 1 << 14) if srv.projected(r) <= c)
+        for f in ("x", "z", "circ", "u", "w"):
+            assert np.array_equal(r.state[f], single[r.rid].state[f]), (r.rid, f)

@@ -304,7 +304,7 @@ def ensemble_phase(ctx, count, window, plain_count, ref_count, seed=3):
     b = Batch(ctx, batch_config(slots=count, slot_capacity=cap, max_panels=512, prec=dvm.F32C,
                                 use_graph=True, remesh=rp))
     t0 = time.perf_counter()
-    slots = [b.admit(px, pz, sp, sd, st) for (px, pz, sp, sd, _), st in zip(jobs, run)]
+    slots = b.admit_many([(px, pz, sp, sd, st) for (px, pz, sp, sd, _), st in zip(jobs, run)])
     torch.cuda.synchronize()
     ta = time.perf_counter() - t0
     ctx.prof_reset()
@@ -313,13 +313,18 @@ def ensemble_phase(ctx, count, window, plain_count, ref_count, seed=3):
     pairs = 0.0
     chunk = 50
     t0 = time.perf_counter()
+    done_steps = 0
     while len(status) < count:
         b.step(chunk)
+        done_steps += chunk
         # sizes after the chunk: a coarse sum of N^2 over the chunk's steps
         for s in slots:
             if s not in status:
                 pairs += float(b.info(s)["n"]) ** 2 * chunk
         status.update(dict(b.poll(count)))
+        if done_steps % 250 == 0:
+            log(f"ensemble batch: step {done_steps}, {count - len(status)} running, "
+                f"{time.perf_counter() - t0:.1f} s")
     torch.cuda.synchronize()
     tb = time.perf_counter() - t0
     ctx.prof_enable(False)
@@ -348,6 +353,7 @@ def ensemble_phase(ctx, count, window, plain_count, ref_count, seed=3):
             t_set += t2 - t1
             t_run += t3 - t2
             steps += run[k]
+            log(f"ensemble plain FP64 ({name}): body {k}, {run[k]} steps in {t3 - t2:.2f} s")
         plain[name] = {"bodies": int(m), "steps": steps, "seconds": t_run, "setup_seconds": t_set,
                        "aggregate_steps_per_s": steps / t_run,
                        "speedup_of_batched": (total / tb) / (steps / t_run),
@@ -455,6 +461,87 @@ def rk3_plateau_phase(ctx, steps, prec=None, fmm=None):
             "steps_per_s": 1.0 / dt,
             "pair_interactions_per_s_in_step": 3.0 * nmean * (nmean - 1) / dt,
             "remesh_ms": tr * 1e3, "remesh_blobs_out": int(out[0].numel())}
+
+
+def graph_sim_phase(ctx):
+    """Single simulations in the chosen form with graph-replayed steps: each
+    runs as a one-slot batch (FP32C sums, device random stream, counts kept on
+    the device, each step two replayed hipGraphs with the RK3 stages inside
+    the first).  (1) The RK3 cylinder at its plateau: the 1.49M-blob
+    remeshed wake loaded into the slot, one step untimed, then two timed.
+    (2) The RK3 cylinder run from rest to step 4000 (M4' remesh every 20
+    steps).  (3) The plain cylinder's 400 Euler steps from rest."""
+    import torch
+
+    import dvmpp_amd as dvm
+    from dvmpp_amd import sim as dsim
+    from dvmpp_amd.batch import Batch, batch_config
+    from dvmpp_amd.workloads import plain_cylinder, rk3_cylinder, wake_snapshot
+    out = {}
+    px, pz, sp, sig = rk3_cylinder()
+    rm = dsim.remesh_params(frequency=20, kernel=dsim.M4PRIME, xmin=-5.0, xmax=25.0, zmin=-10.0,
+                            zmax=10.0, dx=0.01, dz=0.01, sigma=sig, circ_threshold=1e-10)
+
+    def one_slot(scheme, cap, remesh, body, params, steps):
+        b = Batch(ctx, batch_config(slots=1, slot_capacity=cap, max_panels=512, scheme=scheme,
+                                    crossing=dsim.REFLECT, prec=dvm.F32C, use_graph=True,
+                                    remesh=remesh))
+        return b, b.admit(body[0], body[1], params, 0, steps)
+
+    snap = wake_snapshot(sigma=sig)
+    b, slot = one_slot(dsim.RK3, 1 << 21, rm, (px, pz), sp, 1 << 30)
+    b.set_state(slot, snap["x"], snap["z"], snap["circ"], snap["sigma"])
+    b.step(1)
+    torch.cuda.synchronize()
+    n1 = b.info(slot)["n"]
+    t0 = time.perf_counter()
+    b.step(2)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 2
+    n2 = b.info(slot)["n"]
+    b.close()
+    nmean = 0.5 * (n1 + n2)
+    out["rk3_plateau"] = {"blobs": [int(n1), int(n2)], "seconds_per_step": dt,
+                          "steps_per_s": 1.0 / dt,
+                          "pair_interactions_per_s_in_step": 3.0 * nmean * (nmean - 1) / dt}
+    log(f"graph RK3 plateau: 2 steps at N = {n1}..{n2} in {2 * dt:.2f} s")
+
+    run_steps = 4000
+    b, slot = one_slot(dsim.RK3, 1 << 20, rm, (px, pz), sp, run_steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    trace = []
+    for k in range(0, run_steps, 500):
+        b.step(500)
+        info = b.info(slot)
+        trace.append([k + 500, int(info["n"]), round(time.perf_counter() - t0, 3)])
+        log(f"graph RK3 run: step {k + 500}, N = {info['n']}")
+    torch.cuda.synchronize()
+    tr = time.perf_counter() - t0
+    info = b.info(slot)
+    b.close()
+    out["rk3_run_4000"] = {"seconds": tr, "steps_per_s": run_steps / tr,
+                           "final_blobs": int(info["n"]), "status": int(info["status"]),
+                           "final_cd_cl": [info["loads"][2], info["loads"][3]],
+                           "trace_step_blobs_seconds": trace}
+
+    cpx, cpz, csp = plain_cylinder()
+    b, slot = one_slot(dsim.EULER, 1 << 18, dsim.remesh_params(), (cpx, cpz), csp, 400)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.step(400)
+    torch.cuda.synchronize()
+    tc = time.perf_counter() - t0
+    info = b.info(slot)
+    b.close()
+    out["plain_cylinder_400"] = {"seconds": tc, "steps_per_s": 400 / tc,
+                                 "final_blobs": int(info["n"]), "cd": info["loads"][2]}
+    log(f"graph plain cylinder: 400 steps in {tc:.2f} s, N = {info['n']}")
+    out["workload"] = ("one-slot batches, FP32C sums, device random stream, graph-replayed "
+                       "steps: RK3 cylinder (512 panels, nu 5e-4, dt 0.0025, M4' remesh every "
+                       "20 steps at 0.01) at its 1.49M-blob plateau and from rest to step 4000; "
+                       "plain cylinder (256 panels, Euler, dt 0.005) for 400 steps")
+    return out
 
 
 def rk3_run_phase(ctx, steps, fmm=(12, 0.5)):
@@ -668,7 +755,7 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (reference kernel "
                          "for scaling HBM byte counters)")
-    ap.add_argument("--phases", default="sum,cylinder,ensemble,served,rk3,rk3run,merge,bandwidth",
+    ap.add_argument("--phases", default="sum,cylinder,ensemble,served,rk3,graph,merge,bandwidth",
                     help="comma list: sum (isolated sum, always run), cylinder, ensemble, "
                          "served, rk3, rk3run, merge, bandwidth")
     ap.add_argument("--rk3-steps", type=int, default=2)
@@ -805,7 +892,9 @@ def main():
     if "merge" in phases:
         line["merge"] = merge_phase(ctx, args.cylinder_steps)
     if "rk3run" in phases:
-        line["rk3_cylinder_run"] = rk3_run_phase(ctx, args.rk3_run_steps)
+        line["rk3_cylinder_run_fmm"] = rk3_run_phase(ctx, args.rk3_run_steps)
+    if "graph" in phases:
+        line["graph_single_simulation"] = graph_sim_phase(ctx)
     if "rk3" in phases:
         line["rk3_cylinder_plateau"] = rk3_plateau_phase(ctx, args.rk3_steps)
         line["rk3_cylinder_plateau_fmm"] = rk3_plateau_phase(ctx, args.rk3_fmm_steps, dvm.FMM,

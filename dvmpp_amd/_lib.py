@@ -164,6 +164,8 @@ SIGNATURES.update({
     "dvm_batch_poll": (_st, [_vp, _int, _vp, _vp, ctypes.POINTER(_int)]),
     "dvm_batch_slot_info": (_st, [_vp, _int, ctypes.POINTER(_i64), ctypes.POINTER(_int),
                                   ctypes.POINTER(_int), ctypes.POINTER(_int), _vp]),
+    "dvm_batch_admit_many": (_st, [_vp, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "dvm_batch_set_state": (_st, [_vp, _int, _i64, _vp, _vp, _vp, _vp]),
     "dvm_batch_get_state": (_st, [_vp, _int] + [_vp] * 6),
 })
 

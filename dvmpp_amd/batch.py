@@ -54,6 +54,26 @@ class Batch:
                                                  ctypes.byref(slot)))
         return slot.value
 
+    def admit_many(self, jobs):
+        """Admit (px, pz, sheet_params, seed, steps) bodies with their set-up
+        run concurrently; returns their slots."""
+        from ._lib import SheetParams
+        k = len(jobs)
+        pts = [(np.ascontiguousarray(px, dtype=np.float64), np.ascontiguousarray(pz, dtype=np.float64))
+               for px, pz, *_ in jobs]
+        npts = np.array([a.size for a, _ in pts], dtype=np.int32)
+        pxp = (ctypes.c_void_p * k)(*[a.ctypes.data for a, _ in pts])
+        pzp = (ctypes.c_void_p * k)(*[b.ctypes.data for _, b in pts])
+        prm = (SheetParams * k)(*[j[2] for j in jobs])
+        seeds = np.array([j[3] for j in jobs], dtype=np.int32)
+        steps = np.array([j[4] for j in jobs], dtype=np.int32)
+        slots = np.empty(k, dtype=np.int32)
+        check(self.ctx.h, load().dvm_batch_admit_many(
+            self.h, k, ctypes.c_void_p(npts.ctypes.data), pxp, pzp, prm,
+            ctypes.c_void_p(seeds.ctypes.data), ctypes.c_void_p(steps.ctypes.data),
+            ctypes.c_void_p(slots.ctypes.data)))
+        return slots.tolist()
+
     def step(self, steps=1):
         check(self.ctx.h, load().dvm_batch_step(self.h, steps))
 
@@ -78,6 +98,12 @@ class Batch:
                                                      ctypes.c_void_p(loads.ctypes.data)))
         return dict(n=n.value, step=step.value, active=active.value, status=status.value,
                     loads=tuple(loads))
+
+    def set_state(self, slot, x, z, circ, sigma):
+        """Replace an admitted slot's blobs (velocities zero, IDs 0..n-1)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, z, circ, sigma)]
+        check(self.ctx.h, load().dvm_batch_set_state(
+            self.h, slot, arrs[0].size, *[ctypes.c_void_p(a.ctypes.data) for a in arrs]))
 
     def state(self, slot):
         n = self.info(slot)["n"]

@@ -30,6 +30,7 @@ constexpr int kTB = 64;         // targets per FP32C item (one wave)
 constexpr int kMaxChunks = 4;   // source chunks per slot in the FP32C sum
 constexpr int kMinChunkTiles = 8;  // a chunk spans at least this many tiles
 constexpr int kSortBits = 24;   // Morton key bits sorted (12 per axis)
+constexpr int kAdmitStreams = 16;  // bodies set up at once by dvm_batch_admit_many
 
 struct SlotParam {
     double rho, nu, dt, ux, uz, ur, aref, four_nu_dt;
@@ -968,6 +969,7 @@ struct dvm_batch_s {
     std::vector<SlotState> hst;       // last polled states
     hipGraph_t graph_a = nullptr, graph_b = nullptr;
     hipGraphExec_t exec_a = nullptr, exec_b = nullptr;
+    std::vector<hipStream_t> admit_streams;  // set-up of several bodies at once
 };
 
 namespace {
@@ -1195,6 +1197,61 @@ void remesh_due(dvm_batch_t B)
 
 }  // namespace
 
+// Enqueues on `st` the copies that place a set-up body `sh` in slot s and
+// start it (sps, sss: host staging, alive until `st` is synchronised).
+static void admit_copy(dvm_batch_t B, int s, const SheetDev& sh, const dvm_sheet_params& pp,
+                       int seed, int steps, hipStream_t st, SlotParam& sp, SlotState& ss)
+{
+    const dvm_sheet_params* p = &pp;
+    BatchDev& d = B->d;
+    const int nl = sh.nl, mp = d.mp;
+    const int npts = nl + 1;
+    const size_t db = sizeof(double);
+    auto d2d = [&](void* dst, const void* src, size_t bytes) {
+        DVM_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
+    };
+    d2d(d.bx + (size_t)s * (mp + 1), sh.x, db * npts);
+    d2d(d.bz + (size_t)s * (mp + 1), sh.z, db * npts);
+    d2d(d.enx + (size_t)s * mp, sh.enx, db * nl);
+    d2d(d.enz + (size_t)s * mp, sh.enz, db * nl);
+    d2d(d.etx + (size_t)s * mp, sh.etx, db * nl);
+    d2d(d.etz + (size_t)s * mp, sh.etz, db * nl);
+    d2d(d.ds + (size_t)s * mp, sh.ds, db * nl);
+    d2d(d.panel + (size_t)s * mp, sh.panel, sizeof(PanelConst) * nl);
+    d2d(d.P + (size_t)s * mp * (mp + 1), sh.P, db * nl * (nl + 1));
+    DVM_HIP_CHECK(hipMemsetAsync(d.gamma + (size_t)s * mp, 0, db * nl, st));
+    if (B->cfg.prec == DVM_F32C) {
+        const size_t mo = (size_t)s * mp * 2 * kSheetTerms;
+        d2d(d.gcen + (size_t)s * d.mg, sh.gcen, sizeof(double4) * sh.ng);
+        d2d(d.gI0 + mo, sh.gI0, db * 2 * kSheetTerms * nl);
+        d2d(d.gI1 + mo, sh.gI1, db * 2 * kSheetTerms * nl);
+    }
+    sp = SlotParam{};
+    sp.rho = p->rho; sp.nu = p->nu; sp.dt = p->dt; sp.ux = p->ux; sp.uz = p->uz;
+    sp.ur = std::sqrt(p->ux * p->ux + p->uz * p->uz);
+    sp.aref = sh.aref;
+    sp.four_nu_dt = 4.0 * p->nu * p->dt;
+    sp.nl = nl; sp.max_npv = p->max_npv; sp.steps_target = steps;
+    sp.seed = (uint32_t)seed;
+    sp.box = sh.box;
+    sp.nslab = 0;
+    if (sh.slabs.n > 0 && sh.hslab_edge.size() <= (size_t)kSlabCap * mp) {
+        DVM_HIP_CHECK(hipMemcpyAsync(d.slab_off + (size_t)s * (mp + 1), sh.hslab_off.data(),
+                                     sizeof(int) * sh.hslab_off.size(), hipMemcpyHostToDevice,
+                                     st));
+        DVM_HIP_CHECK(hipMemcpyAsync(d.slab_edge + (size_t)s * kSlabCap * mp,
+                                     sh.hslab_edge.data(), sizeof(int) * sh.hslab_edge.size(),
+                                     hipMemcpyHostToDevice, st));
+        sp.nslab = sh.slabs.n;
+        sp.slab_z0 = sh.slabs.z0;
+        sp.slab_inv_h = sh.slabs.inv_h;
+    }
+    ss = SlotState{};
+    ss.active = 1;
+    DVM_HIP_CHECK(hipMemcpyAsync(d.prm + s, &sp, sizeof(sp), hipMemcpyHostToDevice, st));
+    DVM_HIP_CHECK(hipMemcpyAsync(d.st + s, &ss, sizeof(ss), hipMemcpyHostToDevice, st));
+}
+
 extern "C" {
 
 dvm_status dvm_batch_footprint(const dvm_batch_config* cfg, size_t* bytes)
@@ -1266,6 +1323,8 @@ dvm_status dvm_batch_destroy(dvm_batch_t B)
         if (B->graph_a) DVM_HIP_CHECK(hipGraphDestroy(B->graph_a));
         if (B->graph_b) DVM_HIP_CHECK(hipGraphDestroy(B->graph_b));
         if (B->mem) DVM_HIP_CHECK(hipFree(B->mem));
+        for (hipStream_t a : B->admit_streams) DVM_HIP_CHECK(hipStreamDestroy(a));
+        B->admit_streams.clear();
     });
     delete B;
     return st;
@@ -1275,75 +1334,69 @@ dvm_status dvm_batch_admit(dvm_batch_t B, int npts, const double* px, const doub
                            const dvm_sheet_params* p, int seed, int steps, int* slot)
 {
     if (!B || !slot) return DVM_E_INVALID;
+    return dvm_batch_admit_many(B, 1, &npts, &px, &pz, p, &seed, &steps, slot);
+}
+
+dvm_status dvm_batch_admit_many(dvm_batch_t B, int count, const int* npts,
+                                const double* const* px, const double* const* pz,
+                                const dvm_sheet_params* p, const int* seed, const int* steps,
+                                int* slots)
+{
+    if (!B || count < 0 || (count > 0 && (!npts || !px || !pz || !p || !seed || !steps || !slots)))
+        return DVM_E_INVALID;
     dvm_ctx_t ctx = B->ctx;
     return guard(ctx, [&] {
-        *slot = -1;
-        require(steps >= 1, DVM_E_INVALID, "steps must be >= 1");
-        require(npts - 1 <= B->d.mp, DVM_E_SIZE, "body has more panels than max_panels");
-        int s = -1;
+        for (int i = 0; i < count; ++i) slots[i] = -1;
+        std::vector<int> free_slots;
         for (int k = 0; k < B->d.S; ++k)
-            if (!B->busy[k]) { s = k; break; }
-        require(s >= 0, DVM_E_STATE, "no free slot");
-        SheetDev sh;
+            if (!B->busy[k]) free_slots.push_back(k);
+        require((int)free_slots.size() >= count, DVM_E_STATE, "no free slot");
+        for (int i = 0; i < count; ++i) {
+            require(steps[i] >= 1, DVM_E_INVALID, "steps must be >= 1");
+            require(npts[i] - 1 <= B->d.mp, DVM_E_SIZE, "body has more panels than max_panels");
+        }
+        // Bodies set up on up to kAdmitStreams streams at once: each body's
+        // least-squares passes run one workgroup at a time, so concurrent
+        // bodies fill the device.  One host synchronisation at the end.
+        const int ns = count < kAdmitStreams ? count : kAdmitStreams;
+        while ((int)B->admit_streams.size() < ns) {
+            hipStream_t st = nullptr;
+            DVM_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            B->admit_streams.push_back(st);
+        }
+        std::vector<SheetDev> sh((size_t)count);
+        std::vector<SlotParam> sps((size_t)count);
+        std::vector<SlotState> sss((size_t)count);
+        // the set-up reads nothing the context's stream still writes
+        if (count > 1) DVM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        auto wait_all = [&]() {
+            if (count == 1) {
+                DVM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+                return;
+            }
+            for (int k = 0; k < ns; ++k) DVM_HIP_CHECK(hipStreamSynchronize(B->admit_streams[k]));
+        };
         try {
-            sheet_init(ctx, sh, npts, px, pz, p);
+            for (int i = 0; i < count; ++i) {
+                hipStream_t st = count == 1 ? ctx->stream : B->admit_streams[i % ns];
+                sheet_init(ctx, sh[i], npts[i], px[i], pz[i], &p[i], st, false);
+                admit_copy(B, free_slots[i], sh[i], p[i], seed[i], steps[i], st, sps[i], sss[i]);
+            }
+            wait_all();
         } catch (...) {
-            sheet_free(sh);
+            for (int k = 0; k < ns && count > 1; ++k) (void)hipStreamSynchronize(B->admit_streams[k]);
+            (void)hipStreamSynchronize(ctx->stream);
+            for (auto& x : sh) sheet_free(x);
             throw;
         }
-        BatchDev& d = B->d;
-        const int nl = sh.nl, mp = d.mp;
-        hipStream_t st = ctx->stream;
-        const size_t db = sizeof(double);
-        auto d2d = [&](void* dst, const void* src, size_t bytes) {
-            DVM_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
-        };
-        d2d(d.bx + (size_t)s * (mp + 1), sh.x, db * npts);
-        d2d(d.bz + (size_t)s * (mp + 1), sh.z, db * npts);
-        d2d(d.enx + (size_t)s * mp, sh.enx, db * nl);
-        d2d(d.enz + (size_t)s * mp, sh.enz, db * nl);
-        d2d(d.etx + (size_t)s * mp, sh.etx, db * nl);
-        d2d(d.etz + (size_t)s * mp, sh.etz, db * nl);
-        d2d(d.ds + (size_t)s * mp, sh.ds, db * nl);
-        d2d(d.panel + (size_t)s * mp, sh.panel, sizeof(PanelConst) * nl);
-        d2d(d.P + (size_t)s * mp * (mp + 1), sh.P, db * nl * (nl + 1));
-        DVM_HIP_CHECK(hipMemsetAsync(d.gamma + (size_t)s * mp, 0, db * nl, st));
-        if (B->cfg.prec == DVM_F32C) {
-            const size_t mo = (size_t)s * mp * 2 * kSheetTerms;
-            d2d(d.gcen + (size_t)s * d.mg, sh.gcen, sizeof(double4) * sh.ng);
-            d2d(d.gI0 + mo, sh.gI0, db * 2 * kSheetTerms * nl);
-            d2d(d.gI1 + mo, sh.gI1, db * 2 * kSheetTerms * nl);
+        for (int i = 0; i < count; ++i) {
+            sheet_free(sh[i]);
+            const int s = free_slots[i];
+            B->busy[s] = 1;
+            B->host_steps[s] = 0;
+            B->hst[s] = sss[i];
+            slots[i] = s;
         }
-        SlotParam sp{};
-        sp.rho = p->rho; sp.nu = p->nu; sp.dt = p->dt; sp.ux = p->ux; sp.uz = p->uz;
-        sp.ur = std::sqrt(p->ux * p->ux + p->uz * p->uz);
-        sp.aref = sh.aref;
-        sp.four_nu_dt = 4.0 * p->nu * p->dt;
-        sp.nl = nl; sp.max_npv = p->max_npv; sp.steps_target = steps;
-        sp.seed = (uint32_t)seed;
-        sp.box = sh.box;
-        sp.nslab = 0;
-        if (sh.slabs.n > 0 && sh.hslab_edge.size() <= (size_t)kSlabCap * mp) {
-            DVM_HIP_CHECK(hipMemcpyAsync(d.slab_off + (size_t)s * (mp + 1), sh.hslab_off.data(),
-                                         sizeof(int) * sh.hslab_off.size(), hipMemcpyHostToDevice,
-                                         st));
-            DVM_HIP_CHECK(hipMemcpyAsync(d.slab_edge + (size_t)s * kSlabCap * mp,
-                                         sh.hslab_edge.data(), sizeof(int) * sh.hslab_edge.size(),
-                                         hipMemcpyHostToDevice, st));
-            sp.nslab = sh.slabs.n;
-            sp.slab_z0 = sh.slabs.z0;
-            sp.slab_inv_h = sh.slabs.inv_h;
-        }
-        SlotState ss{};
-        ss.active = 1;
-        DVM_HIP_CHECK(hipMemcpyAsync(d.prm + s, &sp, sizeof(sp), hipMemcpyHostToDevice, st));
-        DVM_HIP_CHECK(hipMemcpyAsync(d.st + s, &ss, sizeof(ss), hipMemcpyHostToDevice, st));
-        DVM_HIP_CHECK(hipStreamSynchronize(st));
-        sheet_free(sh);
-        B->busy[s] = 1;
-        B->host_steps[s] = 0;
-        B->hst[s] = ss;
-        *slot = s;
     });
 }
 
@@ -1419,6 +1472,40 @@ dvm_status dvm_batch_get_state(dvm_batch_t B, int slot, double* x, double* z, do
         for (int k = 0; k < 6; ++k)
             if (outs[k] && ss.n) DVM_HIP_CHECK(hipMemcpyAsync(outs[k], srcs[k] + off, b, hipMemcpyDeviceToHost, st));
         DVM_HIP_CHECK(hipStreamSynchronize(st));
+    });
+}
+
+dvm_status dvm_batch_set_state(dvm_batch_t B, int slot, int64_t n, const double* x,
+                               const double* z, const double* circ, const double* sigma)
+{
+    if (!B || slot < 0 || slot >= B->d.S) return DVM_E_INVALID;
+    return guard(B->ctx, [&] {
+        require(B->busy[slot], DVM_E_STATE, "slot not admitted");
+        require(n >= 0 && n <= B->d.cap, DVM_E_SIZE, "blob count exceeds the slot capacity");
+        require(n == 0 || (x && z && circ && sigma), DVM_E_INVALID, "null array");
+        hipStream_t st = B->ctx->stream;
+        const int64_t off = (int64_t)slot * B->d.cap;
+        const size_t b = sizeof(double) * n;
+        SlotState ss;
+        DVM_HIP_CHECK(hipMemcpyAsync(&ss, B->d.st + slot, sizeof(ss), hipMemcpyDeviceToHost, st));
+        DVM_HIP_CHECK(hipStreamSynchronize(st));
+        if (n > 0) {
+            const double* srcs[4] = {x, z, circ, sigma};
+            double* dsts[4] = {B->d.b.x, B->d.b.z, B->d.b.c, B->d.b.s};
+            for (int k = 0; k < 4; ++k)
+                DVM_HIP_CHECK(hipMemcpyAsync(dsts[k] + off, srcs[k], b, hipMemcpyHostToDevice, st));
+            for (double* v : {B->d.b.u, B->d.b.w, B->d.b.uvs, B->d.b.wvs})
+                DVM_HIP_CHECK(hipMemsetAsync(v + off, 0, b, st));
+            std::vector<uint32_t> ids((size_t)n);
+            for (int64_t i = 0; i < n; ++i) ids[(size_t)i] = (uint32_t)i;
+            DVM_HIP_CHECK(hipMemcpyAsync(B->d.b.id + off, ids.data(), sizeof(uint32_t) * n,
+                                         hipMemcpyHostToDevice, st));
+            DVM_HIP_CHECK(hipStreamSynchronize(st));  // ids lives on the host stack
+        }
+        ss.n = n;
+        DVM_HIP_CHECK(hipMemcpyAsync(B->d.st + slot, &ss, sizeof(ss), hipMemcpyHostToDevice, st));
+        DVM_HIP_CHECK(hipStreamSynchronize(st));
+        B->hst[slot] = ss;
     });
 }
 

@@ -109,8 +109,9 @@ __global__ __launch_bounds__(kBlock) void bs_f64_kernel(
     // Sum of the pairs with sources g.r[0..cnt-1] (source index jb + q).
     auto group = [&](const Rec64x4& g, int cnt, int64_t jb, bool self_check) {
         double dx[4], dz[4], r2[4], K[4];
-        bool near[4], take[4];
-        bool any_near = false;
+        bool take[4];
+        // smallest r2 - thr over the group: negative iff some pair is near
+        double gap = 1.0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             dx[q] = xi - g.r[q].x;
@@ -120,13 +121,15 @@ __global__ __launch_bounds__(kBlock) void bs_f64_kernel(
             if (SELF) take[q] = take[q] && !(self_check && jb + q == i);
             if (GUARD) take[q] = take[q] && (r2[q] > r2_skip);
             K[q] = recip64(r2[q]);
-            near[q] = take[q] && r2[q] < g.r[q].w;
-            any_near = any_near || near[q];
+            gap = fmin(gap, take[q] ? r2[q] - g.r[q].w : 1.0);
         }
-        if (any_near) {
+        if (__any(gap < 0.0)) {  // wave-uniform: the group's near-field correction
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (near[q]) K[q] *= 1.0 - exp(-r2[q] * (10.0 / g.r[q].w));
+            for (int q = 0; q < 4; ++q) {
+                const bool near = take[q] && r2[q] < g.r[q].w;
+                const double f = 1.0 - exp(-r2[q] * (10.0 / g.r[q].w));
+                K[q] = near ? K[q] * f : K[q];
+            }
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {

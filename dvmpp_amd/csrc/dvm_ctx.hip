@@ -54,7 +54,7 @@ static hipEvent_t event_take(dvm_ctx_t ctx)
 
 ProfScope::ProfScope(dvm_ctx_t ctx, const char* name) : ctx_(ctx), name_(name)
 {
-    if (!ctx_->prof) return;
+    if (!ctx_ || !ctx_->prof) return;  // no context: work on another stream, not timed
     start_ = event_take(ctx_);
     DVM_HIP_CHECK(hipEventRecord(start_, ctx_->stream));
 }

@@ -214,6 +214,10 @@ struct SheetDev {
     int ng = 0;
     double4* gcen = nullptr;
     double *gI0 = nullptr, *gI1 = nullptr;
+    // host staging of the set-up's uploads, alive until the caller has
+    // synchronised the stream sheet_init ran on
+    std::vector<double> hstage, hgcen;
+    std::vector<PanelConst> hpanel;
 };
 
 constexpr int kSheetTerms = 40;
@@ -233,8 +237,12 @@ void sheet_groups(const SheetDev& sh, std::vector<double>& cen);
 void panel_moments(hipStream_t st, int nl, const PanelConst* pc, double cx, double cz, double r,
                    const double4* gcen, double* i0, double* i1);
 
+// Sets up a body's sheet (geometry, influence matrix, least-squares operator,
+// far-field moments) on `st` (default: the context's stream).  With
+// sync = false it returns with the work queued: the caller synchronises `st`
+// before using or freeing the sheet.
 void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const double* pz,
-                const dvm_sheet_params* p);
+                const dvm_sheet_params* p, hipStream_t st = nullptr, bool sync = true);
 void sheet_free(SheetDev& sh);
 // solvevortexsheet: collocation sum + RHS + gamma = P b.
 // have_uw: the collocation sums are already in sh.uw.

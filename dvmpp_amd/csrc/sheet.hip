@@ -539,8 +539,10 @@ int edge_slabs_build(int npts, const double* px, const double* pz, std::vector<i
 }
 
 void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const double* pz,
-                const dvm_sheet_params* p)
+                const dvm_sheet_params* p, hipStream_t st, bool sync)
 {
+    const bool own_stream = st == nullptr || st == ctx->stream;
+    if (!st) st = ctx->stream;
     require(npts >= 3, DVM_E_SIZE, "a body needs at least 3 points (2 panels)");
     require(npts <= 4097, DVM_E_SIZE, "at most 4096 panels per body");
     require(px && pz && p, DVM_E_INVALID, "null body or parameters");
@@ -561,7 +563,8 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
     sh.aref = mx - mn;
     sh.hxc.resize(nl); sh.hzc.resize(nl); sh.htheta.resize(nl); sh.hds.resize(nl);
     sh.henx.resize(nl); sh.henz.resize(nl); sh.hetx.resize(nl); sh.hetz.resize(nl);
-    std::vector<PanelConst> pc(nl);
+    std::vector<PanelConst>& pc = sh.hpanel;
+    pc.resize(nl);
     for (int i = 0; i < nl; ++i) {
         sh.hxc[i] = 0.5 * (px[i] + px[i + 1]);
         sh.hzc[i] = 0.5 * (pz[i] + pz[i + 1]);
@@ -599,7 +602,7 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
     const size_t d = sizeof(double);
     const size_t nred = 3 * (size_t)nl > 1024 ? 3 * (size_t)nl : 1024;
     const size_t nm = 2 * (size_t)nl * kSheetTerms;
-    std::vector<double> gcen;  // live until the synchronisation at the end
+    std::vector<double>& gcen = sh.hgcen;  // live until the caller's synchronisation
     sheet_groups(sh, gcen);
     const int ng = (int)(gcen.size() / 4);
     const size_t need = Carver::need({d * npts, d * npts, d * nl * 8, sizeof(PanelConst) * nl,
@@ -639,10 +642,11 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
     sh.gI0 = cv.take<double>(nm);
     sh.gI1 = cv.take<double>(nm);
     if (nslab > 0) {
-        DVM_HIP_CHECK(hipMemcpy(slab_off, sh.hslab_off.data(), sizeof(int) * sh.hslab_off.size(),
-                                hipMemcpyHostToDevice));
-        DVM_HIP_CHECK(hipMemcpy(slab_edge, sh.hslab_edge.data(),
-                                sizeof(int) * sh.hslab_edge.size(), hipMemcpyHostToDevice));
+        DVM_HIP_CHECK(hipMemcpyAsync(slab_off, sh.hslab_off.data(),
+                                     sizeof(int) * sh.hslab_off.size(), hipMemcpyHostToDevice, st));
+        DVM_HIP_CHECK(hipMemcpyAsync(slab_edge, sh.hslab_edge.data(),
+                                     sizeof(int) * sh.hslab_edge.size(), hipMemcpyHostToDevice,
+                                     st));
         sh.slabs.off = slab_off;
         sh.slabs.edge = slab_edge;
         sh.slabs.n = nslab;
@@ -652,7 +656,8 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
         sh.slabs = EdgeSlabs{};
     }
 
-    std::vector<double> g((size_t)nl * 8);
+    std::vector<double>& g = sh.hstage;
+    g.resize((size_t)nl * 8);
     std::memcpy(&g[0], sh.hxc.data(), d * nl);
     std::memcpy(&g[nl], sh.hzc.data(), d * nl);
     std::memcpy(&g[2 * nl], sh.htheta.data(), d * nl);
@@ -661,9 +666,8 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
     std::memcpy(&g[5 * nl], sh.henz.data(), d * nl);
     std::memcpy(&g[6 * nl], sh.hetx.data(), d * nl);
     std::memcpy(&g[7 * nl], sh.hetz.data(), d * nl);
-    hipStream_t st = ctx->stream;
-    DVM_HIP_CHECK(hipMemcpyAsync(sh.x, px, d * npts, hipMemcpyHostToDevice, st));
-    DVM_HIP_CHECK(hipMemcpyAsync(sh.z, pz, d * npts, hipMemcpyHostToDevice, st));
+    DVM_HIP_CHECK(hipMemcpyAsync(sh.x, sh.hx.data(), d * npts, hipMemcpyHostToDevice, st));
+    DVM_HIP_CHECK(hipMemcpyAsync(sh.z, sh.hz.data(), d * npts, hipMemcpyHostToDevice, st));
     DVM_HIP_CHECK(hipMemcpyAsync(geo, g.data(), d * nl * 8, hipMemcpyHostToDevice, st));
     DVM_HIP_CHECK(hipMemcpyAsync(sh.panel, pc.data(), sizeof(PanelConst) * nl,
                                  hipMemcpyHostToDevice, st));
@@ -681,13 +685,13 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
     double* U = G + (size_t)nl * W;
     int* pivrow = reinterpret_cast<int*>(U + (size_t)kb * W);
     {
-        ProfScope ps(ctx, "panel_influence");
+        ProfScope ps(own_stream ? ctx : nullptr, "panel_influence");
         influence_kernel<<<dim3(cdiv(nl, 128), nl + 1), 128, 0, st>>>(nl, sh.x, sh.z, sh.xc, sh.zc,
                                                                       sh.theta, sh.ds, sh.A);
         DVM_LAUNCH_CHECK();
     }
     {
-        ProfScope ps(ctx, "sheet_operator");
+        ProfScope ps(own_stream ? ctx : nullptr, "sheet_operator");
         gj_build_kernel<<<dim3(cdiv(W, 128), nl), 128, 0, st>>>(nl, sh.A, G);
         DVM_LAUNCH_CHECK();
         switch (kb) {
@@ -701,7 +705,7 @@ void sheet_init(dvm_ctx_t ctx, SheetDev& sh, int npts, const double* px, const d
         DVM_LAUNCH_CHECK();
     }
     DVM_HIP_CHECK(hipFreeAsync(G, st));
-    DVM_HIP_CHECK(hipStreamSynchronize(st));
+    if (sync) DVM_HIP_CHECK(hipStreamSynchronize(st));
 }
 
 // ----------------------------------------------------------------- reductions

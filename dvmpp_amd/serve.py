@@ -103,12 +103,13 @@ class Server:
                 return c
         return None
 
-    def _place(self, req):
-        """A (batch entry) with a free slot for the request's class, creating
-        a batch if the budget allows; None if it must wait."""
+    def _place(self, req, reserved):
+        """A (batch entry) with a free slot for the request's class (slots
+        `reserved` this round count as taken), creating a batch if the budget
+        allows; None if it must wait."""
         c = self._class_of(req)
         for entry in self.batches:
-            if entry[0] == c and len(entry[2]) < self.cfg.slots:
+            if entry[0] == c and len(entry[2]) + reserved.get(id(entry), 0) < self.cfg.slots:
                 return entry
         if self.allocated_bytes() + self.class_bytes[c] <= self.budget:
             entry = [c, Batch(self.ctx, _class_config(self.cfg, c)), {}]
@@ -128,6 +129,8 @@ class Server:
         while pending or any(e[2] for e in self.batches):
             now = clock() - t0
             waiting = []
+            placed = {}      # id(entry) -> (entry, [requests])
+            reserved = {}
             for req in pending:
                 if req.arrival > now:
                     waiting.append(req)
@@ -137,14 +140,22 @@ class Server:
                     req.admitted = req.finished = clock() - t0
                     done.append(req)
                     continue
-                entry = self._place(req)
+                entry = self._place(req, reserved)
                 if entry is None:
                     waiting.append(req)
                     continue
-                req.slot = entry[1].admit(req.px, req.pz, req.params, req.seed, req.steps)
-                req.capacity = entry[0]
-                req.admitted = clock() - t0
-                entry[2][req.slot] = req
+                placed.setdefault(id(entry), (entry, []))[1].append(req)
+                reserved[id(entry)] = reserved.get(id(entry), 0) + 1
+            # the requests placed in one region are set up together
+            for entry, reqs in placed.values():
+                slots = entry[1].admit_many([(r.px, r.pz, r.params, r.seed, r.steps)
+                                             for r in reqs])
+                t_adm = clock() - t0
+                for req, slot in zip(reqs, slots):
+                    req.slot = slot
+                    req.capacity = entry[0]
+                    req.admitted = t_adm
+                    entry[2][slot] = req
             pending = waiting
             if not any(e[2] for e in self.batches):
                 if not pending:

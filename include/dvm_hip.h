@@ -469,6 +469,15 @@ dvm_status dvm_batch_destroy(dvm_batch_t batch);
  * does; zero blobs. */
 dvm_status dvm_batch_admit(dvm_batch_t batch, int npts, const double* px, const double* pz,
                            const dvm_sheet_params* params, int seed, int steps, int* slot);
+/* dvm_batch_admit for `count` bodies at once (arrays of count entries; px[i],
+ * pz[i] HOST points of body i): their set-up (influence matrix, least-squares
+ * operator, moments) runs concurrently on several streams, with one host
+ * synchronisation in all.  slots[i] receives body i's slot.  No counterpart
+ * in the reference (it has one body). */
+dvm_status dvm_batch_admit_many(dvm_batch_t batch, int count, const int* npts,
+                                const double* const* px, const double* const* pz,
+                                const dvm_sheet_params* params, const int* seed,
+                                const int* steps, int* slots);
 /* Advances every active simulation by `steps` time steps. */
 dvm_status dvm_batch_step(dvm_batch_t batch, int steps);
 /* Reports up to `max` simulations that stopped since the last poll (finished
@@ -480,6 +489,13 @@ dvm_status dvm_batch_slot_info(dvm_batch_t batch, int slot, int64_t* n, int* ste
 /* A slot's blob fields to HOST arrays (n = dvm_batch_slot_info's n). */
 dvm_status dvm_batch_get_state(dvm_batch_t batch, int slot, double* x, double* z, double* circ,
                                double* sigma, double* u, double* w);
+/* Replaces an admitted slot's blobs by n blobs from HOST arrays (velocities
+ * zero, IDs 0..n-1; the step count is kept), as VortexBlobs' public fields
+ * are written directly (src/VortexBlobs.hpp:24-32); n <= slot_capacity.  A
+ * one-slot batch so started from a saved wake runs that simulation with
+ * graph-replayed steps. */
+dvm_status dvm_batch_set_state(dvm_batch_t batch, int slot, int64_t n, const double* x,
+                               const double* z, const double* circ, const double* sigma);
 
 #ifdef __cplusplus
 }

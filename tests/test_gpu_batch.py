@@ -203,3 +203,34 @@ def test_fp64_slots_match_oracle(gpu_ctx, scheme, crossing):
         for k in (2, 3):
             assert abs(info["loads"][k] - lo[k]) <= 1e-10 * max(1.0, abs(lo[k])), (slot, k)
     b.close()
+
+
+def test_set_state_one_slot_matches_single_simulation(gpu_ctx):
+    """A one-slot batch started from a saved wake (dvm_batch_set_state) steps
+    as the single simulation started from the same state: FP32C RK3 steps
+    with remesh, device random stream (positions to 1e-6, Cd to 1e-6)."""
+    from dvmpp_amd.workloads import rk3_cylinder, wake_snapshot
+    px, pz, sp, sig = rk3_cylinder()
+    rp = dsim.remesh_params(frequency=2, kernel=dsim.M4PRIME, xmin=-5.0, xmax=25.0, zmin=-10.0,
+                            zmax=10.0, dx=0.01, dz=0.01, sigma=sig, circ_threshold=1e-10)
+    snap = wake_snapshot(n_target=40_000, sigma=sig)
+    b = Batch(gpu_ctx, batch_config(slots=1, slot_capacity=1 << 17, max_panels=512,
+                                    scheme=dsim.RK3, prec=dvm.F32C, remesh=rp))
+    slot = b.admit(px, pz, sp, 7, 100)
+    b.set_state(slot, snap["x"], snap["z"], snap["circ"], snap["sigma"])
+    assert b.info(slot)["n"] == snap["x"].size
+    b.step(3)  # a remesh at step 2
+    got, info = b.state(slot), b.info(slot)
+    b.close()
+    s = dsim.Sim(gpu_ctx, px, pz, dsim.SimParams(sp, dsim.RK3, dsim.REFLECT, 7, dvm.F32C,
+                                                 dsim.RNG_DEVICE, rp))
+    s.set_state(snap)
+    s.run(3)
+    ref, lo = s.get_state(), s.loads()
+    s.close()
+    assert ref["x"].size == got["x"].size
+    scale = np.max(np.abs(ref["x"]))
+    assert np.max(np.abs(ref["x"] - got["x"])) <= 1e-6 * scale
+    assert np.max(np.abs(ref["z"] - got["z"])) <= 1e-6 * scale
+    assert np.array_equal(ref["sigma"], got["sigma"])
+    assert abs(lo[2] - info["loads"][2]) <= 1e-6 and abs(lo[3] - info["loads"][3]) <= 1e-6

@@ -155,7 +155,9 @@ def test_c4_ensemble_in_one_batch(gpu_ctx):
     steps = 3
     cfg = batch_config(slots=128, slot_capacity=1 << 13, max_panels=512, prec=dvm.F32C)
     b = Batch(gpu_ctx, cfg)
-    slots = [b.admit(px, pz, sp, seed, steps) for px, pz, sp, seed, _ in ens]
+    # set up concurrently (dvm_batch_admit_many); the alone runs below admit
+    # one body at a time on the context's stream
+    slots = b.admit_many([(px, pz, sp, seed, steps) for px, pz, sp, seed, _ in ens])
     b.step(steps)
     done = dict(b.poll(256))
     assert len(done) == 128 and all(st == 0 for st in done.values())

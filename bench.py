@@ -337,6 +337,8 @@ def ensemble_phase(ctx, count, window, plain_count, ref_count, seed=3):
     plain = {}
     for name, mode, m in (("device_stream", dsim.RNG_DEVICE, plain_count),
                           ("reference_stream", dsim.RNG_REFERENCE, ref_count)):
+        if m == 0:
+            continue
         t_set = t_run = 0.0
         steps = 0
         for k in pick[:m]:

@@ -105,6 +105,14 @@ class Batch:
         check(self.ctx.h, load().dvm_batch_set_state(
             self.h, slot, arrs[0].size, *[ctypes.c_void_p(a.ctypes.data) for a in arrs]))
 
+    def sort_order(self, slot, n):
+        """Storage indices of the slot's first n blobs in its last FP32C pair
+        sum's Morton order (n = the slot's count when that sum ran)."""
+        out = np.empty(n, dtype=np.uint32)
+        check(self.ctx.h, load().dvm_batch_sort_order(self.h, slot, n,
+                                                      ctypes.c_void_p(out.ctypes.data)))
+        return out
+
     def state(self, slot):
         n = self.info(slot)["n"]
         arrs = {k: np.empty(n) for k in ("x", "z", "circ", "sigma", "u", "w")}

@@ -15,7 +15,6 @@
 #include "dvm_device.hpp"
 #include "dvm_internal.hpp"
 
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -25,11 +24,16 @@
 namespace dvmi {
 
 constexpr int kBT = 256;       // sources per tile; targets per FP64 item
-constexpr int kColChunks = 64;  // source chunks of the collocation sum per slot
+constexpr int kColChunks = 256;   // most source chunks of the collocation sum per slot
+constexpr int kColSources = 256;  // sources per collocation chunk, at least
 constexpr int kTB = 64;         // targets per FP32C item (one wave)
-constexpr int kMaxChunks = 4;   // source chunks per slot in the FP32C sum
+constexpr int kMaxChunks = 64;  // most source chunks per slot in the FP32C sum
+constexpr int kBaseChunks = 4;  // chunks of every slot large enough for them
+constexpr int kSlotItems = 65536;  // FP32C items a slot is cut into, at least
 constexpr int kMinChunkTiles = 8;  // a chunk spans at least this many tiles
 constexpr int kSortBits = 24;   // Morton key bits sorted (12 per axis)
+constexpr int kSortTile = 4096; // keys per sort workgroup (256 lanes x 16)
+constexpr int kBoxParts = 32;   // workgroups per slot of the bounding-box pass
 constexpr int kAdmitStreams = 16;  // bodies set up at once by dvm_batch_admit_many
 
 struct SlotParam {
@@ -59,7 +63,7 @@ struct WorkList {
     int work_counter;
     int64_t total_blobs;
     int total_items;  // FP32C work items (slot, chunk, 64-target block)
-    int pad;
+    int total_stiles; // sort tiles (kSortTile keys of one slot)
 };
 
 struct BatchDev {
@@ -83,13 +87,15 @@ struct BatchDev {
     // FP32C sum: per-slot Morton sort and chunked work items
     int* item_pre = nullptr;    // S+1, in `order`
     double4* sbox = nullptr;    // S bounding boxes
+    double4* box_part = nullptr;  // S * kBoxParts partial boxes
     uint32_t *keys = nullptr, *keys2 = nullptr, *vals = nullptr, *perm = nullptr;  // S*cap
-    int *seg_begin = nullptr, *seg_end = nullptr;  // S
+    int* stile_pre = nullptr;   // S+1, sort tiles per slot (slot order)
+    int* shist = nullptr;       // 256 digit counts per sort tile, S * max_stiles
+    int max_stiles = 0;
     TileInfo* tboxes = nullptr; // S * max_tboxes, 64-target boxes
     int max_tboxes = 0;
-    double* part = nullptr;     // 2 * kMaxChunks * S * cap chunk partials
-    void* sort_tmp = nullptr;
-    size_t sort_tmp_bytes = 0;
+    double* part = nullptr;     // S * part_slot chunk partials
+    int64_t part_slot = 0;      // partials per slot: 2 * max over n <= cap of n * chunks(n)
     int* flags = nullptr;
     // sheet
     double *bx = nullptr, *bz = nullptr;  // S*(mp+1)
@@ -97,7 +103,7 @@ struct BatchDev {
     PanelConst* panel = nullptr;
     double* P = nullptr;        // S * mp*(mp+1)
     double* gamma = nullptr;
-    double* col = nullptr;      // S * kColChunks * (2*mp + 1)
+    double* col = nullptr;      // S * kColChunks * (2*mp + 1) collocation partials
     int* cnt = nullptr;
     int64_t* off = nullptr;     // S*(mp+1)
     double* cnew = nullptr;
@@ -114,23 +120,50 @@ struct BatchDev {
 
 // ----------------------------------------------------------------- work lists
 
-// One workgroup: per-slot tile and blob counts, their prefixes, the slot order
-// by decreasing tile count, and the work-counter reset.
-// FP32C source chunks of a slot with n live blobs (a function of n alone, so
-// a slot's sum does not depend on the rest of the batch).
-__device__ __forceinline__ int slot_chunks(int64_t n)
+// FP32C source chunks of a slot with n live blobs: enough (slot, chunk,
+// 64-target block) items that a slot alone keeps the device busy, at least
+// kBaseChunks, each chunk at least kMinChunkTiles tiles.  A function of n
+// alone, so a slot's sum does not depend on the rest of the batch.
+__host__ __device__ __forceinline__ int slot_chunks(int64_t n)
 {
     const int ntl = (int)((n + kBT - 1) / kBT);
-    int c = ntl / kMinChunkTiles;
-    return c < 1 ? 1 : (c > kMaxChunks ? kMaxChunks : c);
+    const int nblk = (int)((n + kTB - 1) / kTB);
+    int c = nblk > 0 ? (kSlotItems + nblk - 1) / nblk : 1;
+    c = c > kBaseChunks ? c : kBaseChunks;
+    const int cmax = ntl / kMinChunkTiles;
+    c = c > cmax ? cmax : c;
+    c = c > kMaxChunks ? kMaxChunks : c;
+    return c < 1 ? 1 : c;
 }
 
+// Collocation source chunks of a slot with n live blobs (a function of n).
+__host__ __device__ __forceinline__ int colloc_chunks(int64_t n)
+{
+    const int64_t c = (n + kColSources - 1) / kColSources;
+    return c < 1 ? 1 : (c > kColChunks ? kColChunks : (int)c);
+}
+
+// Chunk partials a slot of capacity cap may need: 2 * max n * slot_chunks(n)
+// over n <= cap (the chunk count is constant between multiples of kTB).
+static int64_t part_slot_size(int64_t cap)
+{
+    // beyond kSlotItems * kTB blobs the count is kBaseChunks: n * chunks grows
+    int64_t m = cap * slot_chunks(cap);
+    for (int64_t n = kTB; n < cap && n <= (int64_t)kSlotItems * kTB; n += kTB) {
+        const int64_t v = n * slot_chunks(n);
+        m = v > m ? v : m;
+    }
+    return 2 * m;
+}
+
+// One workgroup: per-slot tile and blob counts, their prefixes, the slot order
+// by decreasing tile count, the FP32C item and sort-tile prefixes, and the
+// work-counter reset.
 __global__ __launch_bounds__(1024) void bt_prefix_kernel(int S, int64_t cap,
                                                          const SlotState* __restrict__ st,
                                                          WorkList* wl, int* tile_pre, int* order,
                                                          int* otile_pre, int64_t* blob_pre,
-                                                         int* item_pre, int* seg_begin,
-                                                         int* seg_end)
+                                                         int* item_pre, int* stile_pre)
 {
     __shared__ int tiles_s[1024];
     const int t = threadIdx.x;
@@ -148,22 +181,27 @@ __global__ __launch_bounds__(1024) void bt_prefix_kernel(int S, int64_t cap,
     }
     __syncthreads();
     if (t == 0) {
-        int acc = 0, oacc = 0;
+        int acc = 0, oacc = 0, sacc = 0;
         int64_t bacc = 0;
         for (int s = 0; s < S; ++s) {
+            const int64_t n = st[s].active ? st[s].n : 0;
             tile_pre[s] = acc;
             acc += tiles_s[s];
             blob_pre[s] = bacc;
-            bacc += st[s].active ? st[s].n : 0;
+            bacc += n;
             otile_pre[s] = oacc;
             oacc += tiles_s[order[s]];
+            stile_pre[s] = sacc;
+            sacc += (int)((n + kSortTile - 1) / kSortTile);
         }
         tile_pre[S] = acc;
         otile_pre[S] = oacc;
         blob_pre[S] = bacc;
+        stile_pre[S] = sacc;
         wl->total_tiles = acc;
         wl->work_counter = 0;
         wl->total_blobs = bacc;
+        wl->total_stiles = sacc;
         int iacc = 0;
         for (int r = 0; r < S; ++r) {
             const int s = order[r];
@@ -173,11 +211,6 @@ __global__ __launch_bounds__(1024) void bt_prefix_kernel(int S, int64_t cap,
         }
         item_pre[S] = iacc;
         wl->total_items = iacc;
-    }
-    for (int s = t; s < S; s += 1024) {
-        const int64_t n = st[s].active ? st[s].n : 0;
-        seg_begin[s] = (int)(s * cap);
-        seg_end[s] = (int)(s * cap + n);
     }
 }
 
@@ -208,26 +241,32 @@ __device__ __forceinline__ void for_each_blob(const WorkList* wl, const int64_t*
 // ----------------------------------------------------------------- pair sums
 //
 // FP32C (the form of bs_kernels.hip, per slot): each slot's live blobs are
-// Morton-sorted (bounding box, 24-bit keys, segmented radix sort over the
-// slot ranges), cut into 256-source tiles stored as FP32 offsets from their
-// FP64 centres, and summed by a persistent kernel whose waves pull
-// (slot, source chunk, 64-target block) items, largest slots first and chunk
-// by chunk, so co-resident waves stream the same source slice.  Each item
-// writes FP64 chunk partials; a reduce adds them in chunk order and scatters
-// to the storage order.  Chunking and order depend on the slot's own count
-// only.  FP64: one 256-target item per workgroup over all sources, in index
-// order (the reference's summation order).
+// Morton-sorted (bounding box, 24-bit keys, a radix sort with several
+// workgroups per slot), cut into 256-source tiles stored as FP32 offsets
+// from their FP64 centres, and summed by a
+// persistent kernel whose waves pull (slot, source chunk, 64-target block)
+// items, largest slots first and chunk by chunk, so co-resident waves stream
+// the same source slice.  Each item writes FP64 chunk partials; a reduce adds
+// them in chunk order and scatters to the storage order.  Chunking and
+// order depend on the slot's own count only.  FP64: one 256-target item per
+// workgroup over all sources, in index order (the reference's summation
+// order).
 
-// Per-slot bounding box of the live blobs.
+// Per-slot bounding box of the live blobs: kBoxParts workgroups per slot
+// each reduce a contiguous part (min and max are exact, so the box does not
+// depend on the split), then one lane per slot combines the parts.
 __global__ __launch_bounds__(256) void bt_bbox_kernel(int64_t cap, const SlotState* __restrict__ st,
-                                                      BlobBuf b, double4* __restrict__ sbox)
+                                                      BlobBuf b, double4* __restrict__ box_part)
 {
     __shared__ double4 red[256];
-    const int s = blockIdx.x;
+    const int s = blockIdx.y;
     const int64_t n = st[s].active ? st[s].n : 0;
+    const int64_t len = (n + kBoxParts - 1) / kBoxParts;
+    const int64_t j0 = (int64_t)blockIdx.x * len;
+    const int64_t j1 = j0 + len < n ? j0 + len : n;
     const int64_t base = (int64_t)s * cap;
     double4 m = make_double4(INFINITY, -INFINITY, INFINITY, -INFINITY);
-    for (int64_t k = threadIdx.x; k < n; k += 256) {
+    for (int64_t k = j0 + threadIdx.x; k < j1; k += 256) {
         const double x = b.x[base + k], z = b.z[base + k];
         m.x = fmin(m.x, x); m.y = fmax(m.y, x);
         m.z = fmin(m.z, z); m.w = fmax(m.w, z);
@@ -243,7 +282,21 @@ __global__ __launch_bounds__(256) void bt_bbox_kernel(int64_t cap, const SlotSta
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) sbox[s] = red[0];
+    if (threadIdx.x == 0) box_part[(int64_t)s * kBoxParts + blockIdx.x] = red[0];
+}
+
+__global__ void bt_bbox_final_kernel(int S, const double4* __restrict__ box_part,
+                                     double4* __restrict__ sbox)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    double4 r = box_part[(int64_t)s * kBoxParts];
+    for (int p = 1; p < kBoxParts; ++p) {
+        const double4 o = box_part[(int64_t)s * kBoxParts + p];
+        r.x = fmin(r.x, o.x); r.y = fmax(r.y, o.y);
+        r.z = fmin(r.z, o.z); r.w = fmax(r.w, o.w);
+    }
+    sbox[s] = r;
 }
 
 // Morton key (12 bits per axis over the slot's box) and local index.
@@ -263,6 +316,133 @@ __global__ void bt_morton_kernel(int S, int64_t cap, const WorkList* wl,
         keys[o] = spread16((uint32_t)qx) | (spread16((uint32_t)qz) << 1);
         vals[o] = (uint32_t)(o - (int64_t)s * cap);
     });
+}
+
+// Stable LSD radix sort of every slot's live keys by 8-bit digits, several
+// workgroups per slot: each sort tile (kSortTile consecutive keys of one slot)
+// counts its digits; per slot the counts are scanned digit-major over the
+// slot's tiles, giving every (tile, digit) its first output rank; each tile
+// then ranks its keys stably (wave by wave in key order, equal digits by lane
+// ballots) and scatters keys and values into the slot's range.  Counts come
+// from the device, so the sort is captured once and replays at any size; its
+// output equals any stable sort's.
+
+__global__ __launch_bounds__(256) void bt_sort_hist_kernel(int S, int64_t cap, const WorkList* wl,
+                                                           const int* __restrict__ stile_pre,
+                                                           const SlotState* __restrict__ st,
+                                                           const uint32_t* __restrict__ keys,
+                                                           int shift, int* __restrict__ hist)
+{
+    __shared__ int cnt[256];
+    const int total = wl->total_stiles;
+    for (int g = blockIdx.x; g < total; g += gridDim.x) {
+        cnt[threadIdx.x] = 0;
+        __syncthreads();
+        const int s = find_slot(stile_pre, S, g);
+        const int64_t n = st[s].n;
+        const int64_t i0 = (int64_t)(g - stile_pre[s]) * kSortTile;
+        const int64_t i1 = i0 + kSortTile < n ? i0 + kSortTile : n;
+        const uint32_t* __restrict__ k = keys + (int64_t)s * cap;
+        for (int64_t i = i0 + threadIdx.x; i < i1; i += 256)
+            atomicAdd(&cnt[(k[i] >> shift) & 255u], 1);
+        __syncthreads();
+        hist[(int64_t)g * 256 + threadIdx.x] = cnt[threadIdx.x];
+        __syncthreads();
+    }
+}
+
+// One workgroup per slot, one lane per digit.
+__global__ __launch_bounds__(256) void bt_sort_scan_kernel(const int* __restrict__ stile_pre,
+                                                           int* __restrict__ hist)
+{
+    __shared__ int tot[256];
+    const int s = blockIdx.x;
+    const int d = threadIdx.x;
+    const int g0 = stile_pre[s], g1 = stile_pre[s + 1];
+    if (g0 == g1) return;
+    int run = 0;
+    for (int g = g0; g < g1; ++g) {
+        const int v = hist[(int64_t)g * 256 + d];
+        hist[(int64_t)g * 256 + d] = run;
+        run += v;
+    }
+    tot[d] = run;
+    __syncthreads();
+    for (int w = 1; w < 256; w <<= 1) {  // inclusive scan over digits
+        const int v = d >= w ? tot[d - w] : 0;
+        __syncthreads();
+        tot[d] += v;
+        __syncthreads();
+    }
+    const int base = tot[d] - run;
+    for (int g = g0; g < g1; ++g) hist[(int64_t)g * 256 + d] += base;
+}
+
+constexpr int kSortPer = kSortTile / 256;  // keys per lane
+
+__global__ __launch_bounds__(256) void bt_sort_scatter_kernel(
+    int S, int64_t cap, const WorkList* wl, const int* __restrict__ stile_pre,
+    const SlotState* __restrict__ st, const uint32_t* __restrict__ kin,
+    const uint32_t* __restrict__ vin, int shift, const int* __restrict__ hist,
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout)
+{
+    __shared__ int wcnt[4][256];  // per wave: digit counts, then offsets
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const int total = wl->total_stiles;
+    for (int g = blockIdx.x; g < total; g += gridDim.x) {
+        for (int k = threadIdx.x; k < 4 * 256; k += 256) (&wcnt[0][0])[k] = 0;
+        __syncthreads();
+        const int s = find_slot(stile_pre, S, g);
+        const int64_t n = st[s].n;
+        const int64_t base = (int64_t)s * cap;
+        // wave w ranks keys [w * kSortTile / 4, (w + 1) * kSortTile / 4) of the tile
+        const int64_t i0 = (int64_t)(g - stile_pre[s]) * kSortTile + w * (kSortTile / 4);
+        uint32_t kk[kSortPer], vv[kSortPer];
+        int dig[kSortPer], rk[kSortPer];
+#pragma unroll
+        for (int j = 0; j < kSortPer; ++j) {
+            const int64_t i = i0 + j * 64 + lane;
+            const bool valid = i < n;
+            const uint32_t key = valid ? kin[base + i] : 0u;
+            const int dd = (int)((key >> shift) & 255u);
+            uint64_t peers = __ballot(valid);
+#pragma unroll
+            for (int bit = 0; bit < 8; ++bit) {
+                const uint64_t m = __ballot((dd >> bit) & 1);
+                peers &= ((dd >> bit) & 1) ? m : ~m;
+            }
+            int r = -1;
+            if (valid) r = wcnt[w][dd] + (int)__popcll(peers & below);
+            // the lowest lane of each digit group advances the wave's count
+            if (valid && (peers & below) == 0) wcnt[w][dd] += (int)__popcll(peers);
+            kk[j] = key;
+            vv[j] = valid ? vin[base + i] : 0u;
+            dig[j] = dd;
+            rk[j] = r;
+        }
+        __syncthreads();
+        {
+            const int d = threadIdx.x;  // one lane per digit: offsets of the waves
+            int a = 0;
+            for (int v = 0; v < 4; ++v) {
+                const int c = wcnt[v][d];
+                wcnt[v][d] = a;
+                a += c;
+            }
+        }
+        __syncthreads();
+        const int* __restrict__ ho = hist + (int64_t)g * 256;
+#pragma unroll
+        for (int j = 0; j < kSortPer; ++j) {
+            if (rk[j] < 0) continue;
+            const int64_t dst = base + ho[dig[j]] + wcnt[w][dig[j]] + rk[j];
+            kout[dst] = kk[j];
+            vout[dst] = vv[j];
+        }
+        __syncthreads();
+    }
 }
 
 // One workgroup per 256-source tile (sorted order): tile record, FP32
@@ -366,11 +546,10 @@ __global__ __launch_bounds__(256) void bt_bs32_kernel(int S, int64_t cap, int ma
                                                       const double2* __restrict__ p64,
                                                       const TileInfo* __restrict__ tiles,
                                                       const TileInfo* __restrict__ tboxes,
-                                                      double* __restrict__ part)
+                                                      int64_t part_slot, double* __restrict__ part)
 {
     const int lane = threadIdx.x & 63;
     const int total = wl->total_items;
-    const int64_t nb = (int64_t)S * cap;
     for (;;) {
         int item = 0;
         if (lane == 0) item = atomicAdd(&wl->work_counter, 1);
@@ -430,8 +609,9 @@ __global__ __launch_bounds__(256) void bt_bs32_kernel(int S, int64_t cap, int ma
             kahan_add(sw, cw, aw);
         }
         if (i < n) {
-            part[((int64_t)c * 2 + 0) * nb + base + i] = -((double)su - (double)cu);
-            part[((int64_t)c * 2 + 1) * nb + base + i] = (double)sw - (double)cw;
+            double* __restrict__ ps = part + (int64_t)s * part_slot + (int64_t)c * 2 * n;
+            ps[i] = -((double)su - (double)cu);
+            ps[n + i] = (double)sw - (double)cw;
         }
     }
 }
@@ -441,16 +621,17 @@ __global__ __launch_bounds__(256) void bt_bs32_kernel(int S, int64_t cap, int ma
 __global__ void bt_reduce32_kernel(int S, int64_t cap, const WorkList* wl,
                                    const int64_t* __restrict__ blob_pre,
                                    const SlotState* __restrict__ st,
-                                   const uint32_t* __restrict__ perm,
+                                   const uint32_t* __restrict__ perm, int64_t part_slot,
                                    const double* __restrict__ part, BlobBuf b)
 {
-    const int64_t nb = (int64_t)S * cap;
     for_each_blob(wl, blob_pre, S, cap, [&](int s, int64_t o) {
-        const int nch = slot_chunks(st[s].n);
+        const int64_t n = st[s].n;
+        const int nch = slot_chunks(n);
+        const double* __restrict__ ps = part + (int64_t)s * part_slot + (o - (int64_t)s * cap);
         double su = 0.0, sw = 0.0;
         for (int c = 0; c < nch; ++c) {
-            su += part[((int64_t)c * 2 + 0) * nb + o];
-            sw += part[((int64_t)c * 2 + 1) * nb + o];
+            su += ps[(int64_t)c * 2 * n];
+            sw += ps[(int64_t)c * 2 * n + n];
         }
         const int64_t dst = (int64_t)s * cap + perm[o];
         b.u[dst] = su * kRpi2;
@@ -509,7 +690,9 @@ __global__ __launch_bounds__(256) void bt_colloc_kernel(int64_t cap, int mp,
     if (!st[s].active) return;
     const int nl = prm[s].nl;
     const int64_t n = st[s].n;
-    const int64_t chunk = (n + kColChunks - 1) / kColChunks;
+    const int nch = colloc_chunks(n);
+    if (c >= nch) return;
+    const int64_t chunk = (n + nch - 1) / nch;
     const int64_t j0 = (int64_t)c * chunk;
     const int64_t j1 = j0 + chunk < n ? j0 + chunk : n;
     const int64_t base = (int64_t)s * cap;
@@ -536,31 +719,37 @@ __global__ __launch_bounds__(256) void bt_colloc_kernel(int64_t cap, int mp,
 
 // Per slot: b = (U + u).n, b_Nl = -sum circ (the chunk partials added in
 // chunk order), into rhs (mp + 1 per slot).
-__global__ __launch_bounds__(256) void bt_rhs_kernel(int mp, const SlotParam* __restrict__ prm,
+__global__ __launch_bounds__(64) void bt_rhs_kernel(int mp, const SlotParam* __restrict__ prm,
                                                      const SlotState* __restrict__ st,
                                                      const double* __restrict__ enx,
                                                      const double* __restrict__ enz,
                                                      const double* __restrict__ col,
                                                      double* __restrict__ rhs)
 {
-    const int s = blockIdx.x;
+    const int s = blockIdx.y;
     if (!st[s].active) return;
     const int nl = prm[s].nl;
     const bool have = st[s].n > 0;
     const double* cs = col + (int64_t)s * kColChunks * (2 * mp + 1);
-    for (int i = threadIdx.x; i <= nl; i += 256) {
+    const int nch = colloc_chunks(st[s].n);
+    const int64_t cstride = 2 * mp + 1;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= nl) {
         double v;
         if (i == nl) {
             double tot = 0.0;
-            for (int c = 0; c < kColChunks; ++c) tot += cs[(int64_t)c * (2 * mp + 1) + 2 * mp];
+#pragma unroll 8
+            for (int c = 0; c < nch; ++c) tot += cs[c * cstride + 2 * mp];
             v = have ? -tot : -0.0;
         } else {
             double u = 0.0, w = 0.0;
-            if (have)
-                for (int c = 0; c < kColChunks; ++c) {
-                    u += cs[(int64_t)c * (2 * mp + 1) + i];
-                    w += cs[(int64_t)c * (2 * mp + 1) + mp + i];
+            if (have) {
+#pragma unroll 8
+                for (int c = 0; c < nch; ++c) {
+                    u += cs[c * cstride + i];
+                    w += cs[c * cstride + mp + i];
                 }
+            }
             const double ex = enx[(int64_t)s * mp + i], ez = enz[(int64_t)s * mp + i];
             v = have ? (prm[s].ux + u * kRpi2) * ex + (prm[s].uz + w * kRpi2) * ez
                      : prm[s].ux * ex + prm[s].uz * ez;
@@ -1013,15 +1202,15 @@ size_t batch_bytes(const dvm_batch_config& c, BatchDev* d, char* mem)
     o.keys2 = cv.take<uint32_t>(nf);
     o.vals = cv.take<uint32_t>(nf);
     o.perm = cv.take<uint32_t>(nf);
-    o.seg_begin = cv.take<int>(S);
-    o.seg_end = cv.take<int>(S);
+    o.box_part = cv.take<double4>(f32c ? S * kBoxParts : 0);
+    const size_t maxst = (cap + kSortTile - 1) / kSortTile;
+    o.max_stiles = (int)maxst;
+    o.stile_pre = cv.take<int>(S + 1);
+    o.shist = cv.take<int>(f32c ? S * maxst * 256 : 0);
     o.tboxes = cv.take<TileInfo>(f32c ? S * maxtb : 0);
     o.max_tboxes = (int)maxtb;
-    o.part = cv.take<double>(2 * kMaxChunks * nf);
-    // segmented-sort scratch: two key/value buffers' worth plus 1 MiB (the
-    // size the library asks for is checked against it at creation)
-    o.sort_tmp_bytes = f32c ? 8 * nb + (1u << 20) : 0;
-    o.sort_tmp = cv.take<char>(o.sort_tmp_bytes);
+    o.part_slot = f32c ? part_slot_size((int64_t)cap) : 0;
+    o.part = cv.take<double>(S * (size_t)o.part_slot);
     o.flags = cv.take<int>(nb);
     o.bx = cv.take<double>(np1);
     o.bz = cv.take<double>(np1);
@@ -1048,6 +1237,25 @@ size_t batch_bytes(const dvm_batch_config& c, BatchDev* d, char* mem)
     return cv.off + 256;
 }
 
+// The slots' Morton keys (keys, vals) sorted into (keys2, perm): three
+// 8-bit passes, (keys, vals) -> (keys2, perm) -> (keys, vals) -> (keys2, perm).
+static void record_sort(const BatchDev& d, hipStream_t st)
+{
+    static_assert(kSortBits == 24, "three 8-bit passes");
+    const unsigned grid = (unsigned)std::min<int64_t>((int64_t)d.S * d.max_stiles, 4096);
+    uint32_t* kin[3] = {d.keys, d.keys2, d.keys};
+    uint32_t* vin[3] = {d.vals, d.perm, d.vals};
+    uint32_t* kout[3] = {d.keys2, d.keys, d.keys2};
+    uint32_t* vout[3] = {d.perm, d.vals, d.perm};
+    for (int p = 0; p < 3; ++p) {
+        bt_sort_hist_kernel<<<grid, 256, 0, st>>>(d.S, d.cap, d.wl, d.stile_pre, d.st, kin[p],
+                                                  8 * p, d.shist);
+        bt_sort_scan_kernel<<<d.S, 256, 0, st>>>(d.stile_pre, d.shist);
+        bt_sort_scatter_kernel<<<grid, 256, 0, st>>>(d.S, d.cap, d.wl, d.stile_pre, d.st, kin[p],
+                                                     vin[p], 8 * p, d.shist, kout[p], vout[p]);
+    }
+}
+
 void record_convect(dvm_batch_t B)
 {
     BatchDev& d = B->d;
@@ -1064,26 +1272,26 @@ void record_convect(dvm_batch_t B)
     const int stages = rk ? 3 : 1;
     for (int k = 0; k < stages; ++k) {
         bt_prefix_kernel<<<1, 1024, 0, st>>>(S, cap, d.st, d.wl, d.tile_pre, d.order, d.otile_pre,
-                                             d.blob_pre, d.item_pre, d.seg_begin, d.seg_end);
+                                             d.blob_pre, d.item_pre, d.stile_pre);
         if (rk && k == 0) bt_zero_q_kernel<<<gs, 256, 0, st>>>(S, cap, d.wl, d.blob_pre, d.q1, d.q2);
         if (f32c) {
-            bt_bbox_kernel<<<S, 256, 0, st>>>(cap, d.st, d.b, d.sbox);
+            bt_bbox_kernel<<<dim3(kBoxParts, S), 256, 0, st>>>(cap, d.st, d.b, d.box_part);
+            bt_bbox_final_kernel<<<(S + 63) / 64, 64, 0, st>>>(S, d.box_part, d.sbox);
             bt_morton_kernel<<<gs, 256, 0, st>>>(S, cap, d.wl, d.blob_pre, d.sbox, d.b, d.keys, d.vals);
-            size_t tmp_bytes = d.sort_tmp_bytes;
-            DVM_HIP_CHECK(hipcub::DeviceSegmentedRadixSort::SortPairs(
-                d.sort_tmp, tmp_bytes, d.keys, d.keys2, d.vals, d.perm, (int)(S * cap), S,
-                d.seg_begin, d.seg_end, 0, kSortBits, st));
+            record_sort(d, st);
             bt_tiles_kernel<<<gs, kBT, 0, st>>>(S, cap, d.max_tboxes, d.wl, d.tile_pre, d.st, d.perm,
                                                 d.b, d.s32, d.near32, d.p64, d.tiles, d.tboxes);
             bt_bs32_kernel<<<persistent, 256, 0, st>>>(S, cap, d.max_tboxes, d.wl, d.tile_pre, d.order,
                                                        d.item_pre, d.st, d.s32, d.near32, d.p64,
-                                                       d.tiles, d.tboxes, d.part);
-            bt_reduce32_kernel<<<gs, 256, 0, st>>>(S, cap, d.wl, d.blob_pre, d.st, d.perm, d.part, d.b);
+                                                       d.tiles, d.tboxes, d.part_slot, d.part);
+            bt_reduce32_kernel<<<gs, 256, 0, st>>>(S, cap, d.wl, d.blob_pre, d.st, d.perm,
+                                                   d.part_slot, d.part, d.b);
         } else {
             bt_bs64_kernel<<<persistent, kBT, 0, st>>>(S, cap, d.wl, d.order, d.otile_pre, d.st, d.b);
         }
         bt_colloc_kernel<<<dim3(kColChunks, S), 256, 0, st>>>(cap, mp, d.prm, d.st, d.b, d.panel, d.col);
-        bt_rhs_kernel<<<S, 256, 0, st>>>(mp, d.prm, d.st, d.enx, d.enz, d.col, d.rhs);
+        bt_rhs_kernel<<<dim3((mp + 64) / 64, S), 64, 0, st>>>(mp, d.prm, d.st, d.enx, d.enz, d.col,
+                                                              d.rhs);
         bt_gamma_kernel<<<dim3(kGammaBlocks, S), 256, sizeof(double) * (mp + 1), st>>>(
             mp, d.prm, d.st, d.P, d.rhs, d.gamma);
         if (f32c) {
@@ -1111,7 +1319,7 @@ void record_rest(dvm_batch_t B)
     const int mp = d.mp;
     const unsigned gs = 4096;
     bt_prefix_kernel<<<1, 1024, 0, st>>>(S, cap, d.st, d.wl, d.tile_pre, d.order, d.otile_pre,
-                                         d.blob_pre, d.item_pre, d.seg_begin, d.seg_end);
+                                         d.blob_pre, d.item_pre, d.stile_pre);
     bt_diffuse_kernel<<<gs, 256, 0, st>>>(S, cap, d.wl, d.blob_pre, d.prm, d.st, d.b);
     bt_release_count_kernel<<<S, 1024, 0, st>>>(cap, mp, d.prm, d.st, d.gamma, d.ds, d.cnt, d.off, d.cnew);
     bt_release_emit_kernel<<<dim3(16, S), 256, 0, st>>>(cap, mp, d.prm, d.st, d.bx, d.bz, d.ds, d.etx,
@@ -1119,7 +1327,7 @@ void record_rest(dvm_batch_t B)
                                                          d.panel, d.b);
     bt_commit_release_kernel<<<(S + 255) / 256, 256, 0, st>>>(S, d.st);
     bt_prefix_kernel<<<1, 1024, 0, st>>>(S, cap, d.st, d.wl, d.tile_pre, d.order, d.otile_pre,
-                                         d.blob_pre, d.item_pre, d.seg_begin, d.seg_end);
+                                         d.blob_pre, d.item_pre, d.stile_pre);
     const int reflect = B->cfg.crossing != DVM_DELETE ? 1 : 0;
     bt_inside_kernel<<<gs, kBT, 0, st>>>(S, cap, mp, d.wl, d.tile_pre, d.st, d.prm, d.bx, d.bz,
                                          d.enx, d.enz, d.slab_off, d.slab_edge, reflect, d.b,
@@ -1288,14 +1496,6 @@ dvm_status dvm_batch_create(dvm_ctx_t ctx, const dvm_batch_config* cfg, dvm_batc
         const size_t bytes = batch_bytes(c, nullptr, nullptr);
         DVM_HIP_CHECK(hipMalloc(&B->mem, bytes));
         batch_bytes(c, &B->d, B->mem);
-        if (c.prec == DVM_F32C) {
-            size_t need = 0;
-            DVM_HIP_CHECK(hipcub::DeviceSegmentedRadixSort::SortPairs(
-                nullptr, need, B->d.keys, B->d.keys2, B->d.vals, B->d.perm,
-                (int)(c.slots * c.slot_capacity), c.slots, B->d.seg_begin, B->d.seg_end, 0,
-                kSortBits, ctx->stream));
-            require(need <= B->d.sort_tmp_bytes, DVM_E_STATE, "segmented sort scratch too small");
-        }
         DVM_HIP_CHECK(hipMemsetAsync(B->d.st, 0, sizeof(SlotState) * c.slots, ctx->stream));
         DVM_HIP_CHECK(hipMemsetAsync(B->d.prm, 0, sizeof(SlotParam) * c.slots, ctx->stream));
         DVM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -1471,6 +1671,21 @@ dvm_status dvm_batch_get_state(dvm_batch_t B, int slot, double* x, double* z, do
         const double* srcs[6] = {B->d.b.x, B->d.b.z, B->d.b.c, B->d.b.s, B->d.b.u, B->d.b.w};
         for (int k = 0; k < 6; ++k)
             if (outs[k] && ss.n) DVM_HIP_CHECK(hipMemcpyAsync(outs[k], srcs[k] + off, b, hipMemcpyDeviceToHost, st));
+        DVM_HIP_CHECK(hipStreamSynchronize(st));
+    });
+}
+
+dvm_status dvm_batch_sort_order(dvm_batch_t B, int slot, int64_t n, uint32_t* order)
+{
+    if (!B || slot < 0 || slot >= B->d.S) return DVM_E_INVALID;
+    return guard(B->ctx, [&] {
+        require(B->cfg.prec == DVM_F32C, DVM_E_STATE, "only the FP32C sum sorts");
+        require(n >= 0 && n <= B->d.cap, DVM_E_SIZE, "n exceeds the slot capacity");
+        require(n == 0 || order, DVM_E_INVALID, "null array");
+        hipStream_t st = B->ctx->stream;
+        if (n)
+            DVM_HIP_CHECK(hipMemcpyAsync(order, B->d.perm + (int64_t)slot * B->d.cap,
+                                         sizeof(uint32_t) * n, hipMemcpyDeviceToHost, st));
         DVM_HIP_CHECK(hipStreamSynchronize(st));
     });
 }

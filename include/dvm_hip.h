@@ -496,6 +496,10 @@ dvm_status dvm_batch_get_state(dvm_batch_t batch, int slot, double* x, double* z
  * graph-replayed steps. */
 dvm_status dvm_batch_set_state(dvm_batch_t batch, int slot, int64_t n, const double* x,
                                const double* z, const double* circ, const double* sigma);
+/* Diagnostic, no counterpart in the reference: the first n entries of a
+ * slot's blob order in its last FP32C pair sum (storage indices sorted by
+ * Morton key, stable), n = the slot's count when that sum ran. */
+dvm_status dvm_batch_sort_order(dvm_batch_t batch, int slot, int64_t n, uint32_t* order);
 
 #ifdef __cplusplus
 }

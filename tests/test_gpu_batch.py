@@ -234,3 +234,48 @@ def test_set_state_one_slot_matches_single_simulation(gpu_ctx):
     assert np.max(np.abs(ref["z"] - got["z"])) <= 1e-6 * scale
     assert np.array_equal(ref["sigma"], got["sigma"])
     assert abs(lo[2] - info["loads"][2]) <= 1e-6 and abs(lo[3] - info["loads"][3]) <= 1e-6
+
+
+def _spread12(v):
+    v = v.astype(np.uint32) & np.uint32(0xFFFF)
+    for sh, m in ((8, 0x00FF00FF), (4, 0x0F0F0F0F), (2, 0x33333333), (1, 0x55555555)):
+        v = (v | (v << np.uint32(sh))) & np.uint32(m)
+    return v
+
+
+def _morton_order(x, z):
+    """Stable order of 24-bit Morton keys over the set's bounding box (12 bits
+    per axis, truncated; the pair sum's tile order)."""
+    x0, z0 = x.min(), z.min()
+    ext = max(max(x.max() - x0, z.max() - z0), 1e-300)
+    scale = 4095.0 / ext
+    qx = np.clip((x - x0) * scale, 0.0, 4095.0).astype(np.uint32)
+    qz = np.clip((z - z0) * scale, 0.0, 4095.0).astype(np.uint32)
+    keys = _spread12(qx) | (_spread12(qz) << np.uint32(1))
+    return np.argsort(keys, kind="stable").astype(np.uint32)
+
+
+def test_sort_order_is_stable_morton_order(gpu_ctx):
+    """The batch's radix sort (several workgroups per slot, counts on the
+    device) gives each slot the stable Morton order: slots of 70,001 blobs
+    (18 sort tiles), 5,000 (a partial second tile), 100 and one left empty;
+    positions on a coarse lattice so that many keys tie."""
+    rng = np.random.default_rng(5)
+    cfg = batch_config(slots=4, slot_capacity=1 << 17, max_panels=64, prec=dvm.F32C)
+    b = Batch(gpu_ctx, cfg)
+    (px, pz), kw = bodies.cylinder(32), dict(nu=1e-3, dt=0.005)
+    sizes = [70_001, 5_000, 100]
+    sets = []
+    for n in sizes:
+        slot = b.admit(px, pz, dsim.sheet_params(**kw), 3, 5)
+        x = 2.0 + np.round(rng.uniform(0, 6, n) * 40) / 40  # outside the unit body
+        z = np.round(rng.uniform(-2, 2, n) * 40) / 40
+        c = rng.normal(0, 1e-4, n)
+        b.set_state(slot, x, z, c, np.full(n, 0.02))
+        sets.append((slot, x, z))
+    b.admit(px, pz, dsim.sheet_params(**kw), 4, 5)  # its sheet alone: no blobs yet
+    b.step(1)  # the sort runs on the positions set above
+    for slot, x, z in sets:
+        got = b.sort_order(slot, x.size)
+        assert np.array_equal(got, _morton_order(x, z)), slot
+    b.close()

@@ -156,8 +156,33 @@ static int64_t part_slot_size(int64_t cap)
     return 2 * m;
 }
 
-// One workgroup: per-slot tile and blob counts, their prefixes, the slot order
-// by decreasing tile count, the FP32C item and sort-tile prefixes, and the
+// Exclusive prefix sum of v over the workgroup's lanes (1024 lanes, 16
+// waves): wave scans by shuffles, then the wave totals; total gets the sum.
+__device__ __forceinline__ int64_t block_scan_excl(int64_t v, int64_t* wsum, int64_t& total)
+{
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    int64_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    int64_t off = 0, tot = 0;
+    for (int k = 0; k < 16; ++k) {
+        const int64_t t = wsum[k];
+        off += k < wv ? t : 0;
+        tot += t;
+    }
+    __syncthreads();
+    total = tot;
+    return off + x - v;
+}
+
+// One workgroup of 1024 lanes, one per slot (at most 1024 slots): per-slot
+// tile and blob counts, their prefixes, the slot order by decreasing tile
+// count (ties by index), the FP32C item and sort-tile prefixes, and the
 // work-counter reset.
 __global__ __launch_bounds__(1024) void bt_prefix_kernel(int S, int64_t cap,
                                                          const SlotState* __restrict__ st,
@@ -166,51 +191,51 @@ __global__ __launch_bounds__(1024) void bt_prefix_kernel(int S, int64_t cap,
                                                          int* item_pre, int* stile_pre)
 {
     __shared__ int tiles_s[1024];
+    __shared__ int64_t n_s[1024];
+    __shared__ int ord_s[1024];
+    __shared__ int64_t wsum[16];
     const int t = threadIdx.x;
-    for (int s = t; s < S; s += 1024) {
-        const int64_t n = st[s].active ? st[s].n : 0;
-        tiles_s[s] = (int)((n + kBT - 1) / kBT);
-    }
+    const bool slot = t < S;
+    const int64_t n = slot && st[t].active ? st[t].n : 0;
+    const int tiles = (int)((n + kBT - 1) / kBT);
+    tiles_s[t] = tiles;
+    n_s[t] = n;
     __syncthreads();
-    // rank slots by tile count, ties by index
-    for (int s = t; s < S; s += 1024) {
-        int r = 0;
+    int r = 0;
+    if (slot) {
         for (int o = 0; o < S; ++o)
-            r += (tiles_s[o] > tiles_s[s]) || (tiles_s[o] == tiles_s[s] && o < s);
-        order[r] = s;
+            r += (tiles_s[o] > tiles) || (tiles_s[o] == tiles && o < t);
+        order[r] = t;
+        ord_s[r] = t;
     }
     __syncthreads();
+    int64_t tot_tiles, tot_blobs, tot_stiles, tot_otiles, tot_items;
+    const int64_t tp = block_scan_excl(tiles, wsum, tot_tiles);
+    const int64_t bp = block_scan_excl(n, wsum, tot_blobs);
+    const int64_t sp = block_scan_excl((n + kSortTile - 1) / kSortTile, wsum, tot_stiles);
+    // in slot order: lane t holds the slot of rank t
+    const int64_t no = slot ? n_s[ord_s[t]] : 0;
+    const int64_t op = block_scan_excl((no + kBT - 1) / kBT, wsum, tot_otiles);
+    const int64_t items = no > 0 ? ((no + kTB - 1) / kTB) * slot_chunks(no) : 0;
+    const int64_t ip = block_scan_excl(items, wsum, tot_items);
+    if (slot) {
+        tile_pre[t] = (int)tp;
+        blob_pre[t] = bp;
+        stile_pre[t] = (int)sp;
+        otile_pre[t] = (int)op;
+        item_pre[t] = (int)ip;
+    }
     if (t == 0) {
-        int acc = 0, oacc = 0, sacc = 0;
-        int64_t bacc = 0;
-        for (int s = 0; s < S; ++s) {
-            const int64_t n = st[s].active ? st[s].n : 0;
-            tile_pre[s] = acc;
-            acc += tiles_s[s];
-            blob_pre[s] = bacc;
-            bacc += n;
-            otile_pre[s] = oacc;
-            oacc += tiles_s[order[s]];
-            stile_pre[s] = sacc;
-            sacc += (int)((n + kSortTile - 1) / kSortTile);
-        }
-        tile_pre[S] = acc;
-        otile_pre[S] = oacc;
-        blob_pre[S] = bacc;
-        stile_pre[S] = sacc;
-        wl->total_tiles = acc;
+        tile_pre[S] = (int)tot_tiles;
+        blob_pre[S] = tot_blobs;
+        stile_pre[S] = (int)tot_stiles;
+        otile_pre[S] = (int)tot_otiles;
+        item_pre[S] = (int)tot_items;
+        wl->total_tiles = (int)tot_tiles;
         wl->work_counter = 0;
-        wl->total_blobs = bacc;
-        wl->total_stiles = sacc;
-        int iacc = 0;
-        for (int r = 0; r < S; ++r) {
-            const int s = order[r];
-            const int64_t n = st[s].active ? st[s].n : 0;
-            item_pre[r] = iacc;
-            iacc += (int)((n + kTB - 1) / kTB) * (n > 0 ? slot_chunks(n) : 0);
-        }
-        item_pre[S] = iacc;
-        wl->total_items = iacc;
+        wl->total_blobs = tot_blobs;
+        wl->total_stiles = (int)tot_stiles;
+        wl->total_items = (int)tot_items;
     }
 }
 
@@ -678,13 +703,20 @@ __global__ __launch_bounds__(kBT) void bt_bs64_kernel(int S, int64_t cap, WorkLi
 // ----------------------------------------------------------------- sheet
 
 // Collocation sums (unguarded) per (slot, source chunk): u, w partials for the
-// slot's Nl panels and the chunk's circulation sum.
+// slot's Nl panels and the chunk's circulation sum.  The chunk's sources are
+// staged 256 at a time in LDS as records {x, z, circ, 10 sigma^2} and read by
+// every lane (one panel per lane) in index order; the pair is the FP64 form's
+// (src/VelocityKernel.cpp:44-60 with the reciprocal of r^2 from recip64 and
+// fused accumulation), four sources per step, the near-field correction only
+// for a step with a near pair in some lane.
 __global__ __launch_bounds__(256) void bt_colloc_kernel(int64_t cap, int mp,
                                                         const SlotParam* __restrict__ prm,
                                                         const SlotState* __restrict__ st, BlobBuf b,
                                                         const PanelConst* __restrict__ panel,
                                                         double* __restrict__ col)
 {
+    __shared__ double4 srec[256];
+    __shared__ double red[256];
     const int s = blockIdx.y;
     const int c = blockIdx.x;
     if (!st[s].active) return;
@@ -697,15 +729,64 @@ __global__ __launch_bounds__(256) void bt_colloc_kernel(int64_t cap, int mp,
     const int64_t j1 = j0 + chunk < n ? j0 + chunk : n;
     const int64_t base = (int64_t)s * cap;
     double* out = col + ((int64_t)s * kColChunks + c) * (2 * mp + 1);
-    for (int pj = threadIdx.x; pj < nl; pj += 256) {
-        const PanelConst q = panel[(int64_t)s * mp + pj];
+    for (int p0 = 0; p0 < nl; p0 += 256) {  // wave-uniform
+        const int pj = p0 + (int)threadIdx.x;
+        const bool have = pj < nl;
+        double xi = 0.0, zi = 0.0;
+        if (have) {
+            const PanelConst q = panel[(int64_t)s * mp + pj];
+            xi = q.xc;
+            zi = q.zc;
+        }
         double ui = 0.0, wi = 0.0;
-        for (int64_t j = j0; j < j1; ++j)
-            bs_pair_f64(q.xc, q.zc, b.x[base + j], b.z[base + j], b.c[base + j], b.s[base + j], ui, wi);
-        out[pj] = ui;
-        out[mp + pj] = wi;
+        for (int64_t t0 = j0; t0 < j1; t0 += 256) {
+            const int64_t jt = t0 + threadIdx.x;
+            __syncthreads();
+            if (jt < j1) {
+                const double sj = b.s[base + jt];
+                srec[threadIdx.x] = make_double4(b.x[base + jt], b.z[base + jt], b.c[base + jt],
+                                                 10.0 * (sj * sj));
+            } else {
+                srec[threadIdx.x] = make_double4(0.0, 0.0, 0.0, 0.0);
+            }
+            __syncthreads();
+            const int cnt = j1 - t0 < 256 ? (int)(j1 - t0) : 256;
+            for (int k = 0; k < cnt; k += 4) {
+                double dx[4], dz[4], r2[4], K[4];
+                bool take[4];
+                double gap = 1.0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const double4 r = srec[k + q];
+                    dx[q] = xi - r.x;
+                    dz[q] = zi - r.y;
+                    r2[q] = __builtin_fma(dx[q], dx[q], dz[q] * dz[q]);
+                    take[q] = have && k + q < cnt;
+                    K[q] = recip64(r2[q]);
+                    gap = fmin(gap, take[q] ? r2[q] - r.w : 1.0);
+                }
+                if (__any(gap < 0.0)) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const double thr = srec[k + q].w;
+                        const bool near = take[q] && r2[q] < thr;
+                        const double f = 1.0 - exp(-r2[q] * (10.0 / thr));
+                        K[q] = near ? K[q] * f : K[q];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const double kc = take[q] ? K[q] * srec[k + q].z : 0.0;
+                    ui = __builtin_fma(-kc, dz[q], ui);
+                    wi = __builtin_fma(kc, dx[q], wi);
+                }
+            }
+        }
+        if (have) {
+            out[pj] = ui;
+            out[mp + pj] = wi;
+        }
     }
-    __shared__ double red[256];
     double cs = 0.0;
     for (int64_t j = j0 + threadIdx.x; j < j1; j += 256) cs += b.c[base + j];
     red[threadIdx.x] = cs;
@@ -1358,6 +1439,19 @@ void poll_states(dvm_batch_t B)
     DVM_HIP_CHECK(hipStreamSynchronize(B->ctx->stream));
 }
 
+// The remeshed blobs of a slot (written to the alternate buffers) back into
+// its storage, and its new count.
+__global__ void bt_remesh_commit_kernel(int64_t m, BlobBuf from, BlobBuf to, SlotState* st)
+{
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        to.x[j] = from.x[j]; to.z[j] = from.z[j]; to.c[j] = from.c[j]; to.s[j] = from.s[j];
+        to.u[j] = from.u[j]; to.w[j] = from.w[j]; to.uvs[j] = from.uvs[j]; to.wvs[j] = from.wvs[j];
+        to.id[j] = from.id[j];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->n = m;
+}
+
 // Remesh of the slots whose next step is a remesh step (BackgroundMesh
 // specification of remesh.hip), one slot at a time between the two graphs.
 void remesh_due(dvm_batch_t B)
@@ -1392,14 +1486,9 @@ void remesh_due(dvm_batch_t B)
             continue;
         }
         remesh_finish(B->ctx, rp, in, plan, out);
-        const size_t b = sizeof(double) * m;
-        const double* fs[8] = {out.x, out.z, out.c, out.s, out.u, out.w, out.uvs, out.wvs};
-        double* ts[8] = {in.x, in.z, in.c, in.s, in.u, in.w, in.uvs, in.wvs};
-        for (int k = 0; k < 8; ++k)
-            if (m) DVM_HIP_CHECK(hipMemcpyAsync(ts[k], fs[k], b, hipMemcpyDeviceToDevice, st));
-        if (m) DVM_HIP_CHECK(hipMemcpyAsync(in.id, out.id, sizeof(uint32_t) * m, hipMemcpyDeviceToDevice, st));
-        DVM_HIP_CHECK(hipMemcpyAsync(&B->d.st[s].n, &m, sizeof(int64_t), hipMemcpyHostToDevice, st));
-        DVM_HIP_CHECK(hipStreamSynchronize(st));
+        bt_remesh_commit_kernel<<<m > 0 ? (unsigned)std::min<int64_t>(cdiv(m, 256), 1024) : 1, 256, 0,
+                                  st>>>(m, out, in, B->d.st + s);
+        DVM_LAUNCH_CHECK();
     }
 }
 

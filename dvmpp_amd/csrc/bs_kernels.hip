@@ -67,18 +67,6 @@ __global__ void src_rec64_kernel(int64_t n, const double* __restrict__ x,
     rec[j] = make_double4(x[j], z[j], c[j], 10.0 * (sj * sj));
 }
 
-// 1/a from the hardware reciprocal and two Newton steps on the FMA: below one
-// unit in the last place (the correctly rounded quotient the reference's
-// divide gives can differ in that last bit).  a = 0 gives NaN, as 0/0 does.
-__device__ __forceinline__ double recip64(double a)
-{
-    double y = __builtin_amdgcn_rcp(a);
-    double e = __builtin_fma(-a, y, 1.0);
-    y = __builtin_fma(y, e, y);
-    e = __builtin_fma(-a, y, 1.0);
-    return __builtin_fma(y, e, y);
-}
-
 struct Rec64x4 {
     double4 r[4];
 };

@@ -226,6 +226,18 @@ __device__ __forceinline__ void group_moments(int g, int k, int nl, const double
     ga[2 * (g * kSheetTerms + k) + 1] = ai;
 }
 
+// 1/a from the hardware reciprocal and two Newton steps on the FMA: below one
+// unit in the last place (the correctly rounded quotient the reference's
+// divide gives can differ in that last bit).  a = 0 gives NaN, as 0/0 does.
+__device__ __forceinline__ double recip64(double a)
+{
+    double y = __builtin_amdgcn_rcp(a);
+    double e = __builtin_fma(-a, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-a, y, 1.0);
+    return __builtin_fma(y, e, y);
+}
+
 // One pair of the reference kernel (src/VelocityKernel.cpp:44-60), FP64.
 __device__ __forceinline__ void bs_pair_f64(double xi, double zi, double xj, double zj, double cj,
                                             double sj, double& ui, double& wi)

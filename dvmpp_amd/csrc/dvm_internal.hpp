@@ -283,6 +283,7 @@ struct BlobBuf {
 // synchronizes); phase 2 writes the output into `out` (capacity >= count).
 struct RemeshPlan {
     int64_t n_in = 0, nx = 0, nz = 0, n_nodes_out = 0, n_kept = 0;
+    int64_t win_i0 = 0, win_k0 = 0, win_w = 0, win_h = 0;  // node window worked on
     char* mem = nullptr;  // grid values, flags, scans: the context's work arena
 };
 int64_t remesh_count(dvm_ctx_t ctx, const dvm_remesh_params& rp, int64_t n, const BlobBuf& in,

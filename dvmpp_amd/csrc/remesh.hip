@@ -13,10 +13,14 @@
 //   output = nodes with |circ| > circ_threshold in linear order k nx + i with
 //   sigma = remesh sigma, then kept blobs in index order; IDs 0..M-1.
 //
-// Device plan: per-blob cell keys, a stable radix sort of (cell, index) —
-// which yields every cell's blobs in index order — per-cell ranges, then one
-// thread per node gathers its stencil cells (no atomics: the sum order is
-// fixed, so values are bit-identical run to run and to the CPU oracle).
+// Device plan: per-blob cell keys and the rectangle of the remeshed blobs'
+// cells (read back once: it sets the window of nodes and cells worked on, so
+// the cost follows the wake's extent, not the mesh's), a stable radix sort of
+// (cell, index) — which yields every cell's blobs in index order — the cells'
+// start positions, per-blob weight records, then one lane per window node
+// gathers its stencil cells from records staged in LDS (no atomics: the sum
+// order is fixed, so values are bit-identical run to run and to the CPU
+// oracle); nodes outside the window hold no circulation.
 #include "dvm_internal.hpp"
 
 #include <hipcub/hipcub.hpp>
@@ -65,6 +69,19 @@ struct MeshGeo {
     double xmin, zmin, dx, dz;
     int64_t nx, nz;
     int s, kernel;
+};
+
+constexpr int kMaxS = 3;  // stencil half-width: 2 (M4') or 3
+
+// A remeshed blob in (cell, index) order: circ * (its weight on each of the
+// 2s stencil columns of its cell), its weights on the 2s stencil rows
+// (remesh_slot_w, evaluated once per blob instead of once per node; the
+// product circ * wx rounds as in the node sum) and its cell column.
+struct RemeshRec {
+    double cw[2 * kMaxS];
+    double wz[2 * kMaxS];
+    int64_t cx;
+    double pad[3];
 };
 
 // Per blob: its cell key (kKept for a blob whose stencil leaves the mesh)
@@ -153,80 +170,139 @@ __global__ __launch_bounds__(256) void remesh_rect_kernel(int nb, const int* __r
     }
 }
 
-// Sorted copies of the remeshed blobs' fractional cell coordinates and
-// circulation (position q of the (cell, index) order), so the node gather
-// reads them contiguously instead of through the permutation.
+// Records of the remeshed blobs in (cell, index) order (position q), so the
+// node gather reads them contiguously instead of through the permutation.
 __global__ void remesh_sorted_kernel(int64_t n, MeshGeo g, const uint32_t* __restrict__ sidx,
+                                     const uint32_t* __restrict__ key,
                                      const double* __restrict__ x, const double* __restrict__ z,
-                                     const double* __restrict__ c, double* __restrict__ sfx,
-                                     double* __restrict__ sfz, double* __restrict__ sc)
+                                     const double* __restrict__ c, RemeshRec* __restrict__ rec)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
+    if (q >= n || key[q] == kKept) return;
     const uint32_t b = sidx[q];
-    sfx[q] = (x[b] - g.xmin) / g.dx;
-    sfz[q] = (z[b] - g.zmin) / g.dz;
-    sc[q] = c[b];
+    const double fx = (x[b] - g.xmin) / g.dx;
+    const double fz = (z[b] - g.zmin) / g.dz;
+    const int64_t bx = (int64_t)floor(fx) - g.s + 1;
+    const int64_t bz = (int64_t)floor(fz) - g.s + 1;
+    const double cb = c[b];
+    RemeshRec r;
+#pragma unroll
+    for (int t = 0; t < 2 * kMaxS; ++t) {
+        r.cw[t] = t < 2 * g.s ? cb * remesh_slot_w(g.kernel, g.s, fx, bx, t) : 0.0;
+        r.wz[t] = t < 2 * g.s ? remesh_slot_w(g.kernel, g.s, fz, bz, t) : 0.0;
+    }
+    r.cx = bx + g.s - 1;
+    r.pad[0] = r.pad[1] = r.pad[2] = 0.0;
+    rec[q] = r;
 }
 
-__global__ void remesh_range_kernel(int64_t n, const uint32_t* __restrict__ key,
-                                    int* __restrict__ start, int* __restrict__ end)
+// cstart[c] = the first position of the sorted keys holding a key >= c, for
+// the window cells c = 0 .. wcells (kKept sorts after every cell): cell c's
+// blobs are positions cstart[c] .. cstart[c + 1] - 1, and consecutive cells
+// of a row hold consecutive position ranges.
+__global__ void remesh_cstart_kernel(int64_t n, int64_t wcells, const uint32_t* __restrict__ key,
+                                     int* __restrict__ cstart)
 {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const uint32_t k = key[p];
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > wcells) return;
+    int64_t lo = 0, hi = n;  // first p with key[p] >= c
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)key[mid] < c) lo = mid + 1; else hi = mid;
+    }
+    cstart[c] = (int)lo;
+}
+
+// The part of the mesh a remesh works on: the nodes [i0, i0 + W) x [k0, k0 + H)
+// and the cells [cx0, cx0 + CW) x [cz0, cz0 + CH) that hold remeshed blobs;
+// cell arrays are indexed (cz - cz0) * CW + (cx - cx0).  Nodes outside the
+// window have no remeshed blob in reach and take the value 0.
+struct MeshWindow {
+    int64_t i0, k0, W, H;
+    int64_t cx0, cz0, CW, CH;
+};
+
+// Global cell key -> window cell key (kKept stays).
+__global__ void remesh_rekey_kernel(int64_t n, MeshGeo g, MeshWindow w,
+                                    uint32_t* __restrict__ key)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const uint32_t k = key[b];
     if (k == kKept) return;
-    if (p == 0 || key[p - 1] != k) start[k] = (int)p;
-    if (p == n - 1 || key[p + 1] != k) end[k] = (int)p + 1;
+    const int64_t cz = (int64_t)k / g.nx;
+    const int64_t cx = (int64_t)k - cz * g.nx;
+    key[b] = (uint32_t)((cz - w.cz0) * w.CW + (cx - w.cx0));
 }
 
-// One thread per node: the sum over its stencil cells (z-major, ascending)
-// of each cell's blobs in index order.  Nodes outside the rectangle the
-// remeshed blobs' stencils reach (rect, widened by the stencil) have no
-// blobs in reach and take the value 0 without visiting their cells.
-__global__ void remesh_gather_kernel(MeshGeo g, const int* __restrict__ rect,
-                                     const int* __restrict__ start, const int* __restrict__ end,
-                                     const double* __restrict__ sfx,
-                                     const double* __restrict__ sfz,
-                                     const double* __restrict__ sc, double threshold,
-                                     double* __restrict__ G, int* __restrict__ flag)
+// A workgroup per 256 consecutive window nodes of one row; one lane per node
+// sums (circ * wx) * wz over its stencil cells (z-major, ascending) and each
+// cell's blobs in index order.  A node's cells in one stencil row hold one
+// contiguous range of the sorted records, and the workgroup's ranges in that
+// row one contiguous range, staged through LDS 256 records at a time; each
+// lane reads the part of every staged piece that lies in its own range.
+constexpr int kGatherLanes = 256;
+
+__global__ __launch_bounds__(kGatherLanes) void remesh_gather_kernel(
+    MeshGeo g, MeshWindow w, const int* __restrict__ cstart, const RemeshRec* __restrict__ rec,
+    double threshold, double* __restrict__ G, int* __restrict__ flag)
 {
-    const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (node >= g.nx * g.nz) return;
-    const int64_t k = node / g.nx;
-    const int64_t i = node - k * g.nx;
+    __shared__ RemeshRec st[kGatherLanes];
+    const int64_t a = (int64_t)blockIdx.x * kGatherLanes + threadIdx.x;  // node column in the window
+    const bool valid = a < w.W;
+    const int64_t i = w.i0 + a;
     const int s = g.s;
-    double acc = 0.0;
-    if (i >= (int64_t)rect[0] - s + 1 && i <= (int64_t)rect[1] + s &&
-        k >= (int64_t)rect[2] - s + 1 && k <= (int64_t)rect[3] + s) {
+    // the workgroup's node columns and the window's last cell column
+    const int64_t ifirst = w.i0 + (int64_t)blockIdx.x * kGatherLanes;
+    const int64_t ilast = ifirst + kGatherLanes - 1 < w.i0 + w.W - 1 ? ifirst + kGatherLanes - 1
+                                                                   : w.i0 + w.W - 1;
+    const int64_t cxe = w.cx0 + w.CW - 1;
+    for (int64_t kk = blockIdx.y; kk < w.H; kk += gridDim.y) {  // workgroup-uniform
+        const int64_t k = w.k0 + kk;
+        double acc = 0.0;
         for (int64_t cz = k - s; cz <= k + s - 1; ++cz) {
-            if (cz < 0 || cz >= g.nz) continue;
-            for (int64_t cx = i - s; cx <= i + s - 1; ++cx) {
-                if (cx < 0 || cx >= g.nx) continue;
-                const int64_t cell = cz * g.nx + cx;
-                const int q1 = end[cell];
-                for (int q = start[cell]; q < q1; ++q) {
-                    const double wx = remesh_slot_w(g.kernel, s, sfx[q], cx - s + 1,
-                                                    (int)(i - (cx - s + 1)));
-                    const double wz = remesh_slot_w(g.kernel, s, sfz[q], cz - s + 1,
-                                                    (int)(k - (cz - s + 1)));
-                    acc += (sc[q] * wx) * wz;
+            if (cz < w.cz0 || cz >= w.cz0 + w.CH) continue;
+            const int64_t rb = (cz - w.cz0) * w.CW - w.cx0;  // cell index = rb + cx
+            const int64_t ca = ifirst - s > w.cx0 ? ifirst - s : w.cx0;
+            const int64_t cb = ilast + s - 1 < cxe ? ilast + s - 1 : cxe;
+            if (ca > cb) continue;
+            const int A = cstart[rb + ca], B = cstart[rb + cb + 1];
+            const int64_t ta = i - s > w.cx0 ? i - s : w.cx0;
+            const int64_t tb = i + s - 1 < cxe ? i + s - 1 : cxe;
+            const bool mine = valid && ta <= tb;
+            const int qa = mine ? cstart[rb + ta] : A, qb = mine ? cstart[rb + tb + 1] : A;
+            const int u = (int)(k - (cz - s + 1));  // this node's row slot in the cells' stencils
+            for (int q0 = A; q0 < B; q0 += kGatherLanes) {
+                __syncthreads();
+                if (q0 + (int)threadIdx.x < B) st[threadIdx.x] = rec[q0 + threadIdx.x];
+                __syncthreads();
+                const int lo = qa > q0 ? qa : q0;
+                const int hi = qb < q0 + kGatherLanes ? qb : q0 + kGatherLanes;
+                for (int q = lo; q < hi; ++q) {
+                    const RemeshRec& r = st[q - q0];
+                    const int v = (int)(i - (r.cx - s + 1));  // column slot
+                    acc += r.cw[v] * r.wz[u];
                 }
             }
         }
+        if (valid) {
+            const int64_t node = kk * w.W + a;
+            G[node] = acc;
+            flag[node] = fabs(acc) > threshold ? 1 : 0;
+        }
     }
-    G[node] = acc;
-    flag[node] = fabs(acc) > threshold ? 1 : 0;
 }
 
-__global__ void remesh_emit_nodes_kernel(MeshGeo g, double sigma, const double* __restrict__ G,
+__global__ void remesh_emit_nodes_kernel(MeshGeo g, MeshWindow w, double sigma,
+                                         const double* __restrict__ G,
                                          const int* __restrict__ flag, const int* __restrict__ pos,
                                          BlobBuf out)
 {
     const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (node >= g.nx * g.nz || !flag[node]) return;
-    const int64_t k = node / g.nx;
-    const int64_t i = node - k * g.nx;
+    if (node >= w.W * w.H || !flag[node]) return;
+    const int64_t kk = node / w.W;
+    const int64_t i = w.i0 + (node - kk * w.W);
+    const int64_t k = w.k0 + kk;
     const int64_t o = pos[node];
     out.x[o] = g.xmin + (double)i * g.dx;
     out.z[o] = g.zmin + (double)k * g.dz;
@@ -248,6 +324,12 @@ __global__ void remesh_emit_kept_kernel(int64_t n_kept, int64_t first, int64_t b
         out.u[o] = in.u[b]; out.w[o] = in.w[b]; out.uvs[o] = in.uvs[b]; out.wvs[o] = in.wvs[b];
     }
     if (out.id) out.id[o] = (uint32_t)o;
+}
+
+__global__ void remesh_iota_kernel(int64_t n, uint32_t* __restrict__ sidx)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < n) sidx[b] = (uint32_t)b;
 }
 
 __global__ void remesh_tail_kernel(int64_t cells, const int* __restrict__ pos,
@@ -272,8 +354,9 @@ static MeshGeo mesh_geo(const dvm_remesh_params& rp)
 // Plan memory layout (offsets recomputed from the sizes on both phases).
 struct PlanPtrs {
     uint32_t *key, *key2, *idx, *sidx;
-    int *inside, *start, *end, *flag, *pos, *nin, *rect, *rpart;
-    double *G, *sfx, *sfz, *sc;
+    int *inside, *start, *flag, *pos, *nin, *rect, *rpart;
+    double* G;
+    RemeshRec* rec;  // remeshed blobs in (cell, index) order
     void* tmp;
     size_t tmp_bytes;
 };
@@ -287,17 +370,14 @@ static PlanPtrs plan_ptrs(char* mem, int64_t n, int64_t cells, size_t tmp_bytes)
     p.idx = cv.take<uint32_t>(n > 0 ? n : 1);
     p.sidx = cv.take<uint32_t>(n > 0 ? n : 1);
     p.inside = cv.take<int>(n > 0 ? n : 1);
-    p.start = cv.take<int>(cells);
-    p.end = cv.take<int>(cells);
+    p.start = cv.take<int>(cells + 1);  // cell starts of the window (cstart)
     p.flag = cv.take<int>(cells);
     p.pos = cv.take<int>(cells);
     p.nin = cv.take<int>(4);
     p.rect = cv.take<int>(4);
     p.rpart = cv.take<int>(4 * cdiv(n > 0 ? n : 1, 256));
     p.G = cv.take<double>(cells);
-    p.sfx = cv.take<double>(n > 0 ? n : 1);
-    p.sfz = cv.take<double>(n > 0 ? n : 1);
-    p.sc = cv.take<double>(n > 0 ? n : 1);
+    p.rec = cv.take<RemeshRec>(n > 0 ? n : 1);
     p.tmp = cv.take<char>(tmp_bytes);
     p.tmp_bytes = tmp_bytes;
     return p;
@@ -329,7 +409,7 @@ int64_t remesh_count(dvm_ctx_t ctx, const dvm_remesh_params& rp, int64_t n, cons
     const size_t bytes = Carver::need({4 * sizeof(uint32_t) * nn, sizeof(int) * nn,
                                        4 * sizeof(int) * cells, sizeof(int) * 4, sizeof(int) * 4,
                                        4 * sizeof(int) * (size_t)cdiv(nn, 256),
-                                       sizeof(double) * cells, 3 * sizeof(double) * nn, tmp}) +
+                                       sizeof(double) * cells, sizeof(RemeshRec) * nn, tmp}) +
                          20 * 256;
     // the context's work arena: nothing else takes it between this count and
     // remesh_finish, and it persists, so repeated remeshes allocate nothing
@@ -339,8 +419,6 @@ int64_t remesh_count(dvm_ctx_t ctx, const dvm_remesh_params& rp, int64_t n, cons
     plan.nx = g.nx;
     plan.nz = g.nz;
     ProfScope ps(ctx, "remesh");
-    DVM_HIP_CHECK(hipMemsetAsync(p.start, 0, sizeof(int) * cells, st));
-    DVM_HIP_CHECK(hipMemsetAsync(p.end, 0, sizeof(int) * cells, st));
     DVM_HIP_CHECK(hipMemsetAsync(p.nin, 0, sizeof(int) * 4, st));
     const int nb = (int)cdiv(n > 0 ? n : 1, 256);
     if (n > 0) {
@@ -349,28 +427,57 @@ int64_t remesh_count(dvm_ctx_t ctx, const dvm_remesh_params& rp, int64_t n, cons
     }
     remesh_rect_kernel<<<1, 256, 0, st>>>(n > 0 ? nb : 0, p.rpart, p.rect);
     DVM_LAUNCH_CHECK();
-    if (n > 0) {
+    // the rectangle of the remeshed blobs' cells sizes the window
+    int* hv = static_cast<int*>(pinned_get(ctx, sizeof(int) * 8));
+    DVM_HIP_CHECK(hipMemcpyAsync(hv, p.rect, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
+    DVM_HIP_CHECK(hipStreamSynchronize(st));
+    MeshWindow w{};
+    const bool any = hv[1] >= hv[0];
+    if (any) {
+        w.cx0 = hv[0]; w.CW = (int64_t)hv[1] - hv[0] + 1;
+        w.cz0 = hv[2]; w.CH = (int64_t)hv[3] - hv[2] + 1;
+    }
+    if (rp.circ_threshold < 0.0) {  // every node is output: the whole mesh
+        w.i0 = 0; w.W = g.nx;
+        w.k0 = 0; w.H = g.nz;
+    } else if (any) {  // the nodes the remeshed blobs' stencils reach
+        w.i0 = w.cx0 - g.s + 1; w.W = w.CW + 2 * g.s - 1;
+        w.k0 = w.cz0 - g.s + 1; w.H = w.CH + 2 * g.s - 1;
+    }
+    const int64_t wcells = w.CW * w.CH, wnodes = w.W * w.H;
+    plan.win_i0 = w.i0; plan.win_k0 = w.k0; plan.win_w = w.W; plan.win_h = w.H;
+    if (wcells > 0) {
+        remesh_rekey_kernel<<<nb, 256, 0, st>>>(n, g, w, p.key);
+        DVM_LAUNCH_CHECK();
+        // window keys lie below 2^bits - 1, so kKept (all bits set) sorts last
+        int bits = 1;
+        while ((int64_t(1) << bits) <= wcells) ++bits;
         size_t t = p.tmp_bytes;
         DVM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(p.tmp, t, p.key, p.key2, p.idx, p.sidx,
-                                                         (int)n, 0, 32, st));
-        remesh_range_kernel<<<cdiv(n, 256), 256, 0, st>>>(n, p.key2, p.start, p.end);
+                                                         (int)n, 0, bits, st));
+        remesh_cstart_kernel<<<cdiv(wcells + 1, 256), 256, 0, st>>>(n, wcells, p.key2, p.start);
         DVM_LAUNCH_CHECK();
-        remesh_sorted_kernel<<<cdiv(n, 256), 256, 0, st>>>(n, g, p.sidx, in.x, in.z, in.c, p.sfx,
-                                                           p.sfz, p.sc);
+        remesh_sorted_kernel<<<cdiv(n, 256), 256, 0, st>>>(n, g, p.sidx, p.key2, in.x, in.z, in.c,
+                                                           p.rec);
         DVM_LAUNCH_CHECK();
         t = p.tmp_bytes;
         DVM_HIP_CHECK(hipcub::DeviceReduce::Sum(p.tmp, t, p.inside, p.nin, (int)n, st));
+    } else if (n > 0) {
+        // nothing remeshed: every blob is kept, in index order
+        remesh_iota_kernel<<<nb, 256, 0, st>>>(n, p.sidx);
+        DVM_LAUNCH_CHECK();
     }
-    remesh_gather_kernel<<<cdiv(cells, 256), 256, 0, st>>>(g, p.rect, p.start, p.end, p.sfx,
-                                                           p.sfz, p.sc, rp.circ_threshold, p.G,
-                                                           p.flag);
-    DVM_LAUNCH_CHECK();
-    size_t t = p.tmp_bytes;
-    DVM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(p.tmp, t, p.flag, p.pos, (int)cells, st));
-    // one read-back: {remeshed blobs, last node's output offset, its flag}
-    remesh_tail_kernel<<<1, 64, 0, st>>>(cells, p.pos, p.flag, p.nin);
-    DVM_LAUNCH_CHECK();
-    int* hv = static_cast<int*>(pinned_get(ctx, sizeof(int) * 4));
+    if (wnodes > 0) {
+        remesh_gather_kernel<<<dim3(cdiv(w.W, kGatherLanes), (unsigned)std::min<int64_t>(w.H, 65535)),
+                               kGatherLanes, 0, st>>>(
+            g, w, p.start, p.rec, rp.circ_threshold, p.G, p.flag);
+        DVM_LAUNCH_CHECK();
+        size_t t = p.tmp_bytes;
+        DVM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(p.tmp, t, p.flag, p.pos, (int)wnodes, st));
+        // {remeshed blobs, last node's output offset, its flag}
+        remesh_tail_kernel<<<1, 64, 0, st>>>(wnodes, p.pos, p.flag, p.nin);
+        DVM_LAUNCH_CHECK();
+    }
     DVM_HIP_CHECK(hipMemcpyAsync(hv, p.nin, sizeof(int) * 3, hipMemcpyDeviceToHost, st));
     DVM_HIP_CHECK(hipStreamSynchronize(st));
     plan.n_nodes_out = (int64_t)hv[1] + hv[2];
@@ -387,8 +494,13 @@ void remesh_finish(dvm_ctx_t ctx, const dvm_remesh_params& rp, const BlobBuf& in
     const int64_t n = plan.n_in;
     const size_t tmp = plan_tmp_bytes(n, cells, st);
     PlanPtrs p = plan_ptrs(plan.mem, n, cells, tmp);
-    remesh_emit_nodes_kernel<<<cdiv(cells, 256), 256, 0, st>>>(g, rp.sigma, p.G, p.flag, p.pos, out);
-    DVM_LAUNCH_CHECK();
+    MeshWindow w{};
+    w.i0 = plan.win_i0; w.k0 = plan.win_k0; w.W = plan.win_w; w.H = plan.win_h;
+    if (w.W * w.H > 0) {
+        remesh_emit_nodes_kernel<<<cdiv(w.W * w.H, 256), 256, 0, st>>>(g, w, rp.sigma, p.G, p.flag,
+                                                                       p.pos, out);
+        DVM_LAUNCH_CHECK();
+    }
     if (plan.n_kept > 0) {
         remesh_emit_kept_kernel<<<cdiv(plan.n_kept, 256), 256, 0, st>>>(
             plan.n_kept, n - plan.n_kept, plan.n_nodes_out, p.sidx, in, out);

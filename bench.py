@@ -465,14 +465,15 @@ def rk3_plateau_phase(ctx, steps, prec=None, fmm=None):
             "remesh_ms": tr * 1e3, "remesh_blobs_out": int(out[0].numel())}
 
 
-def graph_sim_phase(ctx):
+def graph_sim_phase(ctx, window=40):
     """Single simulations in the chosen form with graph-replayed steps: each
     runs as a one-slot batch (FP32C sums, device random stream, counts kept on
     the device, each step two replayed hipGraphs with the RK3 stages inside
     the first).  (1) The RK3 cylinder at its plateau: the 1.49M-blob
     remeshed wake loaded into the slot, one step untimed, then two timed.
     (2) The RK3 cylinder run from rest to step 4000 (M4' remesh every 20
-    steps).  (3) The plain cylinder's 400 Euler steps from rest."""
+    steps).  (3) The plain cylinder's 400 Euler steps from rest, its first
+    `window` steps also timed alone."""
     import torch
 
     import dvmpp_amd as dvm
@@ -531,12 +532,16 @@ def graph_sim_phase(ctx):
     b, slot = one_slot(dsim.EULER, 1 << 18, dsim.remesh_params(), (cpx, cpz), csp, 400)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    b.step(400)
+    b.step(window)
+    torch.cuda.synchronize()
+    tw = time.perf_counter() - t0
+    b.step(400 - window)
     torch.cuda.synchronize()
     tc = time.perf_counter() - t0
     info = b.info(slot)
     b.close()
     out["plain_cylinder_400"] = {"seconds": tc, "steps_per_s": 400 / tc,
+                                 "window_steps_per_s": window / tw,
                                  "final_blobs": int(info["n"]), "cd": info["loads"][2]}
     log(f"graph plain cylinder: 400 steps in {tc:.2f} s, N = {info['n']}")
     out["workload"] = ("one-slot batches, FP32C sums, device random stream, graph-replayed "
@@ -896,7 +901,7 @@ def main():
     if "rk3run" in phases:
         line["rk3_cylinder_run_fmm"] = rk3_run_phase(ctx, args.rk3_run_steps)
     if "graph" in phases:
-        line["graph_single_simulation"] = graph_sim_phase(ctx)
+        line["graph_single_simulation"] = graph_sim_phase(ctx, args.cylinder_cpu_steps)
     if "rk3" in phases:
         line["rk3_cylinder_plateau"] = rk3_plateau_phase(ctx, args.rk3_steps)
         line["rk3_cylinder_plateau_fmm"] = rk3_plateau_phase(ctx, args.rk3_fmm_steps, dvm.FMM,

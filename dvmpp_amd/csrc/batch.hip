@@ -30,10 +30,11 @@ constexpr int kTB = 64;         // targets per FP32C item (one wave)
 constexpr int kMaxChunks = 64;  // most source chunks per slot in the FP32C sum
 constexpr int kBaseChunks = 4;  // chunks of every slot large enough for them
 constexpr int kSlotItems = 65536;  // FP32C items a slot is cut into, at least
-constexpr int kMinChunkTiles = 8;  // a chunk spans at least this many tiles
+constexpr int kMinChunkTiles = 2;  // a chunk spans at least this many tiles
 constexpr int kSortBits = 24;   // Morton key bits sorted (12 per axis)
 constexpr int kSortTile = 4096; // keys per sort workgroup (256 lanes x 16)
 constexpr int kBoxParts = 32;   // workgroups per slot of the bounding-box pass
+constexpr int kXcds = 8;        // accelerator dies (XCDs) of the device
 constexpr int kAdmitStreams = 16;  // bodies set up at once by dvm_batch_admit_many
 
 struct SlotParam {
@@ -64,6 +65,9 @@ struct WorkList {
     int64_t total_blobs;
     int total_items;  // FP32C work items (slot, chunk, 64-target block)
     int total_stiles; // sort tiles (kSortTile keys of one slot)
+    // FP32C item counters, one per XCD (128 bytes apart): the waves of XCD x
+    // take items x, x + 8, x + 16, ...
+    int xcd_counter[kXcds * 32];
 };
 
 struct BatchDev {
@@ -233,6 +237,7 @@ __global__ __launch_bounds__(1024) void bt_prefix_kernel(int S, int64_t cap,
         item_pre[S] = (int)tot_items;
         wl->total_tiles = (int)tot_tiles;
         wl->work_counter = 0;
+        for (int x = 0; x < kXcds; ++x) wl->xcd_counter[x * 32] = 0;
         wl->total_blobs = tot_blobs;
         wl->total_stiles = (int)tot_stiles;
         wl->total_items = (int)tot_items;
@@ -575,12 +580,17 @@ __global__ __launch_bounds__(256) void bt_bs32_kernel(int S, int64_t cap, int ma
 {
     const int lane = threadIdx.x & 63;
     const int total = wl->total_items;
+    // workgroups are dealt to the XCDs in turn: this one's XCD takes every
+    // kXcds-th item, so the XCDs share no counter and stay in the same chunk
+    const int xcd = blockIdx.x % kXcds;
+    int* ctr = &wl->xcd_counter[xcd * 32];
     for (;;) {
-        int item = 0;
-        if (lane == 0) item = atomicAdd(&wl->work_counter, 1);
+        int k = 0;
+        if (lane == 0) k = atomicAdd(ctr, 1);
         // wave-uniform (an SGPR), so the item's slot, tiles and source
         // records are scalar loads, as in the single-simulation kernel
-        item = __builtin_amdgcn_readfirstlane(item);
+        k = __builtin_amdgcn_readfirstlane(k);
+        const int item = xcd + kXcds * k;
         if (item >= total) break;
         const int p = find_slot(item_pre, S, item);
         const int s = order[p];

@@ -73,15 +73,15 @@ struct MeshGeo {
 
 constexpr int kMaxS = 3;  // stencil half-width: 2 (M4') or 3
 
-// A remeshed blob in (cell, index) order: circ * (its weight on each of the
-// 2s stencil columns of its cell), its weights on the 2s stencil rows
-// (remesh_slot_w, evaluated once per blob instead of once per node; the
-// product circ * wx rounds as in the node sum) and its cell column.
+// A remeshed blob in (cell, index) order, for stencil half-width S: circ *
+// (its weight on each of the 2S stencil columns of its cell) and its weights
+// on the 2S stencil rows (remesh_slot_w, evaluated once per blob instead of
+// once per node; the product circ * wx rounds as in the node sum).  Its cell
+// column is kept beside it (rcx).
+template <int S>
 struct RemeshRec {
-    double cw[2 * kMaxS];
-    double wz[2 * kMaxS];
-    int64_t cx;
-    double pad[3];
+    double cw[2 * S];
+    double wz[2 * S];
 };
 
 // Per blob: its cell key (kKept for a blob whose stencil leaves the mesh)
@@ -172,28 +172,30 @@ __global__ __launch_bounds__(256) void remesh_rect_kernel(int nb, const int* __r
 
 // Records of the remeshed blobs in (cell, index) order (position q), so the
 // node gather reads them contiguously instead of through the permutation.
+template <int S>
 __global__ void remesh_sorted_kernel(int64_t n, MeshGeo g, const uint32_t* __restrict__ sidx,
                                      const uint32_t* __restrict__ key,
                                      const double* __restrict__ x, const double* __restrict__ z,
-                                     const double* __restrict__ c, RemeshRec* __restrict__ rec)
+                                     const double* __restrict__ c, RemeshRec<S>* __restrict__ rec,
+                                     int* __restrict__ rcx)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n || key[q] == kKept) return;
     const uint32_t b = sidx[q];
     const double fx = (x[b] - g.xmin) / g.dx;
     const double fz = (z[b] - g.zmin) / g.dz;
-    const int64_t bx = (int64_t)floor(fx) - g.s + 1;
-    const int64_t bz = (int64_t)floor(fz) - g.s + 1;
+    const int64_t cx = (int64_t)floor(fx);
+    const int64_t bx = cx - S + 1;
+    const int64_t bz = (int64_t)floor(fz) - S + 1;
     const double cb = c[b];
-    RemeshRec r;
+    RemeshRec<S> r;
 #pragma unroll
-    for (int t = 0; t < 2 * kMaxS; ++t) {
-        r.cw[t] = t < 2 * g.s ? cb * remesh_slot_w(g.kernel, g.s, fx, bx, t) : 0.0;
-        r.wz[t] = t < 2 * g.s ? remesh_slot_w(g.kernel, g.s, fz, bz, t) : 0.0;
+    for (int t = 0; t < 2 * S; ++t) {
+        r.cw[t] = cb * remesh_slot_w(g.kernel, S, fx, bx, t);
+        r.wz[t] = remesh_slot_w(g.kernel, S, fz, bz, t);
     }
-    r.cx = bx + g.s - 1;
-    r.pad[0] = r.pad[1] = r.pad[2] = 0.0;
     rec[q] = r;
+    rcx[q] = (int)cx;
 }
 
 // cstart[c] = the first position of the sorted keys holding a key >= c, for
@@ -235,53 +237,64 @@ __global__ void remesh_rekey_kernel(int64_t n, MeshGeo g, MeshWindow w,
     key[b] = (uint32_t)((cz - w.cz0) * w.CW + (cx - w.cx0));
 }
 
-// A workgroup per 256 consecutive window nodes of one row; one lane per node
-// sums (circ * wx) * wz over its stencil cells (z-major, ascending) and each
-// cell's blobs in index order.  A node's cells in one stencil row hold one
-// contiguous range of the sorted records, and the workgroup's ranges in that
-// row one contiguous range, staged through LDS 256 records at a time; each
-// lane reads the part of every staged piece that lies in its own range.
-constexpr int kGatherLanes = 256;
+// A workgroup per 64 x 4 window nodes (64 columns of 4 rows); one lane per
+// node sums (circ * wx) * wz over its stencil cells (z-major, ascending) and
+// each cell's blobs in index order.  A node's cells in one stencil row hold
+// one contiguous range of the sorted records, and the workgroup's ranges in
+// that row one contiguous range: the workgroup walks the cell rows its nodes
+// reach in ascending order, stages each row's range through LDS 256 records
+// at a time, and each lane reads the part of every staged piece that lies in
+// its own range (a record is staged about twice, not once per node row).
+constexpr int kGatherCols = 64;
+constexpr int kGatherRows = 4;
 
-__global__ __launch_bounds__(kGatherLanes) void remesh_gather_kernel(
-    MeshGeo g, MeshWindow w, const int* __restrict__ cstart, const RemeshRec* __restrict__ rec,
-    double threshold, double* __restrict__ G, int* __restrict__ flag)
+template <int S>
+__global__ __launch_bounds__(kGatherCols * kGatherRows) void remesh_gather_kernel(
+    MeshGeo g, MeshWindow w, const int* __restrict__ cstart, const RemeshRec<S>* __restrict__ rec,
+    const int* __restrict__ rcx, double threshold, double* __restrict__ G, int* __restrict__ flag)
 {
-    __shared__ RemeshRec st[kGatherLanes];
-    const int64_t a = (int64_t)blockIdx.x * kGatherLanes + threadIdx.x;  // node column in the window
-    const bool valid = a < w.W;
+    constexpr int kLanes = kGatherCols * kGatherRows;
+    __shared__ RemeshRec<S> st[kLanes];
+    __shared__ int stc[kLanes];
+    const int col = threadIdx.x % kGatherCols;
+    const int row = threadIdx.x / kGatherCols;
+    const int64_t a = (int64_t)blockIdx.x * kGatherCols + col;  // node column in the window
     const int64_t i = w.i0 + a;
-    const int s = g.s;
     // the workgroup's node columns and the window's last cell column
-    const int64_t ifirst = w.i0 + (int64_t)blockIdx.x * kGatherLanes;
-    const int64_t ilast = ifirst + kGatherLanes - 1 < w.i0 + w.W - 1 ? ifirst + kGatherLanes - 1
-                                                                   : w.i0 + w.W - 1;
+    const int64_t ifirst = w.i0 + (int64_t)blockIdx.x * kGatherCols;
+    const int64_t ilast = ifirst + kGatherCols - 1 < w.i0 + w.W - 1 ? ifirst + kGatherCols - 1
+                                                                  : w.i0 + w.W - 1;
     const int64_t cxe = w.cx0 + w.CW - 1;
-    for (int64_t kk = blockIdx.y; kk < w.H; kk += gridDim.y) {  // workgroup-uniform
+    const int64_t ca = ifirst - S > w.cx0 ? ifirst - S : w.cx0;
+    const int64_t cb = ilast + S - 1 < cxe ? ilast + S - 1 : cxe;
+    for (int64_t kb = (int64_t)blockIdx.y * kGatherRows; kb < w.H;
+         kb += (int64_t)gridDim.y * kGatherRows) {  // workgroup-uniform
+        const int64_t kk = kb + row;
+        const bool valid = a < w.W && kk < w.H;
         const int64_t k = w.k0 + kk;
+        const int64_t kl = w.k0 + (kb + kGatherRows - 1 < w.H - 1 ? kb + kGatherRows - 1 : w.H - 1);
         double acc = 0.0;
-        for (int64_t cz = k - s; cz <= k + s - 1; ++cz) {
-            if (cz < w.cz0 || cz >= w.cz0 + w.CH) continue;
+        for (int64_t cz = w.k0 + kb - S; cz <= kl + S - 1; ++cz) {
+            if (cz < w.cz0 || cz >= w.cz0 + w.CH || ca > cb) continue;
             const int64_t rb = (cz - w.cz0) * w.CW - w.cx0;  // cell index = rb + cx
-            const int64_t ca = ifirst - s > w.cx0 ? ifirst - s : w.cx0;
-            const int64_t cb = ilast + s - 1 < cxe ? ilast + s - 1 : cxe;
-            if (ca > cb) continue;
             const int A = cstart[rb + ca], B = cstart[rb + cb + 1];
-            const int64_t ta = i - s > w.cx0 ? i - s : w.cx0;
-            const int64_t tb = i + s - 1 < cxe ? i + s - 1 : cxe;
-            const bool mine = valid && ta <= tb;
+            const int64_t ta = i - S > w.cx0 ? i - S : w.cx0;
+            const int64_t tb = i + S - 1 < cxe ? i + S - 1 : cxe;
+            const bool mine = valid && cz >= k - S && cz <= k + S - 1 && ta <= tb;
             const int qa = mine ? cstart[rb + ta] : A, qb = mine ? cstart[rb + tb + 1] : A;
-            const int u = (int)(k - (cz - s + 1));  // this node's row slot in the cells' stencils
-            for (int q0 = A; q0 < B; q0 += kGatherLanes) {
+            const int u = (int)(k - (cz - S + 1));  // this node's row slot in the cells' stencils
+            for (int q0 = A; q0 < B; q0 += kLanes) {
                 __syncthreads();
-                if (q0 + (int)threadIdx.x < B) st[threadIdx.x] = rec[q0 + threadIdx.x];
+                if (q0 + (int)threadIdx.x < B) {
+                    st[threadIdx.x] = rec[q0 + threadIdx.x];
+                    stc[threadIdx.x] = rcx[q0 + threadIdx.x];
+                }
                 __syncthreads();
                 const int lo = qa > q0 ? qa : q0;
-                const int hi = qb < q0 + kGatherLanes ? qb : q0 + kGatherLanes;
+                const int hi = qb < q0 + kLanes ? qb : q0 + kLanes;
                 for (int q = lo; q < hi; ++q) {
-                    const RemeshRec& r = st[q - q0];
-                    const int v = (int)(i - (r.cx - s + 1));  // column slot
-                    acc += r.cw[v] * r.wz[u];
+                    const int v = (int)(i - (stc[q - q0] - S + 1));  // column slot
+                    acc += st[q - q0].cw[v] * st[q - q0].wz[u];
                 }
             }
         }
@@ -356,7 +369,8 @@ struct PlanPtrs {
     uint32_t *key, *key2, *idx, *sidx;
     int *inside, *start, *flag, *pos, *nin, *rect, *rpart;
     double* G;
-    RemeshRec* rec;  // remeshed blobs in (cell, index) order
+    void* rec;  // RemeshRec<S> of the remeshed blobs in (cell, index) order
+    int* rcx;   // their cell columns
     void* tmp;
     size_t tmp_bytes;
 };
@@ -377,7 +391,8 @@ static PlanPtrs plan_ptrs(char* mem, int64_t n, int64_t cells, size_t tmp_bytes)
     p.rect = cv.take<int>(4);
     p.rpart = cv.take<int>(4 * cdiv(n > 0 ? n : 1, 256));
     p.G = cv.take<double>(cells);
-    p.rec = cv.take<RemeshRec>(n > 0 ? n : 1);
+    p.rec = cv.take<RemeshRec<kMaxS>>(n > 0 ? n : 1);
+    p.rcx = cv.take<int>(n > 0 ? n : 1);
     p.tmp = cv.take<char>(tmp_bytes);
     p.tmp_bytes = tmp_bytes;
     return p;
@@ -409,7 +424,8 @@ int64_t remesh_count(dvm_ctx_t ctx, const dvm_remesh_params& rp, int64_t n, cons
     const size_t bytes = Carver::need({4 * sizeof(uint32_t) * nn, sizeof(int) * nn,
                                        4 * sizeof(int) * cells, sizeof(int) * 4, sizeof(int) * 4,
                                        4 * sizeof(int) * (size_t)cdiv(nn, 256),
-                                       sizeof(double) * cells, sizeof(RemeshRec) * nn, tmp}) +
+                                       sizeof(double) * cells, sizeof(RemeshRec<kMaxS>) * nn,
+                                       sizeof(int) * nn, tmp}) +
                          20 * 256;
     // the context's work arena: nothing else takes it between this count and
     // remesh_finish, and it persists, so repeated remeshes allocate nothing
@@ -457,8 +473,12 @@ int64_t remesh_count(dvm_ctx_t ctx, const dvm_remesh_params& rp, int64_t n, cons
                                                          (int)n, 0, bits, st));
         remesh_cstart_kernel<<<cdiv(wcells + 1, 256), 256, 0, st>>>(n, wcells, p.key2, p.start);
         DVM_LAUNCH_CHECK();
-        remesh_sorted_kernel<<<cdiv(n, 256), 256, 0, st>>>(n, g, p.sidx, p.key2, in.x, in.z, in.c,
-                                                           p.rec);
+        if (g.s == 2)
+            remesh_sorted_kernel<2><<<cdiv(n, 256), 256, 0, st>>>(
+                n, g, p.sidx, p.key2, in.x, in.z, in.c, static_cast<RemeshRec<2>*>(p.rec), p.rcx);
+        else
+            remesh_sorted_kernel<3><<<cdiv(n, 256), 256, 0, st>>>(
+                n, g, p.sidx, p.key2, in.x, in.z, in.c, static_cast<RemeshRec<3>*>(p.rec), p.rcx);
         DVM_LAUNCH_CHECK();
         t = p.tmp_bytes;
         DVM_HIP_CHECK(hipcub::DeviceReduce::Sum(p.tmp, t, p.inside, p.nin, (int)n, st));
@@ -468,9 +488,16 @@ int64_t remesh_count(dvm_ctx_t ctx, const dvm_remesh_params& rp, int64_t n, cons
         DVM_LAUNCH_CHECK();
     }
     if (wnodes > 0) {
-        remesh_gather_kernel<<<dim3(cdiv(w.W, kGatherLanes), (unsigned)std::min<int64_t>(w.H, 65535)),
-                               kGatherLanes, 0, st>>>(
-            g, w, p.start, p.rec, rp.circ_threshold, p.G, p.flag);
+        const dim3 grid(cdiv(w.W, kGatherCols),
+                        (unsigned)std::min<int64_t>(cdiv(w.H, kGatherRows), 65535));
+        if (g.s == 2)
+            remesh_gather_kernel<2><<<grid, kGatherCols * kGatherRows, 0, st>>>(
+                g, w, p.start, static_cast<const RemeshRec<2>*>(p.rec), p.rcx, rp.circ_threshold,
+                p.G, p.flag);
+        else
+            remesh_gather_kernel<3><<<grid, kGatherCols * kGatherRows, 0, st>>>(
+                g, w, p.start, static_cast<const RemeshRec<3>*>(p.rec), p.rcx, rp.circ_threshold,
+                p.G, p.flag);
         DVM_LAUNCH_CHECK();
         size_t t = p.tmp_bytes;
         DVM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(p.tmp, t, p.flag, p.pos, (int)wnodes, st));

@@ -89,7 +89,6 @@ __global__ void gj_build_kernel(int n, const double* __restrict__ A, double* __r
 constexpr int kGjMaxRows = 4096;   // at most 4096 panels per body
 constexpr int kGjThreads = 512;
 constexpr int kGjStrip = 17408;    // doubles of the LDS strip: n rows x (KB + 1)
-constexpr int kGjRows = 8;         // rows per workgroup of the trailing update
 
 // Blocked form of the elimination, KB pivot columns per pass.  Within a pass
 // the steps only touch the strip of columns k0 .. k0+KB-1 and the trailing
@@ -244,24 +243,33 @@ __global__ __launch_bounds__(64) void gj_pivot_rows_kernel(int n, int k0, const 
     }
 }
 
-// The pass's trailing update, rows r0 .. r0+kGjRows-1 and 64 trailing
-// columns per workgroup: T[r][j] -= f(r, m) U_m[j] for m in order (m with
-// p_m != r).
+// The pass's trailing update T -= F U on FP64 matrix cores: F[r][m] =
+// f(r, m) (0 for p_m = r and past the pass's last column), U the pivot rows'
+// values when used (n x KB times KB x trailing columns).  A workgroup of 4
+// waves takes 16 rows and 64 trailing columns, a wave one 16 x 16 tile: KB/4
+// steps of v_mfma_f64_16x16x4f64 (lane l supplies F[l % 16][k], k = l / 16 of
+// the step, and U[k][l % 16]; it receives the product's rows 4v + l / 16,
+// v = 0..3, of column l % 16), then the tile is subtracted from T.  Every row
+// takes the same terms as KB single elimination steps; the products are
+// summed by the matrix unit before the one subtraction, so the trailing
+// values differ from the stepwise order in the last bits.
+constexpr int kGjTile = 16;  // rows per workgroup of the trailing update (the MFMA tile)
+
+typedef double gj_v4d __attribute__((ext_vector_type(4)));
+
 template <int KB>
-__global__ __launch_bounds__(64) void gj_update_kernel(int n, int k0, double* __restrict__ G,
-                                                       const int* __restrict__ pivrow,
-                                                       const double* __restrict__ U)
+__global__ __launch_bounds__(256) void gj_update_kernel(int n, int k0, double* __restrict__ G,
+                                                        const int* __restrict__ pivrow,
+                                                        const double* __restrict__ U)
 {
-    __shared__ double f[kGjRows][KB];
-    __shared__ int pv[KB];
-    const int lane = threadIdx.x;
+    constexpr int KP = (KB + 3) / 4 * 4;  // whole MFMA steps (columns past KB are zero)
+    __shared__ double f[kGjTile][KP];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int W = 2 * n + 1;
     const int kb = min(KB, n - k0);
-    const int r0 = blockIdx.y * kGjRows;
-    const int j = k0 + kb + blockIdx.x * 64 + lane;
-    for (int m = lane; m < KB; m += 64) pv[m] = m < kb ? pivrow[k0 + m] : -1;
-    for (int e = lane; e < kGjRows * KB; e += 64) {
-        const int rr = e / KB, m = e - rr * KB;
+    const int r0 = blockIdx.y * kGjTile;
+    for (int e = t; e < kGjTile * KP; e += 256) {
+        const int rr = e / KP, m = e - rr * KP;
         const int r = r0 + rr;
         double v = 0.0;
         if (r < n && m < kb) {
@@ -271,17 +279,21 @@ __global__ __launch_bounds__(64) void gj_update_kernel(int n, int k0, double* __
         f[rr][m] = v;
     }
     __syncthreads();
+    const int j = k0 + kb + blockIdx.x * 64 + wave * 16 + (lane & 15);  // this lane's column
+    const int kq = lane >> 4;
+    gj_v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int m0 = 0; m0 < KP; m0 += 4) {
+        const int m = m0 + kq;
+        const double a = f[lane & 15][m];
+        const double b = (m < kb && j < W) ? U[(int64_t)m * W + j] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
     if (j >= W) return;
-    double u[KB];
 #pragma unroll
-    for (int m = 0; m < KB; ++m) u[m] = m < kb ? U[(int64_t)m * W + j] : 0.0;
-    const int r1 = min(r0 + kGjRows, n);
-    for (int r = r0; r < r1; ++r) {
-        double acc = G[(int64_t)r * W + j];
-#pragma unroll
-        for (int m = 0; m < KB; ++m)
-            if (pv[m] >= 0 && pv[m] != r) acc -= f[r - r0][m] * u[m];
-        G[(int64_t)r * W + j] = acc;
+    for (int v = 0; v < 4; ++v) {
+        const int r = r0 + 4 * v + kq;
+        if (r < n) G[(int64_t)r * W + j] -= acc[v];
     }
 }
 
@@ -296,7 +308,7 @@ static void gj_passes(hipStream_t st, int n, double* G, int* pivrow, double* U)
         const int wt = W - (k0 + kb);
         gj_pivot_rows_kernel<KB><<<cdiv(wt, 64), 64, 0, st>>>(n, k0, G, pivrow, U);
         DVM_LAUNCH_CHECK();
-        gj_update_kernel<KB><<<dim3(cdiv(wt, 64), cdiv(n, kGjRows)), 64, 0, st>>>(n, k0, G, pivrow, U);
+        gj_update_kernel<KB><<<dim3(cdiv(wt, 64), cdiv(n, kGjTile)), 256, 0, st>>>(n, k0, G, pivrow, U);
         DVM_LAUNCH_CHECK();
     }
 }

@@ -150,6 +150,18 @@ __device__ __forceinline__ double2 reflect_wave(int nl, const double* __restrict
     return make_double2(nx, nz);
 }
 
+// 1/a from the hardware reciprocal and two Newton steps on the FMA: below one
+// unit in the last place (the correctly rounded quotient the reference's
+// divide gives can differ in that last bit).  a = 0 gives NaN, as 0/0 does.
+__device__ __forceinline__ double recip64(double a)
+{
+    double y = __builtin_amdgcn_rcp(a);
+    double e = __builtin_fma(-a, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-a, y, 1.0);
+    return __builtin_fma(y, e, y);
+}
+
 // One (blob, panel) term of src/VortexSheet.cpp:225-272: coefficients
 // px, qx, py, qy with the collocation midpoint as panel origin, against
 // gamma_j and gamma_{j+1}.
@@ -184,20 +196,21 @@ __device__ __forceinline__ void sheet_by_groups(double x, double z, int nl, cons
 {
     const int ng = (nl + kSheetGroup - 1) / kSheetGroup;
     for (int g = 0; g < ng; ++g) {
-        const double4 c = gc[g];
-        const double wr = (x - c.x) / c.z, wi = (z - c.y) / c.z;
+        const double4 c = gc[g];  // centre, radius, 1 / radius (0: never the series)
+        const double wr = (x - c.x) * c.w, wi = (z - c.y) * c.w;
         const double w2 = wr * wr + wi * wi;
         if (w2 > kSheetFar * kSheetFar) {
             const double* a = ga + 2 * g * kSheetTerms;
-            const double tr = wr / w2, ti = -wi / w2;  // 1 / W
+            const double iw2 = recip64(w2);
+            const double tr = wr * iw2, ti = -wi * iw2;  // 1 / W
             double pr = a[2 * (kSheetTerms - 1)], pim = a[2 * (kSheetTerms - 1) + 1];
             for (int k = kSheetTerms - 2; k >= 0; --k) {
                 const double nr = pr * tr - pim * ti + a[2 * k];
                 pim = pr * ti + pim * tr + a[2 * k + 1];
                 pr = nr;
             }
-            su += (pr * ti + pim * tr) / c.z;
-            sw += (pr * tr - pim * ti) / c.z;
+            su += (pr * ti + pim * tr) * c.w;
+            sw += (pr * tr - pim * ti) * c.w;
         } else {
             const int j1 = min((g + 1) * kSheetGroup, nl);
             for (int j = g * kSheetGroup; j < j1; ++j)
@@ -226,17 +239,6 @@ __device__ __forceinline__ void group_moments(int g, int k, int nl, const double
     ga[2 * (g * kSheetTerms + k) + 1] = ai;
 }
 
-// 1/a from the hardware reciprocal and two Newton steps on the FMA: below one
-// unit in the last place (the correctly rounded quotient the reference's
-// divide gives can differ in that last bit).  a = 0 gives NaN, as 0/0 does.
-__device__ __forceinline__ double recip64(double a)
-{
-    double y = __builtin_amdgcn_rcp(a);
-    double e = __builtin_fma(-a, y, 1.0);
-    y = __builtin_fma(y, e, y);
-    e = __builtin_fma(-a, y, 1.0);
-    return __builtin_fma(y, e, y);
-}
 
 // One pair of the reference kernel (src/VelocityKernel.cpp:44-60), FP64.
 __device__ __forceinline__ void bs_pair_f64(double xi, double zi, double xj, double zj, double cj,

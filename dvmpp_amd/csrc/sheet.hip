@@ -480,6 +480,7 @@ void sheet_groups(const SheetDev& sh, std::vector<double>& cen)
         cen[4 * (size_t)g] = ok ? cx : 0.0;
         cen[4 * (size_t)g + 1] = ok ? cz : 0.0;
         cen[4 * (size_t)g + 2] = ok ? r : INFINITY;
+        cen[4 * (size_t)g + 3] = ok ? 1.0 / r : 0.0;
     }
 }
 

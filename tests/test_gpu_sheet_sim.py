@@ -421,6 +421,35 @@ def test_remesh_blobs_in_a_corner_of_the_mesh(gpu_ctx, kernel):
             assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_remesh_dense_cells_and_every_node(gpu_ctx, kernel):
+    # 30,000 blobs in a 0.2 x 0.2 patch (about 300 per cell): a cell row's
+    # records span many 256-record pieces of the gather's staging; then a
+    # negative threshold, which outputs every node of the mesh.
+    rng = np.random.default_rng(60 + kernel)
+    n = 30000
+    x = rng.uniform(0.3, 0.5, n)
+    z = rng.uniform(-0.1, 0.1, n)
+    c = rng.normal(0.0, 1e-3, n)
+    s = np.full(n, 0.03)
+    for thr, lo, hi in ((1e-9, -3.0, 5.0), (-1.0, -1.0, 1.0)):
+        o = oracle.make_params(remesh_frequency=1, remesh_kernel=kernel, remesh_xmin=lo,
+                               remesh_xmax=hi, remesh_zmin=-1.0, remesh_zmax=1.0, remesh_dx=0.02,
+                               remesh_dz=0.02, remesh_sigma=0.03, remesh_circ_threshold=thr)
+        rp = dsim.remesh_params(frequency=1, kernel=kernel, xmin=lo, xmax=hi, zmin=-1.0,
+                                zmax=1.0, dx=0.02, dz=0.02, sigma=0.03, circ_threshold=thr)
+        ox, oz, oc, osg = oracle.remesh(o, x, z, c, s)
+        g = [t.cpu().numpy() for t in dsim.remesh(gpu_ctx, rp, *to_dev(x, z, c, s))]
+        assert g[0].size == ox.size, (thr, g[0].size, ox.size)
+        if thr < 0:
+            assert ox.size == 101 * 101
+        for k, (a, b) in enumerate(zip(g, (ox, oz, oc, osg))):
+            if k == 2 and kernel == 1:
+                assert rel_err(a, b) <= 1e-13
+            else:
+                assert np.array_equal(a, b), (thr, k)
+
+
 def test_release_nan_is_vortex_exception(gpu_ctx):
     px, pz = oracle.cylinder_points(16)
     _, gp = both_params(ux=0.0, uz=0.0)

@@ -160,12 +160,13 @@ static int64_t part_slot_size(int64_t cap)
     return 2 * m;
 }
 
-// Exclusive prefix sum of v over the workgroup's lanes (1024 lanes, 16
-// waves): wave scans by shuffles, then the wave totals; total gets the sum.
+// Exclusive prefix sum of v over the workgroup's lanes (up to 1024 lanes,
+// 16 waves): wave scans by shuffles, then the wave totals; total gets the sum.
 __device__ __forceinline__ int64_t block_scan_excl(int64_t v, int64_t* wsum, int64_t& total)
 {
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
     int64_t x = v;
     for (int o = 1; o < 64; o <<= 1) {
         const int64_t y = __shfl_up(x, o);
@@ -174,7 +175,7 @@ __device__ __forceinline__ int64_t block_scan_excl(int64_t v, int64_t* wsum, int
     if (lane == 63) wsum[wv] = x;
     __syncthreads();
     int64_t off = 0, tot = 0;
-    for (int k = 0; k < 16; ++k) {
+    for (int k = 0; k < nw; ++k) {
         const int64_t t = wsum[k];
         off += k < wv ? t : 0;
         tot += t;
@@ -184,10 +185,13 @@ __device__ __forceinline__ int64_t block_scan_excl(int64_t v, int64_t* wsum, int
     return off + x - v;
 }
 
-// One workgroup of 1024 lanes, one per slot (at most 1024 slots): per-slot
+// One workgroup, a lane per slot (prefix_lanes(S) lanes): per-slot
 // tile and blob counts, their prefixes, the slot order by decreasing tile
 // count (ties by index), the FP32C item and sort-tile prefixes, and the
 // work-counter reset.
+// Lanes of the prefix workgroup: whole waves covering the slots.
+static unsigned prefix_lanes(int S) { return (unsigned)((S + 63) / 64 * 64); }
+
 __global__ __launch_bounds__(1024) void bt_prefix_kernel(int S, int64_t cap,
                                                          const SlotState* __restrict__ st,
                                                          WorkList* wl, int* tile_pre, int* order,
@@ -1362,7 +1366,7 @@ void record_convect(dvm_batch_t B)
     const double bb[3] = {1. / 4, 8. / 9, 3. / 4};
     const int stages = rk ? 3 : 1;
     for (int k = 0; k < stages; ++k) {
-        bt_prefix_kernel<<<1, 1024, 0, st>>>(S, cap, d.st, d.wl, d.tile_pre, d.order, d.otile_pre,
+        bt_prefix_kernel<<<1, prefix_lanes(S), 0, st>>>(S, cap, d.st, d.wl, d.tile_pre, d.order, d.otile_pre,
                                              d.blob_pre, d.item_pre, d.stile_pre);
         if (rk && k == 0) bt_zero_q_kernel<<<gs, 256, 0, st>>>(S, cap, d.wl, d.blob_pre, d.q1, d.q2);
         if (f32c) {
@@ -1409,7 +1413,7 @@ void record_rest(dvm_batch_t B)
     const int64_t cap = d.cap;
     const int mp = d.mp;
     const unsigned gs = 4096;
-    bt_prefix_kernel<<<1, 1024, 0, st>>>(S, cap, d.st, d.wl, d.tile_pre, d.order, d.otile_pre,
+    bt_prefix_kernel<<<1, prefix_lanes(S), 0, st>>>(S, cap, d.st, d.wl, d.tile_pre, d.order, d.otile_pre,
                                          d.blob_pre, d.item_pre, d.stile_pre);
     bt_diffuse_kernel<<<gs, 256, 0, st>>>(S, cap, d.wl, d.blob_pre, d.prm, d.st, d.b);
     bt_release_count_kernel<<<S, 1024, 0, st>>>(cap, mp, d.prm, d.st, d.gamma, d.ds, d.cnt, d.off, d.cnew);
@@ -1417,7 +1421,7 @@ void record_rest(dvm_batch_t B)
                                                          d.etz, d.enx, d.enz, d.cnt, d.off, d.cnew,
                                                          d.panel, d.b);
     bt_commit_release_kernel<<<(S + 255) / 256, 256, 0, st>>>(S, d.st);
-    bt_prefix_kernel<<<1, 1024, 0, st>>>(S, cap, d.st, d.wl, d.tile_pre, d.order, d.otile_pre,
+    bt_prefix_kernel<<<1, prefix_lanes(S), 0, st>>>(S, cap, d.st, d.wl, d.tile_pre, d.order, d.otile_pre,
                                          d.blob_pre, d.item_pre, d.stile_pre);
     const int reflect = B->cfg.crossing != DVM_DELETE ? 1 : 0;
     bt_inside_kernel<<<gs, kBT, 0, st>>>(S, cap, mp, d.wl, d.tile_pre, d.st, d.prm, d.bx, d.bz,

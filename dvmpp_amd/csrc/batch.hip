@@ -493,7 +493,7 @@ __global__ __launch_bounds__(kBT) void bt_tiles_kernel(int S, int64_t cap, int m
                                                        TileInfo* __restrict__ tiles,
                                                        TileInfo* __restrict__ tboxes)
 {
-    __shared__ double red[5][kBT];
+    __shared__ double red[6][kBT];
     const int total = wl->total_tiles;
     const int lane = threadIdx.x & 63;
     for (int g = blockIdx.x; g < total; g += gridDim.x) {
@@ -523,7 +523,7 @@ __global__ __launch_bounds__(kBT) void bt_tiles_kernel(int S, int64_t cap, int m
             tb.hx = (float)(0.5 * (mxx - mnx)) * 1.000001f + 1e-30f;
             tb.hz = (float)(0.5 * (mxz - mnz)) * 1.000001f + 1e-30f;
             tb.thr_max = 0.0f;
-            tb.pad = 0.0f;
+            tb.thr_min = 0.0f;
             tboxes[(int64_t)s * max_tboxes + w0 / kTB] = tb;
         }
         red[0][threadIdx.x] = valid ? xi : INFINITY;
@@ -531,6 +531,7 @@ __global__ __launch_bounds__(kBT) void bt_tiles_kernel(int S, int64_t cap, int m
         red[2][threadIdx.x] = valid ? zi : INFINITY;
         red[3][threadIdx.x] = valid ? zi : -INFINITY;
         red[4][threadIdx.x] = si * si;
+        red[5][threadIdx.x] = valid ? si * si : INFINITY;
         __syncthreads();
         for (int w = kBT / 2; w > 0; w >>= 1) {
             if ((int)threadIdx.x < w) {
@@ -540,6 +541,7 @@ __global__ __launch_bounds__(kBT) void bt_tiles_kernel(int S, int64_t cap, int m
                 red[2][threadIdx.x] = fmin(red[2][threadIdx.x], red[2][p]);
                 red[3][threadIdx.x] = fmax(red[3][threadIdx.x], red[3][p]);
                 red[4][threadIdx.x] = fmax(red[4][threadIdx.x], red[4][p]);
+                red[5][threadIdx.x] = fmin(red[5][threadIdx.x], red[5][p]);
             }
             __syncthreads();
         }
@@ -552,16 +554,19 @@ __global__ __launch_bounds__(kBT) void bt_tiles_kernel(int S, int64_t cap, int m
             ti.hx = (float)(0.5 * (red[1][0] - red[0][0])) * 1.000001f + 1e-30f;
             ti.hz = (float)(0.5 * (red[3][0] - red[2][0])) * 1.000001f + 1e-30f;
             ti.thr_max = (float)(10.0 * red[4][0]);
-            ti.pad = 0.0f;
+            ti.thr_min = (float)(10.0 * red[5][0]);
             tiles[g] = ti;
         }
         if (valid) {
             const int64_t o = base + k;
-            s32[o] = make_float4((float)(xi - cx), (float)(zi - cz), (float)b.c[src], 0.0f);
             const double s2 = si * si;
+            const float thr = (float)(10.0 * s2);
+            const double h = 0.5 * fmax(red[1][0] - red[0][0], red[3][0] - red[2][0]);
+            s32[o] = make_float4((float)(xi - cx), (float)(zi - cz), (float)b.c[src],
+                                 near_thr_hi(thr, h, 10.0 * red[4][0]));
             const double ox = xi - cx, oz = zi - cz;
-            near32[o] = make_float4((float)(10.0 * s2), (float)(1.0 / s2),
-                                    (float)(ox - (double)(float)ox), (float)(oz - (double)(float)oz));
+            near32[o] = make_float4(thr, (float)(1.0 / s2), (float)(ox - (double)(float)ox),
+                                    (float)(oz - (double)(float)oz));
             p64[o] = make_double2(xi, zi);
         }
         __syncthreads();
@@ -614,9 +619,12 @@ __global__ __launch_bounds__(256) void bt_bs32_kernel(int S, int64_t cap, int ma
         const double2 pi = p64[base + it];
         const double xi = pi.x, zi = pi.y;
         const TileInfo tb = tboxes[(int64_t)s * max_tboxes + tbk];
+        const TileInfo* __restrict__ tsl = tiles + tile_pre[s];
+        static_assert(kBT % kTB == 0, "a target block lies in one source tile");
+        const int self_tile = tbk * kTB / kBT;
         float su = 0.0f, cu = 0.0f, sw = 0.0f, cw = 0.0f;
         for (int sti = s0; sti < s1; ++sti) {
-            const TileInfo ts = tiles[tile_pre[s] + sti];
+            const TileInfo ts = tsl[sti];
             const double Dxd = ts.cx - tb.cx;
             const double Dzd = ts.cz - tb.cz;
             const double gx = fmax(fabs(Dxd) - (double)(tb.hx + ts.hx), 0.0);
@@ -641,8 +649,20 @@ __global__ __launch_bounds__(256) void bt_bs32_kernel(int S, int64_t cap, int ma
             } else {
                 const float txl = (float)((xi - ts.cx) - (double)txd);
                 const float tzl = (float)((zi - ts.cz) - (double)tzd);
-                near_range<true, false>(sp, near32 + base + jr, 0, cnt, txd, tzd, txl, tzl, jr, i,
-                                        0.0f, au, aw);
+                // only the tile holding the block's own ranks has the self pair
+                if (sti == self_tile) {
+                    near_range<true, false>(sp, near32 + base + jr, 0, cnt, txd, tzd, txl, tzl, jr,
+                                            i, 0.0f, au, aw);
+                } else if (cnt == kBT) {
+                    const double ex = fabs(Dxd) + (double)(tb.hx + ts.hx);
+                    const double ez = fabs(Dzd) + (double)(tb.hz + ts.hz);
+                    const bool share = ts.thr_min > 1e-8f && ex * ex + ez * ez < 1e8;
+                    near_tile_share4<kBT>(sp, near32 + base + jr, txd, tzd, txl, tzl, share, au,
+                                          aw);
+                } else {
+                    near_range<false, false>(sp, near32 + base + jr, 0, cnt, txd, tzd, txl, tzl, jr,
+                                             i, 0.0f, au, aw);
+                }
             }
             kahan_add(su, cu, au);
             kahan_add(sw, cw, aw);

@@ -268,7 +268,7 @@ __global__ __launch_bounds__(T) void tile_build_kernel(
     float4* __restrict__ p32, float4* __restrict__ near32, double2* __restrict__ p64,
     TileInfo* __restrict__ tiles)
 {
-    __shared__ double red[5][T];
+    __shared__ double red[6][T];
     const int64_t k = (int64_t)blockIdx.x * T + threadIdx.x;
     const bool valid = k < n;
     const int64_t src = valid ? (perm ? (int64_t)perm[k] : k) : 0;
@@ -280,6 +280,7 @@ __global__ __launch_bounds__(T) void tile_build_kernel(
     red[2][threadIdx.x] = valid ? zi : INFINITY;
     red[3][threadIdx.x] = valid ? zi : -INFINITY;
     red[4][threadIdx.x] = si * si;
+    red[5][threadIdx.x] = valid ? si * si : INFINITY;
     __syncthreads();
     for (int st = T / 2; st > 0; st >>= 1) {
         if ((int)threadIdx.x < st) {
@@ -289,6 +290,7 @@ __global__ __launch_bounds__(T) void tile_build_kernel(
             red[2][threadIdx.x] = fmin(red[2][threadIdx.x], red[2][o]);
             red[3][threadIdx.x] = fmax(red[3][threadIdx.x], red[3][o]);
             red[4][threadIdx.x] = fmax(red[4][threadIdx.x], red[4][o]);
+            red[5][threadIdx.x] = fmin(red[5][threadIdx.x], red[5][o]);
         }
         __syncthreads();
     }
@@ -301,15 +303,18 @@ __global__ __launch_bounds__(T) void tile_build_kernel(
         ti.hx = (float)(0.5 * (red[1][0] - red[0][0])) * 1.000001f + 1e-30f;
         ti.hz = (float)(0.5 * (red[3][0] - red[2][0])) * 1.000001f + 1e-30f;
         ti.thr_max = (float)(10.0 * red[4][0]);
-        ti.pad = 0.0f;
+        ti.thr_min = (float)(10.0 * red[5][0]);
         tiles[blockIdx.x] = ti;
     }
     if (!valid || !p32) return;
     const double ci = c[src];
-    p32[k] = make_float4((float)(xi - cx), (float)(zi - cz), (float)ci, 0.0f);
     const double s2 = si * si;
+    const float thr = (float)(10.0 * s2);
+    const double h = 0.5 * fmax(red[1][0] - red[0][0], red[3][0] - red[2][0]);
+    p32[k] = make_float4((float)(xi - cx), (float)(zi - cz), (float)ci,
+                         near_thr_hi(thr, h, 10.0 * red[4][0]));
     const double ox = xi - cx, oz = zi - cz;
-    near32[k] = make_float4((float)(10.0 * s2), (float)(1.0 / s2), (float)(ox - (double)(float)ox),
+    near32[k] = make_float4(thr, (float)(1.0 / s2), (float)(ox - (double)(float)ox),
                             (float)(oz - (double)(float)oz));
     p64[k] = make_double2(xi, zi);
 }
@@ -387,8 +392,21 @@ __global__ __launch_bounds__(BLOCK) void bs_f32c_kernel(
         } else {
             const float txl = (float)((xi - ts.cx) - (double)txd);
             const float tzl = (float)((zi - ts.cz) - (double)tzd);
-            near_range<SELF, GUARD>(sp, near32 + jb, 0, cnt, txd, tzd, txl, tzl, jb, i, r2_skip,
-                                    au, aw);
+            // only tiles overlapping the block's own indices hold self pairs
+            const int64_t b0 = (int64_t)blockIdx.x * BLOCK;
+            const bool self_tile = SELF && jb < b0 + BLOCK && jb + TILE > b0;
+            if (!self_tile && !GUARD && cnt == TILE) {
+                const double ex = fabs(Dxd) + (double)(tb.hx + ts.hx);
+                const double ez = fabs(Dzd) + (double)(tb.hz + ts.hz);
+                const bool share = ts.thr_min > 1e-8f && ex * ex + ez * ez < 1e8;
+                near_tile_share4<TILE>(sp, near32 + jb, txd, tzd, txl, tzl, share, au, aw);
+            } else if (self_tile) {
+                near_range<SELF, GUARD>(sp, near32 + jb, 0, cnt, txd, tzd, txl, tzl, jb, i,
+                                        r2_skip, au, aw);
+            } else {
+                near_range<false, GUARD>(sp, near32 + jb, 0, cnt, txd, tzd, txl, tzl, jb, i,
+                                         r2_skip, au, aw);
+            }
         }
         kahan_add(su, cu, au);
         kahan_add(sw, cw, aw);

@@ -10,12 +10,12 @@
 namespace dvmi {
 
 // Tile record of the FP32C sum: FP64 centre, FP32 half extents (rounded up)
-// and 10 * max(sigma)^2 over the tile's sources.
+// and, for a source tile, the largest and smallest 10 sigma^2 of its sources.
 struct TileInfo {
     double cx, cz;
     float hx, hz;
-    float thr_max;
-    float pad;
+    float thr_max;  // largest 10 sigma^2 of a source tile
+    float thr_min;  // smallest 10 sigma^2 of a source tile
 };
 
 __device__ __forceinline__ void kahan_add(float& s, float& c, float x)
@@ -560,6 +560,133 @@ __device__ __forceinline__ void near_range(const float4* __restrict__ sp,
     for (; k < k1; ++k) {
         const NearPair p = near_prep(sp[k], np[k], txd, tzd, txl, tzl);
         near_add<SELF, GUARD>(p, near_k(p.r2, p.nj), j0 + k, i, r2_skip, au, aw);
+    }
+}
+
+// Enlarged near-field threshold of a source, kept in the fourth word of its
+// FP32 record.  thr = 10 sigma^2 as the near-field constants round it, h the
+// larger half extent of the source tile, thr_max the tile's largest thr.  A
+// one-word separation (txd - x', tzd - z') differs from the exact one by at
+// most 2^-24 (|t - c| + |s - c| + |t - s|) per component; for a pair with
+// |t - s| <= 2 sqrt(thr) that is at most 2^-24 (2h + 4 sqrt(thr_max)), and
+// the returned value exceeds thr by more than the r^2 such an error (and the
+// roundings of both r^2 forms) can move.  So a pair whose one-word r^2 is not
+// below it has an exact two-word r^2 not below thr: it is a 1/r^2 pair of the
+// reference's kernel (src/VelocityKernel.cpp:48-55).
+__device__ __forceinline__ float near_thr_hi(float thr, double h, double thr_max)
+{
+    const double e = 2.0 * h + 4.0 * sqrt(thr_max);
+    const double t = (double)thr;
+    return (float)(t * (1.0 + 0x1p-19) + 1e-6 * sqrt(t) * e + 3e-14 * e * e + 1e-37);
+}
+
+// One quad of a near tile (sources at np[0..3] of the near-field constants).
+// The one-word separations and r^2 come first; if no lane has a pair below
+// its source's enlarged threshold (record word w) the quad is four 1/r^2
+// pairs, summed as in far_quad (one reciprocal for four when `share`, i.e.
+// every r^2 of the tile pair lies in [1e-8, 1e8]); else the whole wave takes
+// the exact two-word path of near_range, whose operations it repeats (same
+// results; it gives 1/r^2 for the pairs at or beyond 10 sigma^2 too).
+__device__ __forceinline__ void near_quad(const float4 sa, const float4 sb, const float4 sc,
+                                          const float4 sd, const float4* __restrict__ np,
+                                          float txd, float tzd, float txl, float tzl, bool share,
+                                          float& au, float& aw)
+{
+    const float dxa = txd - sa.x, dza = tzd - sa.y;
+    const float dxb = txd - sb.x, dzb = tzd - sb.y;
+    const float dxc = txd - sc.x, dzc = tzd - sc.y;
+    const float dxd = txd - sd.x, dzd = tzd - sd.y;
+    const float ra = __builtin_fmaf(dxa, dxa, dza * dza);
+    const float rb = __builtin_fmaf(dxb, dxb, dzb * dzb);
+    const float rc = __builtin_fmaf(dxc, dxc, dzc * dzc);
+    const float rd = __builtin_fmaf(dxd, dxd, dzd * dzd);
+    // wave-uniform: compares into lane masks, ORed on the scalar unit
+    const uint64_t nearm = __builtin_amdgcn_ballot_w64(ra < sa.w) |
+                           __builtin_amdgcn_ballot_w64(rb < sb.w) |
+                           __builtin_amdgcn_ballot_w64(rc < sc.w) |
+                           __builtin_amdgcn_ballot_w64(rd < sd.w);
+    if (nearm != 0) {
+        const float4 na = np[0], nb = np[1], nc = np[2], nd = np[3];
+        NearPair p[4];
+        const float dx[4] = {dxa, dxb, dxc, dxd}, dz[4] = {dza, dzb, dzc, dzd};
+        const float4 n4[4] = {na, nb, nc, nd};
+        const float cc[4] = {sa.z, sb.z, sc.z, sd.z};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            p[q].dx = dx[q] + (txl - n4[q].z);
+            p[q].dz = dz[q] + (tzl - n4[q].w);
+            p[q].r2 = __builtin_fmaf(p[q].dx, p[q].dx, p[q].dz * p[q].dz);
+            p[q].c = cc[q];
+            p[q].nj = make_float2(n4[q].x, n4[q].y);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            near_add<false, false>(p[q], near_k(p[q].r2, p[q].nj), 0, 0, 0.0f, au, aw);
+    } else if (share) {
+        const float pab = ra * rb;
+        const float pcd = rc * rd;
+        const float inv = __builtin_amdgcn_rcpf(pab * pcd);
+        const float iab = inv * pcd;
+        const float icd = inv * pab;
+        const float ka = (rb * iab) * sa.z;
+        const float kb = (ra * iab) * sb.z;
+        const float kc = (rd * icd) * sc.z;
+        const float kd = (rc * icd) * sd.z;
+        au = __builtin_fmaf(ka, dza, au);
+        aw = __builtin_fmaf(ka, dxa, aw);
+        au = __builtin_fmaf(kb, dzb, au);
+        aw = __builtin_fmaf(kb, dxb, aw);
+        au = __builtin_fmaf(kc, dzc, au);
+        aw = __builtin_fmaf(kc, dxc, aw);
+        au = __builtin_fmaf(kd, dzd, au);
+        aw = __builtin_fmaf(kd, dxd, aw);
+    } else {
+        const float ka = __builtin_amdgcn_rcpf(ra) * sa.z;
+        const float kb = __builtin_amdgcn_rcpf(rb) * sb.z;
+        const float kc = __builtin_amdgcn_rcpf(rc) * sc.z;
+        const float kd = __builtin_amdgcn_rcpf(rd) * sd.z;
+        au = __builtin_fmaf(ka, dza, au);
+        aw = __builtin_fmaf(ka, dxa, aw);
+        au = __builtin_fmaf(kb, dzb, au);
+        aw = __builtin_fmaf(kb, dxb, aw);
+        au = __builtin_fmaf(kc, dzc, au);
+        aw = __builtin_fmaf(kc, dxc, aw);
+        au = __builtin_fmaf(kd, dzd, au);
+        aw = __builtin_fmaf(kd, dxd, aw);
+    }
+}
+
+// A full near tile without the self pair or a guard: the source records
+// stream as in far_tile_share4 (batches of eight, the next in flight) and
+// each quad is classified by near_quad; only quads with a near pair read the
+// near-field constants.
+template <int T>
+__device__ __forceinline__ void near_tile_share4(const float4* __restrict__ sp,
+                                                 const float4* __restrict__ np, float txd,
+                                                 float tzd, float txl, float tzl, bool share,
+                                                 float& au, float& aw)
+{
+    const Src8* __restrict__ q = reinterpret_cast<const Src8*>(sp);
+    Src8 a = q[0];
+#pragma unroll 1
+    for (int b = 0; b < T / 8; b += 2) {
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+        __builtin_amdgcn_sched_barrier(0);
+        const int bc = __builtin_amdgcn_readfirstlane(b + 1);
+        const Src8 c = q[bc];
+        __builtin_amdgcn_sched_barrier(0);
+        const float4* __restrict__ na = np + 8 * __builtin_amdgcn_readfirstlane(b);
+        near_quad(a.v[0], a.v[1], a.v[2], a.v[3], na, txd, tzd, txl, tzl, share, au, aw);
+        near_quad(a.v[4], a.v[5], a.v[6], a.v[7], na + 4, txd, tzd, txl, tzl, share, au, aw);
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+        __builtin_amdgcn_sched_barrier(0);
+        a = q[__builtin_amdgcn_readfirstlane(b + 2 < T / 8 ? b + 2 : b + 1)];
+        __builtin_amdgcn_sched_barrier(0);
+        const float4* __restrict__ nc = np + 8 * bc;
+        near_quad(c.v[0], c.v[1], c.v[2], c.v[3], nc, txd, tzd, txl, tzl, share, au, aw);
+        near_quad(c.v[4], c.v[5], c.v[6], c.v[7], nc + 4, txd, tzd, txl, tzl, share, au, aw);
     }
 }
 

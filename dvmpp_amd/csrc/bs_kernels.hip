@@ -98,8 +98,8 @@ __global__ __launch_bounds__(kBlock) void bs_f64_kernel(
     auto group = [&](const Rec64x4& g, int cnt, int64_t jb, bool self_check) {
         double dx[4], dz[4], r2[4], K[4];
         bool take[4];
-        // smallest r2 - thr over the group: negative iff some pair is near
-        double gap = 1.0;
+        // lanes with a near pair in the group (ballots ORed on the scalar unit)
+        uint64_t nearm = 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             dx[q] = xi - g.r[q].x;
@@ -108,10 +108,30 @@ __global__ __launch_bounds__(kBlock) void bs_f64_kernel(
             take[q] = q < cnt;
             if (SELF) take[q] = take[q] && !(self_check && jb + q == i);
             if (GUARD) take[q] = take[q] && (r2[q] > r2_skip);
-            K[q] = recip64(r2[q]);
-            gap = fmin(gap, take[q] ? r2[q] - g.r[q].w : 1.0);
+            nearm |= __builtin_amdgcn_ballot_w64(take[q] && r2[q] < g.r[q].w);
         }
-        if (__any(gap < 0.0)) {  // wave-uniform: the group's near-field correction
+        // 1/r2 of the four from one reciprocal of their product, 1/ra =
+        // rb rc rd / (ra rb rc rd), when the partial products are normal
+        // numbers (then every factor is, and each 1/r2 is within a few units
+        // in the last place); else one reciprocal per pair.
+        const double pab = r2[0] * r2[1];
+        const double pcd = r2[2] * r2[3];
+        const double pp = pab * pcd;
+        const double inv = recip64(pp);
+        const double iab = inv * pcd;
+        const double icd = inv * pab;
+        K[0] = r2[1] * iab;
+        K[1] = r2[0] * iab;
+        K[2] = r2[3] * icd;
+        K[3] = r2[2] * icd;
+        const bool shared = fmin(pab, pcd) > 1e-300 && pp > 1e-300 && pp < 1e300;
+        if (__builtin_amdgcn_ballot_w64(!shared) != 0) {  // wave-uniform, rare
+            // kept a branch: the four reciprocals are not computed ahead
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int q = 0; q < 4; ++q) K[q] = shared ? K[q] : recip64(r2[q]);
+        }
+        if (nearm != 0) {  // wave-uniform: the group's near-field correction
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const bool near = take[q] && r2[q] < g.r[q].w;

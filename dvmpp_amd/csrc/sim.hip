@@ -388,7 +388,9 @@ void release(dvm_sim_t s)
     sheet_release_emit(s->ctx, s->sh, nrv, rrw, (uint32_t)nold, nold, s->b.x, s->b.z, s->b.c,
                        s->b.s, s->b.u, s->b.w, s->b.uvs, s->b.wvs, s->b.id);
     s->n = nold + nrv;
-    DVM_HIP_CHECK(hipStreamSynchronize(st));
+    // the pinned tape of the reference stream is refilled next step: wait for
+    // its copy (the device stream has no host tape)
+    if (s->p.rng_mode != DVM_RNG_DEVICE) DVM_HIP_CHECK(hipStreamSynchronize(st));
 }
 
 // src/DVMBase.cpp:107-118

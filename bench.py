@@ -283,14 +283,16 @@ def c4_remesh():
                               circ_threshold=1e-10)
 
 
-def ensemble_phase(ctx, count, window, plain_count, ref_count, seed=3):
+def ensemble_phase(ctx, count, window, plain_count, ref_count, seed=3, fmm_min=0):
     """The 128-body ensemble (steps ~ U{200..2000}) in one batch (FP32C,
     graph-replayed steps, remesh), every body run to its own step count or to
-    `window` steps if that is smaller (window 0: the full counts).  Beside it
-    the plain FP64 form one simulation at a time, on a random subset of the
-    bodies over the same steps, with the device random stream and with the
-    reference stream; the ratio is batched aggregate steps/s over plain
-    aggregate steps/s of that subset."""
+    `window` steps if that is smaller (window 0: the full counts).  With
+    fmm_min > 0 the batch runs a second time with slots of at least fmm_min
+    blobs taking the fast multipole sum (dvm_batch_set_fmm, order 12, theta
+    0.5).  Beside it the plain FP64 form one simulation at a time, on a
+    random subset of the bodies over the same steps, with the device random
+    stream and with the reference stream; the ratio is batched aggregate
+    steps/s over plain aggregate steps/s of that subset."""
     import torch
 
     import dvmpp_amd as dvm
@@ -301,38 +303,49 @@ def ensemble_phase(ctx, count, window, plain_count, ref_count, seed=3):
     jobs = ensemble(count)
     run = [min(st, window) if window else st for *_, st in jobs]
     cap = 1 << 17 if 0 < window <= 300 else 1 << 20
-    b = Batch(ctx, batch_config(slots=count, slot_capacity=cap, max_panels=512, prec=dvm.F32C,
-                                use_graph=True, remesh=rp))
-    t0 = time.perf_counter()
-    slots = b.admit_many([(px, pz, sp, sd, st) for (px, pz, sp, sd, _), st in zip(jobs, run)])
-    torch.cuda.synchronize()
-    ta = time.perf_counter() - t0
-    ctx.prof_reset()
-    ctx.prof_enable(True)
-    status = {}
-    pairs = 0.0
-    chunk = 50
-    t0 = time.perf_counter()
-    done_steps = 0
-    while len(status) < count:
-        b.step(chunk)
-        done_steps += chunk
-        # sizes after the chunk: a coarse sum of N^2 over the chunk's steps
-        for s in slots:
-            if s not in status:
-                pairs += float(b.info(s)["n"]) ** 2 * chunk
-        status.update(dict(b.poll(count)))
-        if done_steps % 250 == 0:
-            log(f"ensemble batch: step {done_steps}, {count - len(status)} running, "
-                f"{time.perf_counter() - t0:.1f} s")
-    torch.cuda.synchronize()
-    tb = time.perf_counter() - t0
-    ctx.prof_enable(False)
-    blobs = [b.info(s)["n"] for s in slots]
-    ok = sum(1 for st in status.values() if st == 0)
-    b.close()
     total = sum(run)
-    log(f"ensemble batch: {count} bodies, {total} steps in {tb:.2f} s (admission {ta:.2f} s)")
+
+    def batched(fmm):
+        b = Batch(ctx, batch_config(slots=count, slot_capacity=cap, max_panels=512,
+                                    prec=dvm.F32C, use_graph=True, remesh=rp))
+        t0 = time.perf_counter()
+        slots = b.admit_many([(px, pz, sp, sd, st) for (px, pz, sp, sd, _), st in zip(jobs, run)])
+        if fmm:
+            b.set_fmm(fmm, order=12, theta=0.5)
+        torch.cuda.synchronize()
+        ta = time.perf_counter() - t0
+        status = {}
+        pairs = 0.0
+        chunk = 50
+        t0 = time.perf_counter()
+        done_steps = 0
+        while len(status) < count:
+            b.step(chunk)
+            done_steps += chunk
+            # sizes after the chunk: a coarse sum of N^2 over the chunk's steps
+            for s in slots:
+                if s not in status:
+                    pairs += float(b.info(s)["n"]) ** 2 * chunk
+            status.update(dict(b.poll(count)))
+            if done_steps % 250 == 0:
+                log(f"ensemble batch{' (FMM slots)' if fmm else ''}: step {done_steps}, "
+                    f"{count - len(status)} running, {time.perf_counter() - t0:.1f} s")
+        torch.cuda.synchronize()
+        tb = time.perf_counter() - t0
+        blobs = [b.info(s)["n"] for s in slots]
+        ok = sum(1 for st in status.values() if st == 0)
+        b.close()
+        log(f"ensemble batch{' (FMM slots)' if fmm else ''}: {count} bodies, {total} steps in "
+            f"{tb:.2f} s (admission {ta:.2f} s)")
+        return {"aggregate_steps_per_s": total / tb, "seconds": tb, "admission_seconds": ta,
+                "completed_ok": ok, "slot_capacity": cap,
+                "final_blobs_min_median_max": [int(min(blobs)), int(np.median(blobs)),
+                                               int(max(blobs))],
+                "pair_interactions_per_s_estimate": pairs / tb}
+
+    res = batched(0)
+    tb, ta = res["seconds"], res["admission_seconds"]
+    res_fmm = batched(fmm_min) if fmm_min else None
     pick = np.random.default_rng(seed).choice(count, max(plain_count, ref_count), replace=False)
     plain = {}
     for name, mode, m in (("device_stream", dsim.RNG_DEVICE, plain_count),
@@ -367,23 +380,28 @@ def ensemble_phase(ctx, count, window, plain_count, ref_count, seed=3):
                         f"128-512 panels, nu log-U[2e-4,2e-3], dt 0.005, steps ~ U{{200..2000}} "
                         f"({win}: {total} simulation steps), Euler, REFLECT, M4' remesh every 20 "
                         f"steps at 0.01 (sigma 0.0321)",
-            "batched_f32c_graph": {"aggregate_steps_per_s": total / tb, "seconds": tb,
-                                   "admission_seconds": ta, "completed_ok": ok,
-                                   "slot_capacity": cap,
-                                   "final_blobs_min_median_max": [int(min(blobs)),
-                                                                  int(np.median(blobs)),
-                                                                  int(max(blobs))],
-                                   "pair_interactions_per_s_estimate": pairs / tb},
+            "batched_f32c_graph": res,
+            **({"batched_fmm_slots": dict(res_fmm, fmm_min_blobs=fmm_min,
+                                          form="slots of at least fmm_min_blobs blobs: fast "
+                                               "multipole sum (order 12, theta 0.5); the others "
+                                               "FP32C direct",
+                                          speedup_over_plain={
+                                              k: res_fmm["aggregate_steps_per_s"]
+                                              / v["aggregate_steps_per_s"]
+                                              for k, v in plain.items()})}
+               if res_fmm else {}),
             "plain_f64_per_simulation": plain,
             "plain_subset": f"{len(pick)} bodies drawn at random (seed {seed}): "
                             f"{sorted(int(k) for k in pick)}"}
 
 
-def served_phase(ctx, count, rate, steps_range, slots):
+def served_phase(ctx, count, rate, steps_range, slots, fmm_min=0):
     """Poisson arrivals at `rate`/s of ensemble-distribution requests (128-512
     panels, step counts ~ U{steps_range}, remesh), served by continuous
     batching (FP32C, graph-replayed steps) with capacity classes of 128K, 512K
-    and 1M blobs per slot allocated within a 200 GB HBM budget."""
+    and 1M blobs per slot allocated within a 200 GB HBM budget; with
+    fmm_min > 0, slots of at least fmm_min blobs take the fast multipole sum
+    (order 12, theta 0.5)."""
     import dvmpp_amd as dvm
     from dvmpp_amd.batch import batch_config
     from dvmpp_amd.serve import Request, Server
@@ -392,7 +410,7 @@ def served_phase(ctx, count, rate, steps_range, slots):
     reqs = [Request(i, t, px, pz, sp, seed, st) for i, (t, (px, pz, sp, seed, st)) in enumerate(trace)]
     srv = Server(ctx, batch_config(slots=slots, slot_capacity=1 << 20, max_panels=512,
                                    prec=dvm.F32C, use_graph=True, remesh=c4_remesh()),
-                 budget_bytes=200 << 30, classes=[1 << 17, 1 << 19, 1 << 20])
+                 budget_bytes=200 << 30, classes=[1 << 17, 1 << 19, 1 << 20], fmm_min=fmm_min)
     done, wall = srv.run(reqs)
     peak = srv.peak_bytes
     srv.close()
@@ -400,10 +418,12 @@ def served_phase(ctx, count, rate, steps_range, slots):
     lat = np.array([r.latency for r in ok]) if ok else np.array([np.nan])
     steps = sum(r.steps for r in ok)
     log(f"served: {len(done)} requests in {wall:.2f} s")
+    fm = (f", slots of at least {fmm_min} blobs by the fast multipole sum (order 12, theta 0.5)"
+          if fmm_min else "")
     return {"workload": f"{count} requests, Poisson {rate}/s, ensemble bodies (128-512 panels), "
                         f"steps ~ U{{{steps_range[0]}..{steps_range[1]}}}, M4' remesh every 20 "
                         f"steps at 0.01, {slots} slots per batch region, capacity classes "
-                        f"128K/512K/1M blobs, 200 GB budget",
+                        f"128K/512K/1M blobs, 200 GB budget{fm}",
             "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)),
             "aggregate_steps_per_s": steps / wall, "wall_s": wall,
@@ -764,7 +784,8 @@ def main():
                          "for scaling HBM byte counters)")
     ap.add_argument("--phases", default="sum,cylinder,ensemble,served,rk3,graph,merge,bandwidth",
                     help="comma list: sum (isolated sum, always run), cylinder, ensemble, "
-                         "served, rk3, rk3run, merge, bandwidth")
+                         "served, rk3, rk3run, merge, bandwidth; ensemble_full (every body to "
+                         "its own step count, several minutes) only when named")
     ap.add_argument("--rk3-steps", type=int, default=2)
     ap.add_argument("--rk3-fmm-steps", type=int, default=10)
     ap.add_argument("--rk3-run-steps", type=int, default=4000)
@@ -775,8 +796,18 @@ def main():
                     help="steps per body in the ensemble phase (0: each body's full count)")
     ap.add_argument("--ensemble-plain", type=int, default=8)
     ap.add_argument("--ensemble-plain-ref", type=int, default=8)
+    ap.add_argument("--ensemble-full-plain", type=int, default=8,
+                    help="ensemble_full phase: bodies of the plain FP64 subset (device stream)")
+    ap.add_argument("--ensemble-full-plain-ref", type=int, default=4,
+                    help="ensemble_full phase: bodies of the plain FP64 subset (reference stream)")
+    ap.add_argument("--ensemble-fmm-min", type=int, default=65536,
+                    help="ensemble_full phase: slots of at least this many blobs take the fast "
+                         "multipole sum in the second batched run (0: no second run)")
     ap.add_argument("--served-count", type=int, default=8)
     ap.add_argument("--served-steps", default="500,3000")
+    ap.add_argument("--served-fmm-min", type=int, default=65536,
+                    help="a second served run in which slots of at least this many blobs take "
+                         "the fast multipole sum (0: none)")
     args = ap.parse_args()
     phases = set(args.phases.split(","))
 
@@ -891,9 +922,17 @@ def main():
     if "ensemble" in phases:
         line["ensemble"] = ensemble_phase(ctx, args.ensemble_count, args.ensemble_window,
                                           args.ensemble_plain, args.ensemble_plain_ref)
+    if "ensemble_full" in phases:  # several minutes: not in the default list
+        line["ensemble_full"] = ensemble_phase(ctx, args.ensemble_count, 0,
+                                               args.ensemble_full_plain,
+                                               args.ensemble_full_plain_ref,
+                                               fmm_min=args.ensemble_fmm_min)
     if "served" in phases:
         lo, hi = (int(v) for v in args.served_steps.split(","))
         line["served"] = served_phase(ctx, args.served_count, 20.0, (lo, hi), slots=32)
+        if args.served_fmm_min:
+            line["served_fmm_slots"] = served_phase(ctx, args.served_count, 20.0, (lo, hi),
+                                                    slots=32, fmm_min=args.served_fmm_min)
     if "bandwidth" in phases:
         line["bandwidth_kernels"] = bandwidth_phase(ctx)
     if "merge" in phases:

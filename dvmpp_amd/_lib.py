@@ -167,6 +167,7 @@ SIGNATURES.update({
     "dvm_batch_admit_many": (_st, [_vp, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dvm_batch_set_state": (_st, [_vp, _int, _i64, _vp, _vp, _vp, _vp]),
     "dvm_batch_sort_order": (_st, [_vp, _int, _i64, _vp]),
+    "dvm_batch_set_fmm": (_st, [_vp, _i64, _int, _dbl]),
     "dvm_batch_get_state": (_st, [_vp, _int] + [_vp] * 6),
 })
 

@@ -105,6 +105,12 @@ class Batch:
         check(self.ctx.h, load().dvm_batch_set_state(
             self.h, slot, arrs[0].size, *[ctypes.c_void_p(a.ctypes.data) for a in arrs]))
 
+    def set_fmm(self, min_blobs, order=12, theta=0.5):
+        """Slots of at least min_blobs blobs take each step's blob-on-blob sum
+        by the fast multipole form (order, theta); 0 turns it off."""
+        check(self.ctx.h, load().dvm_batch_set_fmm(self.h, int(min_blobs), int(order),
+                                                    float(theta)))
+
     def sort_order(self, slot, n):
         """Storage indices of the slot's first n blobs in its last FP32C pair
         sum's Morton order (n = the slot's count when that sum ran)."""

@@ -36,6 +36,7 @@ constexpr int kSortTile = 4096; // keys per sort workgroup (256 lanes x 16)
 constexpr int kBoxParts = 32;   // workgroups per slot of the bounding-box pass
 constexpr int kXcds = 8;        // accelerator dies (XCDs) of the device
 constexpr int kAdmitStreams = 16;  // bodies set up at once by dvm_batch_admit_many
+constexpr int kFmmStreams = 8;     // fast multipole sums of large slots at once
 
 struct SlotParam {
     double rho, nu, dt, ux, uz, ur, aref, four_nu_dt;
@@ -56,6 +57,8 @@ struct SlotState {
     int status;      // dvm_status of the failure that stopped the slot, else 0
     int nrv;         // nascent blobs of the current step
     double loads[4]; // fx fz cd cl of the last step
+    int ext;         // 1: this step's blob velocities were summed before the
+                     // convection graph (fast multipole); its pair sum skips the slot
 };
 
 // Work lists rebuilt by bt_prefix_kernel at every velocity evaluation.
@@ -603,6 +606,7 @@ __global__ __launch_bounds__(256) void bt_bs32_kernel(int S, int64_t cap, int ma
         if (item >= total) break;
         const int p = find_slot(item_pre, S, item);
         const int s = order[p];
+        if (st[s].ext) continue;  // velocities already summed (dvm_batch_set_fmm)
         const int local = item - item_pre[p];
         const int64_t n = st[s].n;
         const int nblk = (int)((n + kTB - 1) / kTB);
@@ -684,6 +688,7 @@ __global__ void bt_reduce32_kernel(int S, int64_t cap, const WorkList* wl,
                                    const double* __restrict__ part, BlobBuf b)
 {
     for_each_blob(wl, blob_pre, S, cap, [&](int s, int64_t o) {
+        if (st[s].ext) return;  // velocities already summed (dvm_batch_set_fmm)
         const int64_t n = st[s].n;
         const int nch = slot_chunks(n);
         const double* __restrict__ ps = part + (int64_t)s * part_slot + (o - (int64_t)s * cap);
@@ -1274,6 +1279,16 @@ struct dvm_batch_s {
     hipGraph_t graph_a = nullptr, graph_b = nullptr;
     hipGraphExec_t exec_a = nullptr, exec_b = nullptr;
     std::vector<hipStream_t> admit_streams;  // set-up of several bodies at once
+    // dvm_batch_set_fmm: slots of at least fmm_min blobs take the fast
+    // multipole sum (fmm_order, fmm_theta) before the convection graph
+    int64_t fmm_min = 0;
+    int fmm_order = 12;
+    double fmm_theta = 0.5;
+    SlotState* pst = nullptr;  // pinned staging of the slot states (S)
+    // contexts (stream, arenas) of the fast multipole sums: the slots' sums
+    // run on them in turn, concurrently; the main stream waits on fmm_ev
+    std::vector<dvm_ctx_t> fmm_ctx;
+    std::vector<hipEvent_t> fmm_ev;
 };
 
 namespace {
@@ -1526,6 +1541,44 @@ void remesh_due(dvm_batch_t B)
     }
 }
 
+// The blob velocities of the slots with at least fmm_min blobs by the fast
+// multipole sum (fmm.hip, on the slot's own arrays), enqueued before the
+// convection graph on the batch's FMM contexts in turn (their streams run
+// concurrently; the main stream waits for them); the ext flags make the
+// graph's pair sum and reduce skip those slots.  The choice depends on the
+// slot's own count, so a slot's result is the same alone or in any mix.
+// Euler batches only (checked when set).
+void fmm_due(dvm_batch_t B)
+{
+    if (B->fmm_min <= 0) return;
+    poll_states(B);  // the main stream is idle: positions and counts are final
+    const int S = B->d.S;
+    const int K = (int)B->fmm_ctx.size();
+    std::vector<char> used((size_t)K, 0);
+    int j = 0;
+    for (int s = 0; s < S; ++s) {
+        SlotState ss = B->hst[s];
+        ss.ext = 0;
+        if (B->busy[s] && ss.active && ss.n >= B->fmm_min) {
+            const int k = j++ % K;
+            const int64_t off = (int64_t)s * B->d.cap;
+            const BlobBuf& b = B->d.b;
+            used[(size_t)k] = 1;
+            ss.ext = fmm_self(B->fmm_ctx[(size_t)k], ss.n, b.x + off, b.z + off, b.c + off,
+                              b.s + off, b.u + off, b.w + off) ? 1 : 0;
+        }
+        B->pst[s] = ss;
+    }
+    for (int k = 0; k < K; ++k) {
+        if (!used[(size_t)k]) continue;
+        DVM_HIP_CHECK(hipEventRecord(B->fmm_ev[(size_t)k], B->fmm_ctx[(size_t)k]->stream));
+        DVM_HIP_CHECK(hipStreamWaitEvent(B->ctx->stream, B->fmm_ev[(size_t)k], 0));
+    }
+    // the staging is rewritten only after the next poll has synchronised
+    DVM_HIP_CHECK(hipMemcpyAsync(B->d.st, B->pst, sizeof(SlotState) * S, hipMemcpyHostToDevice,
+                                 B->ctx->stream));
+}
+
 }  // namespace
 
 // Enqueues on `st` the copies that place a set-up body `sh` in slot s and
@@ -1646,6 +1699,11 @@ dvm_status dvm_batch_destroy(dvm_batch_t B)
         if (B->graph_a) DVM_HIP_CHECK(hipGraphDestroy(B->graph_a));
         if (B->graph_b) DVM_HIP_CHECK(hipGraphDestroy(B->graph_b));
         if (B->mem) DVM_HIP_CHECK(hipFree(B->mem));
+        if (B->pst) DVM_HIP_CHECK(hipHostFree(B->pst));
+        for (dvm_ctx_t c : B->fmm_ctx) (void)dvm_ctx_destroy(c);  // waits for its stream
+        B->fmm_ctx.clear();
+        for (hipEvent_t e : B->fmm_ev) DVM_HIP_CHECK(hipEventDestroy(e));
+        B->fmm_ev.clear();
         for (hipStream_t a : B->admit_streams) DVM_HIP_CHECK(hipStreamDestroy(a));
         B->admit_streams.clear();
     });
@@ -1730,11 +1788,13 @@ dvm_status dvm_batch_step(dvm_batch_t B, int steps)
         for (int k = 0; k < steps; ++k) {
             if (B->cfg.use_graph) {
                 ProfScope ps(B->ctx, "batch_step");
+                fmm_due(B);
                 DVM_HIP_CHECK(hipGraphLaunch(B->exec_a, B->ctx->stream));
                 remesh_due(B);
                 DVM_HIP_CHECK(hipGraphLaunch(B->exec_b, B->ctx->stream));
             } else {
                 ProfScope ps(B->ctx, "batch_step");
+                fmm_due(B);
                 record_convect(B);
                 remesh_due(B);
                 record_rest(B);
@@ -1844,6 +1904,48 @@ dvm_status dvm_batch_set_state(dvm_batch_t B, int slot, int64_t n, const double*
         DVM_HIP_CHECK(hipMemcpyAsync(B->d.st + slot, &ss, sizeof(ss), hipMemcpyHostToDevice, st));
         DVM_HIP_CHECK(hipStreamSynchronize(st));
         B->hst[slot] = ss;
+    });
+}
+
+dvm_status dvm_batch_set_fmm(dvm_batch_t B, int64_t min_blobs, int expansion_order, double theta)
+{
+    if (!B) return DVM_E_INVALID;
+    return guard(B->ctx, [&] {
+        require(min_blobs >= 0, DVM_E_INVALID, "min_blobs must be >= 0");
+        if (min_blobs > 0) {
+            require(B->cfg.prec == DVM_F32C, DVM_E_INVALID,
+                    "the fast multipole slots need an FP32C batch");
+            require(B->cfg.scheme == DVM_EULER, DVM_E_INVALID,
+                    "the fast multipole slots need an Euler batch (one velocity sum per step)");
+            require(expansion_order >= 1 && fmm_terms(expansion_order) > 0, DVM_E_INVALID,
+                    "FMM expansion order must be 1..32");
+            require(theta > 0.2 && theta <= 1.0, DVM_E_INVALID, "FMM theta must be in (0.2, 1]");
+            if (!B->pst)
+                DVM_HIP_CHECK(hipHostMalloc((void**)&B->pst, sizeof(SlotState) * B->d.S,
+                                            hipHostMallocDefault));
+            while ((int)B->fmm_ctx.size() < kFmmStreams) {
+                dvm_ctx_t c = nullptr;
+                const dvm_status cs = dvm_ctx_create(B->ctx->device, nullptr, &c);
+                require(cs == DVM_OK, cs, "cannot create a context for the fast multipole sums");
+                B->fmm_ctx.push_back(c);
+                hipEvent_t e = nullptr;
+                DVM_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                B->fmm_ev.push_back(e);
+            }
+            for (dvm_ctx_t c : B->fmm_ctx) {
+                c->fmm_order = expansion_order;
+                c->fmm_theta = theta;
+            }
+        }
+        // no slot keeps a flag from an earlier setting
+        poll_states(B);
+        for (SlotState& ss : B->hst) ss.ext = 0;
+        DVM_HIP_CHECK(hipMemcpyAsync(B->d.st, B->hst.data(), sizeof(SlotState) * B->d.S,
+                                     hipMemcpyHostToDevice, B->ctx->stream));
+        DVM_HIP_CHECK(hipStreamSynchronize(B->ctx->stream));
+        B->fmm_min = min_blobs;
+        B->fmm_order = expansion_order;
+        B->fmm_theta = theta;
     });
 }
 

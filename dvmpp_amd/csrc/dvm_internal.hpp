@@ -299,6 +299,9 @@ size_t merge_scratch_bytes(int64_t n);
 int64_t merge_run(dvm_ctx_t ctx, double threshold, int64_t n, const BlobBuf& b, const BlobBuf& out,
                   void* scratch);
 
+// The FMM expansion order used for a requested one (0: unsupported).
+int fmm_terms(int order);
+
 // DVM_FMM blob-on-blob sum (fmm.hip); false when it declined (the caller then
 // runs the direct FP64 sum).
 // With nt > 0 it also writes the sums at targets (tx, tz) into (tu, tw),

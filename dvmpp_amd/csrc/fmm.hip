@@ -740,12 +740,20 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
             dmin = std::min(dmin, std::sqrt(ax * ax + az * az));
         }
 
+    // pinned host memory of this call: the partials read back, then the
+    // interaction tables copied to the device (written after the read-back's
+    // synchronisation, so the previous call's copies on this stream are done)
+    const size_t il_stride = std::max({il[0].size(), il[1].size(), il[2].size(), il[3].size()}) / 2;
+    const size_t rb_bytes = (sizeof(double) * (6 * kRedBlocks + 2 * nt) + 255) & ~size_t(255);
+    const size_t tab_ints = 8 * il_stride + nearo.size();
+    char* pin = static_cast<char*>(pinned_get(ctx, rb_bytes + sizeof(int) * tab_ints));
+
     // extent, largest core, non-finite entries
     double* red = static_cast<double*>(ws_get(ctx, sizeof(double) * 6 * kRedBlocks));
     fmm_bbox_kernel<<<kRedBlocks, 256, 0, st>>>((int)n, x, z, s, c, red);
     DVM_LAUNCH_CHECK();
     // (targets are few: their coordinates come to the host with the partials)
-    double* hr = static_cast<double*>(pinned_get(ctx, sizeof(double) * (6 * kRedBlocks + 2 * nt)));
+    double* hr = reinterpret_cast<double*>(pin);
     DVM_HIP_CHECK(hipMemcpyAsync(hr, red, sizeof(double) * 6 * kRedBlocks, hipMemcpyDeviceToHost, st));
     if (nt > 0) {
         DVM_HIP_CHECK(hipMemcpyAsync(hr + 6 * kRedBlocks, tx, sizeof(double) * nt,
@@ -791,7 +799,6 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
                                                      (uint32_t*)nullptr, (int*)nullptr, (int*)nullptr,
                                                      (int)n, 0, 2 * L, st));
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    const size_t il_stride = std::max({il[0].size(), il[1].size(), il[2].size(), il[3].size()}) / 2;
     const size_t need = al(sizeof(double) * kBinom * kBinom) + al(sizeof(int) * 8 * il_stride) +
                         al(sizeof(int) * nearo.size()) + 2 * al(sizeof(uint32_t) * n) +
                         2 * al(sizeof(int) * n) + al(sizeof(double4) * n) + al(sizeof(int) * (nleaf + 1)) +
@@ -819,15 +826,19 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
     void* sort_tmp = take(sort_bytes);
     int* heavy = reinterpret_cast<int*>(take(sizeof(int) * (n + 1)));
 
-    // tables: interaction lists padded to il_stride entries per child position
-    std::vector<int> ilh(8 * il_stride, 0);
+    // tables: interaction lists padded to il_stride entries per child position,
+    // then the near offsets, staged in the pinned buffer
+    int* ilh = reinterpret_cast<int*>(pin + rb_bytes);
+    std::fill(ilh, ilh + 8 * il_stride, 0);
     FmmTree t{};
     for (int pos = 0; pos < 4; ++pos) {
-        std::copy(il[pos].begin(), il[pos].end(), ilh.begin() + pos * 2 * il_stride);
+        std::copy(il[pos].begin(), il[pos].end(), ilh + pos * 2 * il_stride);
         t.il_n[pos] = (int)(il[pos].size() / 2);
     }
-    DVM_HIP_CHECK(hipMemcpyAsync(ild, ilh.data(), sizeof(int) * ilh.size(), hipMemcpyHostToDevice, st));
-    DVM_HIP_CHECK(hipMemcpyAsync(neard, nearo.data(), sizeof(int) * nearo.size(), hipMemcpyHostToDevice, st));
+    int* nearh = ilh + 8 * il_stride;
+    std::copy(nearo.begin(), nearo.end(), nearh);
+    DVM_HIP_CHECK(hipMemcpyAsync(ild, ilh, sizeof(int) * 8 * il_stride, hipMemcpyHostToDevice, st));
+    DVM_HIP_CHECK(hipMemcpyAsync(neard, nearh, sizeof(int) * nearo.size(), hipMemcpyHostToDevice, st));
     fmm_binom_kernel<<<1, kBinom, 0, st>>>(binom);
     DVM_LAUNCH_CHECK();
 
@@ -870,8 +881,8 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
     case 24: fmm_levels<24>(ctx, t, (int)n, sp, skey, perm, u, w, (int)nt, tx, tz, tu, tw); break;
     default: fmm_levels<32>(ctx, t, (int)n, sp, skey, perm, u, w, (int)nt, tx, tz, tu, tw); break;
     }
-    // the host tables must outlive the copies that read them
-    DVM_HIP_CHECK(hipStreamSynchronize(st));
+    // no wait here: the tables stay in the pinned buffer, which the next call
+    // on this context rewrites only after its read-back has synchronised
     return true;
 }
 

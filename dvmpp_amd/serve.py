@@ -55,11 +55,16 @@ class Server:
     """`cfg` is the template of every batch (slots per batch, largest slot
     capacity, body size limit, scheme, crossing, arithmetic, graph replay,
     remesh); `classes` lists the slot capacities (default: the template's
-    alone).  `budget_bytes` bounds the device bytes of all live batches."""
+    alone).  `budget_bytes` bounds the device bytes of all live batches.
+    With fmm_min > 0 (FP32C Euler batches) every batch gives slots of at least
+    fmm_min blobs the fast multipole sum (dvm_batch_set_fmm, fmm_order,
+    fmm_theta); a request's result still depends on itself alone."""
 
-    def __init__(self, ctx, cfg, budget_bytes, classes=None, steps_per_poll=1, keep_state=False):
+    def __init__(self, ctx, cfg, budget_bytes, classes=None, steps_per_poll=1, keep_state=False,
+                 fmm_min=0, fmm_order=12, fmm_theta=0.5):
         self.ctx = ctx
         self.cfg = cfg
+        self.fmm = (fmm_min, fmm_order, fmm_theta)
         self.classes = sorted(classes) if classes else [int(cfg.slot_capacity)]
         self.class_bytes = {c: footprint(_class_config(cfg, c)) for c in self.classes}
         self.budget = budget_bytes
@@ -113,6 +118,8 @@ class Server:
                 return entry
         if self.allocated_bytes() + self.class_bytes[c] <= self.budget:
             entry = [c, Batch(self.ctx, _class_config(self.cfg, c)), {}]
+            if self.fmm[0] > 0:
+                entry[1].set_fmm(*self.fmm)
             self.batches.append(entry)
             self.peak_bytes = max(self.peak_bytes, self.allocated_bytes())
             return entry

@@ -500,6 +500,17 @@ dvm_status dvm_batch_set_state(dvm_batch_t batch, int slot, int64_t n, const dou
  * slot's blob order in its last FP32C pair sum (storage indices sorted by
  * Morton key, stable), n = the slot's count when that sum ran. */
 dvm_status dvm_batch_sort_order(dvm_batch_t batch, int slot, int64_t n, uint32_t* order);
+/* Slots holding at least min_blobs blobs at the start of a step take that
+ * step's blob-on-blob sum by the fast multipole form of DVM_FMM (the
+ * reference's FMMKernel, src/VelocityKernel.hpp:84-110, with this expansion
+ * order and opening criterion) on their own arrays, before the step's
+ * graphs; the others keep the batch's FP32C direct sum.  The choice depends
+ * on a slot's own count, so its result is the same alone or in any mix.
+ * min_blobs 0 turns it off.  FP32C Euler batches only (DVM_E_INVALID
+ * otherwise: an RK3 step sums three times inside one graph).  No counterpart
+ * in the reference, which runs one simulation with one velocity kernel. */
+dvm_status dvm_batch_set_fmm(dvm_batch_t batch, int64_t min_blobs, int expansion_order,
+                             double theta);
 
 #ifdef __cplusplus
 }

@@ -279,3 +279,58 @@ def test_sort_order_is_stable_morton_order(gpu_ctx):
         got = b.sort_order(slot, x.size)
         assert np.array_equal(got, _morton_order(x, z)), slot
     b.close()
+
+
+def _fmm_batch(ctx, with_small, fmm_min, steps=1):
+    """A cylinder slot started from a 40K-blob wake (Euler, FP32C), beside two
+    small bodies when with_small; fmm_min > 0 gives slots of that many blobs
+    the fast multipole sum (order 16)."""
+    from dvmpp_amd.workloads import rk3_cylinder, wake_snapshot
+    px, pz, sp, sig = rk3_cylinder()
+    snap = wake_snapshot(n_target=40_000, sigma=sig)
+    b = Batch(ctx, batch_config(slots=4, slot_capacity=1 << 17, max_panels=512,
+                                scheme=dsim.EULER, prec=dvm.F32C))
+    slots = []
+    if with_small:
+        slots += [b.admit(cpx, cpz, dsim.sheet_params(**kw), seed, 6)
+                  for (cpx, cpz), kw, seed, _ in CASES[:2]]
+    big = b.admit(px, pz, sp, 7, 6)
+    b.set_state(big, snap["x"], snap["z"], snap["circ"], snap["sigma"])
+    if fmm_min:
+        b.set_fmm(fmm_min, order=16, theta=0.5)
+    b.step(steps)
+    out = {s: b.state(s) for s in slots + [big]}
+    b.close()
+    return out, slots, big, snap["x"].size
+
+
+def test_fmm_slots(gpu_ctx):
+    """dvm_batch_set_fmm: the large slot's velocities come from the fast
+    multipole sum (within the FP32C form's 1e-5 normwise of the FP32C direct
+    sum's at the same state), the small slots keep the FP32C sum bit for bit, and the large
+    slot's result is the same alone and in the mix."""
+    on, small, big, n0 = _fmm_batch(gpu_ctx, True, 20_000)
+    off, _, _, _ = _fmm_batch(gpu_ctx, True, 0)
+    for s in small:
+        for f in ("x", "z", "circ", "sigma", "u", "w"):
+            assert np.array_equal(on[s][f], off[s][f]), (s, f)
+    vu = np.hypot(off[big]["u"][:n0], off[big]["w"][:n0])
+    d = np.hypot(on[big]["u"][:n0] - off[big]["u"][:n0], on[big]["w"][:n0] - off[big]["w"][:n0])
+    assert np.max(d) <= 1e-5 * np.max(vu), np.max(d) / np.max(vu)
+    alone, _, big_alone, _ = _fmm_batch(gpu_ctx, False, 20_000)
+    for f in ("x", "z", "circ", "sigma", "u", "w"):
+        assert np.array_equal(alone[big_alone][f], on[big][f]), f
+    # a second step (the slot has grown) runs as well
+    two, _, big2, _ = _fmm_batch(gpu_ctx, False, 20_000, steps=2)
+    assert two[big2]["x"].size > on[big]["x"].size
+    assert np.all(np.isfinite(two[big2]["u"])) and np.all(np.isfinite(two[big2]["w"]))
+
+
+def test_fmm_slots_need_euler_f32c(gpu_ctx):
+    for scheme, prec in ((dsim.RK3, dvm.F32C), (dsim.EULER, dvm.F64)):
+        b = Batch(gpu_ctx, batch_config(slots=2, slot_capacity=1 << 12, max_panels=64,
+                                        scheme=scheme, prec=prec))
+        with pytest.raises(dvm.DvmError):
+            b.set_fmm(1000)
+        b.set_fmm(0)  # off is always accepted
+        b.close()

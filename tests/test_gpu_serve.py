@@ -18,23 +18,28 @@ def make_requests(count=6):
     return [Request(i, t, px, pz, sp, seed, st) for i, (t, (px, pz, sp, seed, st)) in enumerate(trace)]
 
 
-def serve(ctx, reqs, slots, budget=200 << 30):
+def serve(ctx, reqs, slots, budget=200 << 30, fmm_min=0):
     srv = Server(ctx, batch_config(slots=slots, slot_capacity=1 << 14, max_panels=64,
-                                   prec=dvm.F32C), budget_bytes=budget, keep_state=True)
+                                   prec=dvm.F32C), budget_bytes=budget, keep_state=True,
+                 fmm_min=fmm_min, fmm_order=16)
     done, wall = srv.run(reqs)
     srv.close()
     return {r.rid: r for r in done}, wall
 
 
-def test_served_equals_alone(gpu_ctx):
-    mixed, _ = serve(gpu_ctx, make_requests(), slots=4)
+@pytest.mark.parametrize("fmm_min", [0, 600])
+def test_served_equals_alone(gpu_ctx, fmm_min):
+    """fmm_min 600: once a request's slot passes 600 blobs (mid-run) its sum
+    goes to the fast multipole form (or, where the tree declines such a small
+    set, the direct sum): the result is still the request's alone."""
+    mixed, _ = serve(gpu_ctx, make_requests(), slots=4, fmm_min=fmm_min)
     assert len(mixed) == 6 and all(r.status == 0 for r in mixed.values())
     # requests overlapped: some were admitted before others finished
     spans = sorted((r.admitted, r.finished) for r in mixed.values())
     assert any(spans[k + 1][0] < spans[k][1] for k in range(len(spans) - 1))
     for req in make_requests():
         alone, _ = serve(gpu_ctx, [Request(req.rid, 0.0, req.px, req.pz, req.params, req.seed,
-                                           req.steps)], slots=1)
+                                           req.steps)], slots=1, fmm_min=fmm_min)
         a, m = alone[req.rid].state, mixed[req.rid].state
         for f in ("x", "z", "circ", "sigma", "u", "w"):
             assert np.array_equal(a[f], m[f]), (req.rid, f)

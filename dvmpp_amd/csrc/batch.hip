@@ -1555,25 +1555,41 @@ void fmm_due(dvm_batch_t B)
     const int S = B->d.S;
     const int K = (int)B->fmm_ctx.size();
     std::vector<char> used((size_t)K, 0);
-    int j = 0;
-    for (int s = 0; s < S; ++s) {
-        SlotState ss = B->hst[s];
-        ss.ext = 0;
-        if (B->busy[s] && ss.active && ss.n >= B->fmm_min) {
-            const int k = j++ % K;
-            const int64_t off = (int64_t)s * B->d.cap;
-            const BlobBuf& b = B->d.b;
-            used[(size_t)k] = 1;
-            ss.ext = fmm_self(B->fmm_ctx[(size_t)k], ss.n, b.x + off, b.z + off, b.c + off,
-                              b.s + off, b.u + off, b.w + off) ? 1 : 0;
+    auto join = [&] {  // the main stream waits for the contexts that ran sums
+        for (int k = 0; k < K; ++k) {
+            if (!used[(size_t)k]) continue;
+            DVM_HIP_CHECK(hipEventRecord(B->fmm_ev[(size_t)k], B->fmm_ctx[(size_t)k]->stream));
+            DVM_HIP_CHECK(hipStreamWaitEvent(B->ctx->stream, B->fmm_ev[(size_t)k], 0));
         }
-        B->pst[s] = ss;
+    };
+    int j = 0;
+    try {
+        for (int s = 0; s < S; ++s) {
+            SlotState ss = B->hst[s];
+            ss.ext = 0;
+            if (B->busy[s] && ss.active && ss.n >= B->fmm_min) {
+                const int k = j++ % K;
+                const int64_t off = (int64_t)s * B->d.cap;
+                const BlobBuf& b = B->d.b;
+                used[(size_t)k] = 1;
+                ss.ext = fmm_self(B->fmm_ctx[(size_t)k], ss.n, b.x + off, b.z + off, b.c + off,
+                                  b.s + off, b.u + off, b.w + off) ? 1 : 0;
+            }
+            B->pst[s] = ss;
+        }
+    } catch (...) {
+        // no slot keeps a flag without this step's sum behind it
+        for (int s = 0; s < S; ++s) {
+            B->pst[s] = B->hst[s];
+            B->pst[s].ext = 0;
+        }
+        join();
+        DVM_HIP_CHECK(hipMemcpyAsync(B->d.st, B->pst, sizeof(SlotState) * S,
+                                     hipMemcpyHostToDevice, B->ctx->stream));
+        DVM_HIP_CHECK(hipStreamSynchronize(B->ctx->stream));
+        throw;
     }
-    for (int k = 0; k < K; ++k) {
-        if (!used[(size_t)k]) continue;
-        DVM_HIP_CHECK(hipEventRecord(B->fmm_ev[(size_t)k], B->fmm_ctx[(size_t)k]->stream));
-        DVM_HIP_CHECK(hipStreamWaitEvent(B->ctx->stream, B->fmm_ev[(size_t)k], 0));
-    }
+    join();
     // the staging is rewritten only after the next poll has synchronised
     DVM_HIP_CHECK(hipMemcpyAsync(B->d.st, B->pst, sizeof(SlotState) * S, hipMemcpyHostToDevice,
                                  B->ctx->stream));

@@ -17,8 +17,9 @@ device-resident; results are read back once, on completion.
 import time
 from dataclasses import dataclass, field
 
-from ._lib import BatchConfig
+from ._lib import F32C, BatchConfig
 from .batch import Batch, footprint
+from .sim import EULER
 from .workloads import projected_blobs
 
 STATUS_REJECTED = 2  # DVM_E_SIZE: the request cannot fit any class or the budget
@@ -64,6 +65,8 @@ class Server:
                  fmm_min=0, fmm_order=12, fmm_theta=0.5):
         self.ctx = ctx
         self.cfg = cfg
+        if fmm_min > 0 and (cfg.prec != F32C or cfg.scheme != EULER):
+            raise ValueError("fast multipole slots need an FP32C Euler batch template")
         self.fmm = (fmm_min, fmm_order, fmm_theta)
         self.classes = sorted(classes) if classes else [int(cfg.slot_capacity)]
         self.class_bytes = {c: footprint(_class_config(cfg, c)) for c in self.classes}

@@ -98,3 +98,18 @@ def test_server_rejects_what_cannot_fit_without_gpu():
     assert len(done) == 4 and all(r.status == STATUS_REJECTED for r in done)
     assert not srv.batches and wall < 1.0
     assert np.isfinite(wall)
+
+
+def test_server_fmm_slots_need_euler_f32c_without_gpu():
+    """Fast multipole slots are refused for a template the batch cannot run
+    them in (FP64 pair sums, or RK3 with three sums inside one graph)."""
+    import pytest
+
+    import dvmpp_amd as dvm
+    from dvmpp_amd import sim
+    from dvmpp_amd.batch import batch_config
+    from dvmpp_amd.serve import Server
+    for cfg in (batch_config(prec=dvm.F64), batch_config(prec=dvm.F32C, scheme=sim.RK3)):
+        with pytest.raises(ValueError):
+            Server(None, cfg, budget_bytes=1 << 30, fmm_min=1000)
+    Server(None, batch_config(prec=dvm.F32C), budget_bytes=1 << 30, fmm_min=1000)

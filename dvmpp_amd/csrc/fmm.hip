@@ -133,6 +133,45 @@ __global__ void fmm_key_kernel(int n, const double* __restrict__ x, const double
     idx[i] = i;
 }
 
+// Occupancy of the leaves at level lm: one bit per leaf, in Morton order.
+__global__ void fmm_occ_kernel(int n, const double* __restrict__ x, const double* __restrict__ z,
+                               double x0, double z0, double sl, int nb,
+                               unsigned long long* __restrict__ bm)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int ix = min(max((int)floor((x[i] - x0) / sl), 0), nb - 1);
+    const int iz = min(max((int)floor((z[i] - z0) / sl), 0), nb - 1);
+    const uint32_t code = morton((uint32_t)ix, (uint32_t)iz);
+    atomicOr(&bm[code >> 6], 1ull << (code & 63));
+}
+
+// Occupied boxes of level lm - blockIdx.y from the level-lm bitmap: a box of
+// level l covers 4^(lm - l) consecutive leaf codes.
+__global__ void fmm_occ_count_kernel(const unsigned long long* __restrict__ bm, int lm,
+                                     int* __restrict__ cnt)
+{
+    const int y = blockIdx.y;
+    const int64_t ngroups = int64_t(1) << (2 * (lm - y));
+    const int64_t gbits = int64_t(1) << (2 * y);
+    int local = 0;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ngroups;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        bool any = false;
+        if (gbits <= 64) {
+            const int64_t b0 = g * gbits;
+            const unsigned long long m =
+                gbits == 64 ? ~0ull : ((1ull << gbits) - 1) << (b0 & 63);
+            any = (bm[b0 >> 6] & m) != 0;
+        } else {
+            for (int64_t w = g * gbits / 64; w < (g + 1) * gbits / 64 && !any; ++w) any = bm[w] != 0;
+        }
+        local += any ? 1 : 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o);
+    if ((threadIdx.x & 63) == 0 && local) atomicAdd(&cnt[y], local);
+}
+
 // Sorted particles as (x, z, circulation, 1 / sigma^2).
 __global__ void fmm_gather_kernel(int n, const int* __restrict__ perm, const double* __restrict__ x,
                                   const double* __restrict__ z, const double* __restrict__ c,
@@ -789,6 +828,34 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
     while (L < kFmmMaxLevel && (double)n / (double)(int64_t(1) << (2 * (L + 1))) >= kLeafMean) ++L;
     const double smin = std::sqrt(10.0) * smax * (1.0 + 1e-9) / dmin;
     while (L > 0 && S / (double)(1 << L) < smin) --L;
+    int Ls = L;  // deepest level the core-size rule allows
+    while (Ls < kFmmMaxLevel && S / (double)(1 << (Ls + 1)) >= smin) ++Ls;
+    if (Ls > L && L >= 1) {
+        // A clustered set fills few of the box's leaves: the mean over the box
+        // understates the blobs per occupied leaf.  Count the occupied boxes of
+        // the deeper levels the core rule allows and take the deepest whose
+        // occupied leaves still hold kLeafMean blobs on average.
+        const int64_t words = (int64_t(1) << (2 * Ls)) / 64 > 0 ? (int64_t(1) << (2 * Ls)) / 64 : 1;
+        const int ny = Ls - L;  // levels Ls, Ls - 1, ..., L + 1
+        char* occ = static_cast<char*>(ws_get(ctx, sizeof(unsigned long long) * words + 256));
+        unsigned long long* bm = reinterpret_cast<unsigned long long*>(occ);
+        int* ocnt = reinterpret_cast<int*>(occ + sizeof(unsigned long long) * words);
+        DVM_HIP_CHECK(hipMemsetAsync(occ, 0, sizeof(unsigned long long) * words + 256, st));
+        fmm_occ_kernel<<<cdiv(n, 256), 256, 0, st>>>((int)n, x, z, x0, z0, S / (double)(1 << Ls),
+                                                      1 << Ls, bm);
+        DVM_LAUNCH_CHECK();
+        const unsigned gx = (unsigned)std::min<int64_t>(cdiv(int64_t(1) << (2 * Ls), 256), 1024);
+        fmm_occ_count_kernel<<<dim3(gx, ny), 256, 0, st>>>(bm, Ls, ocnt);
+        DVM_LAUNCH_CHECK();
+        int* hc = reinterpret_cast<int*>(pin);  // the partials are read already
+        DVM_HIP_CHECK(hipMemcpyAsync(hc, ocnt, sizeof(int) * ny, hipMemcpyDeviceToHost, st));
+        DVM_HIP_CHECK(hipStreamSynchronize(st));
+        for (int y = 0; y < ny; ++y)
+            if (hc[y] > 0 && (double)n / (double)hc[y] >= kLeafMean) {
+                L = Ls - y;
+                break;
+            }
+    }
     if (L < 2) return false;  // no well-separated boxes: direct sum
     ctx->fmm_last_level = L;
 

@@ -5,8 +5,10 @@
 //  DVM_F64  (parity form) — one target per lane, sources streamed with
 //           wave-uniform addresses so they arrive through the scalar cache
 //           into SGPRs; the per-target loop visits sources in index order and
-//           evaluates each pair exactly as src/VelocityKernel.cpp:42-61 does
-//           (FP64, correctly rounded divide, no contraction).  When there are
+//           evaluates each pair as src/VelocityKernel.cpp:42-61 does, in FP64
+//           with the reference's near/far test exactly, 1/r^2 within a few
+//           units in the last place (one Newton-refined reciprocal shared by
+//           four pairs) and the sums as fused multiply-adds.  When there are
 //           too few targets to fill 256 CUs the source range is split into
 //           chunks whose partial sums are added in chunk order.
 //
@@ -74,11 +76,12 @@ struct Rec64x4 {
 // One target per lane; sources in index order, four at a time, through
 // wave-uniform (scalar) loads with the next group's records in flight while
 // the current group is summed.  The pair is that of
-// src/VelocityKernel.cpp:44-60 with K = num * (1/r^2) in place of num / r^2
-// and the two accumulations as fused multiply-adds: each pair term is within
-// a few units in the last place of the reference's; the far pairs of a group
-// run without branches and only a group with a near pair in some lane enters
-// the near-field branch.
+// src/VelocityKernel.cpp:44-60 with K = num * (1/r^2) in place of num / r^2,
+// the group's four 1/r^2 from one reciprocal, and the two accumulations as
+// fused multiply-adds: each pair term is within a few units in the last
+// place of the reference's; the far pairs of a group run without branches
+// and only a group with a near pair in some lane enters the near-field
+// branch.
 template <bool SELF, bool GUARD>
 __global__ __launch_bounds__(kBlock) void bs_f64_kernel(
     int64_t nt, const double* __restrict__ tx, const double* __restrict__ tz, int64_t ns,

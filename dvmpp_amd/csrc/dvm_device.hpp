@@ -184,8 +184,20 @@ __device__ __forceinline__ void sheet_pair(double xi, double zi, const PanelCons
     sw += py * gj + qy * gj1;
 }
 
+// Terms of a group's (or the body's) Laurent series at |W|^2 = w2 > 1 (W the
+// point's offset in units of the radius): with the moments scaled by the
+// radius, term k is at most the leading term times |W|^-k, so the series is
+// cut where |W|^-K <= 2^-43 (e = floor(log2 w2) from the exponent bits), at
+// most kSheetTerms (all of them below |W| = 2.8).
+__device__ __forceinline__ int sheet_terms(double w2)
+{
+    const int e = (int)((__double_as_longlong(w2) >> 52) & 0x7ff) - 1023;
+    const int K = e > 0 ? (86 + e - 1) / e : kSheetTerms;
+    return K < kSheetTerms ? K : kSheetTerms;
+}
+
 // The sheet's velocity at (x, z) by panel groups (sheet_groups): for each
-// group, beyond kSheetFar * r of its centre the Laurent series
+// group, beyond kGroupFar * r of its centre the Laurent series
 // phi = (1/r) sum_k A_k / W^(k+1), W = (z - c) / r (u += Im phi, w += Re phi),
 // else sheet_pair over the group's panels.  pc, gm (gamma), gc (centres) and
 // ga (moments, 2 kSheetTerms per group) are one body's; the choice depends on
@@ -199,12 +211,13 @@ __device__ __forceinline__ void sheet_by_groups(double x, double z, int nl, cons
         const double4 c = gc[g];  // centre, radius, 1 / radius (0: never the series)
         const double wr = (x - c.x) * c.w, wi = (z - c.y) * c.w;
         const double w2 = wr * wr + wi * wi;
-        if (w2 > kSheetFar * kSheetFar) {
+        if (w2 > kGroupFar * kGroupFar) {
             const double* a = ga + 2 * g * kSheetTerms;
             const double iw2 = recip64(w2);
             const double tr = wr * iw2, ti = -wi * iw2;  // 1 / W
-            double pr = a[2 * (kSheetTerms - 1)], pim = a[2 * (kSheetTerms - 1) + 1];
-            for (int k = kSheetTerms - 2; k >= 0; --k) {
+            const int K = sheet_terms(w2);
+            double pr = a[2 * (K - 1)], pim = a[2 * (K - 1) + 1];
+            for (int k = K - 2; k >= 0; --k) {
                 const double nr = pr * tr - pim * ti + a[2 * k];
                 pim = pr * ti + pim * tr + a[2 * k + 1];
                 pr = nr;

@@ -225,11 +225,15 @@ constexpr double kSheetFar = 2.0;
 
 // Panel groups of the batched fast sheet sum: kSheetGroup consecutive panels,
 // with centre c (mean of the segments' midpoints) and radius r (largest
-// distance from c to a segment end).  Beyond kSheetFar * r of c a group's
+// distance from c to a segment end).  Beyond kGroupFar * r of c a group's
 // velocity is the Laurent series above with the group's own moments.
 // sheet_groups fills cen (cx, cz, r, 0 per group; r = inf for a degenerate
 // group, which is always summed panel by panel).
 constexpr int kSheetGroup = 16;
+// The groups' series serve the FP32C batch only (1e-5 velocities): they are
+// taken from kGroupFar * r on, where 40 terms leave at most 1.5^-40 (1e-7)
+// of the group's leading term; the body-wide series keeps kSheetFar.
+constexpr double kGroupFar = 1.5;
 void sheet_groups(const SheetDev& sh, std::vector<double>& cen);
 // Geometric integrals (the layout of mI0, mI1) of the nl panels pc on stream
 // st: about (cx, cz) with radius r, or, when gcen is given, about each

@@ -905,8 +905,9 @@ __global__ __launch_bounds__(256) void sheet_on_blobs_far_kernel(
     {
         // t = 1 / W; phi * mr = t (A_0 + t (A_1 + ...))
         const double tr = wr / w2, ti = -wi / w2;
-        double pr = a[2 * (kSheetTerms - 1)], pim = a[2 * (kSheetTerms - 1) + 1];
-        for (int k = kSheetTerms - 2; k >= 0; --k) {
+        const int K = sheet_terms(w2);
+        double pr = a[2 * (K - 1)], pim = a[2 * (K - 1) + 1];
+        for (int k = K - 2; k >= 0; --k) {
             const double nr = pr * tr - pim * ti + a[2 * k];
             pim = pr * ti + pim * tr + a[2 * k + 1];
             pr = nr;

@@ -745,9 +745,10 @@ __global__ __launch_bounds__(kBT) void bt_bs64_kernel(int S, int64_t cap, WorkLi
 // slot's Nl panels and the chunk's circulation sum.  The chunk's sources are
 // staged 256 at a time in LDS as records {x, z, circ, 10 sigma^2} and read by
 // every lane (one panel per lane) in index order; the pair is the FP64 form's
-// (src/VelocityKernel.cpp:44-60 with the reciprocal of r^2 from recip64 and
-// fused accumulation), four sources per step, the near-field correction only
-// for a step with a near pair in some lane.
+// (src/VelocityKernel.cpp:44-60 with fused accumulation), four sources per
+// step with their four 1/r^2 from one Newton-refined reciprocal of the product
+// of the four r^2, the near-field correction only for a step with a near pair
+// in some lane.
 __global__ __launch_bounds__(256) void bt_colloc_kernel(int64_t cap, int mp,
                                                         const SlotParam* __restrict__ prm,
                                                         const SlotState* __restrict__ st, BlobBuf b,
@@ -793,7 +794,7 @@ __global__ __launch_bounds__(256) void bt_colloc_kernel(int64_t cap, int mp,
             for (int k = 0; k < cnt; k += 4) {
                 double dx[4], dz[4], r2[4], K[4];
                 bool take[4];
-                double gap = 1.0;
+                uint64_t nearm = 0;  // lanes with a near pair in the group
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const double4 r = srec[k + q];
@@ -801,10 +802,24 @@ __global__ __launch_bounds__(256) void bt_colloc_kernel(int64_t cap, int mp,
                     dz[q] = zi - r.y;
                     r2[q] = __builtin_fma(dx[q], dx[q], dz[q] * dz[q]);
                     take[q] = have && k + q < cnt;
-                    K[q] = recip64(r2[q]);
-                    gap = fmin(gap, take[q] ? r2[q] - r.w : 1.0);
+                    nearm |= __builtin_amdgcn_ballot_w64(take[q] && r2[q] < r.w);
                 }
-                if (__any(gap < 0.0)) {
+                // the four 1/r2 from one reciprocal of their product, as in the
+                // FP64 blob-on-blob sum (bs_f64_kernel)
+                const double pab = r2[0] * r2[1], pcd = r2[2] * r2[3], pp = pab * pcd;
+                const double inv = recip64(pp);
+                const double iab = inv * pcd, icd = inv * pab;
+                K[0] = r2[1] * iab;
+                K[1] = r2[0] * iab;
+                K[2] = r2[3] * icd;
+                K[3] = r2[2] * icd;
+                const bool shared = fmin(pab, pcd) > 1e-300 && pp > 1e-300 && pp < 1e300;
+                if (__builtin_amdgcn_ballot_w64(!shared) != 0) {  // wave-uniform, rare
+                    asm volatile("" ::: "memory");  // a branch, not computed ahead
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) K[q] = shared ? K[q] : recip64(r2[q]);
+                }
+                if (nearm != 0) {
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const double thr = srec[k + q].w;

@@ -829,6 +829,7 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
     const double smin = std::sqrt(10.0) * smax * (1.0 + 1e-9) / dmin;
     while (L > 0 && S / (double)(1 << L) < smin) --L;
     int Ls = L;  // deepest level the core-size rule allows
+    double occ_leaves = (double)(int64_t(1) << (2 * L));  // occupied leaves of level L (at most)
     while (Ls < kFmmMaxLevel && S / (double)(1 << (Ls + 1)) >= smin) ++Ls;
     if (Ls > L && L >= 1) {
         // A clustered set fills few of the box's leaves: the mean over the box
@@ -853,8 +854,25 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
         for (int y = 0; y < ny; ++y)
             if (hc[y] > 0 && (double)n / (double)hc[y] >= kLeafMean) {
                 L = Ls - y;
+                occ_leaves = (double)hc[y];
                 break;
             }
+    }
+    // Leaf width within the level: the root side is free above the extent, so
+    // the leaves are narrowed to the width at which the occupied ones hold
+    // about kLeafMean blobs (the chosen level's count scaled by area), but no
+    // narrower than the core-size rule allows.  The power-of-two cut of the
+    // extent alone leaves them up to twice that wide, and the near field up to
+    // four times as large (a remeshed wake at its core-size limit).
+    if (L >= 1) {
+        const double wl = S / (double)(1 << L);
+        const double wf = std::fmax(smin, wl * std::sqrt(kLeafMean * occ_leaves / (double)n));
+        int k = L;
+        while (k <= kFmmMaxLevel && wf * (double)(1 << k) < S) ++k;
+        if (wf < wl && k <= kFmmMaxLevel) {
+            S = wf * (double)(1 << k);
+            L = k;
+        }
     }
     if (L < 2) return false;  // no well-separated boxes: direct sum
     ctx->fmm_last_level = L;

@@ -198,6 +198,55 @@ __global__ void fmm_lbeg_kernel(int n, const uint32_t* __restrict__ key, int nle
     lbeg[m] = lo;
 }
 
+// e^-x for 0 <= x < 10, the range of the regularised kernel's factor: with
+// x = k ln2 + r (k = rint(x / ln2), ln2 split in two parts so k ln2 is exact
+// to 2^-106), e^-r = sum_{i<=12} (-r)^i / i! for |r| <= ln2 / 2 (truncation
+// below 1.7e-16 relative), then scaled by 2^-k.  The library exp also handles
+// overflow, underflow and non-finite input, which this range never reaches.
+__device__ __forceinline__ double exp_neg_kernel(double x)
+{
+    constexpr double kLog2e = 1.4426950408889634;      // 1 / ln2
+    constexpr double kLn2Hi = 6.93147180369123816490e-01;  // ln2, high part (exact k ln2_hi)
+    constexpr double kLn2Lo = 1.90821492927058770002e-10;  // ln2 - ln2_hi
+    const double kf = __builtin_rint(x * kLog2e);
+    const double r = fma(-kf, kLn2Lo, fma(-kf, kLn2Hi, x));  // x - k ln2
+    const double q = -r;
+    double p = 1.0 / 479001600.0;  // 1/12!
+    p = fma(p, q, 1.0 / 39916800.0);
+    p = fma(p, q, 1.0 / 3628800.0);
+    p = fma(p, q, 1.0 / 362880.0);
+    p = fma(p, q, 1.0 / 40320.0);
+    p = fma(p, q, 1.0 / 5040.0);
+    p = fma(p, q, 1.0 / 720.0);
+    p = fma(p, q, 1.0 / 120.0);
+    p = fma(p, q, 1.0 / 24.0);
+    p = fma(p, q, 1.0 / 6.0);
+    p = fma(p, q, 0.5);
+    p = fma(p, q, 1.0);
+    p = fma(p, q, 1.0);
+    return ldexp(p, -(int)kf);
+}
+
+// One near pair with the reference's regularised kernel
+// (src/VelocityKernel.cpp:44-60): 1/r^2 from the hardware reciprocal and two
+// Newton steps (about 1 ulp), the factor 1 - exp(-r^2/sigma^2) inside
+// r^2 < 10 sigma^2.  q.w holds 1/sigma^2.
+__device__ __forceinline__ void near_pair64(double xi, double zi, const double4& q, double& nu,
+                                            double& nw)
+{
+    const double dx = xi - q.x;
+    const double dz = zi - q.y;
+    const double r2 = fma(dx, dx, dz * dz);
+    const double xs = r2 * q.w;
+    double inv = __builtin_amdgcn_rcp(r2);
+    inv = fma(inv, fma(-r2, inv, 1.0), inv);
+    inv = fma(inv, fma(-r2, inv, 1.0), inv);
+    const double ci = q.z * inv;  // before the branch: the record is loaded whole
+    const double g = xs < 10.0 ? (1.0 - exp_neg_kernel(xs)) * ci : ci;
+    nu = fma(-g, dz, nu);
+    nw = fma(g, dx, nw);
+}
+
 __device__ __forceinline__ cplx box_centre(const FmmTree& t, int l, uint32_t m)
 {
     const double s = t.S / (double)(1 << l);
@@ -480,19 +529,7 @@ __global__ void fmm_eval_kernel(FmmTree t, int n, const double4* __restrict__ sp
         const uint32_t lm = morton((uint32_t)sx, (uint32_t)sz);
         const int b = t.lbeg[lm], e = t.lbeg[lm + 1];
         for (int j = b; j < e; ++j) {
-            if (j == k) continue;
-            const double4 q4 = sp[j];
-            const double dx = me.x - q4.x;
-            const double dz = me.y - q4.y;
-            const double r2 = dx * dx + dz * dz;
-            const double xs = r2 * q4.w;  // r^2 / sigma^2
-            // 1/r^2: hardware reciprocal and two Newton steps (about 1 ulp)
-            double inv = __builtin_amdgcn_rcp(r2);
-            inv = fma(inv, fma(-r2, inv, 1.0), inv);
-            inv = fma(inv, fma(-r2, inv, 1.0), inv);
-            const double dk = xs < 10.0 ? (1.0 - exp(-xs)) * inv : inv;
-            nu -= dk * dz * q4.z;
-            nw += dk * dx * q4.z;
+            if (j != k) near_pair64(me.x, me.y, sp[j], nu, nw);
         }
     }
     const int i = perm[k];
@@ -529,20 +566,8 @@ __global__ __launch_bounds__(256) void fmm_heavy_kernel(FmmTree t, const double4
             if (sx < 0 || sz < 0 || sx >= side || sz >= side) continue;
             const uint32_t lm = morton((uint32_t)sx, (uint32_t)sz);
             const int b = t.lbeg[lm], e = t.lbeg[lm + 1];
-            for (int j = b + lane; j < e; j += 64) {
-                if (j == k) continue;
-                const double4 q4 = sp[j];
-                const double dx = me.x - q4.x;
-                const double dz = me.y - q4.y;
-                const double r2 = dx * dx + dz * dz;
-                const double xs = r2 * q4.w;
-                double inv = __builtin_amdgcn_rcp(r2);
-                inv = fma(inv, fma(-r2, inv, 1.0), inv);
-                inv = fma(inv, fma(-r2, inv, 1.0), inv);
-                const double dk = xs < 10.0 ? (1.0 - exp(-xs)) * inv : inv;
-                nu -= dk * dz * q4.z;
-                nw += dk * dx * q4.z;
-            }
+            for (int j = b + lane; j < e; j += 64)
+                if (j != k) near_pair64(me.x, me.y, sp[j], nu, nw);
         }
         for (int o = 32; o > 0; o >>= 1) {
             nu += __shfl_xor(nu, o);
@@ -613,19 +638,7 @@ __global__ __launch_bounds__(64) void fmm_target_kernel(FmmTree t, int nt, const
         if (sx < 0 || sz < 0 || sx >= side || sz >= side) continue;
         const uint32_t lm = morton((uint32_t)sx, (uint32_t)sz);
         const int b = t.lbeg[lm], e = t.lbeg[lm + 1];
-        for (int j = b + lane; j < e; j += 64) {
-            const double4 q4 = sp[j];
-            const double dx = xk - q4.x;
-            const double dz = zk - q4.y;
-            const double r2 = dx * dx + dz * dz;
-            const double xs = r2 * q4.w;
-            double inv = __builtin_amdgcn_rcp(r2);
-            inv = fma(inv, fma(-r2, inv, 1.0), inv);
-            inv = fma(inv, fma(-r2, inv, 1.0), inv);
-            const double dk = xs < 10.0 ? (1.0 - exp(-xs)) * inv : inv;
-            nu -= dk * dz * q4.z;
-            nw += dk * dx * q4.z;
-        }
+        for (int j = b + lane; j < e; j += 64) near_pair64(xk, zk, sp[j], nu, nw);
     }
     for (int o = 32; o > 0; o >>= 1) {
         nu += __shfl_xor(nu, o);

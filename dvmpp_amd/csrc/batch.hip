@@ -644,7 +644,9 @@ __global__ __launch_bounds__(256) void bt_bs32_kernel(int S, int64_t cap, int ma
             if (!near) {
                 const double ex = fabs(Dxd) + (double)(tb.hx + ts.hx);
                 const double ez = fabs(Dzd) + (double)(tb.hz + ts.hz);
-                if (cnt == kBT && g2 > 1e-8 && ex * ex + ez * ez < 1e8)
+                if (cnt == kBT && g2 > 1e-4 && ex * ex + ez * ez < 1e4)
+                    far_tile_share8<kBT>(sp, txd, tzd, au, aw);
+                else if (cnt == kBT && g2 > 1e-8 && ex * ex + ez * ez < 1e8)
                     far_tile_share4<kBT>(sp, txd, tzd, au, aw);
                 else if (cnt == kBT)
                     far_tile<kBT>(sp, txd, tzd, au, aw);

@@ -20,8 +20,8 @@
 //           at |x| ~ 15 with 0.01 cores is avoided.  A tile pair whose
 //           bounding boxes (source tile, target block of 64 or 256) are
 //           farther apart than sqrt(10)*max(sigma) takes the far-field loop
-//           (1/r^2 only: 7 VALU ops per pair and one reciprocal per 1, 2 or
-//           4 pairs); the others take the full regularised loop (near_k:
+//           (1/r^2 only: 7 VALU ops per pair and one reciprocal per 1, 2,
+//           4 or 8 pairs); the others take the full regularised loop (near_k:
 //           Taylor series or hardware exponential for 1 - e^-x).  Partial
 //           sums within a tile accumulate in FP32 with FMA;
 //           tile partials are added with Kahan compensation and the result
@@ -343,7 +343,8 @@ __global__ __launch_bounds__(T) void tile_build_kernel(
 }
 
 // TILE sources per source tile, BLOCK targets per block (one per lane),
-// SHARE sources per far-field reciprocal.  Targets: SELF -> the sorted FP64
+// SHARE sources per far-field reciprocal (SHARE 4 takes 8 where every r^2
+// of the tile pair lies in [1e-4, 1e4]).  Targets: SELF -> the sorted FP64
 // copy p64 (target i is source i); otherwise tx, tz in the caller's order.
 // tblk: bounding box of each target block (the near/far test).  Per source
 // tile each target is recentred on the tile centre in FP64 and rounded once,
@@ -406,7 +407,12 @@ __global__ __launch_bounds__(BLOCK) void bs_f32c_kernel(
             if (SHARE == 2 && shared) {
                 far_tile_share2<TILE>(sp, txd, tzd, au, aw);
             } else if (SHARE == 4 && shared) {
-                far_tile_share4<TILE>(sp, txd, tzd, au, aw);
+                const double ex = fabs(Dxd) + (double)(tb.hx + ts.hx);
+                const double ez = fabs(Dzd) + (double)(tb.hz + ts.hz);
+                if (gx * gx + gz * gz > 1e-4 && ex * ex + ez * ez < 1e4)
+                    far_tile_share8<TILE>(sp, txd, tzd, au, aw);
+                else
+                    far_tile_share4<TILE>(sp, txd, tzd, au, aw);
             } else if (cnt == TILE) {
                 far_tile<TILE>(sp, txd, tzd, au, aw);
             } else {

@@ -358,7 +358,7 @@ __device__ inline uint32_t spread16(uint32_t v)
 }
 
 // Far-field loops over one source tile: sum of circ_j * (dz, dx) / r^2.
-// The shared forms take one reciprocal for 2 (or 4) sources:
+// The shared forms take one reciprocal for 2, 4 or 8 sources:
 // 1/ra = rb * (1/(ra rb)); they are used only when every r^2 of the tile pair
 // lies in a range where the product stays a normal FP32 number.
 
@@ -442,6 +442,35 @@ __device__ __forceinline__ void far_quad(const float4 sa, const float4 sb, const
     aw = __builtin_fmaf(kd, dxd, aw);
 }
 
+// Eight far pairs with one reciprocal: the product tree of the eight r^2
+// (pairs, quads, all), its reciprocal, and each 1/r^2 as the product of the
+// other seven taken back down the tree.  The caller keeps every r^2 within
+// [1e-4, 1e4], so every partial product stays within [1e-32, 1e32].
+__device__ __forceinline__ void far_oct(const Src8& s, float txd, float tzd, float& au, float& aw)
+{
+    float dx[8], dz[8], r[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        dx[q] = txd - s.v[q].x;
+        dz[q] = tzd - s.v[q].y;
+        r[q] = __builtin_fmaf(dx[q], dx[q], dz[q] * dz[q]);
+    }
+    const float p01 = r[0] * r[1], p23 = r[2] * r[3], p45 = r[4] * r[5], p67 = r[6] * r[7];
+    const float p03 = p01 * p23, p47 = p45 * p67;
+    const float inv = __builtin_amdgcn_rcpf(p03 * p47);
+    const float i03 = inv * p47, i47 = inv * p03;
+    const float i01 = i03 * p23, i23 = i03 * p01, i45 = i47 * p67, i67 = i47 * p45;
+    const float k[8] = {(r[1] * i01) * s.v[0].z, (r[0] * i01) * s.v[1].z,
+                        (r[3] * i23) * s.v[2].z, (r[2] * i23) * s.v[3].z,
+                        (r[5] * i45) * s.v[4].z, (r[4] * i45) * s.v[5].z,
+                        (r[7] * i67) * s.v[6].z, (r[6] * i67) * s.v[7].z};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        au = __builtin_fmaf(k[q], dz[q], au);
+        aw = __builtin_fmaf(k[q], dx[q], aw);
+    }
+}
+
 // One reciprocal for four sources.  The records go through scalar loads in
 // batches of eight with the next batch in flight while the current one is
 // summed: an explicit wait for all scalar loads (they complete out of order,
@@ -473,6 +502,30 @@ __device__ __forceinline__ void far_tile_share4(const float4* __restrict__ sp, f
         far_quad(c.v[0], c.v[1], c.v[2], c.v[3], txd, tzd, au, aw);
         __builtin_amdgcn_sched_barrier(0);
         far_quad(c.v[4], c.v[5], c.v[6], c.v[7], txd, tzd, au, aw);
+    }
+}
+
+// far_tile_share4 with one reciprocal for each batch of eight (far_oct).
+template <int T>
+__device__ __forceinline__ void far_tile_share8(const float4* __restrict__ sp, float txd,
+                                                float tzd, float& au, float& aw)
+{
+    const Src8* __restrict__ q = reinterpret_cast<const Src8*>(sp);
+    Src8 a = q[0];
+#pragma unroll 1
+    for (int b = 0; b < T / 8; b += 2) {
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+        __builtin_amdgcn_sched_barrier(0);
+        const Src8 c = q[__builtin_amdgcn_readfirstlane(b + 1)];
+        __builtin_amdgcn_sched_barrier(0);
+        far_oct(a, txd, tzd, au, aw);
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+        __builtin_amdgcn_sched_barrier(0);
+        a = q[__builtin_amdgcn_readfirstlane(b + 2 < T / 8 ? b + 2 : b + 1)];
+        __builtin_amdgcn_sched_barrier(0);
+        far_oct(c, txd, tzd, au, aw);
     }
 }
 

@@ -397,6 +397,39 @@ __device__ __forceinline__ cplx shfl_c(cplx v, int src, int width)
     return {__shfl(v.re, src, width), __shfl(v.im, src, width)};
 }
 
+// Lane moves within rows of 16 lanes by DPP (VALU moves, where __shfl takes
+// an LDS permute each): CTRL 0x150 + k broadcasts lane k of each row
+// (row_newbcast), 0x110 + o shifts up by o (row_shr; the first o lanes of a
+// row keep their own value).  A double moves as its two halves.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), CTRL, 0xf, 0xf,
+                                               false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), CTRL, 0xf, 0xf,
+                                               false);
+    return __hiloint2double(hi, lo);
+}
+template <int CTRL>
+__device__ __forceinline__ cplx dpp_c(cplx v)
+{
+    return {dpp_f64<CTRL>(v.re), dpp_f64<CTRL>(v.im)};
+}
+
+// acc += sum_{k < P} C(k + lane, lane) * (lane k's value), lanes in rows of 16.
+template <int K, int P>
+struct RowSum16 {
+    static __device__ __forceinline__ void run(cplx ak, const double* cb, cplx& acc)
+    {
+        acc = cadd(acc, cscale(dpp_c<0x150 + K>(ak), cb[K]));
+        RowSum16<K + 1, P>::run(ak, cb, acc);
+    }
+};
+template <int P>
+struct RowSum16<P, P> {
+    static __device__ __forceinline__ void run(cplx, const double*, cplx&) {}
+};
+
 // Level l locals: L2L from the parent (l > 2) plus M2L from the interaction
 // list, one wavefront per box with particles, made of 64/G groups of G lanes;
 // lane l of a group computes B_l.  Group g takes interaction-list entries g,
@@ -443,26 +476,55 @@ __global__ __launch_bounds__(256) void fmm_down_kernel(FmmTree t, int l)
     const int side = 1 << l;
     const int pos = (ix & 1) | ((iz & 1) << 1);
     const int* il = t.il + pos * t.il_stride;
-    for (int q = grp; q < t.il_n[pos]; q += NG) {
-        const int sx = ix + il[2 * q], sz = iz + il[2 * q + 1];
-        if (sx < 0 || sz < 0 || sx >= side || sz >= side) continue;
-        const int64_t si = level_off(l) + morton((uint32_t)sx, (uint32_t)sz);
-        if (t.cnt[si] == 0) continue;
+    const int nq = t.il_n[pos];
+    // Entry q's source box, its count and its lane's coefficient, loaded one
+    // entry ahead of the arithmetic (each entry is otherwise three dependent
+    // memory round trips); an entry off the grid reads this box's slot.
+    auto fetch = [&](int q, int& sx, int& sz, int64_t& si, int& cn, cplx& a) {
+        sx = ix + il[2 * q];
+        sz = iz + il[2 * q + 1];
+        const bool in = sx >= 0 && sz >= 0 && sx < side && sz < side;
+        si = in ? level_off(l) + morton((uint32_t)sx, (uint32_t)sz) : bi;
+        cn = in ? t.cnt[si] : 0;
+        a = act ? t.A[si * P + lane] : cplx{0.0, 0.0};
+    };
+    int nsx = 0, nsz = 0, ncn = 0;
+    int64_t nsi = 0;
+    cplx na = {0.0, 0.0};
+    if (grp < nq) fetch(grp, nsx, nsz, nsi, ncn, na);
+    for (int q = grp; q < nq; q += NG) {
+        const int sx = nsx, sz = nsz, cn = ncn;
+        const cplx a_src = na;
+        if (q + NG < nq) fetch(q + NG, nsx, nsz, nsi, ncn, na);
+        if (cn == 0) continue;
         const cplx T = {(double)(ix - sx), (double)(iz - sz)};
         const cplx tv = cinv(T);
         // t^(lane+1) by an inclusive prefix product over the group
         cplx pw = tv;
-#pragma unroll
-        for (int o = 1; o < G; o <<= 1) {
-            const cplx v = {__shfl_up(pw.re, o, G), __shfl_up(pw.im, o, G)};
-            if (lane >= o) pw = cmul(pw, v);
-        }
-        const cplx ak = act ? cmul(t.A[si * P + lane], pw) : cplx{0.0, 0.0};  // A_lane t^(lane+1)
         cplx acc = {0.0, 0.0};
+        if constexpr (G == 16) {
+            cplx v = dpp_c<0x111>(pw);
+            if (lane >= 1) pw = cmul(pw, v);
+            v = dpp_c<0x112>(pw);
+            if (lane >= 2) pw = cmul(pw, v);
+            v = dpp_c<0x114>(pw);
+            if (lane >= 4) pw = cmul(pw, v);
+            v = dpp_c<0x118>(pw);
+            if (lane >= 8) pw = cmul(pw, v);
+            const cplx ak = act ? cmul(a_src, pw) : cplx{0.0, 0.0};  // A_lane t^(lane+1)
+            RowSum16<0, P>::run(ak, cb, acc);
+        } else {
 #pragma unroll
-        for (int k = 0; k < P; ++k) {
-            const cplx v = shfl_c(ak, k, G);
-            acc = cadd(acc, cscale(v, cb[k]));
+            for (int o = 1; o < G; o <<= 1) {
+                const cplx v = {__shfl_up(pw.re, o, G), __shfl_up(pw.im, o, G)};
+                if (lane >= o) pw = cmul(pw, v);
+            }
+            const cplx ak = act ? cmul(a_src, pw) : cplx{0.0, 0.0};  // A_lane t^(lane+1)
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                const cplx v = shfl_c(ak, k, G);
+                acc = cadd(acc, cscale(v, cb[k]));
+            }
         }
         // (-1)^lane t^lane = (-1)^lane t^(lane+1) T
         acc = cmul(acc, cmul(pw, T));

@@ -53,12 +53,11 @@ struct cplx {
 };
 __device__ __forceinline__ cplx cmul(cplx a, cplx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
 __device__ __forceinline__ cplx cadd(cplx a, cplx b) { return {a.re + b.re, a.im + b.im}; }
-__device__ __forceinline__ cplx cscale(cplx a, double f) { return {a.re * f, a.im * f}; }
-__device__ __forceinline__ cplx cinv(cplx a)
+__device__ __forceinline__ cplx cmul_fma(cplx a, cplx b)
 {
-    const double d = a.re * a.re + a.im * a.im;
-    return {a.re / d, -a.im / d};
+    return {__builtin_fma(a.re, b.re, -(a.im * b.im)), __builtin_fma(a.re, b.im, a.im * b.re)};
 }
+__device__ __forceinline__ cplx cscale(cplx a, double f) { return {a.re * f, a.im * f}; }
 
 __host__ __device__ __forceinline__ uint32_t spread_bits(uint32_t v)
 {
@@ -91,6 +90,11 @@ struct FmmTree {
     const int* il;        // interaction-list offsets (dx, dz) per child position, il_n each
     int il_n[4];
     int il_stride;
+    // per (child position, entry, lane l): T^-(l+1) and (-1)^l T^-l, with T the
+    // entry's box offset from source to target (box widths; il_cap entries a
+    // position, G lanes an entry, four doubles a lane)
+    const double* m2l;
+    int il_cap;
     const int* nearo;     // near offsets (dx, dz)
     int near_n;
     const int* lbeg;      // leaf particle bounds (4^L + 1)
@@ -421,7 +425,8 @@ template <int K, int P>
 struct RowSum16 {
     static __device__ __forceinline__ void run(cplx ak, const double* cb, cplx& acc)
     {
-        acc = cadd(acc, cscale(dpp_c<0x150 + K>(ak), cb[K]));
+        const cplx v = dpp_c<0x150 + K>(ak);
+        acc = {__builtin_fma(v.re, cb[K], acc.re), __builtin_fma(v.im, cb[K], acc.im)};
         RowSum16<K + 1, P>::run(ak, cb, acc);
     }
 };
@@ -480,55 +485,38 @@ __global__ __launch_bounds__(256) void fmm_down_kernel(FmmTree t, int l)
     // Entry q's source box, its count and its lane's coefficient, loaded one
     // entry ahead of the arithmetic (each entry is otherwise three dependent
     // memory round trips); an entry off the grid reads this box's slot.
-    auto fetch = [&](int q, int& sx, int& sz, int64_t& si, int& cn, cplx& a) {
-        sx = ix + il[2 * q];
-        sz = iz + il[2 * q + 1];
+    auto fetch = [&](int q, int& cn, cplx& a, double4& f) {
+        const int sx = ix + il[2 * q], sz = iz + il[2 * q + 1];
         const bool in = sx >= 0 && sz >= 0 && sx < side && sz < side;
-        si = in ? level_off(l) + morton((uint32_t)sx, (uint32_t)sz) : bi;
+        const int64_t si = in ? level_off(l) + morton((uint32_t)sx, (uint32_t)sz) : bi;
         cn = in ? t.cnt[si] : 0;
         a = act ? t.A[si * P + lane] : cplx{0.0, 0.0};
+        f = *reinterpret_cast<const double4*>(t.m2l + 4 * ((int64_t)(pos * t.il_cap + q) * G + lane));
     };
-    int nsx = 0, nsz = 0, ncn = 0;
-    int64_t nsi = 0;
+    int ncn = 0;
     cplx na = {0.0, 0.0};
-    if (grp < nq) fetch(grp, nsx, nsz, nsi, ncn, na);
+    double4 nf = {0.0, 0.0, 0.0, 0.0};
+    if (grp < nq) fetch(grp, ncn, na, nf);
     for (int q = grp; q < nq; q += NG) {
-        const int sx = nsx, sz = nsz, cn = ncn;
+        const int cn = ncn;
         const cplx a_src = na;
-        if (q + NG < nq) fetch(q + NG, nsx, nsz, nsi, ncn, na);
+        const double4 f = nf;
+        if (q + NG < nq) fetch(q + NG, ncn, na, nf);
         if (cn == 0) continue;
-        const cplx T = {(double)(ix - sx), (double)(iz - sz)};
-        const cplx tv = cinv(T);
-        // t^(lane+1) by an inclusive prefix product over the group
-        cplx pw = tv;
+        // B_l += (-1)^l t^l sum_k C(k+l, l) A_k t^(k+1), t = 1/T: lane k
+        // contributes A_k t^(k+1), each lane sums its binomial row
+        const cplx ak = act ? cmul_fma(a_src, cplx{f.x, f.y}) : cplx{0.0, 0.0};
         cplx acc = {0.0, 0.0};
         if constexpr (G == 16) {
-            cplx v = dpp_c<0x111>(pw);
-            if (lane >= 1) pw = cmul(pw, v);
-            v = dpp_c<0x112>(pw);
-            if (lane >= 2) pw = cmul(pw, v);
-            v = dpp_c<0x114>(pw);
-            if (lane >= 4) pw = cmul(pw, v);
-            v = dpp_c<0x118>(pw);
-            if (lane >= 8) pw = cmul(pw, v);
-            const cplx ak = act ? cmul(a_src, pw) : cplx{0.0, 0.0};  // A_lane t^(lane+1)
             RowSum16<0, P>::run(ak, cb, acc);
         } else {
 #pragma unroll
-            for (int o = 1; o < G; o <<= 1) {
-                const cplx v = {__shfl_up(pw.re, o, G), __shfl_up(pw.im, o, G)};
-                if (lane >= o) pw = cmul(pw, v);
-            }
-            const cplx ak = act ? cmul(a_src, pw) : cplx{0.0, 0.0};  // A_lane t^(lane+1)
-#pragma unroll
             for (int k = 0; k < P; ++k) {
                 const cplx v = shfl_c(ak, k, G);
-                acc = cadd(acc, cscale(v, cb[k]));
+                acc = {__builtin_fma(v.re, cb[k], acc.re), __builtin_fma(v.im, cb[k], acc.im)};
             }
         }
-        // (-1)^lane t^lane = (-1)^lane t^(lane+1) T
-        acc = cmul(acc, cmul(pw, T));
-        b = (lane & 1) ? cadd(b, cscale(acc, -1.0)) : cadd(b, acc);
+        b = cadd(b, cmul_fma(acc, cplx{f.z, f.w}));
     }
     for (int o = G; o < 64; o <<= 1) {
         b.re += __shfl_xor(b.re, o);
@@ -860,7 +848,10 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
     const size_t il_stride = std::max({il[0].size(), il[1].size(), il[2].size(), il[3].size()}) / 2;
     const size_t rb_bytes = (sizeof(double) * (6 * kRedBlocks + 2 * nt) + 255) & ~size_t(255);
     const size_t tab_ints = 8 * il_stride + nearo.size();
-    char* pin = static_cast<char*>(pinned_get(ctx, rb_bytes + sizeof(int) * tab_ints));
+    const int G = P <= 8 ? 8 : P <= 16 ? 16 : 32;  // lanes per coefficient group (Lanes<P>)
+    const size_t m2l_doubles = 4 * il_stride * (size_t)G * 4;
+    const size_t m2l_off = rb_bytes + ((sizeof(int) * tab_ints + 255) & ~size_t(255));
+    char* pin = static_cast<char*>(pinned_get(ctx, m2l_off + sizeof(double) * m2l_doubles));
 
     // extent, largest core, non-finite entries
     double* red = static_cast<double*>(ws_get(ctx, sizeof(double) * 6 * kRedBlocks));
@@ -963,7 +954,7 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
                         al(sizeof(int) * nearo.size()) + 2 * al(sizeof(uint32_t) * n) +
                         2 * al(sizeof(int) * n) + al(sizeof(double4) * n) + al(sizeof(int) * (nleaf + 1)) +
                         2 * al(sizeof(int) * nbox) + 2 * al(sizeof(cplx) * P * nbox) + al(sort_bytes) +
-                        al(sizeof(int) * (n + 1));
+                        al(sizeof(int) * (n + 1)) + al(sizeof(double) * m2l_doubles);
     char* p = static_cast<char*>(ws_get(ctx, need));
     auto take = [&](size_t bytes) {
         char* q = p;
@@ -985,6 +976,7 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
     cplx* B = reinterpret_cast<cplx*>(take(sizeof(cplx) * P * nbox));
     void* sort_tmp = take(sort_bytes);
     int* heavy = reinterpret_cast<int*>(take(sizeof(int) * (n + 1)));
+    double* m2ld = reinterpret_cast<double*>(take(sizeof(double) * m2l_doubles));
 
     // tables: interaction lists padded to il_stride entries per child position,
     // then the near offsets, staged in the pinned buffer
@@ -999,6 +991,33 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
     std::copy(nearo.begin(), nearo.end(), nearh);
     DVM_HIP_CHECK(hipMemcpyAsync(ild, ilh, sizeof(int) * 8 * il_stride, hipMemcpyHostToDevice, st));
     DVM_HIP_CHECK(hipMemcpyAsync(neard, nearh, sizeof(int) * nearo.size(), hipMemcpyHostToDevice, st));
+    // M2L factors of each interaction entry: with T = -(dx, dz) the offset from
+    // the source box to the target box, t = 1/T, lane l gets t^(l+1) and
+    // (-1)^l t^l (the same at every level: offsets are in box widths)
+    double* m2lh = reinterpret_cast<double*>(pin + m2l_off);
+    std::fill(m2lh, m2lh + m2l_doubles, 0.0);
+    for (int pos = 0; pos < 4; ++pos)
+        for (size_t q = 0; q < il[pos].size() / 2; ++q) {
+            const double tre = -(double)il[pos][2 * q], tim = -(double)il[pos][2 * q + 1];
+            const double d = tre * tre + tim * tim;
+            const double vre = tre / d, vim = -tim / d;  // t = 1/T
+            double pre = vre, pim = vim;                // t^(l+1)
+            double fre = 1.0, fim = 0.0;                // t^l
+            double* e = m2lh + 4 * ((size_t)(pos * il_stride + q) * G);
+            for (int ln = 0; ln < G; ++ln) {
+                const double sg = (ln & 1) ? -1.0 : 1.0;
+                e[4 * ln] = pre;
+                e[4 * ln + 1] = pim;
+                e[4 * ln + 2] = sg * fre;
+                e[4 * ln + 3] = sg * fim;
+                const double nre = pre * vre - pim * vim, nim = pre * vim + pim * vre;
+                fre = pre;
+                fim = pim;
+                pre = nre;
+                pim = nim;
+            }
+        }
+    DVM_HIP_CHECK(hipMemcpyAsync(m2ld, m2lh, sizeof(double) * m2l_doubles, hipMemcpyHostToDevice, st));
     fmm_binom_kernel<<<1, kBinom, 0, st>>>(binom);
     DVM_LAUNCH_CHECK();
 
@@ -1020,6 +1039,8 @@ bool fmm_self(dvm_ctx_t ctx, int64_t n, const double* x, const double* z, const 
     t.binom = binom;
     t.il = ild;
     t.il_stride = (int)(2 * il_stride);
+    t.il_cap = (int)il_stride;
+    t.m2l = m2ld;
     t.nearo = neard;
     t.near_n = (int)(nearo.size() / 2);
     t.lbeg = lbeg;

@@ -235,17 +235,20 @@ __device__ __forceinline__ double exp_neg_kernel(double x)
 // (src/VelocityKernel.cpp:44-60): 1/r^2 from the hardware reciprocal and two
 // Newton steps (about 1 ulp), the factor 1 - exp(-r^2/sigma^2) inside
 // r^2 < 10 sigma^2.  q.w holds 1/sigma^2.
+template <bool MASK = false>
 __device__ __forceinline__ void near_pair64(double xi, double zi, const double4& q, double& nu,
-                                            double& nw)
+                                            double& nw, bool self = false)
 {
     const double dx = xi - q.x;
     const double dz = zi - q.y;
-    const double r2 = fma(dx, dx, dz * dz);
+    // a masked (self) pair takes r^2 = 1 and weight 0: it adds -0 * 0, which
+    // leaves both sums as they are
+    const double r2 = MASK && self ? 1.0 : fma(dx, dx, dz * dz);
     const double xs = r2 * q.w;
     double inv = __builtin_amdgcn_rcp(r2);
     inv = fma(inv, fma(-r2, inv, 1.0), inv);
     inv = fma(inv, fma(-r2, inv, 1.0), inv);
-    const double ci = q.z * inv;  // before the branch: the record is loaded whole
+    const double ci = MASK && self ? 0.0 : q.z * inv;  // before the branch: the record loads whole
     const double g = xs < 10.0 ? (1.0 - exp_neg_kernel(xs)) * ci : ci;
     nu = fma(-g, dz, nu);
     nw = fma(g, dx, nw);
@@ -578,9 +581,13 @@ __global__ void fmm_eval_kernel(FmmTree t, int n, const double4* __restrict__ sp
         if (sx < 0 || sz < 0 || sx >= side || sz >= side) continue;
         const uint32_t lm = morton((uint32_t)sx, (uint32_t)sz);
         const int b = t.lbeg[lm], e = t.lbeg[lm + 1];
-        for (int j = b; j < e; ++j) {
-            if (j != k) near_pair64(me.x, me.y, sp[j], nu, nw);
+        int j = b;
+        for (; j + 1 < e; j += 2) {
+            const double4 s0 = sp[j], s1 = sp[j + 1];
+            near_pair64<true>(me.x, me.y, s0, nu, nw, j == k);
+            near_pair64<true>(me.x, me.y, s1, nu, nw, j + 1 == k);
         }
+        if (j < e) near_pair64<true>(me.x, me.y, sp[j], nu, nw, j == k);
     }
     const int i = perm[k];
     u[i] = (fu + nu) * kRpi2;

@@ -582,12 +582,14 @@ __global__ void fmm_eval_kernel(FmmTree t, int n, const double4* __restrict__ sp
         const uint32_t lm = morton((uint32_t)sx, (uint32_t)sz);
         const int b = t.lbeg[lm], e = t.lbeg[lm + 1];
         int j = b;
-        for (; j + 1 < e; j += 2) {
-            const double4 s0 = sp[j], s1 = sp[j + 1];
+        for (; j + 3 < e; j += 4) {
+            const double4 s0 = sp[j], s1 = sp[j + 1], s2 = sp[j + 2], s3 = sp[j + 3];
             near_pair64<true>(me.x, me.y, s0, nu, nw, j == k);
             near_pair64<true>(me.x, me.y, s1, nu, nw, j + 1 == k);
+            near_pair64<true>(me.x, me.y, s2, nu, nw, j + 2 == k);
+            near_pair64<true>(me.x, me.y, s3, nu, nw, j + 3 == k);
         }
-        if (j < e) near_pair64<true>(me.x, me.y, sp[j], nu, nw, j == k);
+        for (; j < e; ++j) near_pair64<true>(me.x, me.y, sp[j], nu, nw, j == k);
     }
     const int i = perm[k];
     u[i] = (fu + nu) * kRpi2;

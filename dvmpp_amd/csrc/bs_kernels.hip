@@ -230,6 +230,70 @@ __global__ void bs_reduce_kernel(int64_t nt, int nchunks, const double* __restri
     w[o] = sw * kRpi2;
 }
 
+// The same sum for few targets (probes, collocation points), where one thread
+// per target leaves most CUs idle and each thread walks hundreds of chunks:
+// a workgroup takes 64 consecutive targets (lane x, coalesced loads) and 16
+// chunk groups (y); thread (x, y) adds chunks y, y + 16, ... in order, and the
+// 16 group sums are added in group order through LDS.
+constexpr int kRedGroups = 16;
+__global__ __launch_bounds__(64 * kRedGroups) void bs_reduce_few_kernel(
+    int64_t nt, int nchunks, const double* __restrict__ part, const uint32_t* __restrict__ out_index,
+    double* __restrict__ u, double* __restrict__ w)
+{
+    __shared__ double ru[kRedGroups][64], rw[kRedGroups][64];
+    const int x = threadIdx.x, y = threadIdx.y;
+    const int64_t i = (int64_t)blockIdx.x * 64 + x;
+    double su = 0.0, sw = 0.0;
+    if (i < nt) {
+        // loads of 8 chunks issued together, the adds in chunk order
+        constexpr int kU = 8;
+        int c = y;
+        for (; c + (kU - 1) * kRedGroups < nchunks; c += kU * kRedGroups) {
+            double a[kU], b[kU];
+#pragma unroll
+            for (int k = 0; k < kU; ++k) {
+                a[k] = part[((int64_t)(c + k * kRedGroups) * 2 + 0) * nt + i];
+                b[k] = part[((int64_t)(c + k * kRedGroups) * 2 + 1) * nt + i];
+            }
+#pragma unroll
+            for (int k = 0; k < kU; ++k) {
+                su += a[k];
+                sw += b[k];
+            }
+        }
+        for (; c < nchunks; c += kRedGroups) {
+            su += part[((int64_t)c * 2 + 0) * nt + i];
+            sw += part[((int64_t)c * 2 + 1) * nt + i];
+        }
+    }
+    ru[y][x] = su;
+    rw[y][x] = sw;
+    __syncthreads();
+    if (y == 0 && i < nt) {
+        double tu = 0.0, tw = 0.0;
+        for (int g = 0; g < kRedGroups; ++g) {
+            tu += ru[g][x];
+            tw += rw[g][x];
+        }
+        const int64_t o = out_index ? (int64_t)out_index[i] : i;
+        u[o] = tu * kRpi2;
+        w[o] = tw * kRpi2;
+    }
+}
+
+// Chunk partials to velocities: one thread per target, or, when that fills
+// fewer workgroups than the device has CUs, bs_reduce_few_kernel.
+static void launch_reduce(dvm_ctx_t ctx, int64_t nt, int nch, const double* part,
+                          const uint32_t* out_index, double* u, double* w)
+{
+    if (cdiv(nt, 256) < ctx->num_cus && nch >= 2 * kRedGroups)
+        bs_reduce_few_kernel<<<(unsigned)cdiv(nt, 64), dim3(64, kRedGroups), 0, ctx->stream>>>(
+            nt, nch, part, out_index, u, w);
+    else
+        bs_reduce_kernel<<<cdiv(nt, 256), 256, 0, ctx->stream>>>(nt, nch, part, out_index, u, w);
+    DVM_LAUNCH_CHECK();
+}
+
 // ----------------------------------------------------------------- F32C form
 
 __global__ void bbox_partial_kernel(int64_t n, const double* __restrict__ x,
@@ -737,8 +801,7 @@ static void launch_f64(dvm_ctx_t ctx, bool self, int64_t ns, const double* sx, c
         DVM_LAUNCH_CHECK();
     }
     if (part) {
-        bs_reduce_kernel<<<cdiv(nt, 256), 256, 0, ctx->stream>>>(nt, nch, part, nullptr, u, w);
-        DVM_LAUNCH_CHECK();
+        launch_reduce(ctx, nt, nch, part, nullptr, u, w);
     }
 }
 
@@ -918,9 +981,7 @@ static void f32c_run(dvm_ctx_t ctx, int kind, int v, int occ, int64_t ns, const 
         DVM_LAUNCH_CHECK();
     }
     if (part) {
-        bs_reduce_kernel<<<cdiv(nt, 256), 256, 0, ctx->stream>>>(nt, nch, part, a.out_index, u,
-                                                                 w);
-        DVM_LAUNCH_CHECK();
+        launch_reduce(ctx, nt, nch, part, a.out_index, u, w);
     }
 }
 

@@ -29,8 +29,9 @@
 // Layout: particles Morton-sorted by leaf (radix sort); boxes of level l are
 // indexed by their Morton code after an offset (4^l - 1) / 3; expansions are
 // complex FP64, `P` per box.  Kernels: key, sort, gather, leaf bounds, P2M
-// (one thread per leaf), M2M per level, M2L + L2L per level (one thread per
-// occupied box), and L2P + P2P (one thread per particle).
+// (one thread per leaf), M2M per level, M2L + L2L per level (a wavefront per
+// occupied box, its lane groups splitting the interaction list), and L2P + P2P
+// (one thread per particle; a wavefront per particle with a dense near field).
 #include <hipcub/hipcub.hpp>
 
 #include <cmath>
@@ -203,8 +204,8 @@ __global__ void fmm_lbeg_kernel(int n, const uint32_t* __restrict__ key, int nle
 }
 
 // e^-x for 0 <= x < 10, the range of the regularised kernel's factor: with
-// x = k ln2 + r (k = rint(x / ln2), ln2 split in two parts so k ln2 is exact
-// to 2^-106), e^-r = sum_{i<=12} (-r)^i / i! for |r| <= ln2 / 2 (truncation
+// x = k ln2 + r (k = rint(x / ln2), ln2 split in two parts as in Cody and
+// Waite's reduction, so k ln2 is exact to 2^-106), e^-r = sum_{i<=12} (-r)^i / i! for |r| <= ln2 / 2 (truncation
 // below 1.7e-16 relative), then scaled by 2^-k.  The library exp also handles
 // overflow, underflow and non-finite input, which this range never reaches.
 __device__ __forceinline__ double exp_neg_kernel(double x)

@@ -793,7 +793,9 @@ __global__ __launch_bounds__(256) void bt_colloc_kernel(int64_t cap, int mp,
             }
             __syncthreads();
             const int cnt = j1 - t0 < 256 ? (int)(j1 - t0) : 256;
-            for (int k = 0; k < cnt; k += 4) {
+            // a wavefront with no panel in this pass only stages sources
+            const int kend = __builtin_amdgcn_ballot_w64(have) != 0 ? cnt : 0;
+            for (int k = 0; k < kend; k += 4) {
                 double dx[4], dz[4], r2[4], K[4];
                 bool take[4];
                 uint64_t nearm = 0;  // lanes with a near pair in the group

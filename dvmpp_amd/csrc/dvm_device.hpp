@@ -176,9 +176,10 @@ __device__ __forceinline__ void sheet_pair(double xi, double zi, const PanelCons
     const double c7 = atan2(c5 * dsj, c2 + c1 * dsj);
     const double c8 = (xi - xj) * q.s2 + (zi - zj) * q.c2;
     const double c9 = (xi - xj) * q.c2 - (zi - zj) * q.s2;
-    const double qx = -q.sn + 0.5 * c8 * c6 / dsj + (c1 * q.cs + c5 * q.sn) * c7 / dsj;
+    const double ids = 1.0 / dsj;  // one division for the four quotients by ds
+    const double qx = -q.sn + 0.5 * c8 * c6 * ids + (c1 * q.cs + c5 * q.sn) * c7 * ids;
     const double px = -0.5 * c6 * q.sn - c7 * q.cs - qx;
-    const double qy = q.cs + 0.5 * c9 * c6 / dsj + (c1 * q.sn - c5 * q.cs) * c7 / dsj;
+    const double qy = q.cs + 0.5 * c9 * c6 * ids + (c1 * q.sn - c5 * q.cs) * c7 * ids;
     const double py = 0.5 * c6 * q.cs - c7 * q.sn - qy;
     su += px * gj + qx * gj1;
     sw += py * gj + qy * gj1;

@@ -245,6 +245,26 @@ def test_f32c_measured_tuning(gpu_ctx):
     assert norm_max <= F32C_TOL and p99 <= F32C_TOL, (norm_max, p99)
 
 
+@pytest.mark.parametrize("prec", [dvm.F32C, dvm.F64])
+def test_probe_sums_rerun_bit_for_bit(gpu_ctx, prec):
+    # 1,024 probes over 65,536 sources: hundreds of source chunks whose
+    # partials the few-target reduce adds in a fixed order (64 targets x 16
+    # chunk groups a workgroup), so a second call gives the same bits; the
+    # sums still match the oracle's
+    x, z, c, s = config2_blobs(65536, seed=31)
+    tx = np.linspace(-1, 15, 1024)
+    tz = np.zeros(1024)
+    u1, w1 = gpu_targets(gpu_ctx, x, z, c, s, tx, tz, prec)
+    u2, w2 = gpu_targets(gpu_ctx, x, z, c, s, tx, tz, prec)
+    assert np.array_equal(u1, u2) and np.array_equal(w1, w2)
+    uo, wo = oracle.bs_targets(x, z, c, s, tx, tz)
+    if prec == dvm.F64:
+        assert rel_err(u1, uo) <= F64_TOL and rel_err(w1, wo) <= F64_TOL
+    else:
+        norm_max, p99 = _f32c_errors(u1, w1, uo, wo)
+        assert norm_max <= F32C_TOL and p99 <= F32C_TOL, (norm_max, p99)
+
+
 def test_f32c_variant_rejected(gpu_ctx):
     with pytest.raises(dvm.DvmError) as e:
         gpu_ctx.set_f32c_variant(512, 64, 2, 4)

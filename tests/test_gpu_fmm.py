@@ -35,6 +35,25 @@ def _fmm(ctx, x, z, c, s, order, theta):
     return u.cpu().numpy(), w.cpu().numpy(), ctx.fmm_last_level()
 
 
+def test_fmm_reruns_bit_for_bit(gpu_ctx):
+    # every sum of the fast form has a fixed order (no atomics in any sum), so
+    # a second call on the same blobs gives the same bits; a dense cluster
+    # (20,000 blobs in a 0.05 square) gives targets with more than 2,048 near
+    # sources, summed by a wavefront each
+    x, z, c, s = _wake(60000, seed=21)
+    rng = np.random.default_rng(22)
+    x = np.concatenate([x, rng.uniform(0.0, 0.05, 20000)])
+    z = np.concatenate([z, rng.uniform(0.0, 0.05, 20000)])
+    c = np.concatenate([c, rng.normal(0.0, 1e-3, 20000)])
+    s = np.full(x.size, 0.01)
+    u1, w1, lv = _fmm(gpu_ctx, x, z, c, s, 12, 0.5)
+    u2, w2, _ = _fmm(gpu_ctx, x, z, c, s, 12, 0.5)
+    assert lv >= 2
+    assert np.array_equal(u1, u2) and np.array_equal(w1, w2)
+    uo, wo = oracle.bs_self(x, z, c, s)
+    assert rel_err(u1, uo) <= 2e-6 and rel_err(w1, wo) <= 2e-6
+
+
 @pytest.mark.parametrize("order,theta,tol", [(10, 0.5, 2e-6), (20, 0.5, 1e-11), (32, 1.0, 1e-8),
                                              (8, 0.7, 1e-3)])
 def test_fmm_matches_direct(gpu_ctx, order, theta, tol):

@@ -812,8 +812,8 @@ def main():
     phases = set(args.phases.split(","))
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(simulations would shard across GPUs with no communication; see DESIGN.md)",
+        print("bench.py measures one GPU only: there is no multi-GPU path "
+              "(an ensemble would shard by simulation across GPUs with no communication)",
               file=sys.stderr)
         sys.exit(2)
 

@@ -113,7 +113,7 @@ class Server:
 
     def _place(self, req, reserved):
         """A (batch entry) with a free slot for the request's class (slots
-        `reserved` this round count as taken), creating a batch if the budget
+        `reserved` in this admission pass count as taken), creating a batch if the budget
         allows; None if it must wait."""
         c = self._class_of(req)
         for entry in self.batches:
